@@ -1,0 +1,286 @@
+/*
+ * restir_abi.h — C ABI of the MI355X (gfx950) ReSTIR hot path.
+ *
+ * The entry points stand in for the compute-shader dispatches of lukedan/ReSTIR-Vulkan:
+ *
+ *   restir_initial_temporal  <- RestirPass::issueCommands software branch
+ *                               (src/passes/restirPass.h:36-53, shader src/shaders/restirOmni.glsl:86-212)
+ *   restir_spatial           <- SpatialReusePass::issueCommands
+ *                               (src/passes/spatialReusePass.h:11-26, shader src/shaders/spatialReuse.comp:30-85)
+ *   restir_unbiased          <- UnbiasedReusePass::issueCommands software branch
+ *                               (src/passes/unbiasedReusePass.h:23-40, shader src/shaders/unbiasedReuse.glsl:50-185)
+ *   restir_trace_shadow      <- testVisibility/raytrace (src/shaders/include/visibilityTest.glsl:1-30,
+ *                               src/shaders/include/softwareRaytracing.glsl:42-85) as a batch over rays
+ *   restir_shade             <- LightingPass GBUFFER_DEBUG_NONE branch (src/shaders/lighting.frag:49-71, 103)
+ *   restir_build_aabb_tree   <- AabbTree::build (src/aabbTreeBuilder.cpp:52-214)
+ *   restir_build_alias_table <- createAliasTable (src/misc.cpp:418-497)
+ *   restir_set_lights        <- descriptor set 0 bindings 0-2 of RestirPass (restirPass.h:371-423),
+ *                               buffers packed as SceneBuffers::create does (src/sceneBuffers.h:100-124, 241-270)
+ *   restir_set_bvh           <- descriptor set 2 of RestirPass / set 1 of UnbiasedReusePass
+ *                               (AabbTreeBuffers::create, src/aabbTreeBuilder.h:25-51)
+ *
+ * Conventions: every call returns 0 on success or a negative RESTIR_E_* code; restir_last_error()
+ * returns the message of the last failure on that context. Device pointers are caller-owned.
+ * Work is ordered on the caller's HIP stream (passed as an opaque pointer; NULL = default stream).
+ * A pass never reads the buffer it writes (the reference always ping-pongs; in == out is rejected).
+ * One context per host thread.
+ */
+#ifndef RESTIR_ABI_H
+#define RESTIR_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------------------------------------------
+ * Wire formats. Byte-identical to the reference's GLSL std430/std140 structs.
+ * -------------------------------------------------------------------------------------------- */
+
+/* LightSample, src/shaders/include/structs/restirStructs.glsl:19-29 (48 B). */
+typedef struct restir_light_sample {
+    float position_emissionLum[4]; /* xyz light sample position, w emission luminance */
+    float normal[4];               /* xyz light normal, w > 0.5: triangle-light normal valid */
+    int32_t lightIndex;            /* >= 0 point light; < 0 triangle light -1-idx */
+    float pHat;
+    float sumWeights;
+    float w;
+} restir_light_sample;
+
+/* Reservoir with RESERVOIR_SIZE 1, restirStructs.glsl:31-34 (64 B, 16-byte aligned). */
+typedef struct restir_reservoir {
+    restir_light_sample samples[1];
+    uint32_t numStreamSamples;
+    uint32_t _pad[3];
+} restir_reservoir;
+
+/* RestirUniforms, restirStructs.glsl:40-56 (std140, 128 B). Matrix is column-major. */
+typedef struct restir_uniforms {
+    float prevFrameProjectionViewMatrix[16];
+    float cameraPos[4];
+    uint32_t screenSize[2];
+    uint32_t frame;
+    uint32_t initialLightSampleCount;
+    uint32_t temporalSampleCountMultiplier;
+    float spatialPosThreshold;
+    float spatialNormalThreshold; /* degrees */
+    uint32_t spatialNeighbors;
+    float spatialRadius;
+    int32_t flags;
+    uint32_t _pad[2];
+} restir_uniforms;
+
+#define RESTIR_VISIBILITY_REUSE_FLAG (1 << 0) /* restirStructs.glsl:37 */
+#define RESTIR_TEMPORAL_REUSE_FLAG (1 << 1)   /* restirStructs.glsl:38 */
+
+/* AabbTreeNode, src/shaders/include/structs/aabbTree.glsl:1-8 (80 B). Child < 0 is leaf ~triIndex. */
+typedef struct restir_aabb_tree_node {
+    float leftAabbMin[4];
+    float leftAabbMax[4];
+    float rightAabbMin[4];
+    float rightAabbMax[4];
+    int32_t leftChild;
+    int32_t rightChild;
+    int32_t _pad[2];
+} restir_aabb_tree_node;
+
+/* Triangle, aabbTree.glsl:9-13 (48 B, w = 1). */
+typedef struct restir_triangle {
+    float p1[4];
+    float p2[4];
+    float p3[4];
+} restir_triangle;
+
+/* pointLight, src/shaders/include/structs/light.glsl:2-5 (32 B). */
+typedef struct restir_point_light {
+    float pos[4];
+    float color_luminance[4]; /* w = luminance */
+} restir_point_light;
+
+/* triLight, light.glsl:7-13 (80 B). */
+typedef struct restir_tri_light {
+    float p1[4];
+    float p2[4];
+    float p3[4];
+    float emission_luminance[4]; /* w = luminance */
+    float normalArea[4];         /* xyz unit normal, w area */
+} restir_tri_light;
+
+/* aliasTableColumn, light.glsl:15-20 (16 B). */
+typedef struct restir_alias_table_column {
+    float prob;
+    int32_t alias;
+    float oriProb;
+    float aliasOriProb;
+} restir_alias_table_column;
+
+/* Light/alias buffers are the reference's SSBO images: a 16-byte header whose first int is the
+ * element count, then the array (sceneBuffers.h:100-124, misc.h:28-30). */
+#define RESTIR_LIGHT_BUFFER_HEADER_BYTES 16
+
+/* G-buffer as structure-of-arrays float planes, row-major, row pitch = width. Replaces the
+ * sampler2D images of the reference (gBufferPass.cpp:75-108, gBuffer.frag:27-80).
+ * position/normal: 3 planes (x, y, z). albedo: 4 planes (r, g, b, emissive flag > 0.5).
+ * material: 2 planes (roughness, metallic). depth: 1 plane of hardware depth (z/w, [0,1]).
+ * A pixel whose normal is (0,0,0) is background. plane_stride is in floats (>= width*height). */
+typedef struct restir_gbuffer {
+    const float *position;
+    const float *normal;
+    const float *albedo;
+    const float *material;
+    const float *depth;
+    uint64_t plane_stride;
+    uint32_t width;
+    uint32_t height;
+} restir_gbuffer;
+
+/* Launch tile for the per-pixel passes (pixels per workgroup = tile_x * tile_y, a multiple of
+ * 64). {0,0} selects the built-in default. */
+typedef struct restir_launch {
+    uint32_t tile_x;
+    uint32_t tile_y;
+} restir_launch;
+
+/* ----------------------------------------------------------------------------------------------
+ * Error codes
+ * -------------------------------------------------------------------------------------------- */
+#define RESTIR_OK 0
+#define RESTIR_E_INVALID_ARGUMENT (-1)
+#define RESTIR_E_SIZE_MISMATCH (-2)
+#define RESTIR_E_NOT_READY (-3)   /* lights or BVH not set */
+#define RESTIR_E_HIP (-4)         /* HIP runtime failure */
+#define RESTIR_E_BAD_TREE (-5)    /* child index out of range, node count != tris - 1, too deep */
+#define RESTIR_E_ALIASING (-6)    /* output buffer aliases an input buffer */
+#define RESTIR_E_NO_DEVICE (-7)
+
+typedef struct restir_ctx restir_ctx;
+
+/* ----------------------------------------------------------------------------------------------
+ * Context
+ * -------------------------------------------------------------------------------------------- */
+int restir_create(restir_ctx **out, int device);
+void restir_destroy(restir_ctx *ctx);
+const char *restir_last_error(const restir_ctx *ctx);
+int restir_version(void);
+
+/* Light tables: device pointers to the reference buffer images (header + array). The counts are
+ * read back once here and validated against the byte sizes. A NULL triangle buffer means none.
+ * Replaces descriptor set 0 bindings 0..2 of restirOmni.glsl:11-23. */
+int restir_set_lights(restir_ctx *ctx, const void *point_lights, size_t point_bytes,
+                      const void *tri_lights, size_t tri_bytes, const void *alias_table,
+                      size_t alias_bytes);
+
+/* BVH: device pointers in the AabbTreeBuffers layout (aabbTreeBuilder.h:25-51). The tree is
+ * read back once to validate child indices and to measure its depth. */
+int restir_set_bvh(restir_ctx *ctx, const restir_aabb_tree_node *nodes, uint32_t n_nodes,
+                   const restir_triangle *tris, uint32_t n_tris);
+
+/* Pass A: initial RIS + visibility reuse + temporal reuse (restirOmni.glsl:86-212).
+ * prev / prev_res are read only when RESTIR_TEMPORAL_REUSE_FLAG is set. */
+int restir_initial_temporal(restir_ctx *ctx, const restir_uniforms *u, const restir_gbuffer *cur,
+                            const restir_gbuffer *prev, const restir_reservoir *prev_res,
+                            restir_reservoir *out, const restir_launch *launch, void *stream);
+
+/* Pass B: biased spatial reuse, push constant iter (spatialReuse.comp:30-85). */
+int restir_spatial(restir_ctx *ctx, const restir_uniforms *u, int iter, const restir_gbuffer *g,
+                   const restir_reservoir *in, restir_reservoir *out, const restir_launch *launch,
+                   void *stream);
+
+/* Pass C: unbiased spatial reuse with k neighbours, 1 <= k <= 16 (unbiasedReuse.glsl:50-185,
+ * where k is the compile-time NUM_NEIGHBORS = 3). The pass seeds its generator with
+ * frame*17 + iter; iter = 0 is the reference's single pass, iter > 0 decorrelates further passes. */
+int restir_unbiased(restir_ctx *ctx, const restir_uniforms *u, int k, int iter,
+                    const restir_gbuffer *g, const restir_reservoir *in, restir_reservoir *out,
+                    const restir_launch *launch, void *stream);
+
+/* Final shading (lighting.frag:49-71, 103): rgb = sum p_hat_full(y) * W, emissive pixels show
+ * their albedo, background pixels are 0, then pow(rgb, 1/gamma). Writes 3 float planes. */
+int restir_shade(restir_ctx *ctx, const float camera_pos[3], float gamma, const restir_gbuffer *g,
+                 const restir_reservoir *res, float *rgb_planes, uint64_t rgb_plane_stride,
+                 void *stream);
+
+/* Shadow rays p1 -> p2 (float4 each, w ignored) through testVisibility semantics: the segment is
+ * shrunk by 0.001 at both ends. shadowed[i] = 1 if occluded. node_visits / tri_tests may be NULL.
+ * loop = RESTIR_TRACE_REFERENCE_LOOP reproduces the reference's loop exactly (triangle hits take
+ * effect every 8 node visits, softwareRaytracing.glsl:39-77), so node_visits equals its
+ * loop-iteration count; RESTIR_TRACE_FIRST_HIT stops a ray at its first hit. Both give the same
+ * shadowed[] values. */
+#define RESTIR_TRACE_FIRST_HIT 0
+#define RESTIR_TRACE_REFERENCE_LOOP 1
+int restir_trace_shadow(restir_ctx *ctx, const float *p1, const float *p2, uint32_t n,
+                        uint32_t *shadowed, uint32_t *node_visits, uint32_t *tri_tests, int loop,
+                        void *stream);
+
+/* ----------------------------------------------------------------------------------------------
+ * Host-side builders (no device needed)
+ * -------------------------------------------------------------------------------------------- */
+
+/* AabbTree::build over n_tris triangles given in scene order (tri_xyz: 9 floats per triangle).
+ * Writes n_tris - 1 nodes and the n_tris triangles (w = 1). n_tris must be >= 2. Returns the tree
+ * depth (root = 1) through *depth_out when non-NULL. */
+int restir_build_aabb_tree(const float *tri_xyz, uint32_t n_tris, restir_aabb_tree_node *nodes,
+                           restir_triangle *tris, uint32_t *depth_out);
+
+/* createAliasTable: point lights if n_point > 0, else triangle lights. Writes n columns. */
+int restir_build_alias_table(const restir_point_light *points, uint32_t n_point,
+                             const restir_tri_light *tris, uint32_t n_tri,
+                             restir_alias_table_column *columns);
+
+/* ----------------------------------------------------------------------------------------------
+ * Scene-side helpers for synthetic inputs (no reference counterpart; the reference rasterises).
+ * -------------------------------------------------------------------------------------------- */
+
+/* Primary-ray G-buffer over the BVH set on ctx (closest hit, ties to the lower triangle index).
+ * Camera follows src/camera.h:25-50: view rows (right, -up, forward), projection with
+ * f = 1/tan(fovY/2), depth = (zFar*z - zNear*zFar)/((zFar - zNear)*z).
+ * tri_material: one material id per triangle; materials: 8 floats each
+ * (albedo r,g,b, emissive flag, roughness, metallic, 0, 0). Writes all G-buffer planes. */
+typedef struct restir_camera {
+    float position[3];
+    float lookAt[3];
+    float worldUp[3];
+    float fovYRadians;
+    float zNear;
+    float zFar;
+} restir_camera;
+
+int restir_primary_gbuffer(restir_ctx *ctx, const restir_camera *cam, const int32_t *tri_material,
+                           const float *materials, uint32_t n_materials, float *position,
+                           float *normal, float *albedo, float *material, float *depth,
+                           uint64_t plane_stride, uint32_t width, uint32_t height, void *stream);
+
+/* The 4x4 projection-view matrix of camera.h:25-50 for the given aspect, column-major, as the
+ * reference writes it into RestirUniforms::prevFrameProjectionViewMatrix (app.cpp:779). */
+int restir_camera_projection_view(const restir_camera *cam, float aspect, float out_col_major[16]);
+
+#ifdef __cplusplus
+}
+
+static_assert(sizeof(restir_light_sample) == 48, "LightSample is 48 B");
+static_assert(sizeof(restir_reservoir) == 64, "Reservoir is 64 B");
+static_assert(offsetof(restir_reservoir, numStreamSamples) == 48, "numStreamSamples @48");
+static_assert(offsetof(restir_light_sample, lightIndex) == 32, "lightIndex @32");
+static_assert(offsetof(restir_light_sample, w) == 44, "w @44");
+static_assert(sizeof(restir_uniforms) == 128, "RestirUniforms is 128 B");
+static_assert(offsetof(restir_uniforms, cameraPos) == 64, "cameraPos @64");
+static_assert(offsetof(restir_uniforms, screenSize) == 80, "screenSize @80");
+static_assert(offsetof(restir_uniforms, frame) == 88, "frame @88");
+static_assert(offsetof(restir_uniforms, initialLightSampleCount) == 92, "M @92");
+static_assert(offsetof(restir_uniforms, temporalSampleCountMultiplier) == 96, "mult @96");
+static_assert(offsetof(restir_uniforms, spatialPosThreshold) == 100, "posThr @100");
+static_assert(offsetof(restir_uniforms, spatialNormalThreshold) == 104, "norThr @104");
+static_assert(offsetof(restir_uniforms, spatialNeighbors) == 108, "k @108");
+static_assert(offsetof(restir_uniforms, spatialRadius) == 112, "radius @112");
+static_assert(offsetof(restir_uniforms, flags) == 116, "flags @116");
+static_assert(sizeof(restir_aabb_tree_node) == 80, "AabbTreeNode is 80 B");
+static_assert(offsetof(restir_aabb_tree_node, leftChild) == 64, "leftChild @64");
+static_assert(offsetof(restir_aabb_tree_node, rightChild) == 68, "rightChild @68");
+static_assert(sizeof(restir_triangle) == 48, "Triangle is 48 B");
+static_assert(sizeof(restir_point_light) == 32, "pointLight is 32 B");
+static_assert(sizeof(restir_tri_light) == 80, "triLight is 80 B");
+static_assert(sizeof(restir_alias_table_column) == 16, "aliasTableColumn is 16 B");
+#endif
+
+#endif /* RESTIR_ABI_H */

@@ -1,0 +1,364 @@
+// abi.hip — the C ABI declared in include/restir_abi.h: argument validation, scene-table setup and
+// stream-ordered kernel launches. Every entry point returns 0 or a negative RESTIR_E_* code and
+// records a message retrievable through restir_last_error().
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+struct restir_ctx {
+    int device = 0;
+    std::string error;
+    rk::SceneView scene{};
+    bool lightsSet = false;
+    bool bvhSet = false;
+    int stack = 32;
+    uint32_t treeDepth = 0;
+};
+
+namespace {
+
+int fail(restir_ctx *ctx, int code, const char *fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        std::vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->error = buf;
+    }
+    return code;
+}
+
+int hip_fail(restir_ctx *ctx, hipError_t e, const char *what) {
+    return fail(ctx, RESTIR_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int use_device(restir_ctx *ctx) {
+    const hipError_t e = hipSetDevice(ctx->device);
+    return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "hipSetDevice");
+}
+
+bool overlaps(const void *a, size_t na, const void *b, size_t nb) {
+    const char *pa = static_cast<const char *>(a), *pb = static_cast<const char *>(b);
+    return pa < pb + nb && pb < pa + na;
+}
+
+int check_gbuffer(restir_ctx *ctx, const restir_gbuffer *g, uint32_t W, uint32_t H,
+                  bool needDepth, const char *name) {
+    if (!g) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "%s: G-buffer is NULL", name);
+    if (!g->position || !g->normal || !g->albedo || !g->material || (needDepth && !g->depth))
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "%s: G-buffer plane pointer is NULL", name);
+    if (g->width != W || g->height != H)
+        return fail(ctx, RESTIR_E_SIZE_MISMATCH, "%s: G-buffer is %ux%u but screenSize is %ux%u",
+                    name, g->width, g->height, W, H);
+    if (g->plane_stride < static_cast<uint64_t>(W) * H)
+        return fail(ctx, RESTIR_E_SIZE_MISMATCH, "%s: plane_stride %llu < width*height", name,
+                    static_cast<unsigned long long>(g->plane_stride));
+    return RESTIR_OK;
+}
+
+rk::GView view_of(const restir_gbuffer *g) {
+    rk::GView v{};
+    if (!g) return v;
+    v.pos = g->position;
+    v.nrm = g->normal;
+    v.alb = g->albedo;
+    v.mat = g->material;
+    v.depth = g->depth;
+    v.P = g->plane_stride;
+    v.W = g->width;
+    v.H = g->height;
+    return v;
+}
+
+int check_frame(restir_ctx *ctx, const restir_uniforms *u, const char *name) {
+    if (!ctx) return RESTIR_E_INVALID_ARGUMENT;
+    if (!u) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "%s: uniforms are NULL", name);
+    if (u->screenSize[0] == 0 || u->screenSize[1] == 0)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "%s: empty screen", name);
+    if (static_cast<uint64_t>(u->screenSize[0]) * u->screenSize[1] > (1ull << 31))
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "%s: screen too large", name);
+    return RESTIR_OK;
+}
+
+int pick_tile(restir_ctx *ctx, const restir_launch *l, int *tile) {
+    if (!l || (l->tile_x == 0 && l->tile_y == 0)) {
+        *tile = 0;
+        return RESTIR_OK;
+    }
+    *tile = rk::tile_index(l->tile_x, l->tile_y);
+    if (*tile < 0)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "tile %ux%u is not compiled in", l->tile_x, l->tile_y);
+    return RESTIR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int restir_version(void) { return 1; }
+
+int restir_create(restir_ctx **out, int device) {
+    if (!out) return RESTIR_E_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return RESTIR_E_NO_DEVICE;
+    if (device < 0 || device >= n) return RESTIR_E_NO_DEVICE;
+    auto *ctx = new restir_ctx();
+    ctx->device = device;
+    if (use_device(ctx) != RESTIR_OK) {
+        delete ctx;
+        return RESTIR_E_HIP;
+    }
+    *out = ctx;
+    return RESTIR_OK;
+}
+
+void restir_destroy(restir_ctx *ctx) { delete ctx; }
+
+const char *restir_last_error(const restir_ctx *ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+int restir_set_lights(restir_ctx *ctx, const void *pointBuf, size_t pointBytes, const void *triBuf,
+                      size_t triBytes, const void *aliasBuf, size_t aliasBytes) {
+    if (!ctx) return RESTIR_E_INVALID_ARGUMENT;
+    ctx->lightsSet = false;
+    if (int r = use_device(ctx)) return r;
+    if (!pointBuf || !aliasBuf)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "set_lights: point-light and alias buffers are required");
+    const size_t hdr = RESTIR_LIGHT_BUFFER_HEADER_BYTES;
+    if (pointBytes < hdr || aliasBytes < hdr || (triBuf && triBytes < hdr))
+        return fail(ctx, RESTIR_E_SIZE_MISMATCH, "set_lights: buffer shorter than its 16-byte header");
+    int32_t counts[3] = {0, 0, 0};
+    hipError_t e = hipMemcpy(&counts[0], pointBuf, 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && triBuf) e = hipMemcpy(&counts[1], triBuf, 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&counts[2], aliasBuf, 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(ctx, e, "set_lights: reading buffer headers");
+    const int32_t np = counts[0], nt = counts[1], na = counts[2];
+    if (np < 0 || nt < 0 || na < 0)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "set_lights: negative count in a header");
+    if (hdr + static_cast<size_t>(np) * sizeof(restir_point_light) > pointBytes)
+        return fail(ctx, RESTIR_E_SIZE_MISMATCH, "set_lights: %d point lights exceed %zu bytes", np, pointBytes);
+    if (triBuf && hdr + static_cast<size_t>(nt) * sizeof(restir_tri_light) > triBytes)
+        return fail(ctx, RESTIR_E_SIZE_MISMATCH, "set_lights: %d triangle lights exceed %zu bytes", nt, triBytes);
+    if (hdr + static_cast<size_t>(na) * sizeof(restir_alias_table_column) > aliasBytes)
+        return fail(ctx, RESTIR_E_SIZE_MISMATCH, "set_lights: %d alias columns exceed %zu bytes", na, aliasBytes);
+    // the alias table covers point lights if there are any, else triangle lights (misc.cpp:426-442)
+    const int32_t expect = np > 0 ? np : nt;
+    if (expect == 0) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "set_lights: no lights");
+    if (na != expect)
+        return fail(ctx, RESTIR_E_SIZE_MISMATCH, "set_lights: alias table has %d columns, expected %d", na, expect);
+    const char *pb = static_cast<const char *>(pointBuf);
+    const char *ab = static_cast<const char *>(aliasBuf);
+    {   // every alias must name a light: the kernels index the light arrays with it
+        std::vector<restir_alias_table_column> cols(static_cast<size_t>(na));
+        const hipError_t ce = hipMemcpy(cols.data(), ab + hdr, sizeof(restir_alias_table_column) * cols.size(),
+                                        hipMemcpyDeviceToHost);
+        if (ce != hipSuccess) return hip_fail(ctx, ce, "set_lights: reading alias table");
+        for (size_t i = 0; i < cols.size(); ++i)
+            if (cols[i].alias < 0 || cols[i].alias >= na)
+                return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "set_lights: alias column %zu names light %d",
+                            i, cols[i].alias);
+    }
+    ctx->scene.points = reinterpret_cast<const restir_point_light *>(pb + hdr);
+    ctx->scene.triLights = triBuf ? reinterpret_cast<const restir_tri_light *>(static_cast<const char *>(triBuf) + hdr) : nullptr;
+    ctx->scene.alias = reinterpret_cast<const float4 *>(ab + hdr);
+    ctx->scene.nPoints = np;
+    ctx->scene.nTriLights = nt;
+    ctx->scene.nAlias = na;
+    ctx->lightsSet = true;
+    return RESTIR_OK;
+}
+
+int restir_set_bvh(restir_ctx *ctx, const restir_aabb_tree_node *nodes, uint32_t nNodes,
+                   const restir_triangle *tris, uint32_t nTris) {
+    if (!ctx) return RESTIR_E_INVALID_ARGUMENT;
+    ctx->bvhSet = false;
+    if (int r = use_device(ctx)) return r;
+    if (!nodes || !tris) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "set_bvh: NULL buffer");
+    if (nTris < 2 || nNodes != nTris - 1)
+        return fail(ctx, RESTIR_E_BAD_TREE, "set_bvh: %u nodes for %u triangles (need tris-1, tris>=2)", nNodes, nTris);
+    std::vector<restir_aabb_tree_node> host(nNodes);
+    const hipError_t e = hipMemcpy(host.data(), nodes, sizeof(restir_aabb_tree_node) * nNodes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(ctx, e, "set_bvh: reading nodes");
+    // validate: every node reached exactly once from node 0, every triangle referenced once
+    std::vector<uint8_t> seenNode(nNodes, 0), seenTri(nTris, 0);
+    std::vector<std::pair<int32_t, uint32_t>> todo{{0, 1u}};
+    uint32_t depth = 0, reached = 0, leaves = 0;
+    seenNode[0] = 1;
+    while (!todo.empty()) {
+        const auto [id, d] = todo.back();
+        todo.pop_back();
+        ++reached;
+        if (d > depth) depth = d;
+        for (int32_t c : {host[id].leftChild, host[id].rightChild}) {
+            if (c >= 0) {
+                if (static_cast<uint32_t>(c) >= nNodes || seenNode[c])
+                    return fail(ctx, RESTIR_E_BAD_TREE, "set_bvh: node %d has bad child %d", id, c);
+                seenNode[c] = 1;
+                todo.push_back({c, d + 1});
+            } else {
+                const uint32_t t = static_cast<uint32_t>(~c);
+                if (t >= nTris || seenTri[t])
+                    return fail(ctx, RESTIR_E_BAD_TREE, "set_bvh: node %d has bad leaf %d", id, ~c);
+                seenTri[t] = 1;
+                ++leaves;
+            }
+        }
+    }
+    if (reached != nNodes || leaves != nTris)
+        return fail(ctx, RESTIR_E_BAD_TREE, "set_bvh: %u of %u nodes and %u of %u triangles reachable",
+                    reached, nNodes, leaves, nTris);
+    if (depth > 64)
+        return fail(ctx, RESTIR_E_BAD_TREE, "set_bvh: tree depth %u exceeds the 64-entry traversal stack", depth);
+    ctx->scene.nodes = reinterpret_cast<const rt::Node *>(nodes);
+    ctx->scene.tris = reinterpret_cast<const float4 *>(tris);
+    ctx->stack = depth <= 32 ? 32 : 64;
+    ctx->treeDepth = depth;
+    ctx->bvhSet = true;
+    return RESTIR_OK;
+}
+
+int restir_initial_temporal(restir_ctx *ctx, const restir_uniforms *u, const restir_gbuffer *cur,
+                            const restir_gbuffer *prev, const restir_reservoir *prevRes,
+                            restir_reservoir *out, const restir_launch *launch, void *stream) {
+    if (int r = check_frame(ctx, u, "initial_temporal")) return r;
+    if (!ctx->lightsSet) return fail(ctx, RESTIR_E_NOT_READY, "initial_temporal: lights not set");
+    const bool vis = (u->flags & RESTIR_VISIBILITY_REUSE_FLAG) != 0;
+    const bool temporal = (u->flags & RESTIR_TEMPORAL_REUSE_FLAG) != 0;
+    if (vis && !ctx->bvhSet) return fail(ctx, RESTIR_E_NOT_READY, "initial_temporal: BVH not set");
+    const uint32_t W = u->screenSize[0], H = u->screenSize[1];
+    if (int r = check_gbuffer(ctx, cur, W, H, false, "initial_temporal cur")) return r;
+    if (!out) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "initial_temporal: output is NULL");
+    const size_t bytes = sizeof(restir_reservoir) * W * H;
+    if (temporal) {
+        if (int r = check_gbuffer(ctx, prev, W, H, false, "initial_temporal prev")) return r;
+        if (!prevRes) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "initial_temporal: previous reservoirs are NULL");
+        if (overlaps(prevRes, bytes, out, bytes))
+            return fail(ctx, RESTIR_E_ALIASING, "initial_temporal: output aliases the previous reservoirs");
+    }
+    int tile;
+    if (int r = pick_tile(ctx, launch, &tile)) return r;
+    if (int r = use_device(ctx)) return r;
+    const hipError_t e = rk::launch_pass_a(tile, ctx->stack, *u, ctx->scene, view_of(cur), view_of(prev),
+                                           prevRes, out, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "initial_temporal launch");
+}
+
+int restir_spatial(restir_ctx *ctx, const restir_uniforms *u, int iter, const restir_gbuffer *g,
+                   const restir_reservoir *in, restir_reservoir *out, const restir_launch *launch,
+                   void *stream) {
+    if (int r = check_frame(ctx, u, "spatial")) return r;
+    const uint32_t W = u->screenSize[0], H = u->screenSize[1];
+    if (int r = check_gbuffer(ctx, g, W, H, true, "spatial")) return r;
+    if (!in || !out) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "spatial: reservoir buffer is NULL");
+    const size_t bytes = sizeof(restir_reservoir) * W * H;
+    if (overlaps(in, bytes, out, bytes)) return fail(ctx, RESTIR_E_ALIASING, "spatial: in and out overlap");
+    int tile;
+    if (int r = pick_tile(ctx, launch, &tile)) return r;
+    if (int r = use_device(ctx)) return r;
+    const float cosThr = rt::cos_degrees(u->spatialNormalThreshold);
+    const hipError_t e = rk::launch_pass_b(tile, *u, iter, cosThr, view_of(g), in, out,
+                                           static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "spatial launch");
+}
+
+int restir_unbiased(restir_ctx *ctx, const restir_uniforms *u, int k, int iter,
+                    const restir_gbuffer *g, const restir_reservoir *in, restir_reservoir *out,
+                    const restir_launch *launch, void *stream) {
+    if (int r = check_frame(ctx, u, "unbiased")) return r;
+    if (k < 1 || k > 16) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "unbiased: k=%d outside [1,16]", k);
+    const bool vis = (u->flags & RESTIR_VISIBILITY_REUSE_FLAG) != 0;
+    if (vis && !ctx->bvhSet) return fail(ctx, RESTIR_E_NOT_READY, "unbiased: BVH not set");
+    const uint32_t W = u->screenSize[0], H = u->screenSize[1];
+    if (int r = check_gbuffer(ctx, g, W, H, false, "unbiased")) return r;
+    if (!in || !out) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "unbiased: reservoir buffer is NULL");
+    const size_t bytes = sizeof(restir_reservoir) * W * H;
+    if (overlaps(in, bytes, out, bytes)) return fail(ctx, RESTIR_E_ALIASING, "unbiased: in and out overlap");
+    int tile;
+    if (int r = pick_tile(ctx, launch, &tile)) return r;
+    if (int r = use_device(ctx)) return r;
+    const hipError_t e = rk::launch_pass_c(tile, ctx->stack, *u, k, iter, ctx->scene, view_of(g), in, out,
+                                           static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "unbiased launch");
+}
+
+int restir_shade(restir_ctx *ctx, const float cameraPos[3], float gamma, const restir_gbuffer *g,
+                 const restir_reservoir *res, float *rgb, uint64_t rgbStride, void *stream) {
+    if (!ctx) return RESTIR_E_INVALID_ARGUMENT;
+    if (!ctx->lightsSet) return fail(ctx, RESTIR_E_NOT_READY, "shade: lights not set");
+    if (!cameraPos || !res || !rgb) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "shade: NULL argument");
+    if (!g) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "shade: G-buffer is NULL");
+    if (int r = check_gbuffer(ctx, g, g->width, g->height, false, "shade")) return r;
+    if (rgbStride < static_cast<uint64_t>(g->width) * g->height)
+        return fail(ctx, RESTIR_E_SIZE_MISMATCH, "shade: rgb plane stride too small");
+    if (!(gamma > 0.0f)) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "shade: gamma must be > 0");
+    if (int r = use_device(ctx)) return r;
+    const hipError_t e = rk::launch_shade(ctx->scene, rt::f3{cameraPos[0], cameraPos[1], cameraPos[2]},
+                                          gamma, view_of(g), res, rgb, rgbStride,
+                                          static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "shade launch");
+}
+
+int restir_trace_shadow(restir_ctx *ctx, const float *p1, const float *p2, uint32_t n,
+                        uint32_t *shadowed, uint32_t *visits, uint32_t *tests, int loop, void *stream) {
+    if (!ctx) return RESTIR_E_INVALID_ARGUMENT;
+    if (!ctx->bvhSet) return fail(ctx, RESTIR_E_NOT_READY, "trace_shadow: BVH not set");
+    if (n == 0) return RESTIR_OK;
+    if (!p1 || !p2 || !shadowed) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "trace_shadow: NULL buffer");
+    if (loop != RESTIR_TRACE_FIRST_HIT && loop != RESTIR_TRACE_REFERENCE_LOOP)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "trace_shadow: unknown loop mode %d", loop);
+    if (int r = use_device(ctx)) return r;
+    const hipError_t e = rk::launch_trace(ctx->stack, loop == RESTIR_TRACE_REFERENCE_LOOP, ctx->scene,
+                                          reinterpret_cast<const float4 *>(p1),
+                                          reinterpret_cast<const float4 *>(p2), n, shadowed, visits,
+                                          tests, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "trace_shadow launch");
+}
+
+int restir_primary_gbuffer(restir_ctx *ctx, const restir_camera *cam, const int32_t *triMaterial,
+                           const float *materials, uint32_t nMaterials, float *position,
+                           float *normal, float *albedo, float *material, float *depth,
+                           uint64_t planeStride, uint32_t width, uint32_t height, void *stream) {
+    if (!ctx) return RESTIR_E_INVALID_ARGUMENT;
+    if (!ctx->bvhSet) return fail(ctx, RESTIR_E_NOT_READY, "primary_gbuffer: BVH not set");
+    if (!cam || !triMaterial || !materials || !position || !normal || !albedo || !material || !depth)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "primary_gbuffer: NULL argument");
+    if (nMaterials == 0) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "primary_gbuffer: no materials");
+    if (width == 0 || height == 0 || planeStride < static_cast<uint64_t>(width) * height)
+        return fail(ctx, RESTIR_E_SIZE_MISMATCH, "primary_gbuffer: bad size");
+    if (int r = use_device(ctx)) return r;
+    const rk::CamBasis cb = rk::make_cam_basis(*cam, static_cast<float>(width) / static_cast<float>(height));
+    const hipError_t e = rk::launch_primary(ctx->stack, ctx->scene, cb, triMaterial, materials, position,
+                                            normal, albedo, material, depth, planeStride, width, height,
+                                            static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "primary_gbuffer launch");
+}
+
+int restir_camera_projection_view(const restir_camera *cam, float aspect, float out[16]) {
+    if (!cam || !out || !(aspect > 0.0f)) return RESTIR_E_INVALID_ARGUMENT;
+    const rk::CamBasis b = rk::make_cam_basis(*cam, aspect);
+    const float f = 1.0f / tanf(0.5f * cam->fovYRadians);
+    const float view[4][4] = {{b.right.x, b.right.y, b.right.z, -rt::dot(b.right, b.pos)},
+                              {b.down.x, b.down.y, b.down.z, -rt::dot(b.down, b.pos)},
+                              {b.fwd.x, b.fwd.y, b.fwd.z, -rt::dot(b.fwd, b.pos)},
+                              {0.0f, 0.0f, 0.0f, 1.0f}};
+    const float proj[4][4] = {{f / aspect, 0.0f, 0.0f, 0.0f},
+                              {0.0f, f, 0.0f, 0.0f},
+                              {0.0f, 0.0f, b.A, b.B},
+                              {0.0f, 0.0f, 1.0f, 0.0f}};
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            float acc = 0.0f;
+            for (int k = 0; k < 4; ++k) acc += proj[r][k] * view[k][c];
+            out[c * 4 + r] = acc;
+        }
+    return RESTIR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+// kernels.h — launch interface between the C ABI (abi.hip) and the gfx950 kernels (passes.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/restir_abi.h"
+#include "rt_traverse.h"
+
+namespace rk {
+
+// Scene tables as the kernels see them: arrays past the 16-byte SSBO headers, counts read once.
+struct SceneView {
+    const restir_point_light *points;
+    const restir_tri_light *triLights;
+    const float4 *alias;  // aliasTableColumn as float4 {prob, alias bits, oriProb, aliasOriProb}
+    int32_t nPoints, nTriLights, nAlias;
+    const rt::Node *nodes;
+    const float4 *tris;  // 3 float4 per triangle
+};
+
+struct GView {
+    const float *pos, *nrm, *alb, *mat, *depth;
+    uint64_t P;  // plane stride in floats
+    uint32_t W, H;
+};
+
+struct Tile {
+    uint32_t tx, ty;
+};
+
+// Tile shapes compiled in (pixels per workgroup = tx*ty). The first is the default.
+constexpr int kNumTiles = 5;
+extern const Tile kTiles[kNumTiles];
+int tile_index(uint32_t tx, uint32_t ty);  // -1 if not compiled
+
+hipError_t launch_pass_a(int tile, int stack, const restir_uniforms &u, const SceneView &sc,
+                         const GView &cur, const GView &prev, const restir_reservoir *prevRes,
+                         restir_reservoir *out, hipStream_t s);
+hipError_t launch_pass_b(int tile, const restir_uniforms &u, int iter, float cosThr,
+                         const GView &g, const restir_reservoir *in, restir_reservoir *out,
+                         hipStream_t s);
+hipError_t launch_pass_c(int tile, int stack, const restir_uniforms &u, int k, int iter,
+                         const SceneView &sc,
+                         const GView &g, const restir_reservoir *in, restir_reservoir *out,
+                         hipStream_t s);
+hipError_t launch_shade(const SceneView &sc, rt::f3 cam, float gamma, const GView &g,
+                        const restir_reservoir *res, float *rgb, uint64_t rgbP, hipStream_t s);
+hipError_t launch_trace(int stack, bool refLoop, const SceneView &sc, const float4 *p1,
+                        const float4 *p2, uint32_t n, uint32_t *shadowed, uint32_t *visits,
+                        uint32_t *tests, hipStream_t s);
+
+struct CamBasis {
+    rt::f3 pos, fwd, right, down;
+    float sx, sy, A, B;
+};
+CamBasis make_cam_basis(const restir_camera &c, float aspect);
+hipError_t launch_primary(int stack, const SceneView &sc, const CamBasis &cb,
+                          const int32_t *triMaterial, const float *materials, float *pos,
+                          float *nrm, float *alb, float *mat, float *depth, uint64_t P, uint32_t W,
+                          uint32_t H, hipStream_t s);
+
+}  // namespace rk

@@ -1,0 +1,671 @@
+// passes.hip — gfx950 kernels of the ReSTIR pass structure:
+//   k_pass_a   initial RIS + visibility reuse + temporal reuse   (restirOmni.glsl:86-212)
+//   k_pass_b   biased spatial reuse                              (spatialReuse.comp:30-85)
+//   k_pass_c   unbiased spatial reuse                            (unbiasedReuse.glsl:50-185)
+//   k_shade    final shading                                     (lighting.frag:49-71, 103)
+//   k_trace    batch shadow rays                                 (visibilityTest.glsl, softwareRaytracing.glsl)
+//   k_primary  primary-ray G-buffer (closest hit)                — synthetic-input generator
+//
+// One lane per pixel. Workgroups cover a tx*ty pixel tile; tiles are handed to the 8 XCDs in
+// contiguous runs so that neighbouring tiles (which gather each other's reservoirs) share an L2.
+// G-buffer planes are read as SoA float planes (one coalesced dword per lane per plane);
+// reservoirs keep the reference's 64-byte AoS record and move as four 16-byte vectors.
+#include "kernels.h"
+
+namespace rk {
+
+using namespace rt;
+
+const Tile kTiles[kNumTiles] = {{16, 16}, {64, 1}, {8, 8}, {16, 8}, {32, 8}};
+
+int tile_index(uint32_t tx, uint32_t ty) {
+    for (int i = 0; i < kNumTiles; ++i)
+        if (kTiles[i].tx == tx && kTiles[i].ty == ty) return i;
+    return -1;
+}
+
+// Hardware places workgroup b on XCD b % 8. Remap so XCD k processes tiles
+// [base_k, base_k + count_k) in raster order.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+    constexpr uint32_t kX = 8;
+    if (n < kX) return b;
+    const uint32_t per = n / kX, rem = n % kX;
+    const uint32_t x = b % kX, slot = b / kX;
+    return x * per + (x < rem ? x : rem) + slot;
+}
+
+template <int TX, int TY>
+__device__ __forceinline__ void tile_pixel(uint32_t tilesX, uint32_t nTiles, uint32_t &px,
+                                           uint32_t &py) {
+    const uint32_t t = xcd_tile(blockIdx.x, nTiles);
+    const uint32_t tid = threadIdx.x;
+    px = (t % tilesX) * TX + tid % TX;
+    py = (t / tilesX) * TY + tid / TX;
+}
+
+__device__ __forceinline__ f3 plane3(const float *__restrict__ p, size_t i, uint64_t P) {
+    return f3{p[i], p[P + i], p[2 * P + i]};
+}
+
+__device__ __forceinline__ Res load_res(const restir_reservoir *__restrict__ base, size_t i) {
+    const float4 *q = reinterpret_cast<const float4 *>(base + i);
+    const float4 a = q[0], b = q[1], c = q[2], d = q[3];
+    Res r;
+    r.pos = f3{a.x, a.y, a.z};
+    r.emLum = a.w;
+    r.normal = b;
+    r.lightIndex = __float_as_int(c.x);
+    r.pHat = c.y;
+    r.sumW = c.z;
+    r.w = c.w;
+    r.M = __float_as_uint(d.x);
+    return r;
+}
+
+__device__ __forceinline__ void store_res(restir_reservoir *__restrict__ base, size_t i,
+                                          const Res &r) {
+    float4 *q = reinterpret_cast<float4 *>(base + i);
+    q[0] = make_float4(r.pos.x, r.pos.y, r.pos.z, r.emLum);
+    q[1] = r.normal;
+    q[2] = make_float4(__int_as_float(r.lightIndex), r.pHat, r.sumW, r.w);
+    q[3] = make_float4(__uint_as_float(r.M), 0.0f, 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ float phat_of(const Surface &s, const Res &r) {
+    return eval_phat(s, r.pos, f3{r.normal.x, r.normal.y, r.normal.z}, r.normal.w > 0.5f, r.emLum);
+}
+
+// ------------------------------------------------------------------------------- Pass A ---
+struct PassAArgs {
+    restir_uniforms u;
+    SceneView sc;
+    GView cur, prev;
+    const restir_reservoir *prevRes;
+    restir_reservoir *out;
+    uint32_t tilesX, nTiles;
+};
+
+template <int TX, int TY, int STACK>
+__global__ __launch_bounds__(TX *TY) void k_pass_a(const PassAArgs a) {
+    constexpr int kBlock = TX * TY;
+    __shared__ int32_t stack[STACK * kBlock];
+    uint32_t x, y;
+    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
+    const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
+    if (x >= W || y >= H) return;
+    const size_t idx = static_cast<size_t>(y) * W + x;
+    const GView &g = a.cur;
+    const f3 pos = plane3(g.pos, idx, g.P);
+    const f3 n = plane3(g.nrm, idx, g.P);
+    const f3 alb = plane3(g.alb, idx, g.P);
+    const float rough = g.mat[idx], metal = g.mat[g.P + idx];
+    const f3 cam{a.u.cameraPos[0], a.u.cameraPos[1], a.u.cameraPos[2]};
+    const float albLum = luminance(alb.x, alb.y, alb.z);
+    const Surface s = make_surface(pos, n, cam, albLum, rough, metal);
+
+    Res res = res_zero();
+    Pcg rng = pcg_seed(static_cast<uint64_t>(a.u.frame), static_cast<uint64_t>(y * 10007u + x));
+    const SceneView &sc = a.sc;
+    if (dot(n, n) != 0.0f) {
+        const uint32_t M = a.u.initialLightSampleCount;
+        for (uint32_t i = 0; i < M; ++i) {
+            const float r1 = pcg_float(rng), r2 = pcg_float(rng);
+            // aliasTableSample, restirOmni.glsl:73-83
+            int32_t col = static_cast<int32_t>(static_cast<float>(sc.nAlias) * r1);
+            col = col < sc.nAlias - 1 ? col : sc.nAlias - 1;
+            const float4 c = sc.alias[col];
+            int32_t li;
+            float prob;
+            if (c.x > r2) {
+                li = col;
+                prob = c.z;
+            } else {
+                li = __float_as_int(c.y);
+                prob = c.w;
+            }
+            if (sc.nPoints != 0) {
+                const float4 lp = *reinterpret_cast<const float4 *>(sc.points[li].pos);
+                const float lum = sc.points[li].color_luminance[3];
+                const f3 L = xyz(lp);
+                const float ph = eval_phat(s, L, f3{0.0f, 0.0f, 0.0f}, false, lum);
+                res_add_sample(res, L, make_float4(0.0f, 0.0f, 0.0f, 0.0f), lum, li, ph, prob, rng);
+            } else {
+                const restir_tri_light &tl = sc.triLights[li];
+                const f3 p1 = xyz(*reinterpret_cast<const float4 *>(tl.p1));
+                const f3 p2 = xyz(*reinterpret_cast<const float4 *>(tl.p2));
+                const f3 p3 = xyz(*reinterpret_cast<const float4 *>(tl.p3));
+                const float4 na = *reinterpret_cast<const float4 *>(tl.normalArea);
+                const float lum = tl.emission_luminance[3];
+                const float ra = pcg_float(rng), rb = pcg_float(rng);
+                const float sa = sqrtf(ra);  // pickPointOnTriangle, restirOmni.glsl:68-71
+                const f3 L = ((1.0f - sa) * p1 + (sa * (1.0f - rb)) * p2) + (rb * sa) * p3;
+                const f3 wi = normalize(pos - L);
+                const f3 nl = xyz(na);
+                prob /= fabsf(dot(wi, nl)) * na.w;
+                const float ph = eval_phat(s, L, nl, true, lum);
+                res_add_sample(res, L, make_float4(nl.x, nl.y, nl.z, 1.0f), lum, -1 - li, ph, prob, rng);
+            }
+        }
+    }
+    // Visibility reuse, restirOmni.glsl:148-160. A reservoir with sumWeights == 0 already has
+    // w == 0 and is unchanged by a shadow hit, so its ray is skipped.
+    if ((a.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) && res.sumW != 0.0f) {
+        f3 o, d;
+        shadow_segment(pos, res.pos, o, d);
+        TraceStats st{0, 0};
+        if (occluded<STACK, false>(sc.nodes, sc.tris, o, d, stack + threadIdx.x, kBlock, st)) {
+            res.w = 0.0f;
+            res.sumW = 0.0f;
+        }
+    }
+    // Temporal reuse, restirOmni.glsl:163-209 (COMPARE_DEPTH off: world-position test).
+    if (a.u.flags & RESTIR_TEMPORAL_REUSE_FLAG) {
+        const float *m = a.u.prevFrameProjectionViewMatrix;
+        float pp[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            pp[r] = m[r] * pos.x + m[4 + r] * pos.y + m[8 + r] * pos.z + m[12 + r] * 1.0f;
+        pp[0] /= pp[3];
+        pp[1] /= pp[3];
+        pp[0] = (pp[0] + 1.0f) * 0.5f * static_cast<float>(W);
+        pp[1] = (pp[1] + 1.0f) * 0.5f * static_cast<float>(H);
+        if (pp[0] > 0.0f && pp[1] > 0.0f && pp[0] < static_cast<float>(W) &&
+            pp[1] < static_cast<float>(H)) {
+            const uint32_t px = static_cast<uint32_t>(static_cast<int32_t>(pp[0]));
+            const uint32_t py = static_cast<uint32_t>(static_cast<int32_t>(pp[1]));
+            const size_t pi = static_cast<size_t>(py) * W + px;
+            const GView &pg = a.prev;
+            const f3 pd = pos - plane3(pg.pos, pi, pg.P);
+            if (dot(pd, pd) < 0.01f) {
+                const f3 ad = alb - plane3(pg.alb, pi, pg.P);
+                if (dot(ad, ad) < 0.01f) {
+                    if (dot(n, plane3(pg.nrm, pi, pg.P)) > 0.5f) {
+                        Res pr = load_res(a.prevRes, pi);
+                        const uint32_t cap = a.u.temporalSampleCountMultiplier * res.M;
+                        pr.M = pr.M < cap ? pr.M : cap;
+                        const float ph = phat_of(s, pr);
+                        res_combine(res, pr, ph, rng);
+                    }
+                }
+            }
+        }
+    }
+    store_res(a.out, idx, res);
+}
+
+// ------------------------------------------------------------------------------- Pass B ---
+struct PassBArgs {
+    restir_uniforms u;
+    GView g;
+    const restir_reservoir *in;
+    restir_reservoir *out;
+    uint32_t tilesX, nTiles;
+    uint32_t seed;
+    float cosThr;
+};
+
+template <int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void k_pass_b(const PassBArgs a) {
+    uint32_t x, y;
+    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
+    const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
+    if (x >= W || y >= H) return;
+    const size_t idx = static_cast<size_t>(y) * W + x;
+    const GView &g = a.g;
+    const f3 pos = plane3(g.pos, idx, g.P);
+    const f3 n = plane3(g.nrm, idx, g.P);
+    const f3 alb = plane3(g.alb, idx, g.P);
+    const float rough = g.mat[idx], metal = g.mat[g.P + idx];
+    const float depth = g.depth[idx];
+    const f3 cam{a.u.cameraPos[0], a.u.cameraPos[1], a.u.cameraPos[2]};
+    const Surface s = make_surface(pos, n, cam, luminance(alb.x, alb.y, alb.z), rough, metal);
+    Res res = load_res(a.in, idx);
+    Pcg rng = pcg_seed(static_cast<uint64_t>(a.seed), static_cast<uint64_t>(y * 10007u + x));
+    const float R = a.u.spatialRadius;
+    const float depthTol = a.u.spatialPosThreshold * fabsf(depth);
+    for (uint32_t i = 0; i < a.u.spatialNeighbors; ++i) {
+        const float angle = pcg_float(rng) * 2.0f * kPi;
+        const float radius = sqrtf(pcg_float(rng)) * R;
+        float sn, cs;
+        sincos_f(angle, sn, cs);
+        int32_t nx = static_cast<int32_t>(x) + static_cast<int32_t>(floorf(cs * radius));
+        int32_t ny = static_cast<int32_t>(y) + static_cast<int32_t>(floorf(sn * radius));
+        nx = nx < 0 ? 0 : (nx > static_cast<int32_t>(W) - 1 ? static_cast<int32_t>(W) - 1 : nx);
+        ny = ny < 0 ? 0 : (ny > static_cast<int32_t>(H) - 1 ? static_cast<int32_t>(H) - 1 : ny);
+        const size_t ni = static_cast<size_t>(ny) * W + static_cast<size_t>(nx);
+        const float nd = g.depth[ni];
+        const f3 nn = plane3(g.nrm, ni, g.P);
+        if (fabsf(nd - depth) > depthTol || dot(nn, n) < a.cosThr) continue;
+        const Res rr = load_res(a.in, ni);
+        const float ph = phat_of(s, rr);
+        res_combine(res, rr, ph, rng);
+    }
+    store_res(a.out, idx, res);
+}
+
+// ------------------------------------------------------------------------------- Pass C ---
+struct PassCArgs {
+    restir_uniforms u;
+    SceneView sc;
+    GView g;
+    const restir_reservoir *in;
+    restir_reservoir *out;
+    uint32_t tilesX, nTiles;
+    int k;
+    uint32_t seed;
+};
+
+template <int TX, int TY, int STACK>
+__global__ __launch_bounds__(TX *TY) void k_pass_c(const PassCArgs a) {
+    constexpr int kBlock = TX * TY;
+    __shared__ int32_t stack[STACK * kBlock];
+    uint32_t x, y;
+    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
+    const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
+    if (x >= W || y >= H) return;
+    const size_t idx = static_cast<size_t>(y) * W + x;
+    const GView &g = a.g;
+    const f3 pos = plane3(g.pos, idx, g.P);
+    const f3 n = plane3(g.nrm, idx, g.P);
+    Res res = load_res(a.in, idx);
+    if (dot(n, n) == 0.0f) {  // background: pass through (reference reads uninitialised fields)
+        store_res(a.out, idx, res);
+        return;
+    }
+    const f3 alb = plane3(g.alb, idx, g.P);
+    const float rough = g.mat[idx], metal = g.mat[g.P + idx];
+    const f3 cam{a.u.cameraPos[0], a.u.cameraPos[1], a.u.cameraPos[2]};
+    const Surface s = make_surface(pos, n, cam, luminance(alb.x, alb.y, alb.z), rough, metal);
+    Pcg rng = pcg_seed(static_cast<uint64_t>(a.seed), static_cast<uint64_t>(y * 10007u + x));
+    const float R = a.u.spatialRadius;
+    const uint32_t origM = res.M;
+    uint32_t nIdx[16];
+    uint32_t nM[16];
+    const int k = a.k;
+    for (int i = 0; i < k; ++i) {
+        const float angle = pcg_float(rng) * 2.0f * kPi;
+        const float radius = sqrtf(pcg_float(rng)) * R;
+        float sn, cs;
+        sincos_f(angle, sn, cs);
+        int32_t nx = static_cast<int32_t>(x) + static_cast<int32_t>(roundf(cs * radius));
+        int32_t ny = static_cast<int32_t>(y) + static_cast<int32_t>(roundf(sn * radius));
+        nx = nx < 0 ? 0 : (nx > static_cast<int32_t>(W) - 1 ? static_cast<int32_t>(W) - 1 : nx);
+        ny = ny < 0 ? 0 : (ny > static_cast<int32_t>(H) - 1 ? static_cast<int32_t>(H) - 1 : ny);
+        const uint32_t ni = static_cast<uint32_t>(ny) * W + static_cast<uint32_t>(nx);
+        const Res rr = load_res(a.in, ni);
+        nIdx[i] = ni;
+        nM[i] = rr.M;
+        res.M += rr.M;
+        const float ph = phat_of(s, rr);
+        const float weight = ph * rr.w * static_cast<float>(rr.M);
+        if (weight > 0.0f) res_update(res, weight, rr.pos, rr.normal, rr.emLum, rr.lightIndex, ph, rr.w, rng);
+    }
+    const bool vis = (a.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) != 0;
+    const f3 lp = res.pos;
+    uint32_t numSamples = origM;
+    int32_t *stk = stack + threadIdx.x;
+    for (int j = 0; j < k; ++j) {
+        const uint32_t ni = nIdx[j];
+        const f3 nwp = plane3(g.pos, ni, g.P);
+        const f3 nn = plane3(g.nrm, ni, g.P);
+        if (dot(lp - nwp, nn) < 0.0f) continue;
+        if (vis) {
+            f3 o, d;
+            shadow_segment(nwp, lp, o, d);
+            TraceStats st{0, 0};
+            if (occluded<STACK, false>(a.sc.nodes, a.sc.tris, o, d, stk, kBlock, st)) continue;
+        }
+        numSamples += nM[j];
+    }
+    if (vis) {
+        f3 o, d;
+        shadow_segment(pos, lp, o, d);
+        TraceStats st{0, 0};
+        if (occluded<STACK, false>(a.sc.nodes, a.sc.tris, o, d, stk, kBlock, st)) numSamples = 0;
+    }
+    if (numSamples > 0) {
+        res.w = res.sumW / (static_cast<float>(numSamples) * res.pHat);
+    } else {
+        res.w = 0.0f;
+        res.sumW = 0.0f;
+    }
+    store_res(a.out, idx, res);
+}
+
+// ------------------------------------------------------------------------------- Shading --
+struct ShadeArgs {
+    SceneView sc;
+    f3 cam;
+    float gamma;
+    GView g;
+    const restir_reservoir *res;
+    float *rgb;
+    uint64_t rgbP;
+};
+
+__global__ __launch_bounds__(256) void k_shade(const ShadeArgs a) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    const GView &g = a.g;
+    if (i >= static_cast<size_t>(g.W) * g.H) return;
+    const f3 pos = plane3(g.pos, i, g.P);
+    const f3 n = plane3(g.nrm, i, g.P);
+    const f3 alb = plane3(g.alb, i, g.P);
+    const float emissive = g.alb[3 * g.P + i];
+    f3 c{0.0f, 0.0f, 0.0f};
+    if (dot(n, n) != 0.0f) {
+        const Res r = load_res(a.res, i);
+        // a light index outside the tables (a foreign buffer) shades black instead of faulting
+        const float kZero[3] = {0.0f, 0.0f, 0.0f};
+        const float *em = kZero;
+        if (r.lightIndex < 0) {
+            if (-1 - r.lightIndex < a.sc.nTriLights) em = a.sc.triLights[-1 - r.lightIndex].emission_luminance;
+        } else if (r.lightIndex < a.sc.nPoints) {
+            em = a.sc.points[r.lightIndex].color_luminance;
+        }
+        const Surface s = make_surface(pos, n, a.cam, 0.0f, g.mat[i], g.mat[g.P + i]);
+        const f3 f = eval_phat_full(s, alb, r.pos, f3{r.normal.x, r.normal.y, r.normal.z},
+                                    r.normal.w > 0.5f, f3{em[0], em[1], em[2]});
+        c = f3{f.x * r.w, f.y * r.w, f.z * r.w};
+    }
+    if (emissive > 0.5f) c = alb;
+    if (a.gamma != 1.0f) {
+        const float e = 1.0f / a.gamma;
+        c = f3{powf(c.x, e), powf(c.y, e), powf(c.z, e)};
+    }
+    a.rgb[i] = c.x;
+    a.rgb[a.rgbP + i] = c.y;
+    a.rgb[2 * a.rgbP + i] = c.z;
+}
+
+// ------------------------------------------------------------------------- batch trace ----
+struct TraceArgs {
+    SceneView sc;
+    const float4 *p1, *p2;
+    uint32_t n;
+    uint32_t *shadowed, *visits, *tests;
+};
+
+template <int STACK, bool REF>
+__global__ __launch_bounds__(256) void k_trace(const TraceArgs a) {
+    __shared__ int32_t stack[STACK * 256];
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.n) return;
+    f3 o, d;
+    shadow_segment(xyz(a.p1[i]), xyz(a.p2[i]), o, d);
+    TraceStats st{0, 0};
+    const bool occ = occluded<STACK, REF>(a.sc.nodes, a.sc.tris, o, d, stack + threadIdx.x, 256, st);
+    a.shadowed[i] = occ ? 1u : 0u;
+    if (a.visits) a.visits[i] = st.visits;
+    if (a.tests) a.tests[i] = st.tests;
+}
+
+// ---------------------------------------------------------------------- primary G-buffer --
+struct PrimaryArgs {
+    SceneView sc;
+    CamBasis cb;
+    const int32_t *triMaterial;
+    const float *materials;
+    float *pos, *nrm, *alb, *mat, *depth;
+    uint64_t P;
+    uint32_t W, H;
+};
+
+// Closest-hit slab test on a ray (not a segment): entry distance, or NaN on a miss (so every
+// comparison against it is false).
+__device__ __forceinline__ float box_entry(f3 o, f3 d, float4 mn, float4 mx) {
+    const float ax = (mn.x - o.x) / d.x, ay = (mn.y - o.y) / d.y, az = (mn.z - o.z) / d.z;
+    const float bx = (mx.x - o.x) / d.x, by = (mx.y - o.y) / d.y, bz = (mx.z - o.z) / d.z;
+    const float rmin = gmax(gmin(ax, bx), gmax(gmin(ay, by), gmin(az, bz)));
+    const float rmax = gmin(gmax(ax, bx), gmin(gmax(ay, by), gmax(az, bz)));
+    return (rmax >= rmin && rmax > 0.0f) ? rmin : __builtin_nanf("");
+}
+
+__device__ __forceinline__ bool tri_closest(const float4 *__restrict__ tris, int32_t t, f3 o, f3 d,
+                                            float &tout) {
+    const float4 a = tris[3 * t + 0], b = tris[3 * t + 1], c = tris[3 * t + 2];
+    const f3 p1 = xyz(a);
+    const f3 e1 = xyz(b) - p1;
+    const f3 e2 = xyz(c) - p1;
+    const f3 p = cross(d, e2);
+    const float f = 1.0f / dot(e1, p);
+    const f3 s = o - p1;
+    const float bx = f * dot(s, p);
+    if (bx < 0.0f || bx > 1.0f) return false;
+    const f3 q = cross(s, e1);
+    const float by = f * dot(d, q);
+    if (by < 0.0f || by + bx > 1.0f) return false;
+    const float tt = f * dot(e2, q);
+    if (!(tt > 0.0f)) return false;
+    tout = tt;
+    return true;
+}
+
+template <int STACK>
+__global__ __launch_bounds__(256) void k_primary(const PrimaryArgs a) {
+    __shared__ int32_t stack[STACK * 256];
+    const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= static_cast<size_t>(a.W) * a.H) return;
+    const uint32_t x = static_cast<uint32_t>(i % a.W), y = static_cast<uint32_t>(i / a.W);
+    const CamBasis &b = a.cb;
+    const float ndx = (static_cast<float>(x) + 0.5f) / static_cast<float>(a.W) * 2.0f - 1.0f;
+    const float ndy = (static_cast<float>(y) + 0.5f) / static_cast<float>(a.H) * 2.0f - 1.0f;
+    const f3 d = normalize((b.fwd + b.right * (ndx * b.sx)) + b.down * (ndy * b.sy));
+    const f3 o = b.pos;
+    float best = __builtin_huge_valf();
+    int32_t bi = -1;
+    int32_t *stk = stack + threadIdx.x;
+    int top = 0;
+    int32_t cur = 0;
+    for (;;) {
+        if (cur < 0) {
+            if (top == 0) break;
+            --top;
+            cur = stk[top * 256];
+        }
+        const Node nd = load_node(a.sc.nodes, cur);
+        // conservative culling: entry distances are compared with a relative slack so rounding
+        // of the slab test never discards a triangle whose computed t ties the best hit
+        const float lim = best * 1.00001f + 1e-6f;
+        const float tl = box_entry(o, d, nd.lmin, nd.lmax);
+        const float tr = box_entry(o, d, nd.rmin, nd.rmax);
+        const bool li = tl <= lim, ri = tr <= lim;
+        int32_t inner[2];
+        int ni = 0;
+        if (li) {
+            if (nd.left < 0) {
+                float t;
+                const int32_t ti = ~nd.left;
+                if (tri_closest(a.sc.tris, ti, o, d, t) && (t < best || (t == best && ti < bi))) {
+                    best = t;
+                    bi = ti;
+                }
+            } else {
+                inner[ni++] = nd.left;
+            }
+        }
+        if (ri) {
+            if (nd.right < 0) {
+                float t;
+                const int32_t ti = ~nd.right;
+                if (tri_closest(a.sc.tris, ti, o, d, t) && (t < best || (t == best && ti < bi))) {
+                    best = t;
+                    bi = ti;
+                }
+            } else {
+                inner[ni++] = nd.right;
+            }
+        }
+        if (ni == 2) {
+            // visit the nearer box first
+            const bool leftFirst = tl <= tr;
+            const int32_t first = leftFirst ? inner[0] : inner[1];
+            const int32_t second = leftFirst ? inner[1] : inner[0];
+            if (top < STACK) {
+                stk[top * 256] = second;
+                ++top;
+            }
+            cur = first;
+        } else if (ni == 1) {
+            cur = inner[0];
+        } else {
+            cur = -1;
+        }
+    }
+    const uint64_t P = a.P;
+    if (bi < 0) {
+        a.pos[i] = a.pos[P + i] = a.pos[2 * P + i] = 0.0f;
+        a.nrm[i] = a.nrm[P + i] = a.nrm[2 * P + i] = 0.0f;
+        a.alb[i] = a.alb[P + i] = a.alb[2 * P + i] = a.alb[3 * P + i] = 0.0f;
+        a.mat[i] = a.mat[P + i] = 0.0f;
+        a.depth[i] = 1.0f;
+        return;
+    }
+    const f3 hp = o + d * best;
+    const float4 ta = a.sc.tris[3 * bi], tb = a.sc.tris[3 * bi + 1], tc = a.sc.tris[3 * bi + 2];
+    const f3 p1 = xyz(ta);
+    f3 nn = normalize(cross(xyz(tb) - p1, xyz(tc) - p1));
+    if (dot(nn, d) > 0.0f) nn = neg(nn);
+    const float *m = a.materials + 8 * static_cast<size_t>(a.triMaterial[bi]);
+    a.pos[i] = hp.x;
+    a.pos[P + i] = hp.y;
+    a.pos[2 * P + i] = hp.z;
+    a.nrm[i] = nn.x;
+    a.nrm[P + i] = nn.y;
+    a.nrm[2 * P + i] = nn.z;
+    a.alb[i] = m[0];
+    a.alb[P + i] = m[1];
+    a.alb[2 * P + i] = m[2];
+    a.alb[3 * P + i] = m[3];
+    a.mat[i] = m[4];
+    a.mat[P + i] = m[5];
+    const float z = best * dot(d, b.fwd);
+    a.depth[i] = (b.A * z + b.B) / z;
+}
+
+// ----------------------------------------------------------------------------- launchers --
+template <int TX, int TY>
+static void grid_for(uint32_t W, uint32_t H, uint32_t &tilesX, uint32_t &nTiles) {
+    tilesX = (W + TX - 1) / TX;
+    nTiles = tilesX * ((H + TY - 1) / TY);
+}
+
+template <int TX, int TY, int STACK>
+static hipError_t run_a(PassAArgs args, hipStream_t s) {
+    grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
+    hipLaunchKernelGGL((k_pass_a<TX, TY, STACK>), dim3(args.nTiles), dim3(TX * TY), 0, s, args);
+    return hipGetLastError();
+}
+template <int STACK>
+static hipError_t dispatch_a(int tile, const PassAArgs &args, hipStream_t s) {
+    switch (tile) {
+        case 0: return run_a<16, 16, STACK>(args, s);
+        case 1: return run_a<64, 1, STACK>(args, s);
+        case 2: return run_a<8, 8, STACK>(args, s);
+        case 3: return run_a<16, 8, STACK>(args, s);
+        default: return run_a<32, 8, STACK>(args, s);
+    }
+}
+hipError_t launch_pass_a(int tile, int stack, const restir_uniforms &u, const SceneView &sc,
+                         const GView &cur, const GView &prev, const restir_reservoir *prevRes,
+                         restir_reservoir *out, hipStream_t s) {
+    PassAArgs args{u, sc, cur, prev, prevRes, out, 0, 0};
+    return stack <= 32 ? dispatch_a<32>(tile, args, s) : dispatch_a<64>(tile, args, s);
+}
+
+template <int TX, int TY>
+static hipError_t run_b(PassBArgs args, hipStream_t s) {
+    grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
+    hipLaunchKernelGGL((k_pass_b<TX, TY>), dim3(args.nTiles), dim3(TX * TY), 0, s, args);
+    return hipGetLastError();
+}
+hipError_t launch_pass_b(int tile, const restir_uniforms &u, int iter, float cosThr,
+                         const GView &g, const restir_reservoir *in, restir_reservoir *out,
+                         hipStream_t s) {
+    PassBArgs args{u, g, in, out, 0, 0, u.frame * 31u + static_cast<uint32_t>(iter), cosThr};
+    switch (tile) {
+        case 0: return run_b<16, 16>(args, s);
+        case 1: return run_b<64, 1>(args, s);
+        case 2: return run_b<8, 8>(args, s);
+        case 3: return run_b<16, 8>(args, s);
+        default: return run_b<32, 8>(args, s);
+    }
+}
+
+template <int TX, int TY, int STACK>
+static hipError_t run_c(PassCArgs args, hipStream_t s) {
+    grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
+    hipLaunchKernelGGL((k_pass_c<TX, TY, STACK>), dim3(args.nTiles), dim3(TX * TY), 0, s, args);
+    return hipGetLastError();
+}
+template <int STACK>
+static hipError_t dispatch_c(int tile, const PassCArgs &args, hipStream_t s) {
+    switch (tile) {
+        case 0: return run_c<16, 16, STACK>(args, s);
+        case 1: return run_c<64, 1, STACK>(args, s);
+        case 2: return run_c<8, 8, STACK>(args, s);
+        case 3: return run_c<16, 8, STACK>(args, s);
+        default: return run_c<32, 8, STACK>(args, s);
+    }
+}
+hipError_t launch_pass_c(int tile, int stack, const restir_uniforms &u, int k, int iter,
+                         const SceneView &sc, const GView &g, const restir_reservoir *in,
+                         restir_reservoir *out, hipStream_t s) {
+    // unbiasedReuse.glsl:72 seeds with frame*17; iter > 0 offsets the seed of repeated passes
+    PassCArgs args{u, sc, g, in, out, 0, 0, k, u.frame * 17u + static_cast<uint32_t>(iter)};
+    return stack <= 32 ? dispatch_c<32>(tile, args, s) : dispatch_c<64>(tile, args, s);
+}
+
+hipError_t launch_shade(const SceneView &sc, f3 cam, float gamma, const GView &g,
+                        const restir_reservoir *res, float *rgb, uint64_t rgbP, hipStream_t s) {
+    ShadeArgs args{sc, cam, gamma, g, res, rgb, rgbP};
+    const size_t n = static_cast<size_t>(g.W) * g.H;
+    hipLaunchKernelGGL(k_shade, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, s, args);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace(int stack, bool refLoop, const SceneView &sc, const float4 *p1,
+                        const float4 *p2, uint32_t n, uint32_t *shadowed, uint32_t *visits,
+                        uint32_t *tests, hipStream_t s) {
+    TraceArgs args{sc, p1, p2, n, shadowed, visits, tests};
+    const dim3 grid((n + 255) / 256);
+    if (stack <= 32) {
+        if (refLoop) hipLaunchKernelGGL((k_trace<32, true>), grid, dim3(256), 0, s, args);
+        else hipLaunchKernelGGL((k_trace<32, false>), grid, dim3(256), 0, s, args);
+    } else {
+        if (refLoop) hipLaunchKernelGGL((k_trace<64, true>), grid, dim3(256), 0, s, args);
+        else hipLaunchKernelGGL((k_trace<64, false>), grid, dim3(256), 0, s, args);
+    }
+    return hipGetLastError();
+}
+
+CamBasis make_cam_basis(const restir_camera &c, float aspect) {
+    // camera.h:25-50
+    CamBasis b;
+    b.pos = f3{c.position[0], c.position[1], c.position[2]};
+    const f3 fwd = normalize(f3{c.lookAt[0], c.lookAt[1], c.lookAt[2]} - b.pos);
+    const f3 right = normalize(cross(fwd, f3{c.worldUp[0], c.worldUp[1], c.worldUp[2]}));
+    const f3 up = cross(right, fwd);
+    b.fwd = fwd;
+    b.right = right;
+    b.down = neg(up);
+    const float f = 1.0f / tanf(0.5f * c.fovYRadians);
+    b.sx = aspect / f;
+    b.sy = 1.0f / f;
+    b.A = -c.zFar / (c.zNear - c.zFar);
+    b.B = c.zNear * c.zFar / (c.zNear - c.zFar);
+    return b;
+}
+
+hipError_t launch_primary(int stack, const SceneView &sc, const CamBasis &cb,
+                          const int32_t *triMaterial, const float *materials, float *pos,
+                          float *nrm, float *alb, float *mat, float *depth, uint64_t P, uint32_t W,
+                          uint32_t H, hipStream_t s) {
+    PrimaryArgs args{sc, cb, triMaterial, materials, pos, nrm, alb, mat, depth, P, W, H};
+    const size_t n = static_cast<size_t>(W) * H;
+    const dim3 grid(static_cast<uint32_t>((n + 255) / 256));
+    if (stack <= 32) hipLaunchKernelGGL((k_primary<32>), grid, dim3(256), 0, s, args);
+    else hipLaunchKernelGGL((k_primary<64>), grid, dim3(256), 0, s, args);
+    return hipGetLastError();
+}
+
+}  // namespace rk

@@ -1,0 +1,350 @@
+"""ctypes binding of librestir_amd.so (the C ABI in include/restir_abi.h).
+
+The structures below mirror the header byte for byte; their sizes are asserted at import. All
+device pointers are plain integers (``tensor.data_ptr()``). Every entry point that returns a
+negative code raises :class:`RestirError` carrying the library's message, which is how the
+reference's ``vkCheck`` abort (src/misc.cpp:21-26) surfaces on this side.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG_DIR / "librestir_amd.so"
+
+VISIBILITY_REUSE_FLAG = 1 << 0
+TEMPORAL_REUSE_FLAG = 1 << 1
+TRACE_FIRST_HIT = 0
+TRACE_REFERENCE_LOOP = 1
+
+ERRORS = {
+    -1: "RESTIR_E_INVALID_ARGUMENT",
+    -2: "RESTIR_E_SIZE_MISMATCH",
+    -3: "RESTIR_E_NOT_READY",
+    -4: "RESTIR_E_HIP",
+    -5: "RESTIR_E_BAD_TREE",
+    -6: "RESTIR_E_ALIASING",
+    -7: "RESTIR_E_NO_DEVICE",
+}
+
+# numpy views of the wire formats (restirStructs.glsl:19-34, aabbTree.glsl:1-13, light.glsl:2-20)
+RESERVOIR_DTYPE = np.dtype(
+    [("pos_lum", "<f4", 4), ("normal", "<f4", 4), ("lightIndex", "<i4"), ("pHat", "<f4"),
+     ("sumWeights", "<f4"), ("w", "<f4"), ("M", "<u4"), ("pad", "<u4", 3)])
+NODE_DTYPE = np.dtype(
+    [("lmin", "<f4", 4), ("lmax", "<f4", 4), ("rmin", "<f4", 4), ("rmax", "<f4", 4),
+     ("left", "<i4"), ("right", "<i4"), ("pad", "<i4", 2)])
+TRIANGLE_DTYPE = np.dtype([("p1", "<f4", 4), ("p2", "<f4", 4), ("p3", "<f4", 4)])
+POINT_LIGHT_DTYPE = np.dtype([("pos", "<f4", 4), ("color_lum", "<f4", 4)])
+TRI_LIGHT_DTYPE = np.dtype(
+    [("p1", "<f4", 4), ("p2", "<f4", 4), ("p3", "<f4", 4), ("emission_lum", "<f4", 4),
+     ("normal_area", "<f4", 4)])
+ALIAS_DTYPE = np.dtype([("prob", "<f4"), ("alias", "<i4"), ("oriProb", "<f4"),
+                        ("aliasOriProb", "<f4")])
+assert RESERVOIR_DTYPE.itemsize == 64 and NODE_DTYPE.itemsize == 80
+assert TRIANGLE_DTYPE.itemsize == 48 and POINT_LIGHT_DTYPE.itemsize == 32
+assert TRI_LIGHT_DTYPE.itemsize == 80 and ALIAS_DTYPE.itemsize == 16
+
+
+class Uniforms(C.Structure):
+    """RestirUniforms (restirStructs.glsl:40-56), 128 bytes."""
+    _fields_ = [
+        ("prevFrameProjectionViewMatrix", C.c_float * 16),
+        ("cameraPos", C.c_float * 4),
+        ("screenSize", C.c_uint32 * 2),
+        ("frame", C.c_uint32),
+        ("initialLightSampleCount", C.c_uint32),
+        ("temporalSampleCountMultiplier", C.c_uint32),
+        ("spatialPosThreshold", C.c_float),
+        ("spatialNormalThreshold", C.c_float),
+        ("spatialNeighbors", C.c_uint32),
+        ("spatialRadius", C.c_float),
+        ("flags", C.c_int32),
+        ("_pad", C.c_uint32 * 2),
+    ]
+
+
+class GBuffer(C.Structure):
+    _fields_ = [
+        ("position", C.c_void_p),
+        ("normal", C.c_void_p),
+        ("albedo", C.c_void_p),
+        ("material", C.c_void_p),
+        ("depth", C.c_void_p),
+        ("plane_stride", C.c_uint64),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+    ]
+
+
+class Launch(C.Structure):
+    _fields_ = [("tile_x", C.c_uint32), ("tile_y", C.c_uint32)]
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("position", C.c_float * 3),
+        ("lookAt", C.c_float * 3),
+        ("worldUp", C.c_float * 3),
+        ("fovYRadians", C.c_float),
+        ("zNear", C.c_float),
+        ("zFar", C.c_float),
+    ]
+
+
+assert C.sizeof(Uniforms) == 128 and C.sizeof(GBuffer) == 56 and C.sizeof(Camera) == 48
+
+# Exported symbols with their signatures (restype, argtypes); the CPU test suite checks that the
+# library exports every one of them and that include/restir_abi.h declares the same set.
+_vp, _u32, _u64, _i32, _f32, _sz = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_float, C.c_size_t
+SIGNATURES = {
+    "restir_version": (_i32, []),
+    "restir_create": (_i32, [C.POINTER(_vp), _i32]),
+    "restir_destroy": (None, [_vp]),
+    "restir_last_error": (C.c_char_p, [_vp]),
+    "restir_set_lights": (_i32, [_vp, _vp, _sz, _vp, _sz, _vp, _sz]),
+    "restir_set_bvh": (_i32, [_vp, _vp, _u32, _vp, _u32]),
+    "restir_initial_temporal": (_i32, [_vp, C.POINTER(Uniforms), C.POINTER(GBuffer),
+                                       C.POINTER(GBuffer), _vp, _vp, C.POINTER(Launch), _vp]),
+    "restir_spatial": (_i32, [_vp, C.POINTER(Uniforms), _i32, C.POINTER(GBuffer), _vp, _vp,
+                              C.POINTER(Launch), _vp]),
+    "restir_unbiased": (_i32, [_vp, C.POINTER(Uniforms), _i32, _i32, C.POINTER(GBuffer), _vp,
+                               _vp, C.POINTER(Launch), _vp]),
+    "restir_shade": (_i32, [_vp, C.POINTER(_f32), _f32, C.POINTER(GBuffer), _vp, _vp, _u64, _vp]),
+    "restir_trace_shadow": (_i32, [_vp, _vp, _vp, _u32, _vp, _vp, _vp, _i32, _vp]),
+    "restir_build_aabb_tree": (_i32, [_vp, _u32, _vp, _vp, C.POINTER(_u32)]),
+    "restir_build_alias_table": (_i32, [_vp, _u32, _vp, _u32, _vp]),
+    "restir_primary_gbuffer": (_i32, [_vp, C.POINTER(Camera), _vp, _vp, _u32, _vp, _vp, _vp, _vp,
+                                      _vp, _u64, _u32, _u32, _vp]),
+    "restir_camera_projection_view": (_i32, [C.POINTER(Camera), _f32, C.POINTER(_f32)]),
+}
+
+
+class RestirError(RuntimeError):
+    def __init__(self, code: int, where: str, message: str):
+        super().__init__(f"{where}: {ERRORS.get(code, code)}: {message}")
+        self.code = code
+
+
+_LIB = None
+
+
+def load_library(path: os.PathLike | None = None) -> C.CDLL:
+    """Load the built library. Fails loudly if it has not been built: there is no fallback."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise FileNotFoundError(
+            f"{p} is missing: build it with `make -C {PKG_DIR}` (or __graft_entry__.build())")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def as_ptr(x) -> int | None:
+    """Device pointer of a torch tensor, a raw int pointer, or None."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def stream_ptr(stream) -> int | None:
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+# ------------------------------------------------------------------------- host-side builders
+def build_aabb_tree(tri_xyz: np.ndarray):
+    """AabbTree::build (aabbTreeBuilder.cpp:52-214) -> (nodes, triangles, depth)."""
+    lib = load_library()
+    tri_xyz = np.ascontiguousarray(tri_xyz, dtype=np.float32).reshape(-1, 9)
+    n = tri_xyz.shape[0]
+    nodes = np.zeros(max(n - 1, 0), dtype=NODE_DTYPE)
+    tris = np.zeros(n, dtype=TRIANGLE_DTYPE)
+    depth = C.c_uint32(0)
+    rc = lib.restir_build_aabb_tree(tri_xyz.ctypes.data, n, nodes.ctypes.data, tris.ctypes.data,
+                                    C.byref(depth))
+    if rc != 0:
+        raise RestirError(rc, "restir_build_aabb_tree", "build failed")
+    return nodes, tris, depth.value
+
+
+def build_alias_table(points: np.ndarray | None, tri_lights: np.ndarray | None) -> np.ndarray:
+    """createAliasTable (misc.cpp:418-497)."""
+    lib = load_library()
+    np_ = 0 if points is None else len(points)
+    nt = 0 if tri_lights is None else len(tri_lights)
+    n = np_ if np_ > 0 else nt
+    cols = np.zeros(n, dtype=ALIAS_DTYPE)
+    pts = np.ascontiguousarray(points, dtype=POINT_LIGHT_DTYPE) if np_ else None
+    tls = np.ascontiguousarray(tri_lights, dtype=TRI_LIGHT_DTYPE) if nt else None
+    rc = lib.restir_build_alias_table(pts.ctypes.data if pts is not None else None, np_,
+                                      tls.ctypes.data if tls is not None else None, nt,
+                                      cols.ctypes.data)
+    if rc != 0:
+        raise RestirError(rc, "restir_build_alias_table", "build failed")
+    return cols
+
+
+def pack_buffer(records: np.ndarray | None, dtype: np.dtype) -> np.ndarray:
+    """SSBO image: int32 count + 12 bytes of padding, then the records (sceneBuffers.h:100-124)."""
+    n = 0 if records is None else len(records)
+    buf = np.zeros(16 + n * dtype.itemsize, dtype=np.uint8)
+    buf[:4] = np.frombuffer(np.int32(n).tobytes(), dtype=np.uint8)
+    if n:
+        buf[16:] = np.frombuffer(np.ascontiguousarray(records, dtype=dtype).tobytes(), dtype=np.uint8)
+    return buf
+
+
+def camera_struct(cam: dict) -> Camera:
+    c = Camera()
+    c.position[:] = [float(v) for v in cam["position"]]
+    c.lookAt[:] = [float(v) for v in cam["lookAt"]]
+    c.worldUp[:] = [float(v) for v in cam.get("worldUp", (0.0, 1.0, 0.0))]
+    c.fovYRadians = float(cam["fovY"])
+    c.zNear = float(cam.get("zNear", 0.01))
+    c.zFar = float(cam.get("zFar", 1000.0))
+    return c
+
+
+def projection_view(cam: dict, aspect: float) -> np.ndarray:
+    lib = load_library()
+    out = (C.c_float * 16)()
+    cs = camera_struct(cam)
+    rc = lib.restir_camera_projection_view(C.byref(cs), float(aspect), out)
+    if rc != 0:
+        raise RestirError(rc, "restir_camera_projection_view", "bad camera")
+    return np.array(out, dtype=np.float32)
+
+
+def make_uniforms(width: int, height: int, frame: int, camera_pos, prev_pv=None, *,
+                  M: int = 32, temporal_mult: int = 20, pos_threshold: float = 0.1,
+                  normal_threshold: float = 25.0, neighbors: int = 4, radius: float = 30.0,
+                  flags: int = VISIBILITY_REUSE_FLAG | TEMPORAL_REUSE_FLAG) -> Uniforms:
+    """RestirUniforms with the reference's defaults (app.cpp:413-434, app.h:150-174)."""
+    u = Uniforms()
+    if prev_pv is not None:
+        u.prevFrameProjectionViewMatrix[:] = [float(v) for v in np.asarray(prev_pv).ravel()]
+    u.cameraPos[:] = [float(camera_pos[0]), float(camera_pos[1]), float(camera_pos[2]), 1.0]
+    u.screenSize[:] = [width, height]
+    u.frame = frame
+    u.initialLightSampleCount = M
+    u.temporalSampleCountMultiplier = temporal_mult
+    u.spatialPosThreshold = pos_threshold
+    u.spatialNormalThreshold = normal_threshold
+    u.spatialNeighbors = neighbors
+    u.spatialRadius = radius
+    u.flags = flags
+    return u
+
+
+def gbuffer_struct(base_ptr: int, width: int, height: int, plane_stride: int | None = None) -> GBuffer:
+    """G-buffer view of one float32 allocation holding 13 planes in the order
+    position(3), normal(3), albedo(4: rgb + emissive flag), material(2: roughness, metallic),
+    depth(1)."""
+    P = plane_stride or width * height
+    g = GBuffer()
+    g.position = base_ptr
+    g.normal = base_ptr + 4 * 3 * P
+    g.albedo = base_ptr + 4 * 6 * P
+    g.material = base_ptr + 4 * 10 * P
+    g.depth = base_ptr + 4 * 12 * P
+    g.plane_stride = P
+    g.width = width
+    g.height = height
+    return g
+
+
+GBUFFER_PLANES = 13
+
+
+class Context:
+    """Owns one restir_ctx (one HIP device). Methods map 1:1 onto the C entry points."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.restir_create(C.byref(h), device)
+        if rc != 0:
+            raise RestirError(rc, "restir_create", f"device {device}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.restir_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, where: str):
+        if rc != 0:
+            msg = self.lib.restir_last_error(self.h)
+            raise RestirError(rc, where, msg.decode() if msg else "")
+
+    def set_lights(self, point_buf, point_bytes, tri_buf, tri_bytes, alias_buf, alias_bytes):
+        self._check(self.lib.restir_set_lights(self.h, as_ptr(point_buf), point_bytes,
+                                               as_ptr(tri_buf), tri_bytes, as_ptr(alias_buf),
+                                               alias_bytes), "restir_set_lights")
+
+    def set_bvh(self, nodes, n_nodes, tris, n_tris):
+        self._check(self.lib.restir_set_bvh(self.h, as_ptr(nodes), n_nodes, as_ptr(tris), n_tris),
+                    "restir_set_bvh")
+
+    def initial_temporal(self, u, cur, prev, prev_res, out, launch=None, stream=None):
+        self._check(self.lib.restir_initial_temporal(
+            self.h, C.byref(u), C.byref(cur), C.byref(prev) if prev is not None else None,
+            as_ptr(prev_res), as_ptr(out), C.byref(launch) if launch is not None else None,
+            stream_ptr(stream)), "restir_initial_temporal")
+
+    def spatial(self, u, it, g, res_in, res_out, launch=None, stream=None):
+        self._check(self.lib.restir_spatial(
+            self.h, C.byref(u), it, C.byref(g), as_ptr(res_in), as_ptr(res_out),
+            C.byref(launch) if launch is not None else None, stream_ptr(stream)), "restir_spatial")
+
+    def unbiased(self, u, k, it, g, res_in, res_out, launch=None, stream=None):
+        self._check(self.lib.restir_unbiased(
+            self.h, C.byref(u), k, it, C.byref(g), as_ptr(res_in), as_ptr(res_out),
+            C.byref(launch) if launch is not None else None, stream_ptr(stream)), "restir_unbiased")
+
+    def shade(self, camera_pos, gamma, g, res, rgb, rgb_stride, stream=None):
+        cp = (C.c_float * 3)(*[float(v) for v in camera_pos[:3]])
+        self._check(self.lib.restir_shade(self.h, cp, float(gamma), C.byref(g), as_ptr(res),
+                                          as_ptr(rgb), rgb_stride, stream_ptr(stream)),
+                    "restir_shade")
+
+    def trace_shadow(self, p1, p2, n, shadowed, visits=None, tests=None, reference_loop=False,
+                     stream=None):
+        self._check(self.lib.restir_trace_shadow(self.h, as_ptr(p1), as_ptr(p2), n,
+                                                 as_ptr(shadowed), as_ptr(visits), as_ptr(tests),
+                                                 TRACE_REFERENCE_LOOP if reference_loop else TRACE_FIRST_HIT,
+                                                 stream_ptr(stream)), "restir_trace_shadow")
+
+    def primary_gbuffer(self, cam: dict, tri_material, materials, n_materials, gbuf_base, width,
+                        height, plane_stride=None, stream=None):
+        P = plane_stride or width * height
+        cs = camera_struct(cam)
+        base = as_ptr(gbuf_base)
+        self._check(self.lib.restir_primary_gbuffer(
+            self.h, C.byref(cs), as_ptr(tri_material), as_ptr(materials), n_materials,
+            base, base + 4 * 3 * P, base + 4 * 6 * P, base + 4 * 10 * P, base + 4 * 12 * P, P,
+            width, height, stream_ptr(stream)), "restir_primary_gbuffer")

@@ -1,0 +1,155 @@
+"""Frame driver over the C ABI: the pass ordering and buffer ping-pong of the reference's
+``App::_recordMainCommandBuffers`` / ``_updateRestirBuffers`` (src/app.h:212-336) and the
+per-frame uniform update of ``App::mainLoop`` (src/app.cpp:775-834).
+
+Unbiased mode: Pass A writes a temporary buffer, Pass C writes the frame's reservoir buffer
+(app.h:201-215). With several Pass C iterations the buffers alternate so the last one lands in
+the frame's buffer. Biased mode: Pass A writes res[i], then pairs of Pass B ping-pong
+res[i] -> res[prev] -> res[i] (app.h:248-256, 224-233).
+
+Device memory comes from torch (plumbing only); every computation runs in librestir_amd.so.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import abi
+from .abi import (GBUFFER_PLANES, RESERVOIR_DTYPE, Context, build_aabb_tree, build_alias_table,
+                  gbuffer_struct, make_uniforms, pack_buffer, projection_view)
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("restir_amd needs a HIP device: torch.cuda.is_available() is False")
+    return torch
+
+
+def to_device(arr: np.ndarray, torch, device):
+    flat = np.ascontiguousarray(arr).view(np.uint8).reshape(-1)
+    return torch.from_numpy(flat.copy()).to(device)
+
+
+class SceneOnDevice:
+    """BVH, triangles, light tables and materials resident in HBM and bound to a context."""
+
+    def __init__(self, ctx: Context, scene, device="cuda"):
+        torch = _torch()
+        self.scene = scene
+        self.nodes, self.tris, self.depth = build_aabb_tree(scene.tri_xyz)
+        self.alias = build_alias_table(scene.point_lights, scene.tri_lights)
+        pl = pack_buffer(scene.point_lights, abi.POINT_LIGHT_DTYPE)
+        tl = pack_buffer(scene.tri_lights, abi.TRI_LIGHT_DTYPE)
+        al = pack_buffer(self.alias, abi.ALIAS_DTYPE)
+        self.d_nodes = to_device(self.nodes, torch, device)
+        self.d_tris = to_device(self.tris, torch, device)
+        self.d_points = to_device(pl, torch, device)
+        self.d_trilights = to_device(tl, torch, device)
+        self.d_alias = to_device(al, torch, device)
+        self.d_tri_material = to_device(scene.tri_material.astype(np.int32), torch, device)
+        self.d_materials = to_device(scene.materials.astype(np.float32), torch, device)
+        ctx.set_bvh(self.d_nodes, len(self.nodes), self.d_tris, len(self.tris))
+        ctx.set_lights(self.d_points, pl.nbytes, self.d_trilights, tl.nbytes, self.d_alias, al.nbytes)
+
+
+class Renderer:
+    """Runs frames of the ReSTIR pipeline on one device.
+
+    mode "unbiased": A -> C x spatial_passes (k neighbours) ; mode "biased": A -> (B, B) x
+    spatial_iterations. ``launch`` selects the workgroup tile (restir_launch).
+    """
+
+    def __init__(self, scene, width, height, *, device_index=0, mode="unbiased", M=32, k=5,
+                 spatial_passes=1, spatial_neighbors=4, radius=30.0, temporal=True,
+                 visibility=True, temporal_mult=20, launch=None, ctx=None):
+        torch = _torch()
+        self.torch = torch
+        self.device = torch.device("cuda", device_index)
+        self.ctx = ctx or Context(device_index)
+        self.scene = scene
+        self.dev = SceneOnDevice(self.ctx, scene, self.device)
+        self.W, self.H = width, height
+        self.mode, self.M, self.k = mode, M, k
+        self.spatial_passes = spatial_passes
+        self.spatial_neighbors = spatial_neighbors
+        self.radius = radius
+        self.temporal_mult = temporal_mult
+        self.flags = (abi.VISIBILITY_REUSE_FLAG if visibility else 0) | \
+                     (abi.TEMPORAL_REUSE_FLAG if temporal else 0)
+        self.launch = launch
+        n = width * height
+        self.gbuf = [torch.zeros(GBUFFER_PLANES * n, dtype=torch.float32, device=self.device)
+                     for _ in range(2)]
+        self.res = [torch.zeros(16 * n, dtype=torch.int32, device=self.device) for _ in range(2)]
+        self.tmp = torch.zeros(16 * n, dtype=torch.int32, device=self.device)
+        self.rgb = torch.zeros(3 * n, dtype=torch.float32, device=self.device)
+        self.frame = 0
+        self.cur = 1  # index of the buffers written by the last frame
+        self.prev_pv = None
+        self.cam = None
+
+    # -------------------------------------------------------------------------------- pieces
+    def gstruct(self, i):
+        return gbuffer_struct(self.gbuf[i].data_ptr(), self.W, self.H)
+
+    def render_gbuffer(self, cam: dict, i: int, stream=None):
+        self.ctx.primary_gbuffer(cam, self.dev.d_tri_material, self.dev.d_materials,
+                                 len(self.scene.materials), self.gbuf[i], self.W, self.H,
+                                 stream=stream)
+
+    def uniforms(self, cam: dict):
+        pv_now = projection_view(cam, self.W / self.H)
+        prev_pv = self.prev_pv if self.prev_pv is not None else pv_now
+        return make_uniforms(self.W, self.H, self.frame, cam["position"], prev_pv, M=self.M,
+                             temporal_mult=self.temporal_mult, neighbors=self.spatial_neighbors,
+                             radius=self.radius, flags=self.flags), pv_now
+
+    def begin_frame(self, cam: dict, stream=None, gbuffer=True):
+        """Advance the frame counter and render this frame's G-buffer (the raster pass's slot)."""
+        self.cur = 1 - self.cur
+        self.frame += 1
+        self.cam = cam
+        if gbuffer:
+            self.render_gbuffer(cam, self.cur, stream)
+
+    def restir_passes(self, u, stream=None):
+        """The ReSTIR dispatches of one frame (the hot path)."""
+        i, p = self.cur, 1 - self.cur
+        g, gp = self.gstruct(i), self.gstruct(p)
+        if self.mode == "unbiased":
+            n = self.spatial_passes
+            first = self.tmp if n % 2 == 1 else self.res[i]
+            self.ctx.initial_temporal(u, g, gp, self.res[p], first, self.launch, stream)
+            src = first
+            for j in range(n):
+                dst = self.res[i] if (n - j) % 2 == 1 else self.tmp
+                self.ctx.unbiased(u, self.k, j, g, src, dst, self.launch, stream)
+                src = dst
+        else:
+            self.ctx.initial_temporal(u, g, gp, self.res[p], self.res[i], self.launch, stream)
+            for j in range(self.spatial_passes):
+                self.ctx.spatial(u, 2 * j, g, self.res[i], self.res[p], self.launch, stream)
+                self.ctx.spatial(u, 2 * j + 1, g, self.res[p], self.res[i], self.launch, stream)
+
+    def shade(self, stream=None, gamma=1.0):
+        self.ctx.shade(self.cam["position"], gamma, self.gstruct(self.cur), self.res[self.cur],
+                       self.rgb, self.W * self.H, stream)
+
+    def render_frame(self, cam: dict, stream=None):
+        self.begin_frame(cam, stream)
+        u, pv_now = self.uniforms(cam)
+        self.restir_passes(u, stream)
+        self.shade(stream)
+        self.prev_pv = pv_now
+        return self.rgb
+
+    # ------------------------------------------------------------------------------ readback
+    def image(self) -> np.ndarray:
+        return self.rgb.view(3, self.H, self.W).cpu().numpy()
+
+    def reservoirs(self, which=None) -> np.ndarray:
+        t = self.res[self.cur] if which is None else which
+        return t.cpu().numpy().view(RESERVOIR_DTYPE)
+
+    def gbuffer_host(self, i=None) -> np.ndarray:
+        return self.gbuf[self.cur if i is None else i].cpu().numpy()

@@ -1,0 +1,197 @@
+"""GPU parity: the HIP path, called through the C ABI, against the CPU oracle on the same inputs.
+
+Bar: bit-exact reservoirs (all 16 words; two NaNs in a float field count as equal), bit-exact
+shadow hit/miss and node-visit counts, bit-exact G-buffer and shaded image.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from restir_amd import abi, scenes
+from restir_amd.pipeline import Renderer
+from rt_helpers import HostScene, floats_equal_nan, random_segments, reservoir_mismatch, segments_from_gbuffer
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert t.cuda.is_available(), "GPU tests need a HIP device"
+    return t
+
+
+def host_planes(r, i):
+    return r.gbuf[i].cpu().numpy()
+
+
+def host_res(t):
+    return t.cpu().numpy().view(abi.RESERVOIR_DTYPE)
+
+
+def oracle_for(r):
+    sc = r.scene
+    return oracle.Scene(r.dev.nodes, r.dev.tris, sc.point_lights if len(sc.point_lights) else None,
+                        sc.tri_lights if len(sc.tri_lights) else None, r.dev.alias)
+
+
+def assert_res_equal(gpu, ref, what):
+    bad = reservoir_mismatch(gpu, ref)
+    assert not bad.any(), f"{what}: {int(bad.sum())} of {len(bad)} reservoirs differ; first {np.flatnonzero(bad)[:5]}"
+
+
+def run_frame_and_compare(r, cam, *, check_shade=True):
+    """One frame through the C ABI; every pass compared with the oracle on the same inputs."""
+    orc = oracle_for(r)
+    r.begin_frame(cam)
+    u, pv_now = r.uniforms(cam)
+    i, p = r.cur, 1 - r.cur
+    W, H = r.W, r.H
+    g_cur, g_prev = host_planes(r, i), host_planes(r, p)
+    og, ogp = oracle.gbuffer(g_cur, W, H), oracle.gbuffer(g_prev, W, H)
+    prev_res = host_res(r.res[p])
+    if r.mode == "unbiased":
+        n = r.spatial_passes
+        first = r.tmp if n % 2 == 1 else r.res[i]
+        r.ctx.initial_temporal(u, r.gstruct(i), r.gstruct(p), r.res[p], first)
+        a_ref = oracle.pass_a(orc, u, og, ogp, prev_res, abi.RESERVOIR_DTYPE)
+        assert_res_equal(host_res(first), a_ref, f"pass A frame {r.frame}")
+        src, src_ref = first, a_ref
+        for j in range(n):
+            dst = r.res[i] if (n - j) % 2 == 1 else r.tmp
+            r.ctx.unbiased(u, r.k, j, r.gstruct(i), src, dst)
+            c_ref = oracle.pass_c(orc, u, r.k, j, og, src_ref)
+            assert_res_equal(host_res(dst), c_ref, f"pass C{j} frame {r.frame}")
+            src, src_ref = dst, c_ref
+        final_ref = src_ref
+    else:
+        r.ctx.initial_temporal(u, r.gstruct(i), r.gstruct(p), r.res[p], r.res[i])
+        a_ref = oracle.pass_a(orc, u, og, ogp, prev_res, abi.RESERVOIR_DTYPE)
+        assert_res_equal(host_res(r.res[i]), a_ref, f"pass A frame {r.frame}")
+        cur_ref = a_ref
+        for j in range(r.spatial_passes):
+            r.ctx.spatial(u, 2 * j, r.gstruct(i), r.res[i], r.res[p])
+            b0 = oracle.pass_b(orc, u, 2 * j, og, cur_ref)
+            assert_res_equal(host_res(r.res[p]), b0, f"pass B{2 * j} frame {r.frame}")
+            r.ctx.spatial(u, 2 * j + 1, r.gstruct(i), r.res[p], r.res[i])
+            b1 = oracle.pass_b(orc, u, 2 * j + 1, og, b0)
+            assert_res_equal(host_res(r.res[i]), b1, f"pass B{2 * j + 1} frame {r.frame}")
+            cur_ref = b1
+        final_ref = cur_ref
+    if check_shade:
+        r.shade()
+        img = r.image()
+        ref_img = oracle.shade(orc, cam["position"], 1.0, og, final_ref)
+        eq = floats_equal_nan(img, ref_img)
+        assert eq.all(), f"shade: {int((~eq).sum())} values differ"
+    r.prev_pv = pv_now
+    return a_ref, final_ref
+
+
+def test_primary_gbuffer_matches_oracle(torch):
+    sc = scenes.cornell_box(1)
+    r = Renderer(sc, 256, 256)
+    r.render_gbuffer(sc.camera, 0)
+    gpu = host_planes(r, 0)
+    hs = HostScene(sc)
+    ref = hs.oracle_gbuffer(256, 256)
+    eq = floats_equal_nan(gpu, ref)
+    assert eq.all(), f"{int((~eq).sum())} G-buffer values differ"
+    normal = gpu[3 * 65536: 6 * 65536].reshape(3, -1)
+    assert (np.abs(normal).sum(axis=0) > 0).mean() > 0.6  # the room fills most of the view
+
+
+@pytest.mark.parametrize("tile", [(16, 16), (64, 1), (8, 8)])
+def test_biased_pipeline_cornell_bit_exact(torch, tile):
+    sc = scenes.cornell_box(1)
+    r = Renderer(sc, 256, 256, mode="biased", spatial_passes=1, spatial_neighbors=5,
+                 launch=abi.Launch(*tile))
+    a1, _ = run_frame_and_compare(r, sc.camera)
+    assert (a1["sumWeights"] > 0).mean() > 0.5
+    # static camera: the second frame reprojects every pixel onto itself and reuses it
+    a2, _ = run_frame_and_compare(r, sc.camera)
+    assert (a2["M"] > 32).mean() > 0.5
+
+
+@pytest.mark.parametrize("k", [3, 5])
+def test_unbiased_pipeline_cornell_bit_exact(torch, k):
+    sc = scenes.cornell_box(1)
+    r = Renderer(sc, 256, 256, mode="unbiased", k=k, spatial_passes=1)
+    run_frame_and_compare(r, sc.camera)
+    run_frame_and_compare(r, sc.camera)
+
+
+def test_unbiased_two_passes_moving_camera_terrain(torch):
+    sc = scenes.terrain(2)
+    r = Renderer(sc, 480, 270, mode="unbiased", k=5, spatial_passes=2)
+    for f in range(3):
+        run_frame_and_compare(r, scenes.orbit_camera(sc.camera, f))
+
+
+def test_trace_shadow_matches_oracle(torch):
+    for sc in (scenes.cornell_box(1), scenes.terrain(2)):
+        hs = HostScene(sc)
+        r = Renderer(sc, 128, 128, ctx=None)
+        lo, hi = sc.aabb()
+        p1, p2 = random_segments(np.random.default_rng(5), lo, hi, 50000)
+        n = len(p1)
+        d1 = torch.from_numpy(p1).cuda()
+        d2 = torch.from_numpy(p2).cuda()
+        sh = torch.zeros(n, dtype=torch.int32, device="cuda")
+        vis = torch.zeros(n, dtype=torch.int32, device="cuda")
+        tst = torch.zeros(n, dtype=torch.int32, device="cuda")
+        r.ctx.trace_shadow(d1, d2, n, sh, vis, tst, reference_loop=True)
+        ref_sh, ref_v, _ = oracle.trace_shadow(hs.oracle, p1, p2)
+        np.testing.assert_array_equal(sh.cpu().numpy().astype(np.uint32), ref_sh)
+        np.testing.assert_array_equal(vis.cpu().numpy().astype(np.uint32), ref_v)
+        # production loop (first hit ends the ray): same hit/miss
+        sh2 = torch.zeros(n, dtype=torch.int32, device="cuda")
+        r.ctx.trace_shadow(d1, d2, n, sh2)
+        np.testing.assert_array_equal(sh2.cpu().numpy().astype(np.uint32), ref_sh)
+        assert 0.05 < ref_sh.mean() < 0.95
+
+
+def test_trace_shadow_on_pass_a_rays(torch):
+    sc = scenes.cornell_box(1)
+    r = Renderer(sc, 256, 256, temporal=False)
+    r.render_frame(sc.camera)
+    planes = host_planes(r, r.cur)
+    # Pass A output before visibility is not exposed; use the final reservoirs' light positions
+    res = r.reservoirs()
+    p1, p2 = segments_from_gbuffer(planes, res, 256, 256)
+    hs = HostScene(sc)
+    n = len(p1)
+    d1, d2 = torch.from_numpy(p1).cuda(), torch.from_numpy(p2).cuda()
+    sh = torch.zeros(n, dtype=torch.int32, device="cuda")
+    vis = torch.zeros(n, dtype=torch.int32, device="cuda")
+    r.ctx.trace_shadow(d1, d2, n, sh, vis, reference_loop=True)
+    ref_sh, ref_v, _ = oracle.trace_shadow(hs.oracle, p1, p2)
+    np.testing.assert_array_equal(sh.cpu().numpy().astype(np.uint32), ref_sh)
+    np.testing.assert_array_equal(vis.cpu().numpy().astype(np.uint32), ref_v)
+
+
+def test_abi_errors_on_device(torch):
+    sc = scenes.cornell_box(1)
+    r = Renderer(sc, 64, 64)
+    g = r.gstruct(0)
+    u, _ = r.uniforms(sc.camera)
+    with pytest.raises(abi.RestirError) as e:
+        r.ctx.spatial(u, 0, g, r.res[0], r.res[0])
+    assert e.value.code == -6
+    u.screenSize[0] = 65
+    with pytest.raises(abi.RestirError) as e:
+        r.ctx.initial_temporal(u, g, g, r.res[1], r.res[0])
+    assert e.value.code == -2
+    u.screenSize[0] = 64
+    with pytest.raises(abi.RestirError) as e:
+        r.ctx.unbiased(u, 17, 0, g, r.res[0], r.res[1])
+    assert e.value.code == -1
+    bad = r.dev.nodes.copy()
+    bad[0]["left"] = 10**6
+    d_bad = torch.from_numpy(bad.view(np.uint8).copy()).cuda()
+    with pytest.raises(abi.RestirError) as e:
+        r.ctx.set_bvh(d_bad, len(bad), r.dev.d_tris, len(r.dev.tris))
+    assert e.value.code == -5
+    with pytest.raises(abi.RestirError) as e:
+        r.ctx.set_bvh(r.dev.d_nodes, len(bad) - 1, r.dev.d_tris, len(r.dev.tris))
+    assert e.value.code == -5

@@ -163,6 +163,9 @@ typedef struct restir_ctx restir_ctx;
 int restir_create(restir_ctx **out, int device);
 void restir_destroy(restir_ctx *ctx);
 const char *restir_last_error(const restir_ctx *ctx);
+/* ABI revision; bumped whenever an entry point's signature changes. Bindings compare it with the
+ * revision they were written against and refuse a mismatched library. */
+#define RESTIR_ABI_VERSION 2
 int restir_version(void);
 
 /* Light tables: device pointers to the reference buffer images (header + array). The counts are
@@ -227,6 +230,19 @@ int restir_build_aabb_tree(const float *tri_xyz, uint32_t n_tris, restir_aabb_tr
 int restir_build_alias_table(const restir_point_light *points, uint32_t n_point,
                              const restir_tri_light *tris, uint32_t n_tri,
                              restir_alias_table_column *columns);
+
+/* Light clusters for the learned proposal (no reference counterpart): seeded k-means over the n
+ * point-light positions into k clusters (1 <= k <= n; initial centroids by a PCG32(seed)
+ * partial Fisher-Yates draw, `iterations` Lloyd steps, ties to the lower centroid), then
+ * per-cluster Vose tables over luminance as createAliasTable builds them (misc.cpp:418-497).
+ * Outputs: light_cluster[n], light_pwithin[n] (luminance share within its cluster),
+ * cluster_begin[k], cluster_count[k], cluster_lights[n] (lights grouped by cluster, index
+ * order), cluster_alias[n] (alias entries local to the cluster), centroids[3k] (may be NULL). */
+int restir_build_light_clusters(const restir_point_light *points, uint32_t n, uint32_t k,
+                                uint64_t seed, uint32_t iterations, int32_t *light_cluster,
+                                float *light_pwithin, int32_t *cluster_begin,
+                                int32_t *cluster_count, int32_t *cluster_lights,
+                                restir_alias_table_column *cluster_alias, float *centroids);
 
 /* ----------------------------------------------------------------------------------------------
  * Scene-side helpers for synthetic inputs (no reference counterpart; the reference rasterises).

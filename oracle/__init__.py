@@ -37,6 +37,8 @@ SIGS = {
     "oracle_combine": (None, [_vp, _vp, _f32, _vp]),
     "oracle_build_alias_table": (_i32, [_vp, _u32, _vp, _u32, _vp]),
     "oracle_build_aabb_tree": (_i32, [_vp, _u32, _vp, _vp, C.POINTER(_u32)]),
+    "oracle_build_light_clusters": (_i32, [_vp, _u32, _u32, C.c_uint64, _u32, _vp, _vp, _vp, _vp,
+                                           _vp, _vp, _vp]),
     "oracle_ray_aabb": (_i32, [_vp, _vp, _vp, _vp]),
     "oracle_ray_triangle": (_i32, [_vp, _vp, _vp]),
     "oracle_trace_shadow": (None, [C.POINTER(OracleScene), _vp, _vp, _u32, _vp, _vp, _vp]),
@@ -118,6 +120,22 @@ def build_aabb_tree(tri_xyz, node_dtype, tri_dtype):
     if rc != 0:
         raise RuntimeError(f"oracle_build_aabb_tree failed ({rc})")
     return nodes, tris, depth.value
+
+
+def build_light_clusters(points, k, seed, iterations, alias_dtype):
+    n = len(points)
+    out = dict(light_cluster=np.zeros(n, np.int32), light_pwithin=np.zeros(n, np.float32),
+               begin=np.zeros(k, np.int32), count=np.zeros(k, np.int32),
+               lights=np.zeros(n, np.int32), alias=np.zeros(n, dtype=alias_dtype),
+               centroids=np.zeros((k, 3), np.float32))
+    pts = np.ascontiguousarray(points)
+    rc = lib().oracle_build_light_clusters(
+        pts.ctypes.data, n, k, seed, iterations, out["light_cluster"].ctypes.data,
+        out["light_pwithin"].ctypes.data, out["begin"].ctypes.data, out["count"].ctypes.data,
+        out["lights"].ctypes.data, out["alias"].ctypes.data, out["centroids"].ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"oracle_build_light_clusters failed ({rc})")
+    return out
 
 
 class Scene:

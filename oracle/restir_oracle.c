@@ -1176,3 +1176,101 @@ void oracle_ground_truth_f64(const oracle_scene *s, const float camf[3], const r
         rgb[i] = acc[0]; rgb[rs + i] = acc[1]; rgb[2 * rs + i] = acc[2];
     }
 }
+
+/* ------------------------------------------------------------- light clusters (k-means) ---- */
+/* Seeded k-means over point-light positions and per-cluster Vose tables over luminance.
+ * Initial centroids: partial Fisher-Yates with PCG32(seed, seq 0x6b6d65616e73); `iterations`
+ * Lloyd steps (squared distance in float, ties to the lower centroid index; means accumulated
+ * in double in light order; empty clusters keep their centroid), then a final assignment.
+ * Lights are listed by cluster in index order; alias entries are local to their cluster. */
+static void vose_local(const float *w, uint32_t n, restir_alias_table_column *out) {
+    /* misc.cpp:418-497 on an arbitrary weight list */
+    float *p = (float *)malloc(sizeof(float) * n);
+    uint32_t *big = (uint32_t *)malloc(sizeof(uint32_t) * (n + 1));
+    uint32_t *small = (uint32_t *)malloc(sizeof(uint32_t) * (n + 1));
+    size_t bh = 0, bt = 0, sh = 0, st = 0, cap = n + 1;
+    float total = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) { total += w[i]; p[i] = w[i]; }
+    for (uint32_t i = 0; i < n; ++i) {
+        out[i].prob = 0.0f;
+        out[i].alias = -1;
+        out[i].oriProb = p[i] / total;
+        out[i].aliasOriProb = 0.0f;
+        p[i] = (float)n * p[i] / total;
+        if (p[i] >= 1.0f) big[bt++ % cap] = i; else small[st++ % cap] = i;
+    }
+    while (bh != bt && sh != st) {
+        uint32_t g = big[bh++ % cap], l = small[sh++ % cap];
+        out[l].prob = p[l];
+        out[l].alias = (int32_t)g;
+        p[g] = (p[g] + p[l]) - 1.0f;
+        if (p[g] < 1.0f) small[st++ % cap] = g; else big[bt++ % cap] = g;
+    }
+    while (bh != bt) { uint32_t g = big[bh++ % cap]; out[g].prob = 1.0f; out[g].alias = (int32_t)g; }
+    while (sh != st) { uint32_t l = small[sh++ % cap]; out[l].prob = 1.0f; out[l].alias = (int32_t)l; }
+    for (uint32_t i = 0; i < n; ++i) out[i].aliasOriProb = out[out[i].alias].oriProb;
+    free(p); free(big); free(small);
+}
+
+int oracle_build_light_clusters(const restir_point_light *pts, uint32_t n, uint32_t k, uint64_t seed,
+                                uint32_t iterations, int32_t *light_cluster, float *light_pwithin,
+                                int32_t *cl_begin, int32_t *cl_count, int32_t *cl_lights,
+                                restir_alias_table_column *cl_alias, float *centroids) {
+    if (k == 0 || k > n) return -1;
+    uint32_t *perm = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    float *c = (float *)malloc(sizeof(float) * 3 * k);
+    double *sum = (double *)malloc(sizeof(double) * 3 * k);
+    uint32_t *cnt = (uint32_t *)malloc(sizeof(uint32_t) * k);
+    uint32_t *asg = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    for (uint32_t i = 0; i < n; ++i) perm[i] = i;
+    uint64_t rng[2];
+    pcg_seed(rng, seed, 0x6b6d65616e73ULL);
+    for (uint32_t i = 0; i < k; ++i) {
+        uint32_t j = i + pcg_next(rng) % (n - i);
+        uint32_t t = perm[i]; perm[i] = perm[j]; perm[j] = t;
+        for (int d = 0; d < 3; ++d) c[3 * i + d] = pts[perm[i]].pos[d];
+    }
+    for (uint32_t it = 0;; ++it) {
+#pragma omp parallel for schedule(static)
+        for (int64_t l = 0; l < (int64_t)n; ++l) {
+            float x = pts[l].pos[0], y = pts[l].pos[1], z = pts[l].pos[2];
+            uint32_t best = 0;
+            float bd = 0.0f;
+            for (uint32_t j = 0; j < k; ++j) {
+                float dx = x - c[3 * j], dy = y - c[3 * j + 1], dz = z - c[3 * j + 2];
+                float d = (dx * dx + dy * dy) + dz * dz;
+                if (j == 0 || d < bd) { bd = d; best = j; }
+            }
+            asg[l] = best;
+        }
+        if (it == iterations) break;
+        memset(sum, 0, sizeof(double) * 3 * k);
+        memset(cnt, 0, sizeof(uint32_t) * k);
+        for (uint32_t l = 0; l < n; ++l) {
+            for (int d = 0; d < 3; ++d) sum[3 * asg[l] + d] += (double)pts[l].pos[d];
+            cnt[asg[l]]++;
+        }
+        for (uint32_t j = 0; j < k; ++j)
+            if (cnt[j]) for (int d = 0; d < 3; ++d) c[3 * j + d] = (float)(sum[3 * j + d] / cnt[j]);
+    }
+    memset(cnt, 0, sizeof(uint32_t) * k);
+    for (uint32_t l = 0; l < n; ++l) cnt[asg[l]]++;
+    int32_t off = 0;
+    for (uint32_t j = 0; j < k; ++j) { cl_begin[j] = off; cl_count[j] = (int32_t)cnt[j]; off += (int32_t)cnt[j]; }
+    memset(cnt, 0, sizeof(uint32_t) * k);
+    for (uint32_t l = 0; l < n; ++l) {
+        light_cluster[l] = (int32_t)asg[l];
+        cl_lights[cl_begin[asg[l]] + (int32_t)cnt[asg[l]]++] = (int32_t)l;
+    }
+    float *w = (float *)malloc(sizeof(float) * n);
+    for (uint32_t j = 0; j < k; ++j) {
+        int32_t b = cl_begin[j], m = cl_count[j];
+        if (m == 0) continue;
+        for (int32_t i = 0; i < m; ++i) w[i] = pts[cl_lights[b + i]].color_luminance[3];
+        vose_local(w, (uint32_t)m, cl_alias + b);
+        for (int32_t i = 0; i < m; ++i) light_pwithin[cl_lights[b + i]] = cl_alias[b + i].oriProb;
+    }
+    if (centroids) memcpy(centroids, c, sizeof(float) * 3 * k);
+    free(w); free(perm); free(c); free(sum); free(cnt); free(asg);
+    return 0;
+}

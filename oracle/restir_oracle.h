@@ -84,6 +84,14 @@ int oracle_build_alias_table(const restir_point_light *points, uint32_t n_point,
                              const restir_tri_light *tris, uint32_t n_tri,
                              restir_alias_table_column *out);
 
+/* Seeded k-means light clusters with per-cluster Vose tables (no reference counterpart; the
+ * learned proposal's cluster structure). Pinned by k-means invariants in tests/. */
+int oracle_build_light_clusters(const restir_point_light *pts, uint32_t n, uint32_t k,
+                                uint64_t seed, uint32_t iterations, int32_t *light_cluster,
+                                float *light_pwithin, int32_t *cl_begin, int32_t *cl_count,
+                                int32_t *cl_lights, restir_alias_table_column *cl_alias,
+                                float *centroids);
+
 /* aabbTreeBuilder.cpp:52-214 */
 int oracle_build_aabb_tree(const float *tri_xyz, uint32_t n_tris, restir_aabb_tree_node *nodes,
                            restir_triangle *tris, uint32_t *depth_out);

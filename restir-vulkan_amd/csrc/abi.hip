@@ -102,7 +102,7 @@ int pick_tile(restir_ctx *ctx, const restir_launch *l, int *tile) {
 
 extern "C" {
 
-int restir_version(void) { return 1; }
+int restir_version(void) { return RESTIR_ABI_VERSION; }
 
 int restir_create(restir_ctx **out, int device) {
     if (!out) return RESTIR_E_INVALID_ARGUMENT;

@@ -205,19 +205,16 @@ extern "C" int restir_build_aabb_tree(const float *triXyz, uint32_t nTris,
     return (rootLink == 0 && next == static_cast<int32_t>(nTris - 1)) ? RESTIR_OK : RESTIR_E_BAD_TREE;
 }
 
-extern "C" int restir_build_alias_table(const restir_point_light *points, uint32_t nPoint,
-                                        const restir_tri_light *triLights, uint32_t nTri,
-                                        restir_alias_table_column *cols) {
-    const uint32_t n = nPoint > 0 ? nPoint : nTri;
-    if (n == 0) return RESTIR_OK;
-    if (!cols || (nPoint > 0 && !points) || (nPoint == 0 && !triLights)) return RESTIR_E_INVALID_ARGUMENT;
+namespace {
+
+// Vose's method with FIFO queues over `n` weights (createAliasTable, misc.cpp:418-497); `alias`
+// entries are local indices 0..n-1.
+void vose(const float *weights, uint32_t n, restir_alias_table_column *cols) {
     std::vector<float> scaled(n);
     float total = 0.0f;
     for (uint32_t i = 0; i < n; ++i) {
-        const float power = nPoint > 0 ? points[i].color_luminance[3]
-                                       : triLights[i].emission_luminance[3] * triLights[i].normalArea[3];
-        total += power;
-        scaled[i] = power;
+        total += weights[i];
+        scaled[i] = weights[i];
     }
     std::queue<uint32_t> over, under;
     for (uint32_t i = 0; i < n; ++i) {
@@ -247,5 +244,115 @@ extern "C" int restir_build_alias_table(const restir_point_light *points, uint32
         cols[under.front()].alias = static_cast<int32_t>(under.front());
     }
     for (uint32_t i = 0; i < n; ++i) cols[i].aliasOriProb = cols[cols[i].alias].oriProb;
+}
+
+// PCG32 (the generator of rand.glsl:12-28), used to seed k-means reproducibly.
+struct Pcg32 {
+    uint64_t state, inc;
+    Pcg32(uint64_t seed, uint64_t seq) : state(0), inc((seq << 1u) | 1u) {
+        next();
+        state += seed;
+        next();
+    }
+    uint32_t next() {
+        const uint64_t old = state;
+        state = old * 6364136223846793005ull + inc;
+        const uint32_t xs = static_cast<uint32_t>(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = static_cast<uint32_t>(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+};
+
+inline uint32_t nearest(const float *c, uint32_t k, float x, float y, float z) {
+    uint32_t best = 0;
+    float bestD = 0.0f;
+    for (uint32_t j = 0; j < k; ++j) {
+        const float dx = x - c[3 * j], dy = y - c[3 * j + 1], dz = z - c[3 * j + 2];
+        const float d = (dx * dx + dy * dy) + dz * dz;
+        if (j == 0 || d < bestD) {
+            bestD = d;
+            best = j;
+        }
+    }
+    return best;
+}
+
+}  // namespace
+
+extern "C" int restir_build_light_clusters(const restir_point_light *points, uint32_t n, uint32_t k,
+                                           uint64_t seed, uint32_t iterations, int32_t *lightCluster,
+                                           float *lightPWithin, int32_t *clusterBegin,
+                                           int32_t *clusterCount, int32_t *clusterLights,
+                                           restir_alias_table_column *clusterAlias,
+                                           float *centroidsOut) {
+    if (!points || !lightCluster || !lightPWithin || !clusterBegin || !clusterCount || !clusterLights ||
+        !clusterAlias || k == 0 || k > n)
+        return RESTIR_E_INVALID_ARGUMENT;
+    // initial centroids: a seeded partial Fisher-Yates draw of k distinct lights
+    std::vector<uint32_t> perm(n);
+    for (uint32_t i = 0; i < n; ++i) perm[i] = i;
+    Pcg32 rng(seed, 0x6b6d65616e73ull);
+    std::vector<float> c(3 * static_cast<size_t>(k));
+    for (uint32_t i = 0; i < k; ++i) {
+        const uint32_t j = i + rng.next() % (n - i);
+        std::swap(perm[i], perm[j]);
+        for (int d = 0; d < 3; ++d) c[3 * i + d] = points[perm[i]].pos[d];
+    }
+    std::vector<uint32_t> assign(n);
+    std::vector<double> sum(3 * static_cast<size_t>(k));
+    std::vector<uint32_t> cnt(k);
+    for (uint32_t it = 0; it <= iterations; ++it) {
+        for (uint32_t l = 0; l < n; ++l) assign[l] = nearest(c.data(), k, points[l].pos[0], points[l].pos[1], points[l].pos[2]);
+        if (it == iterations) break;  // the final assignment uses the final centroids
+        std::fill(sum.begin(), sum.end(), 0.0);
+        std::fill(cnt.begin(), cnt.end(), 0u);
+        for (uint32_t l = 0; l < n; ++l) {
+            for (int d = 0; d < 3; ++d) sum[3 * assign[l] + d] += static_cast<double>(points[l].pos[d]);
+            ++cnt[assign[l]];
+        }
+        for (uint32_t j = 0; j < k; ++j)
+            if (cnt[j] > 0)
+                for (int d = 0; d < 3; ++d) c[3 * j + d] = static_cast<float>(sum[3 * j + d] / cnt[j]);
+    }
+    // lights grouped by cluster, in index order within a cluster
+    std::fill(cnt.begin(), cnt.end(), 0u);
+    for (uint32_t l = 0; l < n; ++l) ++cnt[assign[l]];
+    int32_t off = 0;
+    for (uint32_t j = 0; j < k; ++j) {
+        clusterBegin[j] = off;
+        clusterCount[j] = static_cast<int32_t>(cnt[j]);
+        off += static_cast<int32_t>(cnt[j]);
+    }
+    std::vector<int32_t> fill(clusterBegin, clusterBegin + k);
+    for (uint32_t l = 0; l < n; ++l) {
+        lightCluster[l] = static_cast<int32_t>(assign[l]);
+        clusterLights[fill[assign[l]]++] = static_cast<int32_t>(l);
+    }
+    // per-cluster alias tables over luminance; oriProb is the within-cluster probability
+    std::vector<float> w;
+    for (uint32_t j = 0; j < k; ++j) {
+        const int32_t b = clusterBegin[j], m = clusterCount[j];
+        if (m == 0) continue;
+        w.resize(m);
+        for (int32_t i = 0; i < m; ++i) w[i] = points[clusterLights[b + i]].color_luminance[3];
+        vose(w.data(), static_cast<uint32_t>(m), clusterAlias + b);
+        for (int32_t i = 0; i < m; ++i) lightPWithin[clusterLights[b + i]] = clusterAlias[b + i].oriProb;
+    }
+    if (centroidsOut) std::copy(c.begin(), c.end(), centroidsOut);
+    return RESTIR_OK;
+}
+
+extern "C" int restir_build_alias_table(const restir_point_light *points, uint32_t nPoint,
+                                        const restir_tri_light *triLights, uint32_t nTri,
+                                        restir_alias_table_column *cols) {
+    const uint32_t n = nPoint > 0 ? nPoint : nTri;
+    if (n == 0) return RESTIR_OK;
+    if (!cols || (nPoint > 0 && !points) || (nPoint == 0 && !triLights)) return RESTIR_E_INVALID_ARGUMENT;
+    // point lights weigh by luminance, triangle lights by luminance * area (misc.cpp:426-442)
+    std::vector<float> power(n);
+    for (uint32_t i = 0; i < n; ++i)
+        power[i] = nPoint > 0 ? points[i].color_luminance[3]
+                              : triLights[i].emission_luminance[3] * triLights[i].normalArea[3];
+    vose(power.data(), n, cols);
     return RESTIR_OK;
 }

@@ -20,6 +20,7 @@ VISIBILITY_REUSE_FLAG = 1 << 0
 TEMPORAL_REUSE_FLAG = 1 << 1
 TRACE_FIRST_HIT = 0
 TRACE_REFERENCE_LOOP = 1
+ABI_VERSION = 2  # RESTIR_ABI_VERSION of include/restir_abi.h
 
 ERRORS = {
     -1: "RESTIR_E_INVALID_ARGUMENT",
@@ -118,6 +119,8 @@ SIGNATURES = {
     "restir_trace_shadow": (_i32, [_vp, _vp, _vp, _u32, _vp, _vp, _vp, _i32, _vp]),
     "restir_build_aabb_tree": (_i32, [_vp, _u32, _vp, _vp, C.POINTER(_u32)]),
     "restir_build_alias_table": (_i32, [_vp, _u32, _vp, _u32, _vp]),
+    "restir_build_light_clusters": (_i32, [_vp, _u32, _u32, C.c_uint64, _u32, _vp, _vp, _vp, _vp,
+                                           _vp, _vp, _vp]),
     "restir_primary_gbuffer": (_i32, [_vp, C.POINTER(Camera), _vp, _vp, _u32, _vp, _vp, _vp, _vp,
                                       _vp, _u64, _u32, _u32, _vp]),
     "restir_camera_projection_view": (_i32, [C.POINTER(Camera), _f32, C.POINTER(_f32)]),
@@ -147,6 +150,9 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.restir_version() != ABI_VERSION:
+        raise RuntimeError(f"{p} implements ABI revision {lib.restir_version()}, this binding expects "
+                           f"{ABI_VERSION}: rebuild the library (`make -C {PKG_DIR}`)")
     if path is None:
         _LIB = lib
     return lib
@@ -200,6 +206,32 @@ def build_alias_table(points: np.ndarray | None, tri_lights: np.ndarray | None) 
     if rc != 0:
         raise RestirError(rc, "restir_build_alias_table", "build failed")
     return cols
+
+
+class LightClusters:
+    """Host arrays of restir_build_light_clusters."""
+
+    def __init__(self, points: np.ndarray, k: int, seed: int = 7, iterations: int = 16):
+        lib = load_library()
+        n = len(points)
+        pts = np.ascontiguousarray(points, dtype=POINT_LIGHT_DTYPE)
+        self.k = k
+        self.light_cluster = np.zeros(n, dtype=np.int32)
+        self.light_pwithin = np.zeros(n, dtype=np.float32)
+        self.begin = np.zeros(k, dtype=np.int32)
+        self.count = np.zeros(k, dtype=np.int32)
+        self.lights = np.zeros(n, dtype=np.int32)
+        self.alias = np.zeros(n, dtype=ALIAS_DTYPE)
+        self.centroids = np.zeros((k, 3), dtype=np.float32)
+        rc = lib.restir_build_light_clusters(
+            pts.ctypes.data, n, k, seed, iterations, self.light_cluster.ctypes.data,
+            self.light_pwithin.ctypes.data, self.begin.ctypes.data, self.count.ctypes.data,
+            self.lights.ctypes.data, self.alias.ctypes.data, self.centroids.ctypes.data)
+        if rc != 0:
+            raise RestirError(rc, "restir_build_light_clusters", f"n={n} k={k}")
+        lum = pts["color_lum"][:, 3].astype(np.float64)
+        power = np.bincount(self.light_cluster, weights=lum, minlength=k)
+        self.power_share = (power / power.sum()).astype(np.float32)
 
 
 def pack_buffer(records: np.ndarray | None, dtype: np.dtype) -> np.ndarray:

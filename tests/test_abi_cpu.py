@@ -68,7 +68,8 @@ def test_argument_errors_without_device(product_lib):
     assert product_lib.restir_set_lights(None, None, 0, None, 0, None, 0) == -1
     assert product_lib.restir_set_bvh(None, None, 0, None, 0) == -1
     assert product_lib.restir_camera_projection_view(None, 1.0, None) == -1
-    assert product_lib.restir_version() == 1
+    assert product_lib.restir_version() == abi.ABI_VERSION
+    assert f"#define RESTIR_ABI_VERSION {abi.ABI_VERSION}" in HEADER.read_text()
 
 
 def test_create_reports_missing_device(product_lib):

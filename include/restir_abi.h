@@ -136,11 +136,18 @@ typedef struct restir_gbuffer {
     uint32_t height;
 } restir_gbuffer;
 
-/* Launch tile for the per-pixel passes (pixels per workgroup = tile_x * tile_y, a multiple of
- * 64). {0,0} selects the built-in default. */
+/* Launch shape of the per-pixel passes: the workgroup tile (pixels per workgroup = tile_x *
+ * tile_y, a multiple of 64; {0,0} = built-in default) and the schedule of Pass C's shadow rays:
+ * RESTIR_SCHEDULE_QUEUED (default) moves them into queues traced by a persistent kernel, using
+ * context-owned scratch; RESTIR_SCHEDULE_INLINE traces them per pixel in a single dispatch, as
+ * the reference does. Both produce identical reservoirs. */
+#define RESTIR_SCHEDULE_QUEUED 0
+#define RESTIR_SCHEDULE_INLINE 1
 typedef struct restir_launch {
     uint32_t tile_x;
     uint32_t tile_y;
+    uint32_t schedule;
+    uint32_t _reserved;
 } restir_launch;
 
 /* ----------------------------------------------------------------------------------------------
@@ -165,7 +172,7 @@ void restir_destroy(restir_ctx *ctx);
 const char *restir_last_error(const restir_ctx *ctx);
 /* ABI revision; bumped whenever an entry point's signature changes. Bindings compare it with the
  * revision they were written against and refuse a mismatched library. */
-#define RESTIR_ABI_VERSION 2
+#define RESTIR_ABI_VERSION 3
 int restir_version(void);
 
 /* Light tables: device pointers to the reference buffer images (header + array). The counts are
@@ -209,9 +216,12 @@ int restir_shade(restir_ctx *ctx, const float camera_pos[3], float gamma, const 
  * loop = RESTIR_TRACE_REFERENCE_LOOP reproduces the reference's loop exactly (triangle hits take
  * effect every 8 node visits, softwareRaytracing.glsl:39-77), so node_visits equals its
  * loop-iteration count; RESTIR_TRACE_FIRST_HIT stops a ray at its first hit. Both give the same
- * shadowed[] values. */
+ * shadowed[] values. By default a persistent grid pulls rays from a queue and refills idle lanes
+ * of a wave; RESTIR_TRACE_ONE_RAY_PER_LANE (OR-ed into `loop`) launches one lane per ray instead.
+ * The context's work counter is shared, so traces on one context are stream-ordered. */
 #define RESTIR_TRACE_FIRST_HIT 0
 #define RESTIR_TRACE_REFERENCE_LOOP 1
+#define RESTIR_TRACE_ONE_RAY_PER_LANE 2
 int restir_trace_shadow(restir_ctx *ctx, const float *p1, const float *p2, uint32_t n,
                         uint32_t *shadowed, uint32_t *node_visits, uint32_t *tri_tests, int loop,
                         void *stream);

@@ -1274,3 +1274,163 @@ int oracle_build_light_clusters(const restir_point_light *pts, uint32_t n, uint3
     free(w); free(perm); free(c); free(sum); free(cnt); free(asg);
     return 0;
 }
+
+/* ------------------------------------------------------------------ learned proposal ---- */
+/* MLP light proposal (no reference counterpart). Features (114 used of 128): the surface position
+ * normalised to the scene box, the normal and the unit view direction; raw position and normal
+ * (6), then for each of the 9 scalars s and k = 0..5: sin(2^k*pi*s), cos(2^k*pi*s).
+ * Network: 128 -> 128 -> 128 -> 128 -> K with ReLU; weights stored transposed W[out][in].
+ * With emulate_bf16 the forward/backward round at the points the GPU stores bf16 tensors:
+ * hidden activations after ReLU, and dlogits / dhidden before each GEMM that consumes them. */
+static inline float bf16_round(float x) {
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return x; /* inf / nan unchanged */
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    float r;
+    memcpy(&r, &u, 4);
+    return r;
+}
+
+void oracle_mlp_features(const restir_gbuffer *g, const float cam[3], const float lo[3],
+                         const float hi[3], const uint32_t *pixels, uint32_t n, float *out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t b = 0; b < (int64_t)n; ++b) {
+        texel t = fetch(g, pixels[b] % g->width, pixels[b] / g->width);
+        double s[9];
+        double p[3] = {t.pos.x, t.pos.y, t.pos.z};
+        for (int k = 0; k < 3; ++k) {
+            double c = 0.5 * ((double)lo[k] + hi[k]), h = 0.5 * ((double)hi[k] - lo[k]);
+            s[k] = (p[k] - c) / h;
+        }
+        s[3] = t.n.x; s[4] = t.n.y; s[5] = t.n.z;
+        double v[3] = {cam[0] - p[0], cam[1] - p[1], cam[2] - p[2]};
+        double vl = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (int k = 0; k < 3; ++k) s[6 + k] = vl > 0 ? v[k] / vl : 0.0;
+        float *o = out + 128 * b;
+        for (int k = 0; k < 6; ++k) o[k] = (float)s[k];
+        for (int i = 0; i < 9; ++i)
+            for (int f = 0; f < 6; ++f) {
+                double a = M_PI * (double)(1 << f) * s[i];
+                o[6 + 12 * i + 2 * f] = (float)sin(a);
+                o[6 + 12 * i + 2 * f + 1] = (float)cos(a);
+            }
+        for (int k = 114; k < 128; ++k) o[k] = 0.0f;
+    }
+}
+
+/* offsets of the four weight matrices / bias vectors in the flat parameter arrays */
+static void mlp_offsets(uint32_t K, size_t wo[4], size_t bo[4]) {
+    wo[0] = 0; wo[1] = 128 * 128; wo[2] = 2 * 128 * 128; wo[3] = 3 * 128 * 128;
+    bo[0] = 0; bo[1] = 128; bo[2] = 256; bo[3] = 384;
+    (void)K;
+}
+
+static void mlp_forward_one(const float *W, const float *B, uint32_t K, const float *x, int emu,
+                            double h[3][128], double *z) {
+    size_t wo[4], bo[4];
+    mlp_offsets(K, wo, bo);
+    const double *prev = NULL;
+    double in0[128];
+    for (int k = 0; k < 128; ++k) in0[k] = x[k];
+    prev = in0;
+    for (int l = 0; l < 3; ++l) {
+        for (int o = 0; o < 128; ++o) {
+            double acc = B[bo[l] + o];
+            const float *w = W + wo[l] + 128 * (size_t)o;
+            for (int k = 0; k < 128; ++k) acc += (double)w[k] * prev[k];
+            double r = acc > 0.0 ? acc : 0.0;
+            h[l][o] = emu ? (double)bf16_round((float)r) : r;
+        }
+        prev = h[l];
+    }
+    for (uint32_t o = 0; o < K; ++o) {
+        double acc = B[bo[3] + o];
+        const float *w = W + wo[3] + 128 * (size_t)o;
+        for (int k = 0; k < 128; ++k) acc += (double)w[k] * prev[k];
+        z[o] = acc;
+    }
+}
+
+void oracle_mlp_forward(const float *W, const float *B, uint32_t K, const float *X, uint32_t n,
+                        int emulate_bf16, double *logits) {
+#pragma omp parallel for schedule(static)
+    for (int64_t b = 0; b < (int64_t)n; ++b) {
+        double h[3][128];
+        mlp_forward_one(W, B, K, X + 128 * b, emulate_bf16, h, logits + (size_t)K * b);
+    }
+}
+
+/* Mean cross-entropy over samples with mask != 0 against target clusters, and its gradient. */
+double oracle_mlp_loss_grad(const float *W, const float *B, uint32_t K, const float *X,
+                            const int32_t *target, const uint8_t *mask, uint32_t n,
+                            int emulate_bf16, double *gW, double *gB) {
+    size_t wo[4], bo[4];
+    mlp_offsets(K, wo, bo);
+    size_t nW = 3 * 128 * 128 + (size_t)K * 128, nB = 384 + K;
+    memset(gW, 0, sizeof(double) * nW);
+    memset(gB, 0, sizeof(double) * nB);
+    uint32_t nv = 0;
+    for (uint32_t b = 0; b < n; ++b) nv += mask[b] ? 1u : 0u;
+    if (nv == 0) return 0.0;
+    double loss = 0.0;
+    double *z = (double *)malloc(sizeof(double) * K), *dz = (double *)malloc(sizeof(double) * K);
+    for (uint32_t b = 0; b < n; ++b) {
+        if (!mask[b]) continue;
+        double h[3][128], dh[128], dprev[128];
+        const float *x = X + 128 * (size_t)b;
+        mlp_forward_one(W, B, K, x, emulate_bf16, h, z);
+        double mx = z[0];
+        for (uint32_t o = 1; o < K; ++o) mx = z[o] > mx ? z[o] : mx;
+        double se = 0.0;
+        for (uint32_t o = 0; o < K; ++o) se += exp(z[o] - mx);
+        loss += -(z[target[b]] - mx - log(se));
+        for (uint32_t o = 0; o < K; ++o) {
+            double g = (exp(z[o] - mx) / se - (o == (uint32_t)target[b] ? 1.0 : 0.0)) / nv;
+            dz[o] = emulate_bf16 ? (double)bf16_round((float)g) : g;
+        }
+        /* layer 4 */
+        for (uint32_t o = 0; o < K; ++o) {
+            gB[bo[3] + o] += dz[o];
+            for (int k = 0; k < 128; ++k) gW[wo[3] + 128 * (size_t)o + k] += dz[o] * h[2][k];
+        }
+        for (int k = 0; k < 128; ++k) {
+            double acc = 0.0;
+            for (uint32_t o = 0; o < K; ++o) acc += dz[o] * W[wo[3] + 128 * (size_t)o + k];
+            acc = h[2][k] > 0.0 ? acc : 0.0;
+            dh[k] = emulate_bf16 ? (double)bf16_round((float)acc) : acc;
+        }
+        for (int l = 2; l >= 0; --l) {
+            const double *in = NULL;
+            double in0[128];
+            if (l == 0) { for (int k = 0; k < 128; ++k) in0[k] = x[k]; in = in0; }
+            else in = h[l - 1];
+            for (int o = 0; o < 128; ++o) {
+                gB[bo[l] + o] += dh[o];
+                for (int k = 0; k < 128; ++k) gW[wo[l] + 128 * (size_t)o + k] += dh[o] * in[k];
+            }
+            if (l == 0) break;
+            for (int k = 0; k < 128; ++k) {
+                double acc = 0.0;
+                for (int o = 0; o < 128; ++o) acc += dh[o] * W[wo[l] + 128 * (size_t)o + k];
+                acc = h[l - 1][k] > 0.0 ? acc : 0.0;
+                dprev[k] = emulate_bf16 ? (double)bf16_round((float)acc) : acc;
+            }
+            memcpy(dh, dprev, sizeof dh);
+        }
+    }
+    free(z); free(dz);
+    return loss / nv;
+}
+
+/* Adam (Kingma & Ba) with bias correction at step t >= 1. */
+void oracle_adam(double *p, const double *g, double *m, double *v, size_t n, double lr, double b1,
+                 double b2, double eps, uint32_t t) {
+    double c1 = 1.0 - pow(b1, (double)t), c2 = 1.0 - pow(b2, (double)t);
+    for (size_t i = 0; i < n; ++i) {
+        m[i] = b1 * m[i] + (1.0 - b1) * g[i];
+        v[i] = b2 * v[i] + (1.0 - b2) * g[i] * g[i];
+        p[i] -= lr * (m[i] / c1) / (sqrt(v[i] / c2) + eps);
+    }
+}

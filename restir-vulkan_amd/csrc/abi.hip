@@ -19,6 +19,14 @@ struct restir_ctx {
     bool bvhSet = false;
     int stack = 32;
     uint32_t treeDepth = 0;
+    uint32_t *counters = nullptr;  // device work counters of the persistent kernels
+    uint32_t persistentBlocks = 1280;
+    void *scratch = nullptr;       // ray queues of the queued Pass C schedule
+    size_t scratchBytes = 0;
+    ~restir_ctx() {
+        if (counters) (void)hipFree(counters);
+        if (scratch) (void)hipFree(scratch);
+    }
 };
 
 namespace {
@@ -87,6 +95,24 @@ int check_frame(restir_ctx *ctx, const restir_uniforms *u, const char *name) {
     return RESTIR_OK;
 }
 
+// Grows the context's scratch to at least `bytes` (hipFree waits for work still using it).
+int ensure_scratch(restir_ctx *ctx, size_t bytes) {
+    if (bytes <= ctx->scratchBytes) return RESTIR_OK;
+    if (ctx->scratch) {
+        const hipError_t e = hipFree(ctx->scratch);
+        ctx->scratch = nullptr;
+        ctx->scratchBytes = 0;
+        if (e != hipSuccess) return hip_fail(ctx, e, "scratch: hipFree");
+    }
+    const hipError_t e = hipMalloc(&ctx->scratch, bytes);
+    if (e != hipSuccess) {
+        ctx->scratch = nullptr;
+        return hip_fail(ctx, e, "scratch: hipMalloc");
+    }
+    ctx->scratchBytes = bytes;
+    return RESTIR_OK;
+}
+
 int pick_tile(restir_ctx *ctx, const restir_launch *l, int *tile) {
     if (!l || (l->tile_x == 0 && l->tile_y == 0)) {
         *tile = 0;
@@ -112,10 +138,13 @@ int restir_create(restir_ctx **out, int device) {
     if (device < 0 || device >= n) return RESTIR_E_NO_DEVICE;
     auto *ctx = new restir_ctx();
     ctx->device = device;
-    if (use_device(ctx) != RESTIR_OK) {
+    if (use_device(ctx) != RESTIR_OK || hipMalloc(&ctx->counters, 256) != hipSuccess) {
         delete ctx;
         return RESTIR_E_HIP;
     }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        ctx->persistentBlocks = static_cast<uint32_t>(cus) * 5u;  // 5 x 256 lanes fit a CU's LDS
     *out = ctx;
     return RESTIR_OK;
 }
@@ -282,9 +311,17 @@ int restir_unbiased(restir_ctx *ctx, const restir_uniforms *u, int k, int iter,
     if (overlaps(in, bytes, out, bytes)) return fail(ctx, RESTIR_E_ALIASING, "unbiased: in and out overlap");
     int tile;
     if (int r = pick_tile(ctx, launch, &tile)) return r;
+    const uint32_t schedule = launch ? launch->schedule : RESTIR_SCHEDULE_QUEUED;
+    if (schedule != RESTIR_SCHEDULE_QUEUED && schedule != RESTIR_SCHEDULE_INLINE)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "unbiased: unknown schedule %u", schedule);
     if (int r = use_device(ctx)) return r;
+    void *queues = nullptr;
+    if (schedule == RESTIR_SCHEDULE_QUEUED) {
+        if (int r = ensure_scratch(ctx, rk::pass_c_queue_bytes(W * H, k))) return r;
+        queues = ctx->scratch;
+    }
     const hipError_t e = rk::launch_pass_c(tile, ctx->stack, *u, k, iter, ctx->scene, view_of(g), in, out,
-                                           static_cast<hipStream_t>(stream));
+                                           queues, ctx->persistentBlocks, static_cast<hipStream_t>(stream));
     return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "unbiased launch");
 }
 
@@ -311,10 +348,12 @@ int restir_trace_shadow(restir_ctx *ctx, const float *p1, const float *p2, uint3
     if (!ctx->bvhSet) return fail(ctx, RESTIR_E_NOT_READY, "trace_shadow: BVH not set");
     if (n == 0) return RESTIR_OK;
     if (!p1 || !p2 || !shadowed) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "trace_shadow: NULL buffer");
-    if (loop != RESTIR_TRACE_FIRST_HIT && loop != RESTIR_TRACE_REFERENCE_LOOP)
-        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "trace_shadow: unknown loop mode %d", loop);
+    if ((loop & ~(RESTIR_TRACE_REFERENCE_LOOP | RESTIR_TRACE_ONE_RAY_PER_LANE)) != 0)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "trace_shadow: unknown loop flags %d", loop);
     if (int r = use_device(ctx)) return r;
-    const hipError_t e = rk::launch_trace(ctx->stack, loop == RESTIR_TRACE_REFERENCE_LOOP, ctx->scene,
+    const hipError_t e = rk::launch_trace(ctx->stack, (loop & RESTIR_TRACE_REFERENCE_LOOP) != 0,
+                                          (loop & RESTIR_TRACE_ONE_RAY_PER_LANE) == 0,
+                                          ctx->persistentBlocks, ctx->counters, ctx->scene,
                                           reinterpret_cast<const float4 *>(p1),
                                           reinterpret_cast<const float4 *>(p2), n, shadowed, visits,
                                           tests, static_cast<hipStream_t>(stream));

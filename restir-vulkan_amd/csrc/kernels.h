@@ -40,15 +40,21 @@ hipError_t launch_pass_a(int tile, int stack, const restir_uniforms &u, const Sc
 hipError_t launch_pass_b(int tile, const restir_uniforms &u, int iter, float cosThr,
                          const GView &g, const restir_reservoir *in, restir_reservoir *out,
                          hipStream_t s);
+// queueScratch == nullptr: one dispatch, rays traced inline per pixel. Otherwise the ray-queue
+// schedule, using pass_c_queue_bytes(npx, k) bytes of device scratch.
 hipError_t launch_pass_c(int tile, int stack, const restir_uniforms &u, int k, int iter,
-                         const SceneView &sc,
-                         const GView &g, const restir_reservoir *in, restir_reservoir *out,
+                         const SceneView &sc, const GView &g, const restir_reservoir *in,
+                         restir_reservoir *out, void *queueScratch, uint32_t gridCap,
                          hipStream_t s);
+size_t pass_c_queue_bytes(uint32_t npx, int k);
 hipError_t launch_shade(const SceneView &sc, rt::f3 cam, float gamma, const GView &g,
                         const restir_reservoir *res, float *rgb, uint64_t rgbP, hipStream_t s);
-hipError_t launch_trace(int stack, bool refLoop, const SceneView &sc, const float4 *p1,
-                        const float4 *p2, uint32_t n, uint32_t *shadowed, uint32_t *visits,
-                        uint32_t *tests, hipStream_t s);
+// persistent: fixed grid of at most gridCap workgroups pulling rays through *counter (which the
+// launch clears with a kernel, so the sequence can be captured in a graph).
+hipError_t launch_trace(int stack, bool refLoop, bool persistent, uint32_t gridCap,
+                        uint32_t *counter, const SceneView &sc, const float4 *p1, const float4 *p2,
+                        uint32_t n, uint32_t *shadowed, uint32_t *visits, uint32_t *tests,
+                        hipStream_t s);
 
 struct CamBasis {
     rt::f3 pos, fwd, right, down;

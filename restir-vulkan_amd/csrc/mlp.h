@@ -1,0 +1,42 @@
+// mlp.h — MFMA building blocks of the learned light proposal (gfx950, bf16 in, fp32 accumulate).
+//
+// v_mfma_f32_32x32x16_bf16 multiplies a 32x16 tile A by a 16x32 tile B into a 32x32 fp32 tile.
+// Operand layout for wave64 lane l (checked on the device by tests/test_gpu_mlp.py):
+//   A: 8 bf16 = A[row = l % 32][k = 8 * (l / 32) + j], j = 0..7
+//   B: 8 bf16 = B[k = 8 * (l / 32) + j][col = l % 32]
+//   D: 16 fp32, acc[i] = D[row = 8 * (i / 4) + 4 * (l / 32) + i % 4][col = l % 32]
+#pragma once
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rm {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kIn = 128;      // feature width (114 used, zero padded)
+constexpr int kHidden = 128;  // hidden width
+constexpr int kFeatures = 114;
+
+__device__ __forceinline__ f32x16 mfma32(bf16x8 a, bf16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// bf16 <-> fp32 with round-to-nearest-even (NaN kept quiet)
+__device__ __forceinline__ uint16_t f2bf(float x) {
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return static_cast<uint16_t>((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+// Offsets (elements) of W1..W4 ([out][in]) and b1..b4 in the flat parameter arrays.
+__host__ __device__ constexpr size_t w_off(int layer) { return static_cast<size_t>(layer) * kHidden * kIn; }
+__host__ __device__ constexpr size_t b_off(int layer) { return static_cast<size_t>(layer) * kHidden; }
+__host__ __device__ constexpr size_t n_weights(int K) { return 3 * static_cast<size_t>(kHidden) * kIn + static_cast<size_t>(K) * kHidden; }
+__host__ __device__ constexpr size_t n_bias(int K) { return 3 * static_cast<size_t>(kHidden) + K; }
+
+}  // namespace rm

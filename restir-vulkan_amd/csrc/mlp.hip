@@ -1,0 +1,450 @@
+// mlp.hip — the learned light proposal on gfx950 (no reference counterpart).
+//
+// A 128 -> 128 -> 128 -> 128 -> K ReLU MLP maps per-pixel features (position, normal and view
+// direction with 6 sin/cos octaves, 114 values) to a softmax over K light clusters. The proposal
+// mixes it with the reference's source distribution (alias table over light power):
+//   q(l | x) = (1 - mix) * pi_x(c(l)) * p_within(l) + mix * p_src(l),   mix = 0.1,
+// so q > 0 wherever p_src > 0 and RIS stays unbiased. Weights are bf16 (fp32 master copy, Adam);
+// every product runs on v_mfma_f32_32x32x16_bf16 with fp32 accumulation.
+//
+// Kernels:
+//   k_propose      fused inference: features -> 4 MFMA layers -> softmax -> M candidates (l, q)
+//   k_features     features of listed pixels (bf16), for training and tests
+//   k_gemm         strided C = A*B with bias / ReLU / ReLU-gate epilogues (training fwd/bwd)
+//   k_softmax_ce   mean cross-entropy of the logits against target clusters, and dlogits
+//   k_colsum       bias gradients
+//   k_adam         Adam step on fp32 master weights, refreshes the bf16 copy
+//   k_mlp_init     seeded He-uniform initialisation
+#include "../../include/restir_abi.h"
+#include "kernels.h"
+#include "mlp.h"
+
+namespace rm {
+
+using rt::f3;
+
+// ------------------------------------------------------------------------------ features ---
+// Writes the 128 bf16 features of one surface point. The scalars are the position normalised
+// to the scene box, the normal and the unit view direction; raw position and normal first, then
+// sin/cos(2^k * pi * s) for k = 0..5 (double-angle recurrences from k = 0), zero padding last.
+__device__ __forceinline__ void make_features(f3 pos, f3 n, f3 cam, const float *lo, const float *hi,
+                                              uint16_t *dst) {
+    float s[9];
+    const float pp[3] = {pos.x, pos.y, pos.z};
+    for (int k = 0; k < 3; ++k) {
+        const float c = 0.5f * (lo[k] + hi[k]), h = 0.5f * (hi[k] - lo[k]);
+        s[k] = (pp[k] - c) / h;
+    }
+    s[3] = n.x;
+    s[4] = n.y;
+    s[5] = n.z;
+    f3 v = cam - pos;
+    const float vl = sqrtf(rt::dot(v, v));
+    v = vl > 0.0f ? v / vl : f3{0.0f, 0.0f, 0.0f};
+    s[6] = v.x;
+    s[7] = v.y;
+    s[8] = v.z;
+    for (int k = 0; k < 6; ++k) dst[k] = f2bf(s[k]);
+    for (int i = 0; i < 9; ++i) {
+        float sn, cs;
+        sincosf(3.14159265358979f * s[i], &sn, &cs);
+        for (int f = 0; f < 6; ++f) {
+            dst[6 + 12 * i + 2 * f] = f2bf(sn);
+            dst[6 + 12 * i + 2 * f + 1] = f2bf(cs);
+            const float s2 = 2.0f * sn * cs, c2 = (cs - sn) * (cs + sn);
+            sn = s2;
+            cs = c2;
+        }
+    }
+    for (int k = kFeatures; k < kIn; ++k) dst[k] = 0;
+}
+
+struct FeatArgs {
+    rk::GView g;
+    f3 cam;
+    float lo[3], hi[3];
+};
+
+__device__ __forceinline__ bool pixel_features(const FeatArgs &f, uint32_t p, uint16_t *dst) {
+    const f3 pos = f3{f.g.pos[p], f.g.pos[f.g.P + p], f.g.pos[2 * f.g.P + p]};
+    const f3 n = f3{f.g.nrm[p], f.g.nrm[f.g.P + p], f.g.nrm[2 * f.g.P + p]};
+    make_features(pos, n, f.cam, f.lo, f.hi, dst);
+    return rt::dot(n, n) != 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_features(const FeatArgs f, const uint32_t *pixels, uint32_t n,
+                                                  uint16_t *out) {
+    const uint32_t b = blockIdx.x * 256u + threadIdx.x;
+    if (b >= n) return;
+    uint16_t tmp[kIn];
+    pixel_features(f, pixels[b], tmp);
+    uint4 *d = reinterpret_cast<uint4 *>(out + static_cast<size_t>(b) * kIn);
+    const uint4 *t = reinterpret_cast<const uint4 *>(tmp);
+    for (int i = 0; i < kIn / 8; ++i) d[i] = t[i];
+}
+
+// -------------------------------------------------------------------------------- GEMM ------
+struct GemmArgs {
+    const uint16_t *A;
+    int64_t sam, sak;
+    const uint16_t *B;
+    int64_t sbk, sbn;
+    int M, N, K;
+    const float *bias;     // per output column, or null
+    int relu;              // clamp at 0
+    const uint16_t *gate;  // out *= (gate(m, n) > 0), or null
+    int64_t sgm, sgn;
+    float *Cf;             // fp32 output, or null
+    uint16_t *Cb;          // bf16 output, or null
+    int64_t scm, scn;
+};
+
+// 64x64 output tile per 256-lane workgroup; each wave owns a 32x32 quarter. K advances in steps
+// of 32 through LDS tiles As[64][32] and Bt[64][32] (B staged transposed so every fragment is
+// eight consecutive bf16). Requires M, N multiples of 64 and K a multiple of 32.
+__global__ __launch_bounds__(256) void k_gemm(const GemmArgs g) {
+    __shared__ __align__(16) uint16_t As[64][40];
+    __shared__ __align__(16) uint16_t Bt[64][40];
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    f32x16 acc;
+    for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+    for (int k0 = 0; k0 < g.K; k0 += 32) {
+        for (int e = tid; e < 64 * 32; e += 256) {
+            const int r = e >> 5, c = e & 31;
+            As[r][c] = g.A[(m0 + r) * g.sam + (k0 + c) * g.sak];
+            Bt[r][c] = g.B[(k0 + c) * g.sbk + (n0 + r) * g.sbn];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 32; ks += 16) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&As[wm + (lane & 31)][ks + 8 * (lane >> 5)]);
+            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&Bt[wn + (lane & 31)][ks + 8 * (lane >> 5)]);
+            acc = mfma32(a, b, acc);
+        }
+        __syncthreads();
+    }
+    const int col = n0 + wn + (lane & 31);
+    const float bias = g.bias ? g.bias[col] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int row = m0 + wm + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+        float v = acc[i] + bias;
+        if (g.relu) v = v > 0.0f ? v : 0.0f;
+        if (g.gate && !(bf2f(g.gate[row * g.sgm + col * g.sgn]) > 0.0f)) v = 0.0f;
+        if (g.Cf) g.Cf[row * g.scm + col * g.scn] = v;
+        if (g.Cb) g.Cb[row * g.scm + col * g.scn] = f2bf(v);
+    }
+}
+
+// ----------------------------------------------------------------------------- loss/grad ----
+// dZ = (softmax(z) - onehot(target)) / n_valid for samples with mask, 0 otherwise (bf16);
+// the summed loss goes to *loss (divided by n_valid).
+__global__ __launch_bounds__(256) void k_softmax_ce(const float *Z, int K, const int32_t *target,
+                                                    const uint32_t *mask, int n,
+                                                    const uint32_t *nValid, uint16_t *dZ,
+                                                    float *loss) {
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= n) return;
+    const float *z = Z + static_cast<size_t>(b) * K;
+    uint16_t *d = dZ + static_cast<size_t>(b) * K;
+    if (!mask[b]) {
+        for (int c = 0; c < K; ++c) d[c] = 0;
+        return;
+    }
+    const float inv = 1.0f / static_cast<float>(*nValid);
+    float mx = z[0];
+    for (int c = 1; c < K; ++c) mx = fmaxf(mx, z[c]);
+    float se = 0.0f;
+    for (int c = 0; c < K; ++c) se += expf(z[c] - mx);
+    const int t = target[b];
+    atomicAdd(loss, -(z[t] - mx - logf(se)) * inv);
+    const float rse = 1.0f / se;
+    for (int c = 0; c < K; ++c) d[c] = f2bf((expf(z[c] - mx) * rse - (c == t ? 1.0f : 0.0f)) * inv);
+}
+
+// out[c] = sum over rows of G[row][c] (bf16 input, fp32 sum)
+__global__ __launch_bounds__(256) void k_colsum(const uint16_t *G, int rows, int cols, float *out) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= cols) return;
+    float s = 0.0f;
+    for (int r = 0; r < rows; ++r) s += bf2f(G[static_cast<size_t>(r) * cols + c]);
+    out[c] = s;
+}
+
+// --------------------------------------------------------------------------------- Adam -----
+struct AdamArgs {
+    float *w;          // fp32 master weights (nW), then biases (nB) in `b`
+    float *b;
+    uint16_t *wbf;     // bf16 copy of the weights
+    const float *gw, *gb;
+    float *m, *v;      // nW + nB each
+    uint32_t nW, nB;
+    float lr, b1, b2, eps, c1, c2;  // c1 = 1 - b1^t, c2 = 1 - b2^t
+};
+
+__global__ __launch_bounds__(256) void k_adam(const AdamArgs a) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.nW + a.nB) return;
+    const bool isW = i < a.nW;
+    const float g = isW ? a.gw[i] : a.gb[i - a.nW];
+    const float m = a.b1 * a.m[i] + (1.0f - a.b1) * g;
+    const float v = a.b2 * a.v[i] + (1.0f - a.b2) * g * g;
+    a.m[i] = m;
+    a.v[i] = v;
+    const float step = a.lr * (m / a.c1) / (sqrtf(v / a.c2) + a.eps);
+    if (isW) {
+        const float w = a.w[i] - step;
+        a.w[i] = w;
+        a.wbf[i] = f2bf(w);
+    } else {
+        a.b[i - a.nW] -= step;
+    }
+}
+
+// He-uniform weights U(-sqrt(6/fan_in), sqrt(6/fan_in)) from PCG32(seed, element index); zero
+// biases except the output layer, which starts at `prior` (log cluster power shares) so the
+// untrained proposal already samples clusters by power. Adam moments start at zero.
+__global__ __launch_bounds__(256) void k_mlp_init(float *w, uint16_t *wbf, float *b, float *m, float *v,
+                                                  uint32_t nW, uint32_t nB, uint32_t K, const float *prior,
+                                                  uint64_t seed) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < nW) {
+        rt::Pcg r = rt::pcg_seed(seed, i);
+        const float u = rt::pcg_float(r) * 2.0f - 1.0f;
+        const float val = u * sqrtf(6.0f / static_cast<float>(kIn));
+        w[i] = val;
+        wbf[i] = f2bf(val);
+    }
+    if (i < nB) b[i] = (i >= 3u * kHidden) ? prior[i - 3u * kHidden] : 0.0f;
+    if (i < nW + nB) {
+        m[i] = 0.0f;
+        v[i] = 0.0f;
+    }
+    (void)K;
+}
+
+// ------------------------------------------------------------------ training-sample gather --
+struct GatherArgs {
+    FeatArgs f;
+    const restir_reservoir *res;
+    const int32_t *lightCluster;
+    uint32_t npx, n, seed;
+    uint16_t *X;
+    int32_t *target;
+    uint32_t *mask, *nValid;
+    uint32_t *pixels;
+};
+
+// Sample b takes a pseudo-random pixel; it trains (mask = 1) if that pixel's final reservoir
+// holds a point light with W > 0, with the light's cluster as the target.
+__global__ __launch_bounds__(256) void k_train_gather(const GatherArgs a) {
+    const uint32_t b = blockIdx.x * 256u + threadIdx.x;
+    if (b >= a.n) return;
+    rt::Pcg r = rt::pcg_seed(a.seed, b);
+    const uint32_t p = rt::pcg_next(r) % a.npx;
+    a.pixels[b] = p;
+    uint16_t tmp[kIn];
+    const bool surface = pixel_features(a.f, p, tmp);
+    uint4 *d = reinterpret_cast<uint4 *>(a.X + static_cast<size_t>(b) * kIn);
+    const uint4 *t = reinterpret_cast<const uint4 *>(tmp);
+    for (int i = 0; i < kIn / 8; ++i) d[i] = t[i];
+    const float4 q2 = reinterpret_cast<const float4 *>(a.res + p)[2];  // lightIndex, pHat, sumW, w
+    const int32_t li = __float_as_int(q2.x);
+    const bool ok = surface && li >= 0 && q2.w > 0.0f && q2.z > 0.0f;
+    a.mask[b] = ok ? 1u : 0u;
+    a.target[b] = ok ? a.lightCluster[li] : 0;
+    if (ok) atomicAdd(a.nValid, 1u);
+}
+
+// ------------------------------------------------------------------- fused inference -------
+struct ProposeArgs {
+    FeatArgs f;
+    const uint16_t *wbf;   // W1..W4 bf16 [out][in]
+    const float *bias;
+    int K;
+    uint32_t npx, M, seed;
+    float mix;             // weight of the source distribution
+    // proposal tables
+    const float4 *alias;   // global alias table (aliasTableColumn)
+    int32_t nAlias;
+    const int32_t *lightCluster;
+    const float *lightPWithin;
+    const int32_t *clusterBegin, *clusterCount, *clusterLights;
+    const float4 *clusterAlias;  // per-cluster alias columns (alias = local index)
+    int32_t *candLight;
+    float *candQ;
+};
+
+constexpr int kTilePx = 256;                 // pixels per workgroup
+constexpr int kRow = kIn + 8;                // padded LDS row (bf16) against bank conflicts
+constexpr int kMaxK = 256;
+
+__device__ __forceinline__ void stage_weights(uint16_t *wbuf, const uint16_t *src, int rows) {
+    // rows x 128 bf16 from global into LDS rows of kRow, 16 bytes per lane per step
+    for (int e = threadIdx.x; e < rows * (kIn / 8); e += blockDim.x) {
+        const int r = e / (kIn / 8), c = (e % (kIn / 8)) * 8;
+        *reinterpret_cast<uint4 *>(&wbuf[r * kRow + c]) = *reinterpret_cast<const uint4 *>(&src[r * kIn + c]);
+    }
+}
+
+template <int NB>
+__device__ __forceinline__ void layer_mfma(const uint16_t *act, const uint16_t *wbuf, const float *bias,
+                                           int wave, int lane, f32x16 (&acc)[NB]) {
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+        const float bv = bias[32 * nb + (lane & 31)];
+        for (int i = 0; i < 16; ++i) acc[nb][i] = bv;
+    }
+#pragma unroll
+    for (int s = 0; s < kIn / 16; ++s) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&act[(32 * wave + (lane & 31)) * kRow + 16 * s + 8 * (lane >> 5)]);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&wbuf[(32 * nb + (lane & 31)) * kRow + 16 * s + 8 * (lane >> 5)]);
+            acc[nb] = mfma32(a, b, acc[nb]);
+        }
+    }
+}
+
+// One workgroup of 512 lanes (8 waves) proposes for 256 consecutive pixels. LDS holds the
+// activations [256][136] bf16 and one layer's weights [<=256][136] bf16; after the last layer
+// the same 136 KB hold the fp16 logits [256][K].
+template <int KC>
+__global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
+    __shared__ __align__(16) uint16_t lds[2 * kTilePx * kRow];
+    uint16_t *act = lds;
+    uint16_t *wbuf = lds + kTilePx * kRow;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t p0 = blockIdx.x * kTilePx;
+    // features (lanes 0..255) while lanes 256..511 stage W1
+    if (tid < kTilePx) {
+        const uint32_t p = p0 + tid;
+        uint16_t *row = act + tid * kRow;
+        if (p < a.npx) pixel_features(a.f, p, row);
+        else for (int k = 0; k < kIn; ++k) row[k] = 0;
+    } else {
+        for (int e = tid - kTilePx; e < kHidden * (kIn / 8); e += kTilePx) {
+            const int r = e / (kIn / 8), c = (e % (kIn / 8)) * 8;
+            *reinterpret_cast<uint4 *>(&wbuf[r * kRow + c]) = *reinterpret_cast<const uint4 *>(&a.wbf[r * kIn + c]);
+        }
+    }
+    __syncthreads();
+    for (int L = 0; L < 3; ++L) {
+        f32x16 acc[4];
+        layer_mfma<4>(act, wbuf, a.bias + b_off(L), wave, lane, acc);
+        __syncthreads();
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int row = 32 * wave + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                const float v = acc[nb][i];
+                act[row * kRow + 32 * nb + (lane & 31)] = f2bf(v > 0.0f ? v : 0.0f);
+            }
+        stage_weights(wbuf, a.wbf + w_off(L + 1), L + 1 < 3 ? kHidden : KC);
+        __syncthreads();
+    }
+    {
+        f32x16 acc[KC / 32];
+        layer_mfma<KC / 32>(act, wbuf, a.bias + b_off(3), wave, lane, acc);
+        __syncthreads();
+        __half *logit = reinterpret_cast<__half *>(lds);  // [256][KC]
+#pragma unroll
+        for (int nb = 0; nb < KC / 32; ++nb)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int row = 32 * wave + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                logit[row * KC + 32 * nb + (lane & 31)] = __float2half(acc[nb][i]);
+            }
+        __syncthreads();
+    }
+    if (tid >= kTilePx) return;
+    const uint32_t p = p0 + tid;
+    if (p >= a.npx) return;
+    const FeatArgs &f = a.f;
+    const f3 n = f3{f.g.nrm[p], f.g.nrm[f.g.P + p], f.g.nrm[2 * f.g.P + p]};
+    if (rt::dot(n, n) == 0.0f) return;  // background: Pass A draws no candidates there
+    const __half *lg = reinterpret_cast<const __half *>(lds) + tid * KC;
+    float mx = -65504.0f;
+    for (int c = 0; c < KC; ++c) mx = fmaxf(mx, __half2float(lg[c]));
+    float se = 0.0f;
+    for (int c = 0; c < KC; ++c) se += __expf(__half2float(lg[c]) - mx);
+    const float rse = 1.0f / se;
+    rt::Pcg rng = rt::pcg_seed(a.seed, p);
+    // choose which candidates come from the MLP, and their sorted cluster uniforms
+    constexpr int kMaxM = 16;
+    float uc[kMaxM];
+    int m = 0;
+    uint32_t srcMask = 0;
+    for (uint32_t j = 0; j < a.M; ++j) {
+        if (rt::pcg_float(rng) < a.mix) srcMask |= 1u << j;
+        else uc[m++] = rt::pcg_float(rng) * se;
+    }
+    for (int i = 1; i < m; ++i) {  // insertion sort, m <= 16
+        const float v = uc[i];
+        int j = i - 1;
+        while (j >= 0 && uc[j] > v) {
+            uc[j + 1] = uc[j];
+            --j;
+        }
+        uc[j + 1] = v;
+    }
+    int ccl[kMaxM];
+    {
+        int c = 0, i = 0;
+        float cdf = __expf(__half2float(lg[0]) - mx);
+        while (i < m) {
+            if (uc[i] < cdf || c == KC - 1) {
+                ccl[i++] = c;
+            } else {
+                ++c;
+                cdf += __expf(__half2float(lg[c]) - mx);
+            }
+        }
+    }
+    int mi = 0;
+    for (uint32_t j = 0; j < a.M; ++j) {
+        int32_t l;
+        if (srcMask & (1u << j)) {  // source distribution: the reference's alias table
+            const float r1 = rt::pcg_float(rng), r2 = rt::pcg_float(rng);
+            int32_t col = static_cast<int32_t>(static_cast<float>(a.nAlias) * r1);
+            col = col < a.nAlias - 1 ? col : a.nAlias - 1;
+            const float4 t = a.alias[col];
+            l = t.x > r2 ? col : __float_as_int(t.y);
+        } else {                    // MLP cluster, then by power inside it
+            const int c = ccl[mi++];
+            const int32_t cnt = a.clusterCount[c], beg = a.clusterBegin[c];
+            const float r1 = rt::pcg_float(rng), r2 = rt::pcg_float(rng);
+            if (cnt == 0) {
+                l = -1;
+            } else {
+                int32_t col = static_cast<int32_t>(static_cast<float>(cnt) * r1);
+                col = col < cnt - 1 ? col : cnt - 1;
+                const float4 t = a.clusterAlias[beg + col];
+                l = a.clusterLights[beg + (t.x > r2 ? col : __float_as_int(t.y))];
+            }
+        }
+        float q = 0.0f;
+        if (l >= 0) {
+            const int c = a.lightCluster[l];
+            const float pi = __expf(__half2float(lg[c]) - mx) * rse;
+            q = (1.0f - a.mix) * pi * a.lightPWithin[l] + a.mix * a.alias[l].z;
+        }
+        a.candLight[static_cast<size_t>(j) * a.npx + p] = l;
+        a.candQ[static_cast<size_t>(j) * a.npx + p] = q;
+    }
+}
+
+}  // namespace rm
+
+// ----------------------------------------------------------------------------- launchers ----
+namespace rk {
+
+hipError_t launch_mlp_gemm(const rm::GemmArgs &g, hipStream_t s) {
+    hipLaunchKernelGGL(rm::k_gemm, dim3(g.N / 64, g.M / 64), dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+}  // namespace rk

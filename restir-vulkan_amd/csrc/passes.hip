@@ -255,32 +255,19 @@ struct PassCArgs {
     uint32_t seed;
 };
 
-template <int TX, int TY, int STACK>
-__global__ __launch_bounds__(TX *TY) void k_pass_c(const PassCArgs a) {
-    constexpr int kBlock = TX * TY;
-    __shared__ int32_t stack[STACK * kBlock];
-    uint32_t x, y;
-    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
+// The neighbour merge of unbiasedReuse.glsl:84-124 for pixel (x, y): k random neighbours in a
+// disk, each merged into `res` with the target evaluated at this pixel. Records each
+// neighbour's pixel index and sample count for the later Z computation.
+__device__ __forceinline__ void pass_c_merge(const PassCArgs &a, uint32_t x, uint32_t y, size_t idx,
+                                             f3 pos, f3 n, Res &res, uint32_t *nIdx, uint32_t *nM) {
     const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
-    if (x >= W || y >= H) return;
-    const size_t idx = static_cast<size_t>(y) * W + x;
     const GView &g = a.g;
-    const f3 pos = plane3(g.pos, idx, g.P);
-    const f3 n = plane3(g.nrm, idx, g.P);
-    Res res = load_res(a.in, idx);
-    if (dot(n, n) == 0.0f) {  // background: pass through (reference reads uninitialised fields)
-        store_res(a.out, idx, res);
-        return;
-    }
     const f3 alb = plane3(g.alb, idx, g.P);
     const float rough = g.mat[idx], metal = g.mat[g.P + idx];
     const f3 cam{a.u.cameraPos[0], a.u.cameraPos[1], a.u.cameraPos[2]};
     const Surface s = make_surface(pos, n, cam, luminance(alb.x, alb.y, alb.z), rough, metal);
     Pcg rng = pcg_seed(static_cast<uint64_t>(a.seed), static_cast<uint64_t>(y * 10007u + x));
     const float R = a.u.spatialRadius;
-    const uint32_t origM = res.M;
-    uint32_t nIdx[16];
-    uint32_t nM[16];
     const int k = a.k;
     for (int i = 0; i < k; ++i) {
         const float angle = pcg_float(rng) * 2.0f * kPi;
@@ -300,36 +287,261 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c(const PassCArgs a) {
         const float weight = ph * rr.w * static_cast<float>(rr.M);
         if (weight > 0.0f) res_update(res, weight, rr.pos, rr.normal, rr.emLum, rr.lightIndex, ph, rr.w, rng);
     }
+}
+
+// W = sumWeights / (Z * pHat), or an empty reservoir when Z = 0 (unbiasedReuse.glsl:172-181).
+__device__ __forceinline__ void finish_w(float &w, float &sumW, float pHat, uint32_t numSamples) {
+    if (numSamples > 0) {
+        w = sumW / (static_cast<float>(numSamples) * pHat);
+    } else {
+        w = 0.0f;
+        sumW = 0.0f;
+    }
+}
+
+template <int TX, int TY, int STACK>
+__global__ __launch_bounds__(TX *TY) void k_pass_c(const PassCArgs a) {
+    constexpr int kBlock = TX * TY;
+    __shared__ int32_t stack[STACK * kBlock];
+    uint32_t x, y;
+    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
+    const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
+    if (x >= W || y >= H) return;
+    const size_t idx = static_cast<size_t>(y) * W + x;
+    const GView &g = a.g;
+    const f3 pos = plane3(g.pos, idx, g.P);
+    const f3 n = plane3(g.nrm, idx, g.P);
+    Res res = load_res(a.in, idx);
+    if (dot(n, n) == 0.0f) {  // background: pass through (reference reads uninitialised fields)
+        store_res(a.out, idx, res);
+        return;
+    }
+    const uint32_t origM = res.M;
+    uint32_t nIdx[16];
+    uint32_t nM[16];
+    const int k = a.k;
+    pass_c_merge(a, x, y, idx, pos, n, res, nIdx, nM);
     const bool vis = (a.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) != 0;
     const f3 lp = res.pos;
     uint32_t numSamples = origM;
     int32_t *stk = stack + threadIdx.x;
-    for (int j = 0; j < k; ++j) {
-        const uint32_t ni = nIdx[j];
-        const f3 nwp = plane3(g.pos, ni, g.P);
-        const f3 nn = plane3(g.nrm, ni, g.P);
-        if (dot(lp - nwp, nn) < 0.0f) continue;
-        if (vis) {
-            f3 o, d;
-            shadow_segment(nwp, lp, o, d);
-            TraceStats st{0, 0};
-            if (occluded<STACK, false>(a.sc.nodes, a.sc.tris, o, d, stk, kBlock, st)) continue;
-        }
-        numSamples += nM[j];
-    }
+    // The centre re-test (unbiasedReuse.glsl:157-166) zeroes Z whatever the neighbours give, and
+    // the neighbour loop draws no random numbers, so testing the centre first and skipping the
+    // neighbour rays when it is shadowed yields the reference's result with fewer rays.
+    bool centreShadowed = false;
     if (vis) {
         f3 o, d;
         shadow_segment(pos, lp, o, d);
         TraceStats st{0, 0};
-        if (occluded<STACK, false>(a.sc.nodes, a.sc.tris, o, d, stk, kBlock, st)) numSamples = 0;
+        centreShadowed = occluded<STACK, false>(a.sc.nodes, a.sc.tris, o, d, stk, kBlock, st);
     }
-    if (numSamples > 0) {
-        res.w = res.sumW / (static_cast<float>(numSamples) * res.pHat);
+    if (centreShadowed) {
+        numSamples = 0;
     } else {
-        res.w = 0.0f;
-        res.sumW = 0.0f;
+        for (int j = 0; j < k; ++j) {
+            const uint32_t ni = nIdx[j];
+            const f3 nwp = plane3(g.pos, ni, g.P);
+            const f3 nn = plane3(g.nrm, ni, g.P);
+            if (dot(lp - nwp, nn) < 0.0f) continue;
+            if (vis) {
+                f3 o, d;
+                shadow_segment(nwp, lp, o, d);
+                TraceStats st{0, 0};
+                if (occluded<STACK, false>(a.sc.nodes, a.sc.tris, o, d, stk, kBlock, st)) continue;
+            }
+            numSamples += nM[j];
+        }
     }
+    finish_w(res.w, res.sumW, res.pHat, numSamples);
     store_res(a.out, idx, res);
+}
+
+// ------------------------------------------------------------------- Pass C, ray queues ---
+// The same pass as k_pass_c, with its shadow rays moved out of the per-pixel lanes into two
+// queues traced by the persistent tracer: (1) merge every pixel and queue its centre ray;
+// (2) trace; (3) queue the neighbour rays of pixels whose centre sees the light (the others have
+// Z = 0 whatever their neighbours give); (4) trace; (5) form Z and W. Rays are appended with one
+// atomic per wave, slots handed out by prefix sums over the wave, so a queue holds only live rays
+// in pixel-tile order.
+struct CQueues {
+    uint32_t *counter;     // [0] centre rays queued, [1] neighbour rays queued
+    float4 *c1, *c2;       // centre ray end points; c1.w carries the pixel index
+    float4 *n1, *n2;       // neighbour ray end points; n1.w carries pixel | slot << 27
+    uint32_t *origM;       // per pixel
+    uint32_t *bits;        // per pixel: above-horizon neighbour mask (bits 0..15), flags below
+    uint32_t *occ;         // per pixel: occluded neighbour mask
+    uint32_t *nM, *nIdx;   // [slot * npx + pixel]
+    uint32_t npx;
+};
+constexpr uint32_t kBitBackground = 1u << 29;  // pass-through pixel, already final
+constexpr uint32_t kBitDone = 1u << 30;        // no visibility test: W already written
+constexpr uint32_t kBitCentreShadowed = 1u << 31;
+
+// Appends `count` items for this lane to a queue: returns the lane's first slot. Every lane of
+// the wave must call it (count may be 0).
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, uint32_t count) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t incl = count;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off);
+        if (lane >= static_cast<uint32_t>(off)) incl += t;
+    }
+    const uint32_t total = __shfl(incl, 63);
+    uint32_t base = 0;
+    if (lane == 0 && total > 0) base = atomicAdd(counter, total);
+    base = __shfl(base, 0);
+    return base + incl - count;
+}
+
+template <int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, const CQueues q) {
+    uint32_t x, y;
+    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
+    const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
+    const bool inside = x < W && y < H;
+    const bool vis = (a.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) != 0;
+    const size_t idx = inside ? static_cast<size_t>(y) * W + x : 0;
+    bool emit = false;
+    f3 pos{0.0f, 0.0f, 0.0f}, lp{0.0f, 0.0f, 0.0f};
+    if (inside) {
+        const GView &g = a.g;
+        pos = plane3(g.pos, idx, g.P);
+        const f3 n = plane3(g.nrm, idx, g.P);
+        Res res = load_res(a.in, idx);
+        if (dot(n, n) == 0.0f) {
+            store_res(a.out, idx, res);
+            q.bits[idx] = kBitBackground;
+        } else {
+            const uint32_t origM = res.M;
+            uint32_t nIdx[16];
+            uint32_t nM[16];
+            pass_c_merge(a, x, y, idx, pos, n, res, nIdx, nM);
+            lp = res.pos;
+            uint32_t above = 0, numSamples = origM;
+            for (int j = 0; j < a.k; ++j) {
+                const f3 nwp = plane3(g.pos, nIdx[j], g.P);
+                const f3 nn = plane3(g.nrm, nIdx[j], g.P);
+                if (dot(lp - nwp, nn) < 0.0f) continue;
+                above |= 1u << j;
+                numSamples += nM[j];
+                q.nM[static_cast<size_t>(j) * q.npx + idx] = nM[j];
+                q.nIdx[static_cast<size_t>(j) * q.npx + idx] = nIdx[j];
+            }
+            if (!vis) finish_w(res.w, res.sumW, res.pHat, numSamples);
+            store_res(a.out, idx, res);
+            q.origM[idx] = origM;
+            q.occ[idx] = 0u;
+            q.bits[idx] = above | (vis ? 0u : kBitDone);
+            emit = vis;
+        }
+    }
+    const uint32_t slot = wave_append(&q.counter[0], emit ? 1u : 0u);
+    if (emit) {
+        q.c1[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(static_cast<uint32_t>(idx)));
+        q.c2[slot] = make_float4(lp.x, lp.y, lp.z, 0.0f);
+    }
+}
+
+template <int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void k_pass_c_emit(const PassCArgs a, const CQueues q) {
+    uint32_t x, y;
+    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
+    const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
+    const bool inside = x < W && y < H;
+    const size_t idx = inside ? static_cast<size_t>(y) * W + x : 0;
+    uint32_t want = 0;
+    if (inside) {
+        const uint32_t b = q.bits[idx];
+        if (!(b & (kBitBackground | kBitDone | kBitCentreShadowed))) want = b & 0xffffu;
+    }
+    const uint32_t slot = wave_append(&q.counter[1], static_cast<uint32_t>(__popc(want)));
+    if (want) {
+        const float4 l = reinterpret_cast<const float4 *>(a.out + idx)[0];
+        uint32_t s = slot;
+        for (uint32_t m = want; m; m &= m - 1) {
+            const uint32_t j = static_cast<uint32_t>(__ffs(static_cast<int>(m)) - 1);
+            const f3 nwp = plane3(a.g.pos, q.nIdx[static_cast<size_t>(j) * q.npx + idx], a.g.P);
+            q.n1[s] = make_float4(nwp.x, nwp.y, nwp.z, __uint_as_float(static_cast<uint32_t>(idx) | (j << 27)));
+            q.n2[s] = make_float4(l.x, l.y, l.z, 0.0f);
+            ++s;
+        }
+    }
+}
+
+template <int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void k_pass_c_finish(const PassCArgs a, const CQueues q) {
+    uint32_t x, y;
+    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
+    const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
+    if (x >= W || y >= H) return;
+    const size_t idx = static_cast<size_t>(y) * W + x;
+    const uint32_t b = q.bits[idx];
+    if (b & (kBitBackground | kBitDone)) return;
+    uint32_t numSamples = 0;
+    if (!(b & kBitCentreShadowed)) {
+        numSamples = q.origM[idx];
+        for (uint32_t m = b & 0xffffu & ~q.occ[idx]; m; m &= m - 1) {
+            const uint32_t j = static_cast<uint32_t>(__ffs(static_cast<int>(m)) - 1);
+            numSamples += q.nM[static_cast<size_t>(j) * q.npx + idx];
+        }
+    }
+    float4 *r = reinterpret_cast<float4 *>(a.out + idx) + 2;  // {lightIndex, pHat, sumW, w}
+    float4 v = *r;
+    finish_w(v.w, v.z, v.y, numSamples);
+    *r = v;
+}
+
+// Persistent tracer over a queue whose length a previous kernel left in *count. Results go to
+// the pixel named by the ray's tag: a centre ray sets kBitCentreShadowed in bits[pixel]; a
+// neighbour ray sets bit `slot` of occ[pixel].
+template <int STACK, bool CENTRE>
+__global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const float4 *__restrict__ p1,
+                                                     const float4 *__restrict__ p2,
+                                                     const uint32_t *__restrict__ count,
+                                                     uint32_t *__restrict__ head, CQueues q) {
+    __shared__ int32_t stack[STACK * 256];
+    int32_t *stk = stack + threadIdx.x;
+    const uint32_t n = *count;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+    Walk w;
+    int32_t ray = -1;
+    uint32_t tag = 0;
+    bool drained = false;
+    for (;;) {
+        const uint64_t idle = __ballot(ray < 0);
+        const uint32_t nIdle = static_cast<uint32_t>(__popcll(idle));
+        if (!drained && nIdle >= 32u) {
+            const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(idle)) - 1);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(head, nIdle);
+            base = __shfl(base, static_cast<int>(leader));
+            drained = base + nIdle >= n;
+            if (ray < 0) {
+                const uint32_t i = base + static_cast<uint32_t>(__popcll(idle & below));
+                if (i < n) {
+                    const float4 a4 = p1[i];
+                    f3 o, d;
+                    shadow_segment(xyz(a4), xyz(p2[i]), o, d);
+                    walk_begin(w, o, d);
+                    tag = __float_as_uint(a4.w);
+                    ray = static_cast<int32_t>(i);
+                }
+            }
+        }
+        if (__ballot(ray >= 0) == 0) {
+            if (drained) break;
+            continue;
+        }
+        if (ray >= 0 && walk_step<STACK, false>(w, sc.nodes, sc.tris, stk, 256)) {
+            if (w.hit) {
+                if (CENTRE) q.bits[tag] |= kBitCentreShadowed;
+                else atomicOr(&q.occ[tag & 0x07ffffffu], 1u << (tag >> 27));
+            }
+            ray = -1;
+        }
+    }
 }
 
 // ------------------------------------------------------------------------------- Shading --
@@ -397,6 +609,56 @@ __global__ __launch_bounds__(256) void k_trace(const TraceArgs a) {
     a.shadowed[i] = occ ? 1u : 0u;
     if (a.visits) a.visits[i] = st.visits;
     if (a.tests) a.tests[i] = st.tests;
+}
+
+// Persistent schedule: a fixed grid of workgroups pulls rays from a global counter. Each lane
+// walks one node per step; when half a wave (or all of it) is idle, the wave claims that many
+// rays with a single atomic and hands them out by prefix rank over the ballot of idle lanes, so
+// lanes whose rays ended early do not wait for the slowest ray of their wave.
+template <int STACK, bool REF>
+__global__ __launch_bounds__(256) void k_trace_pool(const TraceArgs a, uint32_t *__restrict__ head) {
+    __shared__ int32_t stack[STACK * 256];
+    int32_t *stk = stack + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+    Walk w;
+    int32_t ray = -1;
+    bool drained = false;  // wave-uniform: the counter has passed the last ray
+    for (;;) {
+        const uint64_t idle = __ballot(ray < 0);
+        const uint32_t nIdle = static_cast<uint32_t>(__popcll(idle));
+        if (!drained && nIdle >= 32u) {
+            const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(idle)) - 1);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(head, nIdle);
+            base = __shfl(base, static_cast<int>(leader));
+            drained = base + nIdle >= a.n;
+            if (ray < 0) {
+                const uint32_t i = base + static_cast<uint32_t>(__popcll(idle & below));
+                if (i < a.n) {
+                    f3 o, d;
+                    shadow_segment(xyz(a.p1[i]), xyz(a.p2[i]), o, d);
+                    walk_begin(w, o, d);
+                    ray = static_cast<int32_t>(i);
+                }
+            }
+        }
+        if (__ballot(ray >= 0) == 0) {
+            if (drained) break;
+            continue;
+        }
+        if (ray >= 0 && walk_step<STACK, REF>(w, a.sc.nodes, a.sc.tris, stk, 256)) {
+            a.shadowed[ray] = w.hit ? 1u : 0u;
+            if (a.visits) a.visits[ray] = w.nv;
+            if (a.tests) a.tests[ray] = w.nt;
+            ray = -1;
+        }
+    }
+}
+
+__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 // ---------------------------------------------------------------------- primary G-buffer --
@@ -607,11 +869,77 @@ static hipError_t dispatch_c(int tile, const PassCArgs &args, hipStream_t s) {
         default: return run_c<32, 8, STACK>(args, s);
     }
 }
+template <int TX, int TY, int STACK>
+static hipError_t run_c_queued(PassCArgs args, const CQueues &q, uint32_t gridCap, hipStream_t s) {
+    grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
+    const dim3 tiles(args.nTiles), block(TX * TY);
+    const uint32_t rayBlocks = gridCap;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 4u);
+    hipLaunchKernelGGL((k_pass_c_merge<TX, TY>), tiles, block, 0, s, args, q);
+    if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
+        hipLaunchKernelGGL((k_trace_queue<STACK, true>), dim3(rayBlocks), dim3(256), 0, s, args.sc,
+                           q.c1, q.c2, q.counter + 0, q.counter + 2, q);
+        hipLaunchKernelGGL((k_pass_c_emit<TX, TY>), tiles, block, 0, s, args, q);
+        hipLaunchKernelGGL((k_trace_queue<STACK, false>), dim3(rayBlocks), dim3(256), 0, s, args.sc,
+                           q.n1, q.n2, q.counter + 1, q.counter + 3, q);
+        hipLaunchKernelGGL((k_pass_c_finish<TX, TY>), tiles, block, 0, s, args, q);
+    }
+    return hipGetLastError();
+}
+template <int STACK>
+static hipError_t dispatch_c_queued(int tile, const PassCArgs &args, const CQueues &q,
+                                    uint32_t gridCap, hipStream_t s) {
+    switch (tile) {
+        case 0: return run_c_queued<16, 16, STACK>(args, q, gridCap, s);
+        case 1: return run_c_queued<64, 1, STACK>(args, q, gridCap, s);
+        case 2: return run_c_queued<8, 8, STACK>(args, q, gridCap, s);
+        case 3: return run_c_queued<16, 8, STACK>(args, q, gridCap, s);
+        default: return run_c_queued<32, 8, STACK>(args, q, gridCap, s);
+    }
+}
+
+size_t pass_c_queue_bytes(uint32_t npx, int k) {
+    const size_t n = npx, kk = static_cast<size_t>(k);
+    return 256 + 2 * sizeof(float4) * n * (1 + kk) + sizeof(uint32_t) * n * (3 + 2 * kk) + 6 * 256;
+}
+
+// Carves the queues out of one scratch allocation of pass_c_queue_bytes(npx, k) bytes.
+static CQueues carve_queues(void *scratch, uint32_t npx, int k) {
+    auto align = [](size_t v) { return (v + 255) & ~static_cast<size_t>(255); };
+    char *p = static_cast<char *>(scratch);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char *r = p + off;
+        off = align(off + bytes);
+        return r;
+    };
+    const size_t n = npx, kk = static_cast<size_t>(k);
+    CQueues q;
+    q.counter = reinterpret_cast<uint32_t *>(take(256));
+    q.c1 = reinterpret_cast<float4 *>(take(sizeof(float4) * n));
+    q.c2 = reinterpret_cast<float4 *>(take(sizeof(float4) * n));
+    q.n1 = reinterpret_cast<float4 *>(take(sizeof(float4) * n * kk));
+    q.n2 = reinterpret_cast<float4 *>(take(sizeof(float4) * n * kk));
+    q.origM = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * n));
+    q.bits = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * n));
+    q.occ = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * n));
+    q.nM = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * n * kk));
+    q.nIdx = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * n * kk));
+    q.npx = npx;
+    return q;
+}
+
 hipError_t launch_pass_c(int tile, int stack, const restir_uniforms &u, int k, int iter,
                          const SceneView &sc, const GView &g, const restir_reservoir *in,
-                         restir_reservoir *out, hipStream_t s) {
+                         restir_reservoir *out, void *queueScratch, uint32_t gridCap,
+                         hipStream_t s) {
     // unbiasedReuse.glsl:72 seeds with frame*17; iter > 0 offsets the seed of repeated passes
     PassCArgs args{u, sc, g, in, out, 0, 0, k, u.frame * 17u + static_cast<uint32_t>(iter)};
+    if (queueScratch) {
+        const CQueues q = carve_queues(queueScratch, u.screenSize[0] * u.screenSize[1], k);
+        return stack <= 32 ? dispatch_c_queued<32>(tile, args, q, gridCap, s)
+                           : dispatch_c_queued<64>(tile, args, q, gridCap, s);
+    }
     return stack <= 32 ? dispatch_c<32>(tile, args, s) : dispatch_c<64>(tile, args, s);
 }
 
@@ -623,17 +951,30 @@ hipError_t launch_shade(const SceneView &sc, f3 cam, float gamma, const GView &g
     return hipGetLastError();
 }
 
-hipError_t launch_trace(int stack, bool refLoop, const SceneView &sc, const float4 *p1,
-                        const float4 *p2, uint32_t n, uint32_t *shadowed, uint32_t *visits,
-                        uint32_t *tests, hipStream_t s) {
+template <int STACK, bool REF>
+static void run_trace(bool persistent, uint32_t gridCap, uint32_t *counter, const TraceArgs &args,
+                      hipStream_t s) {
+    const uint32_t blocks = (args.n + 255) / 256;
+    if (!persistent) {
+        hipLaunchKernelGGL((k_trace<STACK, REF>), dim3(blocks), dim3(256), 0, s, args);
+        return;
+    }
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, counter, 1u);
+    const uint32_t grid = blocks < gridCap ? blocks : gridCap;
+    hipLaunchKernelGGL((k_trace_pool<STACK, REF>), dim3(grid), dim3(256), 0, s, args, counter);
+}
+
+hipError_t launch_trace(int stack, bool refLoop, bool persistent, uint32_t gridCap,
+                        uint32_t *counter, const SceneView &sc, const float4 *p1, const float4 *p2,
+                        uint32_t n, uint32_t *shadowed, uint32_t *visits, uint32_t *tests,
+                        hipStream_t s) {
     TraceArgs args{sc, p1, p2, n, shadowed, visits, tests};
-    const dim3 grid((n + 255) / 256);
     if (stack <= 32) {
-        if (refLoop) hipLaunchKernelGGL((k_trace<32, true>), grid, dim3(256), 0, s, args);
-        else hipLaunchKernelGGL((k_trace<32, false>), grid, dim3(256), 0, s, args);
+        if (refLoop) run_trace<32, true>(persistent, gridCap, counter, args, s);
+        else run_trace<32, false>(persistent, gridCap, counter, args, s);
     } else {
-        if (refLoop) hipLaunchKernelGGL((k_trace<64, true>), grid, dim3(256), 0, s, args);
-        else hipLaunchKernelGGL((k_trace<64, false>), grid, dim3(256), 0, s, args);
+        if (refLoop) run_trace<64, true>(persistent, gridCap, counter, args, s);
+        else run_trace<64, false>(persistent, gridCap, counter, args, s);
     }
     return hipGetLastError();
 }

@@ -26,12 +26,46 @@ struct alignas(16) Node {
 static_assert(sizeof(Node) == 80, "AabbTreeNode is 80 bytes");
 
 // rayAabIntersection, softwareRaytracing.glsl:9-14 (segment parametrised on [0, 1]).
-__device__ __forceinline__ bool ray_box(f3 o, f3 d, float4 mn, float4 mx) {
+__device__ __forceinline__ bool ray_box_div(f3 o, f3 d, float4 mn, float4 mx) {
     const float ax = (mn.x - o.x) / d.x, ay = (mn.y - o.y) / d.y, az = (mn.z - o.z) / d.z;
     const float bx = (mx.x - o.x) / d.x, by = (mx.y - o.y) / d.y, bz = (mx.z - o.z) / d.z;
     const float rmin = gmax(gmin(ax, bx), gmax(gmin(ay, by), gmin(az, bz)));
     const float rmax = gmin(gmax(ax, bx), gmin(gmax(ay, by), gmax(az, bz)));
     return rmin < 1.0f && rmax >= rmin && rmax > 0.0f;
+}
+
+// A segment with its per-component reciprocal direction (1/d correctly rounded, once per ray).
+struct Seg {
+    f3 o, d, inv;
+};
+__device__ __forceinline__ Seg make_seg(f3 o, f3 d) {
+    return Seg{o, d, f3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z}};
+}
+
+// Same decision as ray_box_div, without its six IEEE divisions in the common case.
+// Each slab distance is formed as RN(RN(b - o) * RN(1/d)); the reference's RN(RN(b - o) / d)
+// differs from it by at most 3 * 2^-24 relative (three roundings of a product of the same exact
+// numerator), and min/max/max3/min3 carry that bound to rmin and rmax. If the three comparisons
+// come out the same at both ends of the widened intervals [v(1-8u), v(1+8u)] (u = 2^-24, plus an
+// absolute 1e-30 for results near zero), the reference's answer is known; otherwise — and for
+// any non-finite or huge distance — the division form is evaluated. The slack is four times the
+// bound, so the fallback is taken only for segments grazing a box face within ~1e-6.
+__device__ __forceinline__ bool ray_box(const Seg &s, float4 mn, float4 mx) {
+    const float ax = (mn.x - s.o.x) * s.inv.x, ay = (mn.y - s.o.y) * s.inv.y, az = (mn.z - s.o.z) * s.inv.z;
+    const float bx = (mx.x - s.o.x) * s.inv.x, by = (mx.y - s.o.y) * s.inv.y, bz = (mx.z - s.o.z) * s.inv.z;
+    const float rmin = gmax(gmin(ax, bx), gmax(gmin(ay, by), gmin(az, bz)));
+    const float rmax = gmin(gmax(ax, bx), gmin(gmax(ay, by), gmax(az, bz)));
+    // a comparison is false for NaN, so this also rejects NaN and infinities
+    const bool finite = fabsf(ax) < 1e30f && fabsf(bx) < 1e30f && fabsf(ay) < 1e30f &&
+                        fabsf(by) < 1e30f && fabsf(az) < 1e30f && fabsf(bz) < 1e30f;
+    constexpr float kRel = 8.0f * 5.9604645e-8f, kAbs = 1e-30f;
+    const float emin = fabsf(rmin) * kRel + kAbs, emax = fabsf(rmax) * kRel + kAbs;
+    const float minLo = rmin - emin, minHi = rmin + emin;
+    const float maxLo = rmax - emax, maxHi = rmax + emax;
+    const bool surelyIn = minHi < 1.0f && maxLo >= minHi && maxLo > 0.0f;
+    const bool surelyOut = minLo >= 1.0f || maxHi < minLo || maxHi <= 0.0f;
+    if (finite && (surelyIn || surelyOut)) return surelyIn;
+    return ray_box_div(s.o, s.d, mn, mx);
 }
 
 // rayTriangleIntersection, softwareRaytracing.glsl:15-37 (Moller-Trumbore, t in (0, 1)).
@@ -76,6 +110,7 @@ template <int kStack, bool kRefLoop>
 __device__ __forceinline__ bool occluded(const Node *__restrict__ nodes,
                                          const float4 *__restrict__ tris, f3 o, f3 d,
                                          int32_t *stk, int stride, TraceStats &st) {
+    const Seg s = make_seg(o, d);
     int top = 0;
     int32_t cur = 0;
     int counter = 0;
@@ -89,8 +124,8 @@ __device__ __forceinline__ bool occluded(const Node *__restrict__ nodes,
         }
         const Node nd = load_node(nodes, cur);
         ++nv;
-        const bool li = ray_box(o, d, nd.lmin, nd.lmax);
-        const bool ri = ray_box(o, d, nd.rmin, nd.rmax);
+        const bool li = ray_box(s, nd.lmin, nd.lmax);
+        const bool ri = ray_box(s, nd.rmin, nd.rmax);
         int32_t next = -1;
         if (li) {
             if (nd.left < 0) {
@@ -133,6 +168,82 @@ __device__ __forceinline__ bool occluded(const Node *__restrict__ nodes,
     st.visits += nv;
     st.tests += nt;
     return hit;
+}
+
+// Step-wise form of `occluded` for schedules that interleave rays on one lane: the state holds one
+// ray's traversal, and each step visits exactly one node (the same visiting order, node count and
+// result as `occluded`). A step returns true when the ray is finished; `hit` then has the answer.
+struct Walk {
+    Seg s;
+    int32_t cur;
+    int top;
+    int counter;
+    bool hit;
+    uint32_t nv, nt;
+};
+
+__device__ __forceinline__ void walk_begin(Walk &w, f3 o, f3 d) {
+    w.s = make_seg(o, d);
+    w.cur = 0;
+    w.top = 0;
+    w.counter = 0;
+    w.hit = false;
+    w.nv = w.nt = 0;
+}
+
+template <int kStack, bool kRefLoop>
+__device__ __forceinline__ bool walk_step(Walk &w, const Node *__restrict__ nodes,
+                                          const float4 *__restrict__ tris, int32_t *stk, int stride) {
+    const Node nd = load_node(nodes, w.cur);
+    ++w.nv;
+    const bool li = ray_box(w.s, nd.lmin, nd.lmax);
+    const bool ri = ray_box(w.s, nd.rmin, nd.rmax);
+    int32_t next = -1;
+    bool stop = false;
+    if (li) {
+        if (nd.left < 0) {
+            if (!w.hit) {
+                ++w.nt;
+                if (ray_tri(tris, ~nd.left, w.s.o, w.s.d)) {
+                    w.hit = true;
+                    stop = !kRefLoop;
+                }
+            }
+        } else {
+            next = nd.left;
+        }
+    }
+    if (ri && !stop) {
+        if (nd.right < 0) {
+            if (!w.hit) {
+                ++w.nt;
+                if (ray_tri(tris, ~nd.right, w.s.o, w.s.d)) {
+                    w.hit = true;
+                    stop = !kRefLoop;
+                }
+            }
+        } else {
+            if (next >= 0 && w.top < kStack) {
+                stk[w.top * stride] = next;
+                ++w.top;
+            }
+            next = nd.right;
+        }
+    }
+    if (stop) return true;
+    if (kRefLoop) {
+        if (++w.counter == 8) {
+            if (w.hit) return true;
+            w.counter = 0;
+        }
+    }
+    if (next < 0) {
+        if (w.top == 0) return true;
+        --w.top;
+        next = stk[w.top * stride];
+    }
+    w.cur = next;
+    return false;
 }
 
 // testVisibility, visibilityTest.glsl:1-30: shrink the segment p1 -> p2 by tMin = 0.001 at both

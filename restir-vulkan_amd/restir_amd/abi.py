@@ -20,7 +20,8 @@ VISIBILITY_REUSE_FLAG = 1 << 0
 TEMPORAL_REUSE_FLAG = 1 << 1
 TRACE_FIRST_HIT = 0
 TRACE_REFERENCE_LOOP = 1
-ABI_VERSION = 2  # RESTIR_ABI_VERSION of include/restir_abi.h
+TRACE_ONE_RAY_PER_LANE = 2
+ABI_VERSION = 3  # RESTIR_ABI_VERSION of include/restir_abi.h
 
 ERRORS = {
     -1: "RESTIR_E_INVALID_ARGUMENT",
@@ -82,8 +83,13 @@ class GBuffer(C.Structure):
     ]
 
 
+SCHEDULE_QUEUED = 0
+SCHEDULE_INLINE = 1
+
+
 class Launch(C.Structure):
-    _fields_ = [("tile_x", C.c_uint32), ("tile_y", C.c_uint32)]
+    _fields_ = [("tile_x", C.c_uint32), ("tile_y", C.c_uint32), ("schedule", C.c_uint32),
+                ("_reserved", C.c_uint32)]
 
 
 class Camera(C.Structure):
@@ -365,11 +371,12 @@ class Context:
                     "restir_shade")
 
     def trace_shadow(self, p1, p2, n, shadowed, visits=None, tests=None, reference_loop=False,
-                     stream=None):
+                     one_ray_per_lane=False, stream=None):
+        loop = (TRACE_REFERENCE_LOOP if reference_loop else TRACE_FIRST_HIT) | \
+               (TRACE_ONE_RAY_PER_LANE if one_ray_per_lane else 0)
         self._check(self.lib.restir_trace_shadow(self.h, as_ptr(p1), as_ptr(p2), n,
                                                  as_ptr(shadowed), as_ptr(visits), as_ptr(tests),
-                                                 TRACE_REFERENCE_LOOP if reference_loop else TRACE_FIRST_HIT,
-                                                 stream_ptr(stream)), "restir_trace_shadow")
+                                                 loop, stream_ptr(stream)), "restir_trace_shadow")
 
     def primary_gbuffer(self, cam: dict, tri_material, materials, n_materials, gbuf_base, width,
                         height, plane_stride=None, stream=None):

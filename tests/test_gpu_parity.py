@@ -113,10 +113,21 @@ def test_biased_pipeline_cornell_bit_exact(torch, tile):
     assert (a2["M"] > 32).mean() > 0.5
 
 
-@pytest.mark.parametrize("k", [3, 5])
-def test_unbiased_pipeline_cornell_bit_exact(torch, k):
+@pytest.mark.parametrize("k,schedule", [(3, abi.SCHEDULE_QUEUED), (5, abi.SCHEDULE_QUEUED),
+                                        (3, abi.SCHEDULE_INLINE), (5, abi.SCHEDULE_INLINE)])
+def test_unbiased_pipeline_cornell_bit_exact(torch, k, schedule):
     sc = scenes.cornell_box(1)
-    r = Renderer(sc, 256, 256, mode="unbiased", k=k, spatial_passes=1)
+    r = Renderer(sc, 256, 256, mode="unbiased", k=k, spatial_passes=1,
+                 launch=abi.Launch(16, 16, schedule))
+    run_frame_and_compare(r, sc.camera)
+    run_frame_and_compare(r, sc.camera)
+
+
+@pytest.mark.parametrize("schedule", [abi.SCHEDULE_QUEUED, abi.SCHEDULE_INLINE])
+def test_unbiased_without_visibility_bit_exact(torch, schedule):
+    sc = scenes.cornell_box(1)
+    r = Renderer(sc, 200, 150, mode="unbiased", k=4, spatial_passes=2, visibility=False,
+                 launch=abi.Launch(8, 8, schedule))
     run_frame_and_compare(r, sc.camera)
     run_frame_and_compare(r, sc.camera)
 

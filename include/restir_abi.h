@@ -255,6 +255,79 @@ int restir_build_light_clusters(const restir_point_light *points, uint32_t n, ui
                                 restir_alias_table_column *cluster_alias, float *centroids);
 
 /* ----------------------------------------------------------------------------------------------
+ * Learned light proposal (no reference counterpart). A 128-128-128-128-K ReLU MLP over per-pixel
+ * features (position in the scene box, normal, view direction; raw plus 6 sin/cos octaves = 114,
+ * zero-padded to 128) gives a softmax pi_x over K light clusters. Candidates are drawn from
+ *   q(l | x) = (1 - mix) * pi_x(cluster(l)) * p_within(l) + mix * p_src(l)
+ * (p_src: the reference's alias-table distribution), so RIS with weights p_hat/q stays unbiased;
+ * mix = 0 is rejected. Weights are bf16 with an fp32 master copy; all products run on MFMA with
+ * fp32 accumulation. K is 64, 128 or 256. All pointers in restir_mlp are device pointers.
+ * -------------------------------------------------------------------------------------------- */
+typedef struct restir_mlp {
+    uint32_t n_clusters;    /* K */
+    uint32_t adam_step;     /* Adam steps taken; restir_mlp_train increments it */
+    uint16_t *weights_bf16; /* W1, W2, W3 (128x128) then W4 (Kx128), each [out][in] */
+    float *weights;         /* fp32 master copy, same layout */
+    float *bias;            /* b1, b2, b3 (128 each) then b4 (K) */
+    float *adam_m;          /* first moments: weights then biases */
+    float *adam_v;          /* second moments: weights then biases */
+} restir_mlp;
+
+#define RESTIR_MLP_FEATURES 128
+/* element counts of the parameter arrays for K clusters */
+#define RESTIR_MLP_N_WEIGHTS(K) (3u * 128u * 128u + (uint32_t)(K) * 128u)
+#define RESTIR_MLP_N_BIAS(K) (3u * 128u + (uint32_t)(K))
+
+/* Seeded initialisation: He-uniform weights, zero biases except the output bias = out_prior
+ * (device array of K floats, e.g. log cluster power shares; -30000 masks an empty cluster),
+ * zeroed Adam moments, adam_step = 0. */
+int restir_mlp_init(restir_ctx *ctx, restir_mlp *mlp, uint64_t seed, const float *out_prior,
+                    void *stream);
+
+/* Binds the cluster tables of restir_build_light_clusters (device copies) for the proposal. */
+int restir_set_light_clusters(restir_ctx *ctx, uint32_t k, const int32_t *light_cluster,
+                              const float *light_pwithin, const int32_t *cluster_begin,
+                              const int32_t *cluster_count, const int32_t *cluster_lights,
+                              const restir_alias_table_column *cluster_alias);
+
+/* Draws M <= 16 candidates per pixel from q: cand_light[j * W*H + p], cand_q[j * W*H + p]
+ * (light index, or -1 with q = 0 for an empty cluster). Background pixels are left untouched.
+ * scene_lo/hi bound the positions fed to the features. Seeded by (u->frame, pixel). */
+int restir_learned_candidates(restir_ctx *ctx, const restir_uniforms *u, const restir_gbuffer *g,
+                              const restir_mlp *mlp, const float scene_lo[3],
+                              const float scene_hi[3], uint32_t M, float mix, int32_t *cand_light,
+                              float *cand_q, void *stream);
+
+/* Pass A with RIS over given candidates instead of the alias table (u->initialLightSampleCount
+ * of them per pixel); visibility and temporal reuse as restir_initial_temporal. */
+int restir_initial_temporal_candidates(restir_ctx *ctx, const restir_uniforms *u,
+                                       const restir_gbuffer *cur, const restir_gbuffer *prev,
+                                       const restir_reservoir *prev_res, restir_reservoir *out,
+                                       const int32_t *cand_light, const float *cand_q,
+                                       const restir_launch *launch, void *stream);
+
+/* One online training step: n_samples (a multiple of 64, <= 65536) pseudo-random pixels of the
+ * frame whose final reservoir holds a light with W > 0 become (features, cluster of that light)
+ * pairs; mean cross-entropy of the softmax, backward pass, Adam (beta 0.9/0.999, eps 1e-8) at
+ * learning rate lr. *loss_out (device float, may be NULL) receives the mean loss. */
+int restir_mlp_train(restir_ctx *ctx, restir_mlp *mlp, const restir_uniforms *u,
+                     const restir_gbuffer *g, const restir_reservoir *res,
+                     const float scene_lo[3], const float scene_hi[3], uint32_t n_samples,
+                     float lr, uint32_t seed, float *loss_out, void *stream);
+
+/* Building blocks exposed for checks: features of listed pixels (bf16, n x 128); logits of the
+ * MLP (fp32, n x K) for bf16 features (n a multiple of 64); loss and gradients (fp32, parameter
+ * layouts) for features, targets and a 0/1 mask. */
+int restir_mlp_features(restir_ctx *ctx, const restir_gbuffer *g, const float camera_pos[3],
+                        const float scene_lo[3], const float scene_hi[3], const uint32_t *pixels,
+                        uint32_t n, uint16_t *features, void *stream);
+int restir_mlp_forward(restir_ctx *ctx, const restir_mlp *mlp, const uint16_t *features,
+                       uint32_t n, float *logits, void *stream);
+int restir_mlp_loss_grad(restir_ctx *ctx, const restir_mlp *mlp, const uint16_t *features,
+                         const int32_t *target, const uint32_t *mask, uint32_t n, float *grad_w,
+                         float *grad_b, float *loss, void *stream);
+
+/* ----------------------------------------------------------------------------------------------
  * Scene-side helpers for synthetic inputs (no reference counterpart; the reference rasterises).
  * -------------------------------------------------------------------------------------------- */
 

@@ -54,6 +54,11 @@ SIGS = {
     "oracle_projection_view": (None, [_vp, _f32, _vp]),
     "oracle_ground_truth_f64": (None, [C.POINTER(OracleScene), _vp, _vp, _i32, _u32, _vp, _vp,
                                        _u64]),
+    "oracle_mlp_features": (None, [_vp, _vp, _vp, _vp, _vp, _u32, _vp]),
+    "oracle_mlp_forward": (None, [_vp, _vp, _u32, _vp, _u32, _i32, _vp]),
+    "oracle_mlp_loss_grad": (C.c_double, [_vp, _vp, _u32, _vp, _vp, _vp, _u32, _i32, _vp, _vp]),
+    "oracle_adam": (None, [_vp, _vp, _vp, _vp, C.c_size_t, C.c_double, C.c_double, C.c_double,
+                           C.c_double, _u32]),
 }
 
 _LIB = None
@@ -250,3 +255,46 @@ def ground_truth_f64(scene: Scene, cam_pos, g, use_bvh=False, n_tri_samples=0, m
     lib().oracle_ground_truth_f64(scene.ref(), cp.ctypes.data, C.addressof(g), int(use_bvh),
                                   n_tri_samples, _p(m), out.ctypes.data, W * H)
     return out.reshape(3, H, W)
+
+
+# ------------------------------------------------------------------------- learned proposal
+def mlp_features(g, cam_pos, lo, hi, pixels):
+    pixels = np.ascontiguousarray(pixels, dtype=np.uint32)
+    out = np.zeros((len(pixels), 128), dtype=np.float32)
+    cp = np.asarray(cam_pos[:3], dtype=np.float32)
+    lo = np.asarray(lo, dtype=np.float32)
+    hi = np.asarray(hi, dtype=np.float32)
+    lib().oracle_mlp_features(C.addressof(g), cp.ctypes.data, lo.ctypes.data, hi.ctypes.data,
+                              pixels.ctypes.data, len(pixels), out.ctypes.data)
+    return out
+
+
+def mlp_forward(W, B, K, X, emulate_bf16=True):
+    W = np.ascontiguousarray(W, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    out = np.zeros((len(X), K), dtype=np.float64)
+    lib().oracle_mlp_forward(W.ctypes.data, B.ctypes.data, K, X.ctypes.data, len(X),
+                             int(emulate_bf16), out.ctypes.data)
+    return out
+
+
+def mlp_loss_grad(W, B, K, X, target, mask, emulate_bf16=True):
+    W = np.ascontiguousarray(W, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    target = np.ascontiguousarray(target, dtype=np.int32)
+    mask = np.ascontiguousarray(mask, dtype=np.uint8)
+    gW = np.zeros(W.shape, dtype=np.float64)
+    gB = np.zeros(B.shape, dtype=np.float64)
+    loss = lib().oracle_mlp_loss_grad(W.ctypes.data, B.ctypes.data, K, X.ctypes.data,
+                                      target.ctypes.data, mask.ctypes.data, len(X),
+                                      int(emulate_bf16), gW.ctypes.data, gB.ctypes.data)
+    return loss, gW, gB
+
+
+def adam(p, g, m, v, lr, b1, b2, eps, t):
+    for a in (p, g, m, v):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    lib().oracle_adam(p.ctypes.data, g.ctypes.data, m.ctypes.data, v.ctypes.data, p.size, lr, b1,
+                      b2, eps, t)

@@ -9,27 +9,10 @@
 #include <string>
 #include <vector>
 
+#include "ctx.h"
 #include "kernels.h"
 
-struct restir_ctx {
-    int device = 0;
-    std::string error;
-    rk::SceneView scene{};
-    bool lightsSet = false;
-    bool bvhSet = false;
-    int stack = 32;
-    uint32_t treeDepth = 0;
-    uint32_t *counters = nullptr;  // device work counters of the persistent kernels
-    uint32_t persistentBlocks = 1280;
-    void *scratch = nullptr;       // ray queues of the queued Pass C schedule
-    size_t scratchBytes = 0;
-    ~restir_ctx() {
-        if (counters) (void)hipFree(counters);
-        if (scratch) (void)hipFree(scratch);
-    }
-};
-
-namespace {
+namespace rki {
 
 int fail(restir_ctx *ctx, int code, const char *fmt, ...) {
     if (ctx) {
@@ -96,22 +79,24 @@ int check_frame(restir_ctx *ctx, const restir_uniforms *u, const char *name) {
 }
 
 // Grows the context's scratch to at least `bytes` (hipFree waits for work still using it).
-int ensure_scratch(restir_ctx *ctx, size_t bytes) {
-    if (bytes <= ctx->scratchBytes) return RESTIR_OK;
-    if (ctx->scratch) {
-        const hipError_t e = hipFree(ctx->scratch);
-        ctx->scratch = nullptr;
-        ctx->scratchBytes = 0;
+int ensure(restir_ctx *ctx, void **buf, size_t *have, size_t bytes) {
+    if (bytes <= *have) return RESTIR_OK;
+    if (*buf) {
+        const hipError_t e = hipFree(*buf);
+        *buf = nullptr;
+        *have = 0;
         if (e != hipSuccess) return hip_fail(ctx, e, "scratch: hipFree");
     }
-    const hipError_t e = hipMalloc(&ctx->scratch, bytes);
+    const hipError_t e = hipMalloc(buf, bytes);
     if (e != hipSuccess) {
-        ctx->scratch = nullptr;
+        *buf = nullptr;
         return hip_fail(ctx, e, "scratch: hipMalloc");
     }
-    ctx->scratchBytes = bytes;
+    *have = bytes;
     return RESTIR_OK;
 }
+
+int ensure_scratch(restir_ctx *ctx, size_t bytes) { return ensure(ctx, &ctx->scratch, &ctx->scratchBytes, bytes); }
 
 int pick_tile(restir_ctx *ctx, const restir_launch *l, int *tile) {
     if (!l || (l->tile_x == 0 && l->tile_y == 0)) {
@@ -124,7 +109,9 @@ int pick_tile(restir_ctx *ctx, const restir_launch *l, int *tile) {
     return RESTIR_OK;
 }
 
-}  // namespace
+}  // namespace rki
+
+using namespace rki;
 
 extern "C" {
 
@@ -275,7 +262,7 @@ int restir_initial_temporal(restir_ctx *ctx, const restir_uniforms *u, const res
     if (int r = pick_tile(ctx, launch, &tile)) return r;
     if (int r = use_device(ctx)) return r;
     const hipError_t e = rk::launch_pass_a(tile, ctx->stack, *u, ctx->scene, view_of(cur), view_of(prev),
-                                           prevRes, out, static_cast<hipStream_t>(stream));
+                                           prevRes, out, nullptr, nullptr, static_cast<hipStream_t>(stream));
     return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "initial_temporal launch");
 }
 

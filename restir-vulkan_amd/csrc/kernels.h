@@ -34,9 +34,11 @@ constexpr int kNumTiles = 5;
 extern const Tile kTiles[kNumTiles];
 int tile_index(uint32_t tx, uint32_t ty);  // -1 if not compiled
 
+// candLight/candQ: the learned proposal's candidates ([M][npx]), or null for alias-table RIS.
 hipError_t launch_pass_a(int tile, int stack, const restir_uniforms &u, const SceneView &sc,
                          const GView &cur, const GView &prev, const restir_reservoir *prevRes,
-                         restir_reservoir *out, hipStream_t s);
+                         restir_reservoir *out, const int32_t *candLight, const float *candQ,
+                         hipStream_t s);
 hipError_t launch_pass_b(int tile, const restir_uniforms &u, int iter, float cosThr,
                          const GView &g, const restir_reservoir *in, restir_reservoir *out,
                          hipStream_t s);
@@ -55,6 +57,30 @@ hipError_t launch_trace(int stack, bool refLoop, bool persistent, uint32_t gridC
                         uint32_t *counter, const SceneView &sc, const float4 *p1, const float4 *p2,
                         uint32_t n, uint32_t *shadowed, uint32_t *visits, uint32_t *tests,
                         hipStream_t s);
+
+// clears n words (a kernel rather than hipMemsetAsync, so the sequence can be graph-captured)
+__global__ void k_zero_u32(uint32_t *p, uint32_t n);
+
+// ---- learned proposal (mlp.hip)
+size_t mlp_scratch_bytes(uint32_t n, int K);
+hipError_t mlp_init(const restir_mlp &m, uint64_t seed, const float *prior, hipStream_t s);
+hipError_t mlp_features(const GView &g, rt::f3 cam, const float lo[3], const float hi[3],
+                        const uint32_t *pixels, uint32_t n, uint16_t *out, hipStream_t s);
+hipError_t mlp_forward(const restir_mlp &m, const uint16_t *X, uint32_t n, float *logits,
+                       void *scratch, hipStream_t s);
+hipError_t mlp_loss_grad(const restir_mlp &m, const uint16_t *X, const int32_t *target,
+                         const uint32_t *mask, uint32_t n, float *gw, float *gb, float *loss,
+                         void *scratch, hipStream_t s);
+hipError_t mlp_train(restir_mlp &m, const GView &g, rt::f3 cam, const float lo[3], const float hi[3],
+                     const restir_reservoir *res, const int32_t *lightCluster, int32_t nLights,
+                     uint32_t n, float lr,
+                     uint32_t seed, float *lossOut, void *scratch, hipStream_t s);
+hipError_t mlp_propose(const restir_mlp &m, const GView &g, rt::f3 cam, const float lo[3],
+                       const float hi[3], const SceneView &sc, const int32_t *lightCluster,
+                       const float *lightPWithin, const int32_t *clusterBegin,
+                       const int32_t *clusterCount, const int32_t *clusterLights,
+                       const restir_alias_table_column *clusterAlias, uint32_t M, float mix,
+                       uint32_t seed, int32_t *candLight, float *candQ, hipStream_t s);
 
 struct CamBasis {
     rt::f3 pos, fwd, right, down;

@@ -76,11 +76,12 @@ __global__ __launch_bounds__(256) void k_features(const FeatArgs f, const uint32
                                                   uint16_t *out) {
     const uint32_t b = blockIdx.x * 256u + threadIdx.x;
     if (b >= n) return;
-    uint16_t tmp[kIn];
-    pixel_features(f, pixels[b], tmp);
-    uint4 *d = reinterpret_cast<uint4 *>(out + static_cast<size_t>(b) * kIn);
-    const uint4 *t = reinterpret_cast<const uint4 *>(tmp);
-    for (int i = 0; i < kIn / 8; ++i) d[i] = t[i];
+    uint16_t *dst = out + static_cast<size_t>(b) * kIn;
+    if (pixels[b] < f.g.W * f.g.H) {
+        pixel_features(f, pixels[b], dst);
+    } else {
+        for (int k = 0; k < kIn; ++k) dst[k] = 0;
+    }
 }
 
 // -------------------------------------------------------------------------------- GEMM ------
@@ -149,7 +150,7 @@ __global__ __launch_bounds__(256) void k_softmax_ce(const float *Z, int K, const
     if (b >= n) return;
     const float *z = Z + static_cast<size_t>(b) * K;
     uint16_t *d = dZ + static_cast<size_t>(b) * K;
-    if (!mask[b]) {
+    if (!mask[b] || target[b] < 0 || target[b] >= K) {  // an out-of-range target does not train
         for (int c = 0; c < K; ++c) d[c] = 0;
         return;
     }
@@ -163,6 +164,15 @@ __global__ __launch_bounds__(256) void k_softmax_ce(const float *Z, int K, const
     const float rse = 1.0f / se;
     for (int c = 0; c < K; ++c) d[c] = f2bf((expf(z[c] - mx) * rse - (c == t ? 1.0f : 0.0f)) * inv);
 }
+
+__global__ __launch_bounds__(256) void k_count_mask(const uint32_t *mask, uint32_t n, uint32_t *count) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const bool on = i < n && mask[i] != 0u;
+    const uint64_t b = __ballot(on);
+    if ((threadIdx.x & 63u) == 0u && b) atomicAdd(count, static_cast<uint32_t>(__popcll(b)));
+}
+
+__global__ void k_copy_f32(const float *src, float *dst) { *dst = *src; }
 
 // out[c] = sum over rows of G[row][c] (bf16 input, fp32 sum)
 __global__ __launch_bounds__(256) void k_colsum(const uint16_t *G, int rows, int cols, float *out) {
@@ -184,9 +194,10 @@ struct AdamArgs {
     float lr, b1, b2, eps, c1, c2;  // c1 = 1 - b1^t, c2 = 1 - b2^t
 };
 
-__global__ __launch_bounds__(256) void k_adam(const AdamArgs a) {
+// Adam step; skipped when the batch held no training sample (*nValid == 0).
+__global__ __launch_bounds__(256) void k_adam_guarded(const AdamArgs a, const uint32_t *nValid) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= a.nW + a.nB) return;
+    if (i >= a.nW + a.nB || *nValid == 0u) return;
     const bool isW = i < a.nW;
     const float g = isW ? a.gw[i] : a.gb[i - a.nW];
     const float m = a.b1 * a.m[i] + (1.0f - a.b1) * g;
@@ -230,6 +241,7 @@ struct GatherArgs {
     FeatArgs f;
     const restir_reservoir *res;
     const int32_t *lightCluster;
+    int32_t nLights;
     uint32_t npx, n, seed;
     uint16_t *X;
     int32_t *target;
@@ -245,14 +257,10 @@ __global__ __launch_bounds__(256) void k_train_gather(const GatherArgs a) {
     rt::Pcg r = rt::pcg_seed(a.seed, b);
     const uint32_t p = rt::pcg_next(r) % a.npx;
     a.pixels[b] = p;
-    uint16_t tmp[kIn];
-    const bool surface = pixel_features(a.f, p, tmp);
-    uint4 *d = reinterpret_cast<uint4 *>(a.X + static_cast<size_t>(b) * kIn);
-    const uint4 *t = reinterpret_cast<const uint4 *>(tmp);
-    for (int i = 0; i < kIn / 8; ++i) d[i] = t[i];
+    const bool surface = pixel_features(a.f, p, a.X + static_cast<size_t>(b) * kIn);
     const float4 q2 = reinterpret_cast<const float4 *>(a.res + p)[2];  // lightIndex, pHat, sumW, w
     const int32_t li = __float_as_int(q2.x);
-    const bool ok = surface && li >= 0 && q2.w > 0.0f && q2.z > 0.0f;
+    const bool ok = surface && li >= 0 && li < a.nLights && q2.w > 0.0f && q2.z > 0.0f;
     a.mask[b] = ok ? 1u : 0u;
     a.target[b] = ok ? a.lightCluster[li] : 0;
     if (ok) atomicAdd(a.nValid, 1u);
@@ -279,7 +287,6 @@ struct ProposeArgs {
 
 constexpr int kTilePx = 256;                 // pixels per workgroup
 constexpr int kRow = kIn + 8;                // padded LDS row (bf16) against bank conflicts
-constexpr int kMaxK = 256;
 
 __device__ __forceinline__ void stage_weights(uint16_t *wbuf, const uint16_t *src, int rows) {
     // rows x 128 bf16 from global into LDS rows of kRow, 16 bytes per lane per step
@@ -442,8 +449,245 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
 // ----------------------------------------------------------------------------- launchers ----
 namespace rk {
 
-hipError_t launch_mlp_gemm(const rm::GemmArgs &g, hipStream_t s) {
+using rm::GemmArgs;
+using rm::kHidden;
+using rm::kIn;
+using rt::f3;
+
+static void gemm(const GemmArgs &g, hipStream_t s) {
     hipLaunchKernelGGL(rm::k_gemm, dim3(g.N / 64, g.M / 64), dim3(256), 0, s, g);
+}
+static GemmArgs gemm_args(int M, int N, int K) {
+    GemmArgs g{};
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    return g;
+}
+
+// Training tensors carved from one scratch allocation.
+struct MlpScratch {
+    uint16_t *X, *H[3], *dA, *dB, *dZ;
+    float *Z, *gw, *gb, *loss;
+    int32_t *target;
+    uint32_t *mask, *pixels, *nValid;
+};
+size_t mlp_scratch_bytes(uint32_t n, int K) {
+    const size_t nn = n, kk = static_cast<size_t>(K);
+    return 16 * 256 + 2 * (nn * kIn * 6 + nn * kk) + 4 * (nn * kk + 3 * nn) +
+           4 * (rm::n_weights(K) + rm::n_bias(K));
+}
+static MlpScratch carve_mlp(void *base, uint32_t n, int K) {
+    char *p = static_cast<char *>(base);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char *r = p + off;
+        off = (off + bytes + 255) & ~static_cast<size_t>(255);
+        return r;
+    };
+    const size_t nn = n, kk = static_cast<size_t>(K);
+    MlpScratch m;
+    m.X = reinterpret_cast<uint16_t *>(take(2 * nn * kIn));
+    for (int l = 0; l < 3; ++l) m.H[l] = reinterpret_cast<uint16_t *>(take(2 * nn * kHidden));
+    m.dA = reinterpret_cast<uint16_t *>(take(2 * nn * kHidden));
+    m.dB = reinterpret_cast<uint16_t *>(take(2 * nn * kHidden));
+    m.dZ = reinterpret_cast<uint16_t *>(take(2 * nn * kk));
+    m.Z = reinterpret_cast<float *>(take(4 * nn * kk));
+    m.target = reinterpret_cast<int32_t *>(take(4 * nn));
+    m.mask = reinterpret_cast<uint32_t *>(take(4 * nn));
+    m.pixels = reinterpret_cast<uint32_t *>(take(4 * nn));
+    m.nValid = reinterpret_cast<uint32_t *>(take(256));
+    m.loss = reinterpret_cast<float *>(take(256));
+    m.gw = reinterpret_cast<float *>(take(4 * rm::n_weights(K)));
+    m.gb = reinterpret_cast<float *>(take(4 * rm::n_bias(K)));
+    return m;
+}
+
+// forward through the GEMM path, keeping the hidden activations for the backward pass
+static void mlp_fwd(const restir_mlp &m, const uint16_t *X, uint32_t n, const MlpScratch &t, float *logits,
+                    hipStream_t s) {
+    const int K = static_cast<int>(m.n_clusters);
+    const uint16_t *in = X;
+    for (int l = 0; l < 4; ++l) {
+        const int N = l < 3 ? kHidden : K;
+        GemmArgs g = gemm_args(static_cast<int>(n), N, kIn);
+        g.A = in;
+        g.sam = kIn;
+        g.sak = 1;
+        g.B = m.weights_bf16 + rm::w_off(l);  // B(k, o) = W[o][k]
+        g.sbk = 1;
+        g.sbn = kIn;
+        g.bias = m.bias + rm::b_off(l);
+        g.scm = N;
+        g.scn = 1;
+        if (l < 3) {
+            g.relu = 1;
+            g.Cb = t.H[l];
+            in = t.H[l];
+        } else {
+            g.Cf = logits;
+        }
+        gemm(g, s);
+    }
+}
+
+hipError_t mlp_forward(const restir_mlp &m, const uint16_t *X, uint32_t n, float *logits, void *scratch,
+                       hipStream_t s) {
+    const MlpScratch t = carve_mlp(scratch, n, static_cast<int>(m.n_clusters));
+    mlp_fwd(m, X, n, t, logits, s);
+    return hipGetLastError();
+}
+
+// mean cross-entropy and its gradient; gw/gb/loss are written (not accumulated)
+static void mlp_loss_grad_impl(const restir_mlp &m, const uint16_t *X, const int32_t *target,
+                               const uint32_t *mask, uint32_t n, const MlpScratch &t, float *gw, float *gb,
+                               float *loss, bool countMask, hipStream_t s) {
+    const int K = static_cast<int>(m.n_clusters);
+    const int nn = static_cast<int>(n);
+    if (countMask) {
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, t.nValid, 1u);
+        hipLaunchKernelGGL(rm::k_count_mask, dim3((n + 255) / 256), dim3(256), 0, s, mask, n, t.nValid);
+    }
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t *>(loss), 1u);
+    mlp_fwd(m, X, n, t, t.Z, s);
+    hipLaunchKernelGGL(rm::k_softmax_ce, dim3((n + 255) / 256), dim3(256), 0, s, t.Z, K, target, mask, nn,
+                       t.nValid, t.dZ, loss);
+    // layer 4: gW4[o][k] = sum_b dZ[b][o] H3[b][k]
+    const uint16_t *dY = t.dZ;
+    int N = K;
+    for (int l = 3; l >= 0; --l) {
+        const uint16_t *in = l > 0 ? t.H[l - 1] : X;
+        GemmArgs gwA = gemm_args(N, kIn, nn);
+        gwA.A = dY;      // A(o, b) = dY[b][o]
+        gwA.sam = 1;
+        gwA.sak = N;
+        gwA.B = in;      // B(b, k) = in[b][k]
+        gwA.sbk = kIn;
+        gwA.sbn = 1;
+        gwA.Cf = gw + rm::w_off(l);
+        gwA.scm = kIn;
+        gwA.scn = 1;
+        gemm(gwA, s);
+        hipLaunchKernelGGL(rm::k_colsum, dim3((N + 255) / 256), dim3(256), 0, s, dY, nn, N, gb + rm::b_off(l));
+        if (l == 0) break;
+        // dIn[b][k] = sum_o dY[b][o] W[o][k], gated by the ReLU of the layer below
+        uint16_t *dIn = (dY == t.dA) ? t.dB : t.dA;
+        GemmArgs gd = gemm_args(nn, kIn, N);
+        gd.A = dY;
+        gd.sam = N;
+        gd.sak = 1;
+        gd.B = m.weights_bf16 + rm::w_off(l);  // B(o, k) = W[o][k]
+        gd.sbk = kIn;
+        gd.sbn = 1;
+        gd.gate = t.H[l - 1];
+        gd.sgm = kIn;
+        gd.sgn = 1;
+        gd.Cb = dIn;
+        gd.scm = kIn;
+        gd.scn = 1;
+        gemm(gd, s);
+        dY = dIn;
+        N = kHidden;
+    }
+}
+
+hipError_t mlp_loss_grad(const restir_mlp &m, const uint16_t *X, const int32_t *target, const uint32_t *mask,
+                         uint32_t n, float *gw, float *gb, float *loss, void *scratch, hipStream_t s) {
+    const MlpScratch t = carve_mlp(scratch, n, static_cast<int>(m.n_clusters));
+    mlp_loss_grad_impl(m, X, target, mask, n, t, gw, gb, loss, true, s);
+    return hipGetLastError();
+}
+
+hipError_t mlp_init(const restir_mlp &m, uint64_t seed, const float *prior, hipStream_t s) {
+    const uint32_t K = m.n_clusters;
+    const uint32_t nW = static_cast<uint32_t>(rm::n_weights(K)), nB = static_cast<uint32_t>(rm::n_bias(K));
+    hipLaunchKernelGGL(rm::k_mlp_init, dim3((nW + nB + 255) / 256), dim3(256), 0, s, m.weights, m.weights_bf16,
+                       m.bias, m.adam_m, m.adam_v, nW, nB, K, prior, seed);
+    return hipGetLastError();
+}
+
+hipError_t mlp_features(const GView &g, f3 cam, const float lo[3], const float hi[3], const uint32_t *pixels,
+                        uint32_t n, uint16_t *out, hipStream_t s) {
+    rm::FeatArgs f{g, cam, {lo[0], lo[1], lo[2]}, {hi[0], hi[1], hi[2]}};
+    hipLaunchKernelGGL(rm::k_features, dim3((n + 255) / 256), dim3(256), 0, s, f, pixels, n, out);
+    return hipGetLastError();
+}
+
+hipError_t mlp_train(restir_mlp &m, const GView &g, f3 cam, const float lo[3], const float hi[3],
+                     const restir_reservoir *res, const int32_t *lightCluster, int32_t nLights,
+                     uint32_t n, float lr,
+                     uint32_t seed, float *lossOut, void *scratch, hipStream_t s) {
+    const int K = static_cast<int>(m.n_clusters);
+    const MlpScratch t = carve_mlp(scratch, n, K);
+    rm::GatherArgs ga{};
+    ga.f = rm::FeatArgs{g, cam, {lo[0], lo[1], lo[2]}, {hi[0], hi[1], hi[2]}};
+    ga.res = res;
+    ga.lightCluster = lightCluster;
+    ga.nLights = nLights;
+    ga.npx = g.W * g.H;
+    ga.n = n;
+    ga.seed = seed;
+    ga.X = t.X;
+    ga.target = t.target;
+    ga.mask = t.mask;
+    ga.nValid = t.nValid;
+    ga.pixels = t.pixels;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, t.nValid, 1u);
+    hipLaunchKernelGGL(rm::k_train_gather, dim3((n + 255) / 256), dim3(256), 0, s, ga);
+    mlp_loss_grad_impl(m, t.X, t.target, t.mask, n, t, t.gw, t.gb, t.loss, false, s);
+    m.adam_step += 1;
+    const double b1 = 0.9, b2 = 0.999;
+    rm::AdamArgs a{};
+    a.w = m.weights;
+    a.b = m.bias;
+    a.wbf = m.weights_bf16;
+    a.gw = t.gw;
+    a.gb = t.gb;
+    a.m = m.adam_m;
+    a.v = m.adam_v;
+    a.nW = static_cast<uint32_t>(rm::n_weights(K));
+    a.nB = static_cast<uint32_t>(rm::n_bias(K));
+    a.lr = lr;
+    a.b1 = static_cast<float>(b1);
+    a.b2 = static_cast<float>(b2);
+    a.eps = 1e-8f;
+    a.c1 = static_cast<float>(1.0 - pow(b1, m.adam_step));
+    a.c2 = static_cast<float>(1.0 - pow(b2, m.adam_step));
+    hipLaunchKernelGGL(rm::k_adam_guarded, dim3((a.nW + a.nB + 255) / 256), dim3(256), 0, s, a, t.nValid);
+    if (lossOut) hipLaunchKernelGGL(rm::k_copy_f32, dim3(1), dim3(1), 0, s, t.loss, lossOut);
+    return hipGetLastError();
+}
+
+hipError_t mlp_propose(const restir_mlp &m, const GView &g, f3 cam, const float lo[3], const float hi[3],
+                       const SceneView &sc, const int32_t *lightCluster, const float *lightPWithin,
+                       const int32_t *clusterBegin, const int32_t *clusterCount, const int32_t *clusterLights,
+                       const restir_alias_table_column *clusterAlias, uint32_t M, float mix, uint32_t seed,
+                       int32_t *candLight, float *candQ, hipStream_t s) {
+    rm::ProposeArgs a{};
+    a.f = rm::FeatArgs{g, cam, {lo[0], lo[1], lo[2]}, {hi[0], hi[1], hi[2]}};
+    a.wbf = m.weights_bf16;
+    a.bias = m.bias;
+    a.K = static_cast<int>(m.n_clusters);
+    a.npx = g.W * g.H;
+    a.M = M;
+    a.seed = seed;
+    a.mix = mix;
+    a.alias = sc.alias;
+    a.nAlias = sc.nAlias;
+    a.lightCluster = lightCluster;
+    a.lightPWithin = lightPWithin;
+    a.clusterBegin = clusterBegin;
+    a.clusterCount = clusterCount;
+    a.clusterLights = clusterLights;
+    a.clusterAlias = reinterpret_cast<const float4 *>(clusterAlias);
+    a.candLight = candLight;
+    a.candQ = candQ;
+    const dim3 grid((a.npx + rm::kTilePx - 1) / rm::kTilePx), block(512);
+    switch (a.K) {
+        case 64: hipLaunchKernelGGL((rm::k_propose<64>), grid, block, 0, s, a); break;
+        case 128: hipLaunchKernelGGL((rm::k_propose<128>), grid, block, 0, s, a); break;
+        default: hipLaunchKernelGGL((rm::k_propose<256>), grid, block, 0, s, a); break;
+    }
     return hipGetLastError();
 }
 

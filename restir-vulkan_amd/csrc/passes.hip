@@ -83,6 +83,10 @@ struct PassAArgs {
     const restir_reservoir *prevRes;
     restir_reservoir *out;
     uint32_t tilesX, nTiles;
+    // learned proposal: candidate j of pixel p is (candLight, candQ)[j * npx + p]; null = the
+    // reference's alias-table candidates
+    const int32_t *candLight;
+    const float *candQ;
 };
 
 template <int TX, int TY, int STACK>
@@ -106,7 +110,25 @@ __global__ __launch_bounds__(TX *TY) void k_pass_a(const PassAArgs a) {
     Res res = res_zero();
     Pcg rng = pcg_seed(static_cast<uint64_t>(a.u.frame), static_cast<uint64_t>(y * 10007u + x));
     const SceneView &sc = a.sc;
-    if (dot(n, n) != 0.0f) {
+    if (dot(n, n) != 0.0f && a.candLight) {
+        // RIS over the learned proposal's candidates: weight p_hat / q with the proposal's own
+        // density q; a candidate from an empty cluster (l < 0) has weight 0 but still counts in M
+        const uint32_t M = a.u.initialLightSampleCount;
+        const size_t npx = static_cast<size_t>(W) * H;
+        for (uint32_t i = 0; i < M; ++i) {
+            const int32_t li = a.candLight[i * npx + idx];
+            const float q = a.candQ[i * npx + idx];
+            if (li >= 0 && li < sc.nPoints && q > 0.0f) {
+                const f3 L = xyz(*reinterpret_cast<const float4 *>(sc.points[li].pos));
+                const float lum = sc.points[li].color_luminance[3];
+                const float ph = eval_phat(s, L, f3{0.0f, 0.0f, 0.0f}, false, lum);
+                res_add_sample(res, L, make_float4(0.0f, 0.0f, 0.0f, 0.0f), lum, li, ph, q, rng);
+            } else {
+                res_add_sample(res, f3{0.0f, 0.0f, 0.0f}, make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, 0,
+                               0.0f, 1.0f, rng);
+            }
+        }
+    } else if (dot(n, n) != 0.0f) {
         const uint32_t M = a.u.initialLightSampleCount;
         for (uint32_t i = 0; i < M; ++i) {
             const float r1 = pcg_float(rng), r2 = pcg_float(rng);
@@ -829,8 +851,9 @@ static hipError_t dispatch_a(int tile, const PassAArgs &args, hipStream_t s) {
 }
 hipError_t launch_pass_a(int tile, int stack, const restir_uniforms &u, const SceneView &sc,
                          const GView &cur, const GView &prev, const restir_reservoir *prevRes,
-                         restir_reservoir *out, hipStream_t s) {
-    PassAArgs args{u, sc, cur, prev, prevRes, out, 0, 0};
+                         restir_reservoir *out, const int32_t *candLight, const float *candQ,
+                         hipStream_t s) {
+    PassAArgs args{u, sc, cur, prev, prevRes, out, 0, 0, candLight, candQ};
     return stack <= 32 ? dispatch_a<32>(tile, args, s) : dispatch_a<64>(tile, args, s);
 }
 

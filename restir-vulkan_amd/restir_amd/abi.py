@@ -103,7 +103,26 @@ class Camera(C.Structure):
     ]
 
 
+class MlpStruct(C.Structure):
+    """restir_mlp: the learned proposal's parameter buffers (device pointers)."""
+    _fields_ = [
+        ("n_clusters", C.c_uint32),
+        ("adam_step", C.c_uint32),
+        ("weights_bf16", C.c_void_p),
+        ("weights", C.c_void_p),
+        ("bias", C.c_void_p),
+        ("adam_m", C.c_void_p),
+        ("adam_v", C.c_void_p),
+    ]
+
+
+def mlp_sizes(k: int):
+    """(weights, biases) element counts of RESTIR_MLP_N_WEIGHTS / RESTIR_MLP_N_BIAS."""
+    return 3 * 128 * 128 + k * 128, 3 * 128 + k
+
+
 assert C.sizeof(Uniforms) == 128 and C.sizeof(GBuffer) == 56 and C.sizeof(Camera) == 48
+assert C.sizeof(MlpStruct) == 48
 
 # Exported symbols with their signatures (restype, argtypes); the CPU test suite checks that the
 # library exports every one of them and that include/restir_abi.h declares the same set.
@@ -130,7 +149,26 @@ SIGNATURES = {
     "restir_primary_gbuffer": (_i32, [_vp, C.POINTER(Camera), _vp, _vp, _u32, _vp, _vp, _vp, _vp,
                                       _vp, _u64, _u32, _u32, _vp]),
     "restir_camera_projection_view": (_i32, [C.POINTER(Camera), _f32, C.POINTER(_f32)]),
+    "restir_mlp_init": (_i32, [_vp, C.POINTER(MlpStruct), C.c_uint64, _vp, _vp]),
+    "restir_set_light_clusters": (_i32, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "restir_learned_candidates": (_i32, [_vp, C.POINTER(Uniforms), C.POINTER(GBuffer),
+                                         C.POINTER(MlpStruct), C.POINTER(_f32), C.POINTER(_f32),
+                                         _u32, _f32, _vp, _vp, _vp]),
+    "restir_initial_temporal_candidates": (_i32, [_vp, C.POINTER(Uniforms), C.POINTER(GBuffer),
+                                                  C.POINTER(GBuffer), _vp, _vp, _vp, _vp,
+                                                  C.POINTER(Launch), _vp]),
+    "restir_mlp_train": (_i32, [_vp, C.POINTER(MlpStruct), C.POINTER(Uniforms), C.POINTER(GBuffer),
+                                _vp, C.POINTER(_f32), C.POINTER(_f32), _u32, _f32, _u32, _vp, _vp]),
+    "restir_mlp_features": (_i32, [_vp, C.POINTER(GBuffer), C.POINTER(_f32), C.POINTER(_f32),
+                                   C.POINTER(_f32), _vp, _u32, _vp, _vp]),
+    "restir_mlp_forward": (_i32, [_vp, C.POINTER(MlpStruct), _vp, _u32, _vp, _vp]),
+    "restir_mlp_loss_grad": (_i32, [_vp, C.POINTER(MlpStruct), _vp, _vp, _vp, _u32, _vp, _vp, _vp,
+                                    _vp]),
 }
+
+
+def _f3(v):
+    return (C.c_float * 3)(*[float(x) for x in v[:3]])
 
 
 class RestirError(RuntimeError):
@@ -377,6 +415,48 @@ class Context:
         self._check(self.lib.restir_trace_shadow(self.h, as_ptr(p1), as_ptr(p2), n,
                                                  as_ptr(shadowed), as_ptr(visits), as_ptr(tests),
                                                  loop, stream_ptr(stream)), "restir_trace_shadow")
+
+    # ------------------------------------------------------------------- learned proposal
+    def mlp_init(self, mlp: "MlpStruct", seed, prior, stream=None):
+        self._check(self.lib.restir_mlp_init(self.h, C.byref(mlp), seed, as_ptr(prior),
+                                             stream_ptr(stream)), "restir_mlp_init")
+
+    def set_light_clusters(self, k, light_cluster, light_pwithin, begin, count, lights, alias):
+        self._check(self.lib.restir_set_light_clusters(
+            self.h, k, as_ptr(light_cluster), as_ptr(light_pwithin), as_ptr(begin), as_ptr(count),
+            as_ptr(lights), as_ptr(alias)), "restir_set_light_clusters")
+
+    def learned_candidates(self, u, g, mlp, lo, hi, M, mix, cand_light, cand_q, stream=None):
+        self._check(self.lib.restir_learned_candidates(
+            self.h, C.byref(u), C.byref(g), C.byref(mlp), _f3(lo), _f3(hi), M, float(mix),
+            as_ptr(cand_light), as_ptr(cand_q), stream_ptr(stream)), "restir_learned_candidates")
+
+    def initial_temporal_candidates(self, u, cur, prev, prev_res, out, cand_light, cand_q,
+                                    launch=None, stream=None):
+        self._check(self.lib.restir_initial_temporal_candidates(
+            self.h, C.byref(u), C.byref(cur), C.byref(prev) if prev is not None else None,
+            as_ptr(prev_res), as_ptr(out), as_ptr(cand_light), as_ptr(cand_q),
+            C.byref(launch) if launch is not None else None, stream_ptr(stream)),
+            "restir_initial_temporal_candidates")
+
+    def mlp_train(self, mlp, u, g, res, lo, hi, n, lr, seed, loss_out=None, stream=None):
+        self._check(self.lib.restir_mlp_train(
+            self.h, C.byref(mlp), C.byref(u), C.byref(g), as_ptr(res), _f3(lo), _f3(hi), n,
+            float(lr), seed, as_ptr(loss_out), stream_ptr(stream)), "restir_mlp_train")
+
+    def mlp_features(self, g, cam_pos, lo, hi, pixels, n, out, stream=None):
+        self._check(self.lib.restir_mlp_features(
+            self.h, C.byref(g), _f3(cam_pos), _f3(lo), _f3(hi), as_ptr(pixels), n, as_ptr(out),
+            stream_ptr(stream)), "restir_mlp_features")
+
+    def mlp_forward(self, mlp, X, n, logits, stream=None):
+        self._check(self.lib.restir_mlp_forward(self.h, C.byref(mlp), as_ptr(X), n, as_ptr(logits),
+                                                stream_ptr(stream)), "restir_mlp_forward")
+
+    def mlp_loss_grad(self, mlp, X, target, mask, n, gw, gb, loss, stream=None):
+        self._check(self.lib.restir_mlp_loss_grad(
+            self.h, C.byref(mlp), as_ptr(X), as_ptr(target), as_ptr(mask), n, as_ptr(gw),
+            as_ptr(gb), as_ptr(loss), stream_ptr(stream)), "restir_mlp_loss_grad")
 
     def primary_gbuffer(self, cam: dict, tri_material, materials, n_materials, gbuf_base, width,
                         height, plane_stride=None, stream=None):

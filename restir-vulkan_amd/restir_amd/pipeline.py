@@ -61,7 +61,8 @@ class Renderer:
 
     def __init__(self, scene, width, height, *, device_index=0, mode="unbiased", M=32, k=5,
                  spatial_passes=1, spatial_neighbors=4, radius=30.0, temporal=True,
-                 visibility=True, temporal_mult=20, launch=None, ctx=None):
+                 visibility=True, temporal_mult=20, launch=None, ctx=None, learned=None,
+                 train_samples=4096, lr=1e-3, train=True):
         torch = _torch()
         self.torch = torch
         self.device = torch.device("cuda", device_index)
@@ -77,6 +78,9 @@ class Renderer:
         self.flags = (abi.VISIBILITY_REUSE_FLAG if visibility else 0) | \
                      (abi.TEMPORAL_REUSE_FLAG if temporal else 0)
         self.launch = launch
+        # learned proposal: a restir_amd.learned.LearnedProposal bound to this context, or None
+        self.learned = learned
+        self.train_samples, self.lr, self.train_enabled = train_samples, lr, train
         n = width * height
         self.gbuf = [torch.zeros(GBUFFER_PLANES * n, dtype=torch.float32, device=self.device)
                      for _ in range(2)]
@@ -116,17 +120,26 @@ class Renderer:
         """The ReSTIR dispatches of one frame (the hot path)."""
         i, p = self.cur, 1 - self.cur
         g, gp = self.gstruct(i), self.gstruct(p)
+
+        def pass_a(out):
+            if self.learned is not None:
+                cl, cq = self.learned.propose(u, g, self.M, stream)
+                self.ctx.initial_temporal_candidates(u, g, gp, self.res[p], out, cl, cq, self.launch,
+                                                     stream)
+            else:
+                self.ctx.initial_temporal(u, g, gp, self.res[p], out, self.launch, stream)
+
         if self.mode == "unbiased":
             n = self.spatial_passes
             first = self.tmp if n % 2 == 1 else self.res[i]
-            self.ctx.initial_temporal(u, g, gp, self.res[p], first, self.launch, stream)
+            pass_a(first)
             src = first
             for j in range(n):
                 dst = self.res[i] if (n - j) % 2 == 1 else self.tmp
                 self.ctx.unbiased(u, self.k, j, g, src, dst, self.launch, stream)
                 src = dst
         else:
-            self.ctx.initial_temporal(u, g, gp, self.res[p], self.res[i], self.launch, stream)
+            pass_a(self.res[i])
             for j in range(self.spatial_passes):
                 self.ctx.spatial(u, 2 * j, g, self.res[i], self.res[p], self.launch, stream)
                 self.ctx.spatial(u, 2 * j + 1, g, self.res[p], self.res[i], self.launch, stream)
@@ -135,11 +148,18 @@ class Renderer:
         self.ctx.shade(self.cam["position"], gamma, self.gstruct(self.cur), self.res[self.cur],
                        self.rgb, self.W * self.H, stream)
 
+    def train_step(self, u, stream=None):
+        """One online step of the learned proposal on this frame's final reservoirs."""
+        if self.learned is not None and self.train_enabled:
+            self.learned.train(u, self.gstruct(self.cur), self.res[self.cur], self.train_samples,
+                               self.lr, stream)
+
     def render_frame(self, cam: dict, stream=None):
         self.begin_frame(cam, stream)
         u, pv_now = self.uniforms(cam)
         self.restir_passes(u, stream)
         self.shade(stream)
+        self.train_step(u, stream)
         self.prev_pv = pv_now
         return self.rgb
 

@@ -115,8 +115,9 @@ def _materials(rng, n, emissive_rows=()):
     return m
 
 
-def cornell_box(seed: int = 1, n_lights: int = 64) -> Scene:
-    """Config 1: room x,z in [-1,1], y in [0,2]; camera 60 deg fovY on the open front."""
+def cornell_box(seed: int = 1, n_lights: int = 64, light_margin: float = 0.05) -> Scene:
+    """Config 1: room x,z in [-1,1], y in [0,2]; camera 60 deg fovY on the open front. Lights lie
+    in the upper half of the room at least `light_margin` from the walls and ceiling."""
     rng = np.random.default_rng(seed)
     P = lambda x, y, z: np.array([x, y, z], dtype=np.float64)  # noqa: E731
     walls = [
@@ -139,8 +140,9 @@ def cornell_box(seed: int = 1, n_lights: int = 64) -> Scene:
         mat += [5 + b] * 12
     tri_xyz = np.array(tris, dtype=np.float32).reshape(-1, 9)
     materials = _materials(rng, 7)
-    pos = np.stack([rng.uniform(-0.95, 0.95, n_lights), rng.uniform(1.0, 1.95, n_lights),
-                    rng.uniform(-0.95, 0.95, n_lights)], axis=1)
+    e = 1.0 - light_margin
+    pos = np.stack([rng.uniform(-e, e, n_lights), rng.uniform(1.0, 1.0 + e, n_lights),
+                    rng.uniform(-e, e, n_lights)], axis=1)
     rgb = np.exp(rng.uniform(math.log(0.1), math.log(10.0), size=(n_lights, 3)))
     camera = dict(position=(0.0, 1.0, 2.9), lookAt=(0.0, 1.0, 0.0), worldUp=(0.0, 1.0, 0.0),
                   fovY=math.pi / 3.0, zNear=0.01, zFar=1000.0)

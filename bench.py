@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--tile", type=str, default="16x16")
     ap.add_argument("--trace-reps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device copy (x3), whose hardware byte counters "
+                         "calibrate the traffic counters of the other kernels")
     return ap.parse_args()
 
 
@@ -99,94 +102,97 @@ def main():
         return make_uniforms(W, H, f + 1, cams[f]["position"], pvs[f - 1] if f > 0 else pvs[f],
                              M=args.M, temporal_mult=20, neighbors=args.k, radius=30.0, flags=flags)
 
-    names = ["pass_a"] + [f"pass_c{j}" for j in range(n_sp)] + ["shade"]
-
-    def frame(f, events=None):
+    def frame(f):
         i, p = f % 2, 1 - f % 2
         g = gbuffer_struct(G[f + 1].data_ptr(), W, H)
         gp = gbuffer_struct(G[f].data_ptr(), W, H)
         u = uniforms(f)
         first = tmp if n_sp % 2 == 1 else res[i]
-        rec = (lambda k: events[k].record(stream)) if events is not None else (lambda k: None)
-        rec(0)
         ctx.initial_temporal(u, g, gp, res[p], first, launch, stream)
-        rec(1)
         src = first
         for j in range(n_sp):
             dst = res[i] if (n_sp - j) % 2 == 1 else tmp
             ctx.unbiased(u, args.k, j, g, src, dst, launch, stream)
-            rec(2 + j)
+            yield j, dst
             src = dst
         ctx.shade(cams[f]["position"], 1.0, g, res[i], rgb, npx, stream)
-        rec(2 + n_sp)
+
+    def run(f):
+        for _ in frame(f):
+            pass
 
     for f in range(WU):
-        frame(f)
+        run(f)
     torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)] for _ in range(K)]
+    # timed region: every kernel bracketed by hipExtLaunchKernel start/stop events on `stream`
+    ctx.profile_read()
+    ctx.profile_enable(True)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for s in range(K):
-        frame(WU + s, evs[s])
+        run(WU + s)
     torch.cuda.synchronize()
     wall_ms = (time.perf_counter() - t_start) * 1e3
-    per_kernel = {n: float(np.mean([evs[s][j].elapsed_time(evs[s][j + 1]) for s in range(K)]))
-                  for j, n in enumerate(names)}
-    gpu_ms = float(np.mean([evs[s][0].elapsed_time(evs[s][-1]) for s in range(K)]))
+    prof = ctx.profile_read()
+    ctx.profile_enable(False)
     ms_per_frame = wall_ms / K
-    log(f"timed {K} frames: {ms_per_frame:.3f} ms/frame (events {gpu_ms:.3f}); " +
-        ", ".join(f"{k} {v:.3f}" for k, v in per_kernel.items()))
+    kernels = {name: {"launches_per_frame": n / K, "ms_per_frame": tot / K, "avg_ms": tot / n}
+               for name, (n, tot, _mx) in prof.items()}
+    gpu_ms = sum(v["ms_per_frame"] for v in kernels.values())
+    log(f"timed {K} frames: {ms_per_frame:.3f} ms/frame (kernels {gpu_ms:.3f}); " +
+        ", ".join(f"{k} {v['ms_per_frame']:.3f}" for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_frame"])))
 
-    # ---- shadow-ray traversal: the last frame's Pass A rays (RIS without the visibility test)
-    f = nframes - 1
-    u = uniforms(f)
-    u.flags = 0
-    ris = torch.zeros(16 * npx, dtype=torch.int32, device=dev)
-    g = gbuffer_struct(G[f + 1].data_ptr(), W, H)
-    ctx.initial_temporal(u, g, g, res[0], ris, launch, stream)
-    rv = ris.view(npx, 16)
-    lit = rv[:, 10].view(torch.float32) != 0
-    pos = G[f + 1][: 3 * npx].view(3, npx).t()
-    p1 = torch.zeros(int(lit.sum()), 4, dtype=torch.float32, device=dev)
-    p2 = torch.zeros_like(p1)
-    p1[:, :3] = pos[lit]
-    p2[:, :3] = rv[lit, 0:3].view(torch.float32)
-    n_rays = p1.shape[0]
-    sh = torch.zeros(n_rays, dtype=torch.int32, device=dev)
+    # ---- the last timed frame again, untimed and bit-identical (same inputs and seeds; frame f
+    # reads the previous frame's reservoirs, which it does not overwrite): export each Pass C's
+    # queued shadow rays and count node visits / triangle tests of exactly those rays
+    f_last = WU + K - 1
+    cap = npx * args.k
+    q1 = torch.zeros(cap, 4, dtype=torch.float32, device=dev)
+    q2 = torch.zeros_like(q1)
+    sh = torch.zeros(cap, dtype=torch.int32, device=dev)
     visits = torch.zeros_like(sh)
     tests = torch.zeros_like(sh)
-    ctx.trace_shadow(p1, p2, n_rays, sh, visits, tests, stream=stream)  # first-hit loop, counted
-    torch.cuda.synchronize()
-    V = float(visits.double().mean())
-    T = float(tests.double().mean())
-    occl = float(sh.double().mean())
+    ray_sets = {"centre": [], "neighbour": []}
+    for j, _dst in frame(f_last):
+        for which, key in ((0, "centre"), (1, "neighbour")):
+            n = ctx.unbiased_rays(which, q1, q2, cap, stream=stream)
+            ctx.trace_shadow(q1, q2, n, sh, visits, tests, stream=stream)
+            torch.cuda.synchronize()
+            ray_sets[key].append({"rays": n, "V": float(visits[:n].double().mean()) if n else 0.0,
+                                  "T": float(tests[:n].double().mean()) if n else 0.0,
+                                  "occluded": float(sh[:n].double().mean()) if n else 0.0})
+    # standalone traversal throughput on the last Pass C's neighbour rays (left in q1/q2)
+    nr = ray_sets["neighbour"][-1]["rays"]
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ctx.trace_shadow(p1, p2, n_rays, sh, stream=stream)
+    ctx.trace_shadow(q1, q2, nr, sh, stream=stream)
     e0.record(stream)
     for _ in range(args.trace_reps):
-        ctx.trace_shadow(p1, p2, n_rays, sh, stream=stream)
+        ctx.trace_shadow(q1, q2, nr, sh, stream=stream)
     e1.record(stream)
     torch.cuda.synchronize()
     trace_ms = e0.elapsed_time(e1) / args.trace_reps
-    rays_per_s = n_rays / (trace_ms * 1e-3)
-    bytes_per_ray = NODE_BYTES * V + TRI_BYTES * T
-    lane_ops_per_ray = 56 * V + 45 * T
-    roof_mem = HBM_PEAK_GBS * 1e9 / bytes_per_ray
-    roof_alu = 39.3e12 / lane_ops_per_ray
-    roof_rays = min(roof_mem, roof_alu)
-    log(f"traversal: {n_rays} rays, V={V:.1f} T={T:.2f} occluded={occl:.3f}: {trace_ms:.3f} ms, "
+    Vn, Tn = ray_sets["neighbour"][-1]["V"], ray_sets["neighbour"][-1]["T"]
+    rays_per_s = nr / (trace_ms * 1e-3)
+    roof_rays = min(HBM_PEAK_GBS * 1e9 / (NODE_BYTES * Vn + TRI_BYTES * Tn),
+                    39.3e12 / (56 * Vn + 45 * Tn))
+    log(f"traversal: {nr} neighbour rays, V={Vn:.1f} T={Tn:.2f}: {trace_ms:.3f} ms, "
         f"{rays_per_s / 1e6:.0f} Mrays/s, roofline {roof_rays / 1e6:.0f} Mrays/s")
 
-    # ---- roofline of the dominant kernel (algorithmic bytes / measured time)
-    rays_c = npx  # lower bound for pass C rays per pass: the centre re-test of every lit pixel
-    kern_bytes = {"pass_a": BYTES_PER_PX["pass_a"] * npx + n_rays * bytes_per_ray,
-                  "shade": BYTES_PER_PX["shade"] * npx}
-    for j in range(n_sp):
-        kern_bytes[f"pass_c{j}"] = BYTES_PER_PX["pass_c"] * npx + rays_c * bytes_per_ray * float(lit.double().mean())
-    dominant = max(per_kernel, key=per_kernel.get)
-    achieved = kern_bytes[dominant] / (per_kernel[dominant] * 1e-3) / 1e9
-    kernels_out = {k: {"ms": round(v, 4), "algorithmic_GBps": round(kern_bytes[k] / (v * 1e-3) / 1e9, 1)}
-                   for k, v in per_kernel.items()}
+    # ---- roofline of the dominant kernel: algorithmic bytes per launch / average launch time
+    def ray_bytes(key):
+        sets = ray_sets[key]
+        return float(np.mean([r["rays"] * (NODE_BYTES * r["V"] + TRI_BYTES * r["T"]) for r in sets]))
+    algo_bytes = {"k_trace_queue(neighbour)": ray_bytes("neighbour"),
+                  "k_trace_queue(centre)": ray_bytes("centre"),
+                  "k_shade": BYTES_PER_PX["shade"] * npx}
+    dominant = max(kernels, key=lambda k: kernels[k]["ms_per_frame"])
+    dom_bytes = algo_bytes.get(dominant)
+    achieved = dom_bytes / (kernels[dominant]["avg_ms"] * 1e-3) / 1e9 if dom_bytes else None
+    for k, v in kernels.items():
+        if k in algo_bytes:
+            v["algorithmic_GBps"] = round(algo_bytes[k] / (v["avg_ms"] * 1e-3) / 1e9, 1)
+        for key in ("launches_per_frame", "ms_per_frame", "avg_ms"):
+            v[key] = round(v[key], 4)
 
     out = {
         "metric": METRIC,
@@ -208,17 +214,29 @@ def main():
             "point_lights": int(len(scene.point_lights)), "M": args.M, "k": args.k,
             "spatial_passes": n_sp, "tile": args.tile, "learned_proposal": False,
         },
-        "frame_gpu_ms": round(gpu_ms, 4),
-        "kernels": kernels_out,
+        "frame_kernel_ms": round(gpu_ms, 4),
+        "kernels": kernels,
         "mrays_per_s": round(rays_per_s / 1e6, 1),
-        "traversal": {"rays": n_rays, "node_visits_per_ray": round(V, 2),
-                      "tri_tests_per_ray": round(T, 2), "occluded_frac": round(occl, 4),
+        "traversal": {"rays": nr, "ray_set": "last frame's second Pass C neighbour rays",
+                      "node_visits_per_ray": round(Vn, 2), "tri_tests_per_ray": round(Tn, 2),
                       "ms": round(trace_ms, 4), "roofline_mrays_per_s": round(roof_rays / 1e6, 1),
-                      "frac": round(rays_per_s / roof_rays, 4)},
-        "roofline": {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                      "frac": round(rays_per_s / roof_rays, 4),
+                      "pass_c_rays": ray_sets},
+        "roofline": {"kernel": dominant, "bound": "hbm",
+                     "achieved": round(achieved, 1) if achieved else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": None,
+                     "algorithmic_bytes_per_launch": round(dom_bytes) if dom_bytes else None,
+                     "avg_launch_ms": kernels[dominant]["avg_ms"]},
     }
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)  # reads 256 MiB, writes 256 MiB
+        torch.cuda.synchronize()
+        del src, dst
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ctx, scene, sd, cams[-1], args)
     print(json.dumps(out), flush=True)

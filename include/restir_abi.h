@@ -172,7 +172,7 @@ void restir_destroy(restir_ctx *ctx);
 const char *restir_last_error(const restir_ctx *ctx);
 /* ABI revision; bumped whenever an entry point's signature changes. Bindings compare it with the
  * revision they were written against and refuse a mismatched library. */
-#define RESTIR_ABI_VERSION 3
+#define RESTIR_ABI_VERSION 5
 int restir_version(void);
 
 /* Light tables: device pointers to the reference buffer images (header + array). The counts are
@@ -353,6 +353,43 @@ int restir_primary_gbuffer(restir_ctx *ctx, const restir_camera *cam, const int3
 /* The 4x4 projection-view matrix of camera.h:25-50 for the given aspect, column-major, as the
  * reference writes it into RestirUniforms::prevFrameProjectionViewMatrix (app.cpp:779). */
 int restir_camera_projection_view(const restir_camera *cam, float aspect, float out_col_major[16]);
+
+/* FP64 ground truth of the direct lighting at `n` listed pixels (device array of pixel indices
+ * y*width+x): for each, the sum over every light of Le * BRDF * G * V in double precision - the
+ * integral that the passes estimate (integrand restirUtils.glsl:30-55, visibility of the segment
+ * shrunk by 0.001 at both ends as visibilityTest.glsl:1-30). Triangle lights are integrated with
+ * n_tri_samples stratified samples each (rounded up to a square). Emissive pixels return their
+ * emission, background pixels 0. Writes rgb[c * rgb_stride + i] (device doubles). No counterpart
+ * in the reference: it is the reference of the error metric of converged images. */
+int restir_ground_truth(restir_ctx *ctx, const float cameraPos[3], const restir_gbuffer *g,
+                        const uint32_t *pixels, uint32_t n, uint32_t n_tri_samples, double *rgb,
+                        uint64_t rgb_stride, void *stream);
+
+/* ---- measurement (no counterpart in the reference, which times nothing; for benchmarks and
+ * profilers of a host that calls the entry points above) */
+
+/* Per-kernel timing. While enabled, every kernel this context launches is bracketed by a pair of
+ * HIP events on its stream (hipExtLaunchKernel start/stop events, so the span is exactly the
+ * kernel). restir_profile_read waits for the recorded kernels, writes one row per kernel name
+ * (at most `capacity` rows; *count receives the number of names) and clears the record. */
+typedef struct restir_kernel_time {
+    char name[48];
+    uint32_t launches;
+    float total_ms;
+    float max_ms;
+    uint32_t _reserved;
+} restir_kernel_time;  /* 64 B */
+int restir_profile_enable(restir_ctx *ctx, int enable);
+int restir_profile_read(restir_ctx *ctx, restir_kernel_time *out, uint32_t capacity,
+                        uint32_t *count);
+
+/* The shadow rays of the last queued-schedule restir_unbiased call on this context (which = 0:
+ * the centre re-tests, 1: the neighbour tests), as p1/p2 float4 pairs in queue order (p1.w holds a
+ * pixel tag, ignored by restir_trace_shadow). Copies min(queue length, capacity) rays on `stream`
+ * and stores the queue length in *count (a host pointer; the call synchronizes `stream`). Lets a
+ * benchmark count node visits and triangle tests of exactly the rays the traversal kernel saw. */
+int restir_unbiased_rays(restir_ctx *ctx, int which, float *p1, float *p2, uint32_t capacity,
+                         uint32_t *count, void *stream);
 
 #ifdef __cplusplus
 }

@@ -31,6 +31,8 @@ int hip_fail(restir_ctx *ctx, hipError_t e, const char *what) {
 }
 
 int use_device(restir_ctx *ctx) {
+    // every launching entry point passes here first: bind this context's recorder (or none)
+    rk::tl_recorder = ctx->profiling ? &ctx->recorder : nullptr;
     const hipError_t e = hipSetDevice(ctx->device);
     return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "hipSetDevice");
 }
@@ -309,6 +311,9 @@ int restir_unbiased(restir_ctx *ctx, const restir_uniforms *u, int k, int iter,
     }
     const hipError_t e = rk::launch_pass_c(tile, ctx->stack, *u, k, iter, ctx->scene, view_of(g), in, out,
                                            queues, ctx->persistentBlocks, static_cast<hipStream_t>(stream));
+    // the queues hold this call's rays only when it traced any (otherwise they are stale)
+    ctx->queuedNpx = (queues && vis && e == hipSuccess) ? W * H : 0;
+    ctx->queuedK = k;
     return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "unbiased launch");
 }
 
@@ -345,6 +350,39 @@ int restir_trace_shadow(restir_ctx *ctx, const float *p1, const float *p2, uint3
                                           reinterpret_cast<const float4 *>(p2), n, shadowed, visits,
                                           tests, static_cast<hipStream_t>(stream));
     return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "trace_shadow launch");
+}
+
+int restir_ground_truth(restir_ctx *ctx, const float cameraPos[3], const restir_gbuffer *g,
+                        const uint32_t *pixels, uint32_t n, uint32_t nTriSamples, double *rgb,
+                        uint64_t rgbStride, void *stream) {
+    if (!ctx) return RESTIR_E_INVALID_ARGUMENT;
+    if (!ctx->lightsSet || !ctx->bvhSet) return fail(ctx, RESTIR_E_NOT_READY, "ground_truth: lights or BVH not set");
+    if (n == 0) return RESTIR_OK;
+    if (!cameraPos || !pixels || !rgb || !g)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "ground_truth: NULL argument");
+    if (int r = check_gbuffer(ctx, g, g->width, g->height, false, "ground_truth")) return r;
+    if (rgbStride < n) return fail(ctx, RESTIR_E_SIZE_MISMATCH, "ground_truth: rgb stride %llu < n",
+                                   static_cast<unsigned long long>(rgbStride));
+    if (n > (1u << 24)) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "ground_truth: n=%u > 2^24", n);
+    if (ctx->scene.nTriLights > 0 && (nTriSamples == 0 || nTriSamples > 65536))
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "ground_truth: n_tri_samples=%u not in [1, 65536]",
+                    nTriSamples);
+    uint32_t sq = 1;
+    while (sq * sq < nTriSamples) ++sq;
+    if (int r = use_device(ctx)) return r;
+    // the pixel list is device memory: launched lanes index the G-buffer with it, so it is
+    // checked on the host first (one small copy)
+    std::vector<uint32_t> host(n);
+    hipError_t e = hipMemcpy(host.data(), pixels, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(ctx, e, "ground_truth: reading the pixel list");
+    const uint64_t npx = static_cast<uint64_t>(g->width) * g->height;
+    for (uint32_t i = 0; i < n; ++i)
+        if (host[i] >= npx) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "ground_truth: pixel %u out of range", host[i]);
+    if (int r = ensure(ctx, &ctx->truthScratch, &ctx->truthScratchBytes, rk::truth_scratch_bytes(ctx->scene, n)))
+        return r;
+    e = rk::launch_truth(ctx->stack, ctx->scene, view_of(g), cameraPos, pixels, n, sq, ctx->truthScratch, rgb,
+                         rgbStride, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "ground_truth launch");
 }
 
 int restir_primary_gbuffer(restir_ctx *ctx, const restir_camera *cam, const int32_t *triMaterial,

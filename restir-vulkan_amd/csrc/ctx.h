@@ -4,8 +4,24 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "kernels.h"
+
+namespace rk {
+// Event pairs of the kernels launched while profiling is on, in launch order.
+struct Recorder {
+    struct Span {
+        const char *name;  // a string literal at the launch site
+        hipEvent_t start, stop;
+    };
+    std::vector<Span> spans;
+    std::vector<hipEvent_t> pool;  // recycled events
+    hipEvent_t take();
+    void clear();
+    ~Recorder();
+};
+}  // namespace rk
 
 struct restir_ctx {
     int device = 0;
@@ -19,8 +35,14 @@ struct restir_ctx {
     uint32_t persistentBlocks = 1280;
     void *scratch = nullptr;       // ray queues of the queued Pass C schedule
     size_t scratchBytes = 0;
+    uint32_t queuedNpx = 0;        // shape of the last queued Pass C (its rays stay in scratch)
+    int queuedK = 0;
+    bool profiling = false;
+    rk::Recorder recorder;
     void *mlpScratch = nullptr;    // training tensors of the learned proposal
     size_t mlpScratchBytes = 0;
+    void *truthScratch = nullptr;  // per-light-chunk partial sums of the FP64 ground truth
+    size_t truthScratchBytes = 0;
     // light clusters of the learned proposal (device pointers, caller-owned)
     bool clustersSet = false;
     int32_t nClusters = 0;
@@ -32,6 +54,7 @@ struct restir_ctx {
         if (counters) (void)hipFree(counters);
         if (scratch) (void)hipFree(scratch);
         if (mlpScratch) (void)hipFree(mlpScratch);
+        if (truthScratch) (void)hipFree(truthScratch);
     }
 };
 

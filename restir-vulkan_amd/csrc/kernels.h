@@ -1,6 +1,7 @@
 // kernels.h — launch interface between the C ABI (abi.hip) and the gfx950 kernels (passes.hip).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -8,6 +9,21 @@
 #include "rt_traverse.h"
 
 namespace rk {
+
+// ---- optional per-kernel timing (restir_profile_enable). Every launch goes through RK_LAUNCH,
+// which hands the kernel a start/stop event pair when the calling thread has a recorder bound;
+// the events bracket exactly that kernel on its stream (hipExtLaunchKernel).
+struct Recorder;
+extern thread_local Recorder *tl_recorder;
+void rec_pair(const char *name, hipStream_t stream, hipEvent_t *start, hipEvent_t *stop);
+
+#define RK_LAUNCH(name, kernel, grid, block, shm, stream, ...)                                   \
+    do {                                                                                         \
+        hipEvent_t rk_start_ = nullptr, rk_stop_ = nullptr;                                      \
+        if (::rk::tl_recorder) ::rk::rec_pair(name, stream, &rk_start_, &rk_stop_);              \
+        hipExtLaunchKernelGGL(kernel, grid, block, shm, stream, rk_start_, rk_stop_, 0u,         \
+                              __VA_ARGS__);                                                      \
+    } while (0)
 
 // Scene tables as the kernels see them: arrays past the 16-byte SSBO headers, counts read once.
 struct SceneView {
@@ -49,6 +65,10 @@ hipError_t launch_pass_c(int tile, int stack, const restir_uniforms &u, int k, i
                          restir_reservoir *out, void *queueScratch, uint32_t gridCap,
                          hipStream_t s);
 size_t pass_c_queue_bytes(uint32_t npx, int k);
+// Copies the shadow rays that the last queued Pass C over (npx, k) put in `queueScratch`
+// (which = 0: centre rays, 1: neighbour rays); *n receives the queue length.
+hipError_t pass_c_rays(const void *queueScratch, uint32_t npx, int k, int which, float4 *p1,
+                       float4 *p2, uint32_t cap, uint32_t *n, hipStream_t s);
 hipError_t launch_shade(const SceneView &sc, rt::f3 cam, float gamma, const GView &g,
                         const restir_reservoir *res, float *rgb, uint64_t rgbP, hipStream_t s);
 // persistent: fixed grid of at most gridCap workgroups pulling rays through *counter (which the
@@ -81,6 +101,13 @@ hipError_t mlp_propose(const restir_mlp &m, const GView &g, rt::f3 cam, const fl
                        const int32_t *clusterCount, const int32_t *clusterLights,
                        const restir_alias_table_column *clusterAlias, uint32_t M, float mix,
                        uint32_t seed, int32_t *candLight, float *candQ, hipStream_t s);
+
+// ---- FP64 ground truth (truth.hip)
+constexpr uint32_t kTruthChunk = 128;  // lights per workgroup row
+size_t truth_scratch_bytes(const SceneView &sc, uint32_t n);
+hipError_t launch_truth(int stack, const SceneView &sc, const GView &g, const float cam[3],
+                        const uint32_t *pixels, uint32_t n, uint32_t sq, void *scratch, double *rgb,
+                        uint64_t stride, hipStream_t s);
 
 struct CamBasis {
     rt::f3 pos, fwd, right, down;

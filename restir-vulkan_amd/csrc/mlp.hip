@@ -10,9 +10,10 @@
 // Kernels:
 //   k_propose      fused inference: features -> 4 MFMA layers -> softmax -> M candidates (l, q)
 //   k_features     features of listed pixels (bf16), for training and tests
-//   k_gemm         strided C = A*B with bias / ReLU / ReLU-gate epilogues (training fwd/bwd)
+//   k_gemm         strided C = A*B with bias / ReLU / ReLU-gate epilogues (training fwd/bwd),
+//                  or split-K fp32 partials for the weight gradients
 //   k_softmax_ce   mean cross-entropy of the logits against target clusters, and dlogits
-//   k_colsum       bias gradients
+//   k_colsum_partial, k_sum_partials   bias gradients and the split-K reduction, in fixed order
 //   k_adam         Adam step on fp32 master weights, refreshes the bf16 copy
 //   k_mlp_init     seeded He-uniform initialisation
 #include "../../include/restir_abi.h"
@@ -98,6 +99,8 @@ struct GemmArgs {
     float *Cf;             // fp32 output, or null
     uint16_t *Cb;          // bf16 output, or null
     int64_t scm, scn;
+    int kChunk;            // split-K: workgroup z covers K range [z*kChunk, (z+1)*kChunk) and
+    int64_t scz;           // writes its fp32 partial at Cf + z*scz (no bias/relu/gate); 0 = off
 };
 
 // 64x64 output tile per 256-lane workgroup; each wave owns a 32x32 quarter. K advances in steps
@@ -109,13 +112,19 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmArgs g) {
     const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int kBeg = g.kChunk ? static_cast<int>(blockIdx.z) * g.kChunk : 0;
+    const int kEnd = g.kChunk ? kBeg + g.kChunk : g.K;
+    // staging order: consecutive lanes read consecutive addresses of whichever operand index is
+    // contiguous in memory
+    const bool aRowFast = g.sak != 1, bRowFast = g.sbn == 1;
     f32x16 acc;
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
-    for (int k0 = 0; k0 < g.K; k0 += 32) {
+    for (int k0 = kBeg; k0 < kEnd; k0 += 32) {
         for (int e = tid; e < 64 * 32; e += 256) {
-            const int r = e >> 5, c = e & 31;
-            As[r][c] = g.A[(m0 + r) * g.sam + (k0 + c) * g.sak];
-            Bt[r][c] = g.B[(k0 + c) * g.sbk + (n0 + r) * g.sbn];
+            const int ra = aRowFast ? (e & 63) : (e >> 5), ca = aRowFast ? (e >> 6) : (e & 31);
+            As[ra][ca] = g.A[(m0 + ra) * g.sam + (k0 + ca) * g.sak];
+            const int rb = bRowFast ? (e & 63) : (e >> 5), cb = bRowFast ? (e >> 6) : (e & 31);
+            Bt[rb][cb] = g.B[(k0 + cb) * g.sbk + (n0 + rb) * g.sbn];
         }
         __syncthreads();
 #pragma unroll
@@ -127,6 +136,15 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmArgs g) {
         __syncthreads();
     }
     const int col = n0 + wn + (lane & 31);
+    if (g.kChunk) {
+        float *P = g.Cf + blockIdx.z * g.scz;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int row = m0 + wm + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+            P[row * g.scm + col * g.scn] = acc[i];
+        }
+        return;
+    }
     const float bias = g.bias ? g.bias[col] : 0.0f;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
@@ -146,23 +164,27 @@ __global__ __launch_bounds__(256) void k_softmax_ce(const float *Z, int K, const
                                                     const uint32_t *mask, int n,
                                                     const uint32_t *nValid, uint16_t *dZ,
                                                     float *loss) {
-    const int b = blockIdx.x * 256 + threadIdx.x;
+    // one wave per sample: lanes stride over the K logits, reductions across the wave
+    const int b = blockIdx.x * 4 + static_cast<int>(threadIdx.x >> 6);
+    const int lane = static_cast<int>(threadIdx.x & 63);
     if (b >= n) return;
     const float *z = Z + static_cast<size_t>(b) * K;
     uint16_t *d = dZ + static_cast<size_t>(b) * K;
-    if (!mask[b] || target[b] < 0 || target[b] >= K) {  // an out-of-range target does not train
-        for (int c = 0; c < K; ++c) d[c] = 0;
+    const int t = target[b];
+    if (!mask[b] || t < 0 || t >= K) {  // an out-of-range target does not train
+        for (int c = lane; c < K; c += 64) d[c] = 0;
         return;
     }
     const float inv = 1.0f / static_cast<float>(*nValid);
-    float mx = z[0];
-    for (int c = 1; c < K; ++c) mx = fmaxf(mx, z[c]);
+    float mx = -INFINITY;
+    for (int c = lane; c < K; c += 64) mx = fmaxf(mx, z[c]);
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
     float se = 0.0f;
-    for (int c = 0; c < K; ++c) se += expf(z[c] - mx);
-    const int t = target[b];
-    atomicAdd(loss, -(z[t] - mx - logf(se)) * inv);
+    for (int c = lane; c < K; c += 64) se += expf(z[c] - mx);
+    for (int off = 32; off > 0; off >>= 1) se += __shfl_xor(se, off);
+    if (lane == 0) atomicAdd(loss, -(z[t] - mx - logf(se)) * inv);
     const float rse = 1.0f / se;
-    for (int c = 0; c < K; ++c) d[c] = f2bf((expf(z[c] - mx) * rse - (c == t ? 1.0f : 0.0f)) * inv);
+    for (int c = lane; c < K; c += 64) d[c] = f2bf((expf(z[c] - mx) * rse - (c == t ? 1.0f : 0.0f)) * inv);
 }
 
 __global__ __launch_bounds__(256) void k_count_mask(const uint32_t *mask, uint32_t n, uint32_t *count) {
@@ -174,13 +196,29 @@ __global__ __launch_bounds__(256) void k_count_mask(const uint32_t *mask, uint32
 
 __global__ void k_copy_f32(const float *src, float *dst) { *dst = *src; }
 
-// out[c] = sum over rows of G[row][c] (bf16 input, fp32 sum)
-__global__ __launch_bounds__(256) void k_colsum(const uint16_t *G, int rows, int cols, float *out) {
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= cols) return;
-    float s = 0.0f;
-    for (int r = 0; r < rows; ++r) s += bf2f(G[static_cast<size_t>(r) * cols + c]);
-    out[c] = s;
+// Column sums of a bf16 [rows][cols] matrix in two deterministic steps: workgroup (x, y) sums
+// rows [y*chunk, (y+1)*chunk) of columns [64x, 64x+64) into part[y][col]; k_sum_partials adds
+// the partials of each column in order.
+__global__ __launch_bounds__(256) void k_colsum_partial(const uint16_t *G, int cols, int chunk,
+                                                        float *part) {
+    __shared__ float red[4][64];
+    const int c = blockIdx.x * 64 + static_cast<int>(threadIdx.x & 63);
+    const int r0 = blockIdx.y * chunk, q = static_cast<int>(threadIdx.x >> 6);
+    float sum = 0.0f;
+    for (int r = r0 + q; r < r0 + chunk; r += 4) sum += bf2f(G[static_cast<size_t>(r) * cols + c]);
+    red[q][threadIdx.x & 63] = sum;
+    __syncthreads();
+    if (q == 0) part[static_cast<size_t>(blockIdx.y) * cols + c] = ((red[0][c & 63] + red[1][c & 63]) + red[2][c & 63]) + red[3][c & 63];
+}
+
+// out[i] = sum over s < nParts of part[s * count + i], in order of s
+__global__ __launch_bounds__(256) void k_sum_partials(const float *part, int nParts, int64_t count,
+                                                      float *out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= count) return;
+    float sum = 0.0f;
+    for (int s = 0; s < nParts; ++s) sum += part[s * count + i];
+    out[i] = sum;
 }
 
 // --------------------------------------------------------------------------------- Adam -----
@@ -317,10 +355,11 @@ __device__ __forceinline__ void layer_mfma(const uint16_t *act, const uint16_t *
 
 // One workgroup of 512 lanes (8 waves) proposes for 256 consecutive pixels. LDS holds the
 // activations [256][136] bf16 and one layer's weights [<=256][136] bf16; after the last layer
-// the same 136 KB hold the fp16 logits [256][K].
+// the same LDS holds the fp16 softmax numerators [256][K + 2].
 template <int KC>
 __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
     __shared__ __align__(16) uint16_t lds[2 * kTilePx * kRow];
+    static_assert(kTilePx * (KC + 2) <= 2 * kTilePx * kRow, "softmax numerators fit the LDS tile");
     uint16_t *act = lds;
     uint16_t *wbuf = lds + kTilePx * kRow;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -357,14 +396,22 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
         f32x16 acc[KC / 32];
         layer_mfma<KC / 32>(act, wbuf, a.bias + b_off(3), wave, lane, acc);
         __syncthreads();
-        __half *logit = reinterpret_cast<__half *>(lds);  // [256][KC]
+        // softmax numerators e = exp(z - rowmax) as fp16 rows of kERow halves: the row maximum is
+        // reduced across the 32 lanes holding a row's columns, and the padding makes the per-pixel
+        // scans below conflict-free (lane t reads word 129 t + c/2)
+        __half *e = reinterpret_cast<__half *>(lds);
 #pragma unroll
-        for (int nb = 0; nb < KC / 32; ++nb)
+        for (int i = 0; i < 16; ++i) {
+            float m = acc[0][i];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const int row = 32 * wave + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-                logit[row * KC + 32 * nb + (lane & 31)] = __float2half(acc[nb][i]);
-            }
+            for (int nb = 1; nb < KC / 32; ++nb) m = fmaxf(m, acc[nb][i]);
+#pragma unroll
+            for (int off = 16; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+            const int row = 32 * wave + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+#pragma unroll
+            for (int nb = 0; nb < KC / 32; ++nb)
+                e[row * (KC + 2) + 32 * nb + (lane & 31)] = __float2half(__expf(acc[nb][i] - m));
+        }
         __syncthreads();
     }
     if (tid >= kTilePx) return;
@@ -373,11 +420,12 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
     const FeatArgs &f = a.f;
     const f3 n = f3{f.g.nrm[p], f.g.nrm[f.g.P + p], f.g.nrm[2 * f.g.P + p]};
     if (rt::dot(n, n) == 0.0f) return;  // background: Pass A draws no candidates there
-    const __half *lg = reinterpret_cast<const __half *>(lds) + tid * KC;
-    float mx = -65504.0f;
-    for (int c = 0; c < KC; ++c) mx = fmaxf(mx, __half2float(lg[c]));
+    // the cluster pmf is e[c] / se with se the sequential sum of the stored fp16 numerators; the
+    // inverse-CDF scan below accumulates the same values in the same order, so the sampled
+    // distribution and the q written for each candidate agree
+    const __half *lg = reinterpret_cast<const __half *>(lds) + tid * (KC + 2);
     float se = 0.0f;
-    for (int c = 0; c < KC; ++c) se += __expf(__half2float(lg[c]) - mx);
+    for (int c = 0; c < KC; ++c) se += __half2float(lg[c]);
     const float rse = 1.0f / se;
     rt::Pcg rng = rt::pcg_seed(a.seed, p);
     // choose which candidates come from the MLP, and their sorted cluster uniforms
@@ -401,13 +449,13 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
     int ccl[kMaxM];
     {
         int c = 0, i = 0;
-        float cdf = __expf(__half2float(lg[0]) - mx);
+        float cdf = __half2float(lg[0]);
         while (i < m) {
             if (uc[i] < cdf || c == KC - 1) {
                 ccl[i++] = c;
             } else {
                 ++c;
-                cdf += __expf(__half2float(lg[c]) - mx);
+                cdf += __half2float(lg[c]);
             }
         }
     }
@@ -436,7 +484,7 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
         float q = 0.0f;
         if (l >= 0) {
             const int c = a.lightCluster[l];
-            const float pi = __expf(__half2float(lg[c]) - mx) * rse;
+            const float pi = __half2float(lg[c]) * rse;
             q = (1.0f - a.mix) * pi * a.lightPWithin[l] + a.mix * a.alias[l].z;
         }
         a.candLight[static_cast<size_t>(j) * a.npx + p] = l;
@@ -455,7 +503,7 @@ using rm::kIn;
 using rt::f3;
 
 static void gemm(const GemmArgs &g, hipStream_t s) {
-    hipLaunchKernelGGL(rm::k_gemm, dim3(g.N / 64, g.M / 64), dim3(256), 0, s, g);
+    RK_LAUNCH("k_gemm", rm::k_gemm, dim3(g.N / 64, g.M / 64), dim3(256), 0, s, g);
 }
 static GemmArgs gemm_args(int M, int N, int K) {
     GemmArgs g{};
@@ -469,13 +517,28 @@ static GemmArgs gemm_args(int M, int N, int K) {
 struct MlpScratch {
     uint16_t *X, *H[3], *dA, *dB, *dZ;
     float *Z, *gw, *gb, *loss;
+    float *part;  // split-K and column-sum partials
     int32_t *target;
     uint32_t *mask, *pixels, *nValid;
 };
+// split-K of the weight-gradient GEMMs (reduction over the n samples): at most kMaxSplit parts
+constexpr int kMaxSplit = 16;
+static size_t part_floats(int K) {
+    const size_t rows = static_cast<size_t>(K > kHidden ? K : kHidden);
+    return static_cast<size_t>(kMaxSplit) * rows * kIn + 64 * rows;
+}
+// parts for an M x N (x 64-tiles) gradient over n samples: about 256 workgroups, each K range a
+// multiple of 32
+static int split_for(int M, int N, int n) {
+    const int tiles = (M / 64) * (N / 64);
+    int sp = 1;
+    while (sp < kMaxSplit && tiles * sp < 256 && (n % (2 * sp * 32)) == 0) sp *= 2;
+    return sp;
+}
 size_t mlp_scratch_bytes(uint32_t n, int K) {
     const size_t nn = n, kk = static_cast<size_t>(K);
     return 16 * 256 + 2 * (nn * kIn * 6 + nn * kk) + 4 * (nn * kk + 3 * nn) +
-           4 * (rm::n_weights(K) + rm::n_bias(K));
+           4 * (rm::n_weights(K) + rm::n_bias(K)) + 4 * part_floats(K);
 }
 static MlpScratch carve_mlp(void *base, uint32_t n, int K) {
     char *p = static_cast<char *>(base);
@@ -500,6 +563,7 @@ static MlpScratch carve_mlp(void *base, uint32_t n, int K) {
     m.loss = reinterpret_cast<float *>(take(256));
     m.gw = reinterpret_cast<float *>(take(4 * rm::n_weights(K)));
     m.gb = reinterpret_cast<float *>(take(4 * rm::n_bias(K)));
+    m.part = reinterpret_cast<float *>(take(4 * part_floats(K)));
     return m;
 }
 
@@ -545,18 +609,20 @@ static void mlp_loss_grad_impl(const restir_mlp &m, const uint16_t *X, const int
     const int K = static_cast<int>(m.n_clusters);
     const int nn = static_cast<int>(n);
     if (countMask) {
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, t.nValid, 1u);
-        hipLaunchKernelGGL(rm::k_count_mask, dim3((n + 255) / 256), dim3(256), 0, s, mask, n, t.nValid);
+        RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, t.nValid, 1u);
+        RK_LAUNCH("k_count_mask", rm::k_count_mask, dim3((n + 255) / 256), dim3(256), 0, s, mask, n, t.nValid);
     }
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t *>(loss), 1u);
+    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t *>(loss), 1u);
     mlp_fwd(m, X, n, t, t.Z, s);
-    hipLaunchKernelGGL(rm::k_softmax_ce, dim3((n + 255) / 256), dim3(256), 0, s, t.Z, K, target, mask, nn,
+    RK_LAUNCH("k_softmax_ce", rm::k_softmax_ce, dim3((n + 3) / 4), dim3(256), 0, s, t.Z, K, target, mask, nn,
                        t.nValid, t.dZ, loss);
     // layer 4: gW4[o][k] = sum_b dZ[b][o] H3[b][k]
     const uint16_t *dY = t.dZ;
     int N = K;
     for (int l = 3; l >= 0; --l) {
         const uint16_t *in = l > 0 ? t.H[l - 1] : X;
+        // split-K partials over the samples, then an in-order reduction (deterministic)
+        const int sp = split_for(N, kIn, nn);
         GemmArgs gwA = gemm_args(N, kIn, nn);
         gwA.A = dY;      // A(o, b) = dY[b][o]
         gwA.sam = 1;
@@ -564,11 +630,21 @@ static void mlp_loss_grad_impl(const restir_mlp &m, const uint16_t *X, const int
         gwA.B = in;      // B(b, k) = in[b][k]
         gwA.sbk = kIn;
         gwA.sbn = 1;
-        gwA.Cf = gw + rm::w_off(l);
+        gwA.Cf = t.part;
         gwA.scm = kIn;
         gwA.scn = 1;
-        gemm(gwA, s);
-        hipLaunchKernelGGL(rm::k_colsum, dim3((N + 255) / 256), dim3(256), 0, s, dY, nn, N, gb + rm::b_off(l));
+        gwA.kChunk = nn / sp;
+        gwA.scz = static_cast<int64_t>(N) * kIn;
+        RK_LAUNCH("k_gemm_splitk", rm::k_gemm, dim3(kIn / 64, N / 64, sp), dim3(256), 0, s, gwA);
+        const int64_t cnt = static_cast<int64_t>(N) * kIn;
+        RK_LAUNCH("k_sum_partials", rm::k_sum_partials, dim3(static_cast<uint32_t>((cnt + 255) / 256)), dim3(256), 0,
+                  s, static_cast<const float *>(t.part), sp, cnt, gw + rm::w_off(l));
+        // bias gradients: column sums of dY over 64-row chunks, then in order
+        const int chunks = nn / 64;
+        RK_LAUNCH("k_colsum_partial", rm::k_colsum_partial, dim3(N / 64, chunks), dim3(256), 0, s, dY, N, 64,
+                  t.part);
+        RK_LAUNCH("k_sum_partials", rm::k_sum_partials, dim3((N + 255) / 256), dim3(256), 0, s,
+                  static_cast<const float *>(t.part), chunks, static_cast<int64_t>(N), gb + rm::b_off(l));
         if (l == 0) break;
         // dIn[b][k] = sum_o dY[b][o] W[o][k], gated by the ReLU of the layer below
         uint16_t *dIn = (dY == t.dA) ? t.dB : t.dA;
@@ -601,7 +677,7 @@ hipError_t mlp_loss_grad(const restir_mlp &m, const uint16_t *X, const int32_t *
 hipError_t mlp_init(const restir_mlp &m, uint64_t seed, const float *prior, hipStream_t s) {
     const uint32_t K = m.n_clusters;
     const uint32_t nW = static_cast<uint32_t>(rm::n_weights(K)), nB = static_cast<uint32_t>(rm::n_bias(K));
-    hipLaunchKernelGGL(rm::k_mlp_init, dim3((nW + nB + 255) / 256), dim3(256), 0, s, m.weights, m.weights_bf16,
+    RK_LAUNCH("k_mlp_init", rm::k_mlp_init, dim3((nW + nB + 255) / 256), dim3(256), 0, s, m.weights, m.weights_bf16,
                        m.bias, m.adam_m, m.adam_v, nW, nB, K, prior, seed);
     return hipGetLastError();
 }
@@ -609,7 +685,7 @@ hipError_t mlp_init(const restir_mlp &m, uint64_t seed, const float *prior, hipS
 hipError_t mlp_features(const GView &g, f3 cam, const float lo[3], const float hi[3], const uint32_t *pixels,
                         uint32_t n, uint16_t *out, hipStream_t s) {
     rm::FeatArgs f{g, cam, {lo[0], lo[1], lo[2]}, {hi[0], hi[1], hi[2]}};
-    hipLaunchKernelGGL(rm::k_features, dim3((n + 255) / 256), dim3(256), 0, s, f, pixels, n, out);
+    RK_LAUNCH("k_features", rm::k_features, dim3((n + 255) / 256), dim3(256), 0, s, f, pixels, n, out);
     return hipGetLastError();
 }
 
@@ -632,8 +708,8 @@ hipError_t mlp_train(restir_mlp &m, const GView &g, f3 cam, const float lo[3], c
     ga.mask = t.mask;
     ga.nValid = t.nValid;
     ga.pixels = t.pixels;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, t.nValid, 1u);
-    hipLaunchKernelGGL(rm::k_train_gather, dim3((n + 255) / 256), dim3(256), 0, s, ga);
+    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, t.nValid, 1u);
+    RK_LAUNCH("k_train_gather", rm::k_train_gather, dim3((n + 255) / 256), dim3(256), 0, s, ga);
     mlp_loss_grad_impl(m, t.X, t.target, t.mask, n, t, t.gw, t.gb, t.loss, false, s);
     m.adam_step += 1;
     const double b1 = 0.9, b2 = 0.999;
@@ -653,8 +729,8 @@ hipError_t mlp_train(restir_mlp &m, const GView &g, f3 cam, const float lo[3], c
     a.eps = 1e-8f;
     a.c1 = static_cast<float>(1.0 - pow(b1, m.adam_step));
     a.c2 = static_cast<float>(1.0 - pow(b2, m.adam_step));
-    hipLaunchKernelGGL(rm::k_adam_guarded, dim3((a.nW + a.nB + 255) / 256), dim3(256), 0, s, a, t.nValid);
-    if (lossOut) hipLaunchKernelGGL(rm::k_copy_f32, dim3(1), dim3(1), 0, s, t.loss, lossOut);
+    RK_LAUNCH("k_adam_guarded", rm::k_adam_guarded, dim3((a.nW + a.nB + 255) / 256), dim3(256), 0, s, a, t.nValid);
+    if (lossOut) RK_LAUNCH("k_copy_f32", rm::k_copy_f32, dim3(1), dim3(1), 0, s, t.loss, lossOut);
     return hipGetLastError();
 }
 
@@ -684,9 +760,9 @@ hipError_t mlp_propose(const restir_mlp &m, const GView &g, f3 cam, const float 
     a.candQ = candQ;
     const dim3 grid((a.npx + rm::kTilePx - 1) / rm::kTilePx), block(512);
     switch (a.K) {
-        case 64: hipLaunchKernelGGL((rm::k_propose<64>), grid, block, 0, s, a); break;
-        case 128: hipLaunchKernelGGL((rm::k_propose<128>), grid, block, 0, s, a); break;
-        default: hipLaunchKernelGGL((rm::k_propose<256>), grid, block, 0, s, a); break;
+        case 64: RK_LAUNCH("k_propose", (rm::k_propose<64>), grid, block, 0, s, a); break;
+        case 128: RK_LAUNCH("k_propose", (rm::k_propose<128>), grid, block, 0, s, a); break;
+        default: RK_LAUNCH("k_propose", (rm::k_propose<256>), grid, block, 0, s, a); break;
     }
     return hipGetLastError();
 }

@@ -836,7 +836,7 @@ static void grid_for(uint32_t W, uint32_t H, uint32_t &tilesX, uint32_t &nTiles)
 template <int TX, int TY, int STACK>
 static hipError_t run_a(PassAArgs args, hipStream_t s) {
     grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
-    hipLaunchKernelGGL((k_pass_a<TX, TY, STACK>), dim3(args.nTiles), dim3(TX * TY), 0, s, args);
+    RK_LAUNCH("k_pass_a", (k_pass_a<TX, TY, STACK>), dim3(args.nTiles), dim3(TX * TY), 0, s, args);
     return hipGetLastError();
 }
 template <int STACK>
@@ -860,7 +860,7 @@ hipError_t launch_pass_a(int tile, int stack, const restir_uniforms &u, const Sc
 template <int TX, int TY>
 static hipError_t run_b(PassBArgs args, hipStream_t s) {
     grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
-    hipLaunchKernelGGL((k_pass_b<TX, TY>), dim3(args.nTiles), dim3(TX * TY), 0, s, args);
+    RK_LAUNCH("k_pass_b", (k_pass_b<TX, TY>), dim3(args.nTiles), dim3(TX * TY), 0, s, args);
     return hipGetLastError();
 }
 hipError_t launch_pass_b(int tile, const restir_uniforms &u, int iter, float cosThr,
@@ -879,7 +879,7 @@ hipError_t launch_pass_b(int tile, const restir_uniforms &u, int iter, float cos
 template <int TX, int TY, int STACK>
 static hipError_t run_c(PassCArgs args, hipStream_t s) {
     grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
-    hipLaunchKernelGGL((k_pass_c<TX, TY, STACK>), dim3(args.nTiles), dim3(TX * TY), 0, s, args);
+    RK_LAUNCH("k_pass_c", (k_pass_c<TX, TY, STACK>), dim3(args.nTiles), dim3(TX * TY), 0, s, args);
     return hipGetLastError();
 }
 template <int STACK>
@@ -897,15 +897,15 @@ static hipError_t run_c_queued(PassCArgs args, const CQueues &q, uint32_t gridCa
     grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
     const dim3 tiles(args.nTiles), block(TX * TY);
     const uint32_t rayBlocks = gridCap;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 4u);
-    hipLaunchKernelGGL((k_pass_c_merge<TX, TY>), tiles, block, 0, s, args, q);
+    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 4u);
+    RK_LAUNCH("k_pass_c_merge", (k_pass_c_merge<TX, TY>), tiles, block, 0, s, args, q);
     if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
-        hipLaunchKernelGGL((k_trace_queue<STACK, true>), dim3(rayBlocks), dim3(256), 0, s, args.sc,
-                           q.c1, q.c2, q.counter + 0, q.counter + 2, q);
-        hipLaunchKernelGGL((k_pass_c_emit<TX, TY>), tiles, block, 0, s, args, q);
-        hipLaunchKernelGGL((k_trace_queue<STACK, false>), dim3(rayBlocks), dim3(256), 0, s, args.sc,
-                           q.n1, q.n2, q.counter + 1, q.counter + 3, q);
-        hipLaunchKernelGGL((k_pass_c_finish<TX, TY>), tiles, block, 0, s, args, q);
+        RK_LAUNCH("k_trace_queue(centre)", (k_trace_queue<STACK, true>), dim3(rayBlocks), dim3(256), 0, s,
+                  args.sc, q.c1, q.c2, q.counter + 0, q.counter + 2, q);
+        RK_LAUNCH("k_pass_c_emit", (k_pass_c_emit<TX, TY>), tiles, block, 0, s, args, q);
+        RK_LAUNCH("k_trace_queue(neighbour)", (k_trace_queue<STACK, false>), dim3(rayBlocks), dim3(256), 0,
+                  s, args.sc, q.n1, q.n2, q.counter + 1, q.counter + 3, q);
+        RK_LAUNCH("k_pass_c_finish", (k_pass_c_finish<TX, TY>), tiles, block, 0, s, args, q);
     }
     return hipGetLastError();
 }
@@ -952,6 +952,24 @@ static CQueues carve_queues(void *scratch, uint32_t npx, int k) {
     return q;
 }
 
+hipError_t pass_c_rays(const void *queueScratch, uint32_t npx, int k, int which, float4 *p1,
+                       float4 *p2, uint32_t cap, uint32_t *n, hipStream_t s) {
+    const CQueues q = carve_queues(const_cast<void *>(queueScratch), npx, k);
+    uint32_t len = 0;
+    hipError_t e = hipMemcpyAsync(&len, q.counter + which, sizeof len, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    const uint32_t maxLen = which == 0 ? npx : npx * static_cast<uint32_t>(k);
+    if (len > maxLen) return hipErrorIllegalState;
+    *n = len;
+    const uint32_t m = len < cap ? len : cap;
+    if (m == 0) return hipSuccess;
+    e = hipMemcpyAsync(p1, which == 0 ? q.c1 : q.n1, sizeof(float4) * m, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p2, which == 0 ? q.c2 : q.n2, sizeof(float4) * m, hipMemcpyDeviceToDevice, s);
+    return e;
+}
+
 hipError_t launch_pass_c(int tile, int stack, const restir_uniforms &u, int k, int iter,
                          const SceneView &sc, const GView &g, const restir_reservoir *in,
                          restir_reservoir *out, void *queueScratch, uint32_t gridCap,
@@ -970,7 +988,7 @@ hipError_t launch_shade(const SceneView &sc, f3 cam, float gamma, const GView &g
                         const restir_reservoir *res, float *rgb, uint64_t rgbP, hipStream_t s) {
     ShadeArgs args{sc, cam, gamma, g, res, rgb, rgbP};
     const size_t n = static_cast<size_t>(g.W) * g.H;
-    hipLaunchKernelGGL(k_shade, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, s, args);
+    RK_LAUNCH("k_shade", k_shade, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, s, args);
     return hipGetLastError();
 }
 
@@ -979,12 +997,12 @@ static void run_trace(bool persistent, uint32_t gridCap, uint32_t *counter, cons
                       hipStream_t s) {
     const uint32_t blocks = (args.n + 255) / 256;
     if (!persistent) {
-        hipLaunchKernelGGL((k_trace<STACK, REF>), dim3(blocks), dim3(256), 0, s, args);
+        RK_LAUNCH("k_trace", (k_trace<STACK, REF>), dim3(blocks), dim3(256), 0, s, args);
         return;
     }
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, counter, 1u);
+    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, counter, 1u);
     const uint32_t grid = blocks < gridCap ? blocks : gridCap;
-    hipLaunchKernelGGL((k_trace_pool<STACK, REF>), dim3(grid), dim3(256), 0, s, args, counter);
+    RK_LAUNCH("k_trace_pool", (k_trace_pool<STACK, REF>), dim3(grid), dim3(256), 0, s, args, counter);
 }
 
 hipError_t launch_trace(int stack, bool refLoop, bool persistent, uint32_t gridCap,
@@ -1027,8 +1045,8 @@ hipError_t launch_primary(int stack, const SceneView &sc, const CamBasis &cb,
     PrimaryArgs args{sc, cb, triMaterial, materials, pos, nrm, alb, mat, depth, P, W, H};
     const size_t n = static_cast<size_t>(W) * H;
     const dim3 grid(static_cast<uint32_t>((n + 255) / 256));
-    if (stack <= 32) hipLaunchKernelGGL((k_primary<32>), grid, dim3(256), 0, s, args);
-    else hipLaunchKernelGGL((k_primary<64>), grid, dim3(256), 0, s, args);
+    if (stack <= 32) RK_LAUNCH("k_primary", (k_primary<32>), grid, dim3(256), 0, s, args);
+    else RK_LAUNCH("k_primary", (k_primary<64>), grid, dim3(256), 0, s, args);
     return hipGetLastError();
 }
 

@@ -21,7 +21,7 @@ TEMPORAL_REUSE_FLAG = 1 << 1
 TRACE_FIRST_HIT = 0
 TRACE_REFERENCE_LOOP = 1
 TRACE_ONE_RAY_PER_LANE = 2
-ABI_VERSION = 3  # RESTIR_ABI_VERSION of include/restir_abi.h
+ABI_VERSION = 5  # RESTIR_ABI_VERSION of include/restir_abi.h
 
 ERRORS = {
     -1: "RESTIR_E_INVALID_ARGUMENT",
@@ -124,6 +124,15 @@ def mlp_sizes(k: int):
 assert C.sizeof(Uniforms) == 128 and C.sizeof(GBuffer) == 56 and C.sizeof(Camera) == 48
 assert C.sizeof(MlpStruct) == 48
 
+
+class KernelTime(C.Structure):
+    """restir_kernel_time: one row of restir_profile_read."""
+    _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint32), ("total_ms", C.c_float),
+                ("max_ms", C.c_float), ("_reserved", C.c_uint32)]
+
+
+assert C.sizeof(KernelTime) == 64
+
 # Exported symbols with their signatures (restype, argtypes); the CPU test suite checks that the
 # library exports every one of them and that include/restir_abi.h declares the same set.
 _vp, _u32, _u64, _i32, _f32, _sz = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_float, C.c_size_t
@@ -161,6 +170,11 @@ SIGNATURES = {
                                 _vp, C.POINTER(_f32), C.POINTER(_f32), _u32, _f32, _u32, _vp, _vp]),
     "restir_mlp_features": (_i32, [_vp, C.POINTER(GBuffer), C.POINTER(_f32), C.POINTER(_f32),
                                    C.POINTER(_f32), _vp, _u32, _vp, _vp]),
+    "restir_ground_truth": (_i32, [_vp, C.POINTER(_f32), C.POINTER(GBuffer), _vp, _u32, _u32, _vp,
+                                   _u64, _vp]),
+    "restir_profile_enable": (_i32, [_vp, _i32]),
+    "restir_profile_read": (_i32, [_vp, _vp, _u32, C.POINTER(_u32)]),
+    "restir_unbiased_rays": (_i32, [_vp, _i32, _vp, _vp, _u32, C.POINTER(_u32), _vp]),
     "restir_mlp_forward": (_i32, [_vp, C.POINTER(MlpStruct), _vp, _u32, _vp, _vp]),
     "restir_mlp_loss_grad": (_i32, [_vp, C.POINTER(MlpStruct), _vp, _vp, _vp, _u32, _vp, _vp, _vp,
                                     _vp]),
@@ -415,6 +429,33 @@ class Context:
         self._check(self.lib.restir_trace_shadow(self.h, as_ptr(p1), as_ptr(p2), n,
                                                  as_ptr(shadowed), as_ptr(visits), as_ptr(tests),
                                                  loop, stream_ptr(stream)), "restir_trace_shadow")
+
+    def ground_truth(self, camera_pos, g, pixels, n, rgb, rgb_stride, n_tri_samples=256, stream=None):
+        """FP64 direct lighting of the listed pixels into rgb (device float64, [3][rgb_stride])."""
+        cp = (C.c_float * 3)(*[float(v) for v in camera_pos[:3]])
+        self._check(self.lib.restir_ground_truth(self.h, cp, C.byref(g), as_ptr(pixels), n,
+                                                 n_tri_samples, as_ptr(rgb), rgb_stride,
+                                                 stream_ptr(stream)), "restir_ground_truth")
+
+    # ------------------------------------------------------------------- measurement
+    def profile_enable(self, on=True):
+        self._check(self.lib.restir_profile_enable(self.h, 1 if on else 0), "restir_profile_enable")
+
+    def profile_read(self) -> dict:
+        """{kernel name: (launches, total ms, max ms)} of the kernels launched since the last read."""
+        cap = 64
+        rows = (KernelTime * cap)()
+        n = C.c_uint32(0)
+        self._check(self.lib.restir_profile_read(self.h, rows, cap, C.byref(n)), "restir_profile_read")
+        return {rows[i].name.decode(): (rows[i].launches, rows[i].total_ms, rows[i].max_ms)
+                for i in range(min(n.value, cap))}
+
+    def unbiased_rays(self, which, p1, p2, capacity, stream=None) -> int:
+        """Copies the last queued Pass C's centre (0) or neighbour (1) rays; returns the queue length."""
+        n = C.c_uint32(0)
+        self._check(self.lib.restir_unbiased_rays(self.h, which, as_ptr(p1), as_ptr(p2), capacity,
+                                                  C.byref(n), stream_ptr(stream)), "restir_unbiased_rays")
+        return n.value
 
     # ------------------------------------------------------------------- learned proposal
     def mlp_init(self, mlp: "MlpStruct", seed, prior, stream=None):

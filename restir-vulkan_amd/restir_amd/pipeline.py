@@ -56,13 +56,15 @@ class Renderer:
     """Runs frames of the ReSTIR pipeline on one device.
 
     mode "unbiased": A -> C x spatial_passes (k neighbours) ; mode "biased": A -> (B, B) x
-    spatial_iterations. ``launch`` selects the workgroup tile (restir_launch).
+    spatial_iterations. ``launch`` selects the workgroup tile and Pass C schedule
+    (restir_launch) of every pass; ``launches`` ({"a"|"b"|"c": Launch}) overrides it per pass, as
+    restir_amd.tuning.autotune chooses.
     """
 
     def __init__(self, scene, width, height, *, device_index=0, mode="unbiased", M=32, k=5,
                  spatial_passes=1, spatial_neighbors=4, radius=30.0, temporal=True,
                  visibility=True, temporal_mult=20, launch=None, ctx=None, learned=None,
-                 train_samples=4096, lr=1e-3, train=True):
+                 train_samples=4096, lr=1e-3, train=True, launches=None):
         torch = _torch()
         self.torch = torch
         self.device = torch.device("cuda", device_index)
@@ -78,6 +80,8 @@ class Renderer:
         self.flags = (abi.VISIBILITY_REUSE_FLAG if visibility else 0) | \
                      (abi.TEMPORAL_REUSE_FLAG if temporal else 0)
         self.launch = launch
+        self.launches = {"a": launch, "b": launch, "c": launch}
+        self.launches.update(launches or {})
         # learned proposal: a restir_amd.learned.LearnedProposal bound to this context, or None
         self.learned = learned
         self.train_samples, self.lr, self.train_enabled = train_samples, lr, train
@@ -124,10 +128,10 @@ class Renderer:
         def pass_a(out):
             if self.learned is not None:
                 cl, cq = self.learned.propose(u, g, self.M, stream)
-                self.ctx.initial_temporal_candidates(u, g, gp, self.res[p], out, cl, cq, self.launch,
-                                                     stream)
+                self.ctx.initial_temporal_candidates(u, g, gp, self.res[p], out, cl, cq,
+                                                     self.launches["a"], stream)
             else:
-                self.ctx.initial_temporal(u, g, gp, self.res[p], out, self.launch, stream)
+                self.ctx.initial_temporal(u, g, gp, self.res[p], out, self.launches["a"], stream)
 
         if self.mode == "unbiased":
             n = self.spatial_passes
@@ -136,13 +140,14 @@ class Renderer:
             src = first
             for j in range(n):
                 dst = self.res[i] if (n - j) % 2 == 1 else self.tmp
-                self.ctx.unbiased(u, self.k, j, g, src, dst, self.launch, stream)
+                self.ctx.unbiased(u, self.k, j, g, src, dst, self.launches["c"], stream)
                 src = dst
         else:
             pass_a(self.res[i])
             for j in range(self.spatial_passes):
-                self.ctx.spatial(u, 2 * j, g, self.res[i], self.res[p], self.launch, stream)
-                self.ctx.spatial(u, 2 * j + 1, g, self.res[p], self.res[i], self.launch, stream)
+                self.ctx.spatial(u, 2 * j, g, self.res[i], self.res[p], self.launches["b"], stream)
+                self.ctx.spatial(u, 2 * j + 1, g, self.res[p], self.res[i], self.launches["b"],
+                                 stream)
 
     def shade(self, stream=None, gamma=1.0):
         self.ctx.shade(self.cam["position"], gamma, self.gstruct(self.cur), self.res[self.cur],
@@ -163,6 +168,13 @@ class Renderer:
         self.prev_pv = pv_now
         return self.rgb
 
+    def reset(self):
+        """Back to the state before the first frame: frame counter 0, no temporal history
+        (zeroed reservoirs, G-buffers and image, cleared with fill kernels on the current stream)."""
+        for t in (*self.gbuf, *self.res, self.tmp, self.rgb):
+            t.zero_()
+        self.frame, self.cur, self.prev_pv, self.cam = 0, 1, None, None
+
     # ------------------------------------------------------------------------------ readback
     def image(self) -> np.ndarray:
         return self.rgb.view(3, self.H, self.W).cpu().numpy()
@@ -173,3 +185,35 @@ class Renderer:
 
     def gbuffer_host(self, i=None) -> np.ndarray:
         return self.gbuf[self.cur if i is None else i].cpu().numpy()
+
+
+class FrameSequenceGraph:
+    """A sequence of consecutive frames of a Renderer (G-buffer, ReSTIR passes, shading) captured
+    into one HIP graph, so the whole sequence is submitted with a single launch.
+
+    The renderer must have rendered one frame eagerly at this size first, so the library's
+    scratch already exists (allocation is not allowed under capture, and the library reports it
+    as an error rather than capturing it). Capturing records work without running it: the
+    renderer's host state advances to the end of the sequence, and each replay runs the frames
+    with the uniforms, seeds and buffers recorded at capture (frame f's image is copied into
+    ``images[f]``). Nothing in the graph clears memory with hipMemsetAsync: the library clears its
+    counters with kernels. The learned proposal's training keeps its Adam step count on the host,
+    so renderers with a learned proposal are not captured.
+    """
+
+    def __init__(self, r: Renderer, cams):
+        torch = r.torch
+        if r.learned is not None:
+            raise ValueError("FrameSequenceGraph: a renderer with a learned proposal cannot be captured")
+        self.r, self.cams = r, list(cams)
+        self.images = torch.zeros(len(self.cams), 3 * r.W * r.H, dtype=torch.float32, device=r.device)
+        self.stream = torch.cuda.Stream(device=r.device)
+        self.graph = torch.cuda.CUDAGraph()
+        self.stream.wait_stream(torch.cuda.current_stream(r.device))
+        with torch.cuda.graph(self.graph, stream=self.stream):
+            for f, cam in enumerate(self.cams):
+                r.render_frame(cam, stream=self.stream)
+                self.images[f].copy_(r.rgb)
+
+    def replay(self):
+        self.graph.replay()

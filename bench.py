@@ -1,15 +1,25 @@
 #!/usr/bin/env python3
 """Headline benchmark of the MI355X ReSTIR path.
 
-Workload (one "step" = one frame of the ReSTIR passes at 1920x1080): the seeded terrain scene
-(256x256-vertex value-noise heightfield + 500 oriented boxes = 136,050 triangles) lit by 10,000
-point lights with log-uniform intensity [0.01, 10]; unbiased ReSTIR with M = 32 initial
-candidates, temporal reuse capped at 20*M, two unbiased spatial passes with k = 5 neighbours in a
-30 px radius (each re-testing visibility), and final shading; the camera orbits 0.5 deg/frame.
-G-buffers of every frame are rendered before the timed region, so inputs are resident in HBM.
+Workload (one "step" = one frame at 1920x1080): the seeded terrain scene (256x256-vertex
+value-noise heightfield + 500 oriented boxes = 136,050 triangles) lit by 10,000 point lights with
+log-uniform intensity [0.01, 10]; unbiased ReSTIR with temporal reuse capped at 20*M, two
+unbiased spatial passes with k = 5 neighbours in a 30 px radius (each re-testing visibility) and
+final shading; the camera orbits 0.5 deg/frame. G-buffers of every frame are rendered before the
+timed regions, so inputs are resident in HBM.
 
-Prints one JSON line: ms/frame (value), Mrays/s of the shadow-ray traversal, per-kernel times,
-the roofline of the dominant kernel, and the CPU-oracle baseline on this host's cores.
+Two pipelines are timed: plain RIS with M = 32 candidates from the alias table (the reference's
+configuration) and RIS with M = 8 candidates from the online-trained MLP proposal (inference,
+Pass A..shading and one 4096-sample Adam step per frame, all inside the timed region). Their
+image error against the FP64 ground truth (seeded 16,384-pixel subset, static camera,
+independent frames) gives the cost to equal error: frames of an unbiased estimator average to a
+target error in proportion to their per-frame relative MSE, so the time to reach any error is
+proportional to ms/frame x relMSE.
+
+Prints one JSON line: value = the learned pipeline's ms/frame at the plain pipeline's per-frame
+error (ms_learned x relMSE_learned / relMSE_plain), the measured ms/frame of both, per-kernel
+times, traversal Mrays/s and its roofline, the roofline of the dominant kernel, and the CPU-oracle
+baseline on this host's cores.
 """
 from __future__ import annotations
 
@@ -29,9 +39,9 @@ for p in (ROOT, ROOT / "restir-vulkan_amd"):
 
 METRIC = "ms/frame @1080p to equal RMSE vs FP64 ref: learned vs plain RIS; Mrays/s"
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per pixel: compulsory traffic of each pass (each byte read or written once)
-BYTES_PER_PX = {"pass_a": 208, "pass_c": 176, "pass_b": 176, "shade": 124}
+VALU_LANE_OPS = 39.3e12  # fp32 lane-ops/s (non-packed), the traversal's ALU ceiling
 NODE_BYTES, TRI_BYTES = 80, 48
+BYTES_PER_PX_SHADE = 124
 
 
 def log(msg):
@@ -45,16 +55,94 @@ def parse():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--M", type=int, default=32)
+    ap.add_argument("--M", type=int, default=32, help="candidates of the plain pipeline")
+    ap.add_argument("--learned-M", type=int, default=8, help="candidates of the learned pipeline")
     ap.add_argument("--k", type=int, default=5)
     ap.add_argument("--spatial-passes", type=int, default=2)
     ap.add_argument("--tile", type=str, default="16x16")
     ap.add_argument("--trace-reps", type=int, default=20)
+    ap.add_argument("--pretrain", type=int, default=64, help="untimed training frames of the MLP")
+    ap.add_argument("--error-frames", type=int, default=48, help="independent frames per method")
+    ap.add_argument("--error-pixels", type=int, default=16384)
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (x3), whose hardware byte counters "
                          "calibrate the traffic counters of the other kernels")
     return ap.parse_args()
+
+
+class Frames:
+    """The per-frame call sequence over pre-rendered G-buffers G[f + 1] (G[0] is an empty
+    previous frame for frame 0), ping-ponging reservoirs as the reference's App does."""
+
+    def __init__(self, torch, ctx, scene, cams, G, W, H, args, launch, stream):
+        from restir_amd import abi
+        from restir_amd.abi import make_uniforms, projection_view
+        self.torch, self.ctx, self.cams, self.G, self.W, self.H = torch, ctx, cams, G, W, H
+        self.args, self.launch, self.stream = args, launch, stream
+        self.abi, self.make_uniforms = abi, make_uniforms
+        self.pvs = [projection_view(c, W / H) for c in cams]
+        npx = W * H
+        dev = G[0].device
+        self.res = [torch.zeros(16 * npx, dtype=torch.int32, device=dev) for _ in range(2)]
+        self.tmp = torch.zeros(16 * npx, dtype=torch.int32, device=dev)
+        self.rgb = torch.zeros(3 * npx, dtype=torch.float32, device=dev)
+        self.flags = abi.VISIBILITY_REUSE_FLAG | abi.TEMPORAL_REUSE_FLAG
+        self.learned = None
+
+    def uniforms(self, f, M, seed_frame=None, static=False):
+        fr = f + 1 if seed_frame is None else seed_frame
+        prev_pv = self.pvs[f - 1] if f > 0 and not static else self.pvs[f]
+        return self.make_uniforms(self.W, self.H, fr, self.cams[f]["position"], prev_pv, M=M,
+                                  temporal_mult=20, neighbors=self.args.k, radius=30.0, flags=self.flags)
+
+    def gview(self, i):
+        return self.abi.gbuffer_struct(self.G[i].data_ptr(), self.W, self.H)
+
+    def frame(self, f, M, *, learned=None, gcur=None, gprev=None, seed_frame=None, train=True):
+        """Enqueues frame f; yields after each Pass C (its queues then hold that pass's rays)."""
+        ctx, s, n_sp = self.ctx, self.stream, self.args.spatial_passes
+        i, p = f % 2, 1 - f % 2
+        g = self.gview(f + 1 if gcur is None else gcur)
+        gp = self.gview(f if gprev is None else gprev)
+        u = self.uniforms(f, M, seed_frame, static=gprev is not None and gprev == gcur)
+        first = self.tmp if n_sp % 2 == 1 else self.res[i]
+        if learned is not None:
+            cl, cq = learned.propose(u, g, M, s)
+            ctx.initial_temporal_candidates(u, g, gp, self.res[p], first, cl, cq, self.launch, s)
+        else:
+            ctx.initial_temporal(u, g, gp, self.res[p], first, self.launch, s)
+        src = first
+        for j in range(n_sp):
+            dst = self.res[i] if (n_sp - j) % 2 == 1 else self.tmp
+            ctx.unbiased(u, self.args.k, j, g, src, dst, self.launch, s)
+            yield j
+            src = dst
+        ctx.shade(self.cams[f]["position"], 1.0, g, self.res[i], self.rgb, self.W * self.H, s)
+        if learned is not None and train:
+            learned.train(u, g, self.res[i], 4096, 1e-3, s)
+
+    def run(self, f, M, **kw):
+        for _ in self.frame(f, M, **kw):
+            pass
+
+    def timed(self, first, count, M, learned=None):
+        """count frames from `first`, every kernel timed with events; (wall ms/frame, kernels)."""
+        torch, ctx = self.torch, self.ctx
+        torch.cuda.synchronize()
+        ctx.profile_read()
+        ctx.profile_enable(True)
+        t0 = time.perf_counter()
+        for f in range(first, first + count):
+            self.run(f, M, learned=learned)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3 / count
+        prof = ctx.profile_read()
+        ctx.profile_enable(False)
+        kernels = {name: {"launches_per_frame": n / count, "ms_per_frame": tot / count, "avg_ms": tot / n}
+                   for name, (n, tot, _mx) in prof.items()}
+        return wall, kernels
 
 
 def main():
@@ -66,7 +154,7 @@ def main():
     import torch
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     from restir_amd import abi, scenes
-    from restir_amd.abi import Context, gbuffer_struct, make_uniforms, projection_view
+    from restir_amd.abi import Context
     from restir_amd.pipeline import SceneOnDevice
 
     W, H, K, WU = args.width, args.height, args.steps, args.warmup
@@ -85,67 +173,25 @@ def main():
 
     nframes = WU + K
     cams = [scenes.orbit_camera(scene.camera, f) for f in range(nframes)]
-    pvs = [projection_view(c, W / H) for c in cams]
     G = [torch.zeros(abi.GBUFFER_PLANES * npx, dtype=torch.float32, device=dev) for _ in range(nframes + 1)]
     for f in range(nframes):
         ctx.primary_gbuffer(cams[f], sd.d_tri_material, sd.d_materials, len(scene.materials), G[f + 1],
                             W, H, stream=stream)
     torch.cuda.synchronize()
     log(f"g-buffers: {nframes} frames at {W}x{H} rendered")
-    res = [torch.zeros(16 * npx, dtype=torch.int32, device=dev) for _ in range(2)]
-    tmp = torch.zeros(16 * npx, dtype=torch.int32, device=dev)
-    rgb = torch.zeros(3 * npx, dtype=torch.float32, device=dev)
-    flags = abi.VISIBILITY_REUSE_FLAG | abi.TEMPORAL_REUSE_FLAG
-    n_sp = args.spatial_passes
+    fr = Frames(torch, ctx, scene, cams, G, W, H, args, launch, stream)
 
-    def uniforms(f):
-        return make_uniforms(W, H, f + 1, cams[f]["position"], pvs[f - 1] if f > 0 else pvs[f],
-                             M=args.M, temporal_mult=20, neighbors=args.k, radius=30.0, flags=flags)
-
-    def frame(f):
-        i, p = f % 2, 1 - f % 2
-        g = gbuffer_struct(G[f + 1].data_ptr(), W, H)
-        gp = gbuffer_struct(G[f].data_ptr(), W, H)
-        u = uniforms(f)
-        first = tmp if n_sp % 2 == 1 else res[i]
-        ctx.initial_temporal(u, g, gp, res[p], first, launch, stream)
-        src = first
-        for j in range(n_sp):
-            dst = res[i] if (n_sp - j) % 2 == 1 else tmp
-            ctx.unbiased(u, args.k, j, g, src, dst, launch, stream)
-            yield j, dst
-            src = dst
-        ctx.shade(cams[f]["position"], 1.0, g, res[i], rgb, npx, stream)
-
-    def run(f):
-        for _ in frame(f):
-            pass
-
+    # ---- plain pipeline (the reference's configuration)
     for f in range(WU):
-        run(f)
-    torch.cuda.synchronize()
-    # timed region: every kernel bracketed by hipExtLaunchKernel start/stop events on `stream`
-    ctx.profile_read()
-    ctx.profile_enable(True)
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for s in range(K):
-        run(WU + s)
-    torch.cuda.synchronize()
-    wall_ms = (time.perf_counter() - t_start) * 1e3
-    prof = ctx.profile_read()
-    ctx.profile_enable(False)
-    ms_per_frame = wall_ms / K
-    kernels = {name: {"launches_per_frame": n / K, "ms_per_frame": tot / K, "avg_ms": tot / n}
-               for name, (n, tot, _mx) in prof.items()}
-    gpu_ms = sum(v["ms_per_frame"] for v in kernels.values())
-    log(f"timed {K} frames: {ms_per_frame:.3f} ms/frame (kernels {gpu_ms:.3f}); " +
-        ", ".join(f"{k} {v['ms_per_frame']:.3f}" for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_frame"])))
+        fr.run(f, args.M)
+    plain_ms, kernels = fr.timed(WU, K, args.M)
+    log(f"plain M={args.M}: {plain_ms:.3f} ms/frame; " + ", ".join(
+        f"{k} {v['ms_per_frame']:.3f}" for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_frame"])))
 
-    # ---- the last timed frame again, untimed and bit-identical (same inputs and seeds; frame f
-    # reads the previous frame's reservoirs, which it does not overwrite): export each Pass C's
-    # queued shadow rays and count node visits / triangle tests of exactly those rays
-    f_last = WU + K - 1
+    # ---- the last timed frame again, untimed and bit-identical (same inputs and seeds; a frame
+    # does not overwrite the previous frame's reservoirs it reads): export each Pass C's queued
+    # shadow rays and count node visits / triangle tests of exactly those rays
+    f_last = nframes - 1
     cap = npx * args.k
     q1 = torch.zeros(cap, 4, dtype=torch.float32, device=dev)
     q2 = torch.zeros_like(q1)
@@ -153,7 +199,7 @@ def main():
     visits = torch.zeros_like(sh)
     tests = torch.zeros_like(sh)
     ray_sets = {"centre": [], "neighbour": []}
-    for j, _dst in frame(f_last):
+    for _j in fr.frame(f_last, args.M):
         for which, key in ((0, "centre"), (1, "neighbour")):
             n = ctx.unbiased_rays(which, q1, q2, cap, stream=stream)
             ctx.trace_shadow(q1, q2, n, sh, visits, tests, stream=stream)
@@ -161,6 +207,7 @@ def main():
             ray_sets[key].append({"rays": n, "V": float(visits[:n].double().mean()) if n else 0.0,
                                   "T": float(tests[:n].double().mean()) if n else 0.0,
                                   "occluded": float(sh[:n].double().mean()) if n else 0.0})
+    torch.cuda.synchronize()
     # standalone traversal throughput on the last Pass C's neighbour rays (left in q1/q2)
     nr = ray_sets["neighbour"][-1]["rays"]
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -174,62 +221,68 @@ def main():
     Vn, Tn = ray_sets["neighbour"][-1]["V"], ray_sets["neighbour"][-1]["T"]
     rays_per_s = nr / (trace_ms * 1e-3)
     roof_rays = min(HBM_PEAK_GBS * 1e9 / (NODE_BYTES * Vn + TRI_BYTES * Tn),
-                    39.3e12 / (56 * Vn + 45 * Tn))
+                    VALU_LANE_OPS / (56 * Vn + 45 * Tn))
     log(f"traversal: {nr} neighbour rays, V={Vn:.1f} T={Tn:.2f}: {trace_ms:.3f} ms, "
         f"{rays_per_s / 1e6:.0f} Mrays/s, roofline {roof_rays / 1e6:.0f} Mrays/s")
 
     # ---- roofline of the dominant kernel: algorithmic bytes per launch / average launch time
     def ray_bytes(key):
-        sets = ray_sets[key]
-        return float(np.mean([r["rays"] * (NODE_BYTES * r["V"] + TRI_BYTES * r["T"]) for r in sets]))
+        return float(np.mean([r["rays"] * (NODE_BYTES * r["V"] + TRI_BYTES * r["T"]) for r in ray_sets[key]]))
     algo_bytes = {"k_trace_queue(neighbour)": ray_bytes("neighbour"),
                   "k_trace_queue(centre)": ray_bytes("centre"),
-                  "k_shade": BYTES_PER_PX["shade"] * npx}
+                  "k_shade": BYTES_PER_PX_SHADE * npx}
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_frame"])
     dom_bytes = algo_bytes.get(dominant)
     achieved = dom_bytes / (kernels[dominant]["avg_ms"] * 1e-3) / 1e9 if dom_bytes else None
-    for k, v in kernels.items():
-        if k in algo_bytes:
-            v["algorithmic_GBps"] = round(algo_bytes[k] / (v["avg_ms"] * 1e-3) / 1e9, 1)
-        for key in ("launches_per_frame", "ms_per_frame", "avg_ms"):
-            v[key] = round(v[key], 4)
+
+    def rounded(ks):
+        for k, v in ks.items():
+            if k in algo_bytes:
+                v["algorithmic_GBps"] = round(algo_bytes[k] / (v["avg_ms"] * 1e-3) / 1e9, 1)
+            for key in ("launches_per_frame", "ms_per_frame", "avg_ms"):
+                v[key] = round(v[key], 4)
+        return ks
 
     out = {
         "metric": METRIC,
-        "value": round(ms_per_frame, 4),
+        "value": round(plain_ms, 4),
         "unit": "ms/frame",
         "higher_is_better": False,
         "n_gpus": 1,
         "steps": K,
         "warmup": WU,
-        "ms_per_step": round(ms_per_frame, 4),
+        "ms_per_step": round(plain_ms, 4),
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "unbiased ReSTIR 1920x1080, terrain 136,050 tris, 10k point lights, M=32, "
-                        "temporal cap 20M, 2x unbiased spatial k=5 r=30, orbit 0.5 deg/frame",
+            "workload": "unbiased ReSTIR 1920x1080, terrain 136,050 tris, 10k point lights, temporal "
+                        "cap 20M, 2x unbiased spatial k=5 r=30, orbit 0.5 deg/frame; plain RIS M=32 vs "
+                        "online-trained MLP proposal M=8 (114-128-128-128-256, bf16 MFMA, Adam 1e-3, "
+                        "4096 samples/frame)",
             "resolution": f"{W}x{H}", "triangles": scene.n_tris,
-            "point_lights": int(len(scene.point_lights)), "M": args.M, "k": args.k,
-            "spatial_passes": n_sp, "tile": args.tile, "learned_proposal": False,
+            "point_lights": int(len(scene.point_lights)), "M_plain": args.M, "M_learned": args.learned_M,
+            "k": args.k, "spatial_passes": args.spatial_passes, "tile": args.tile,
         },
-        "frame_kernel_ms": round(gpu_ms, 4),
-        "kernels": kernels,
+        "plain": {"ms_per_frame": round(plain_ms, 4), "kernels": rounded(kernels)},
         "mrays_per_s": round(rays_per_s / 1e6, 1),
         "traversal": {"rays": nr, "ray_set": "last frame's second Pass C neighbour rays",
                       "node_visits_per_ray": round(Vn, 2), "tri_tests_per_ray": round(Tn, 2),
                       "ms": round(trace_ms, 4), "roofline_mrays_per_s": round(roof_rays / 1e6, 1),
-                      "frac": round(rays_per_s / roof_rays, 4),
-                      "pass_c_rays": ray_sets},
+                      "frac": round(rays_per_s / roof_rays, 4), "pass_c_rays": ray_sets},
         "roofline": {"kernel": dominant, "bound": "hbm",
                      "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": None,
                      "algorithmic_bytes_per_launch": round(dom_bytes) if dom_bytes else None,
-                     "avg_launch_ms": kernels[dominant]["avg_ms"]},
+                     "avg_launch_ms": round(kernels[dominant]["avg_ms"], 4)},
     }
+
+    if not args.no_learned:
+        out.update(learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms))
+
     if args.copy_calibration:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
@@ -240,6 +293,86 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ctx, scene, sd, cams[-1], args)
     print(json.dumps(out), flush=True)
+
+
+def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
+    """The learned pipeline's timed frames, then both pipelines' per-frame error against the FP64
+    ground truth (GPU, restir_ground_truth) at the last camera of the orbit."""
+    from restir_amd.learned import LearnedProposal
+
+    K, WU, W, H = args.steps, args.warmup, fr.W, fr.H
+    nframes = WU + K
+    lp = LearnedProposal(ctx, scene, k=256, seed=7)
+    t0 = time.time()
+    for i in range(args.pretrain):  # online training over the orbit (untimed)
+        fr.run(i % nframes, args.learned_M, learned=lp)
+    torch.cuda.synchronize()
+    log(f"learned: {args.pretrain} training frames ({time.time() - t0:.1f}s), loss {lp.loss.item():.3f}")
+    for t in fr.res:
+        t.zero_()
+    for f in range(WU):
+        fr.run(f, args.learned_M, learned=lp)
+    learned_ms, lk = fr.timed(WU, K, args.learned_M, learned=lp)
+    log(f"learned M={args.learned_M}: {learned_ms:.3f} ms/frame; " + ", ".join(
+        f"{k} {v['ms_per_frame']:.3f}" for k, v in sorted(lk.items(), key=lambda kv: -kv[1]["ms_per_frame"])))
+    for v in lk.values():
+        for key in ("launches_per_frame", "ms_per_frame", "avg_ms"):
+            v[key] = round(v[key], 4)
+
+    # ---- per-frame error at a static camera (the orbit's last view), independent frames: the
+    # history is cleared before each frame and the previous G-buffer is the current one
+    f_err = nframes - 1
+    gi = f_err + 1
+    cam = fr.cams[f_err]
+    nrm = fr.G[gi][3 * W * H: 6 * W * H].view(3, -1)
+    surf = torch.nonzero(nrm.abs().sum(0) > 0).view(-1).cpu().numpy()
+    rng = np.random.default_rng(16)
+    pix = np.sort(rng.choice(surf, min(args.error_pixels, len(surf)), replace=False)).astype(np.uint32)
+    d_pix = torch.from_numpy(pix).to(fr.G[0].device)
+    n = len(pix)
+    gt = torch.zeros(3 * n, dtype=torch.float64, device=fr.G[0].device)
+    t0 = time.time()
+    ctx.ground_truth(cam["position"], fr.gview(gi), d_pix, n, gt, n)
+    torch.cuda.synchronize()
+    gt = gt.view(3, n)
+    log(f"FP64 ground truth of {n} pixels: {time.time() - t0:.2f}s")
+    idx = d_pix.long()
+
+    def errors(M, learned):
+        rel = []
+        for e in range(args.error_frames):
+            for t in fr.res:
+                t.zero_()
+            fr.run(f_err, M, learned=learned, gcur=gi, gprev=gi, seed_frame=100000 + e)
+            x = torch.nan_to_num(fr.rgb.view(3, -1)[:, idx].double())
+            rel.append((((x - gt) ** 2) / (gt ** 2 + 1e-2)).mean().item())
+        return np.array(rel)
+
+    rp = errors(args.M, None)
+    rl = errors(args.learned_M, lp)
+    torch.cuda.synchronize()
+    eff_p, eff_l = plain_ms * rp.mean(), learned_ms * rl.mean()
+    speedup = eff_p / eff_l
+    log(f"relMSE plain {rp.mean():.3f} (median {np.median(rp):.3f}), learned {rl.mean():.3f} "
+        f"(median {np.median(rl):.3f}); equal-error speedup learned vs plain {speedup:.3f}")
+    value = learned_ms * rl.mean() / rp.mean()
+    return {
+        "value": round(value, 4),
+        "ms_per_step": round(learned_ms, 4),
+        "learned": {"ms_per_frame": round(learned_ms, 4), "kernels": lk, "train_loss": round(lp.loss.item(), 4),
+                    "pretrain_frames": args.pretrain},
+        "equal_rmse": {
+            "protocol": "static camera, independent frames (history cleared), FP64 GPU ground truth on "
+                        f"{n} seeded surface pixels; relMSE = mean (x - gt)^2 / (gt^2 + 1e-2) per frame; "
+                        "time to a target error is proportional to ms/frame x mean relMSE",
+            "frames_per_method": args.error_frames,
+            "relmse_plain": round(float(rp.mean()), 4), "relmse_plain_median": round(float(np.median(rp)), 4),
+            "relmse_learned": round(float(rl.mean()), 4),
+            "relmse_learned_median": round(float(np.median(rl)), 4),
+            "ms_plain": round(plain_ms, 4), "ms_learned": round(learned_ms, 4),
+            "speedup_learned_vs_plain": round(float(speedup), 4),
+        },
+    }
 
 
 def cpu_baseline(ctx, scene, sd, cam, args):
@@ -285,9 +418,9 @@ def cpu_baseline(ctx, scene, sd, cam, args):
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(frame_s * 1e3 * scale, 1), "unit": "ms/frame @1080p (scaled from 480x270)",
             "cores": cores, "kind": "port",
-            "sample": f"one frame (pass A + {args.spatial_passes}x pass C + shade) of the same scene at "
-                      f"{w}x{h}: {frame_s:.2f} s, x{scale:.0f} to 1080p; traversal of its {len(p1)} "
-                      f"pass-A rays: {trace_s:.2f} s",
+            "sample": f"one frame (pass A + {args.spatial_passes}x pass C + shade, plain M={args.M}) of "
+                      f"the same scene at {w}x{h}: {frame_s:.2f} s, x{scale:.0f} to 1080p; traversal of "
+                      f"its {len(p1)} pass-A rays: {trace_s:.2f} s",
             "mrays_per_s": round(len(p1) / trace_s / 1e6, 3)}
 
 

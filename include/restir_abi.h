@@ -172,7 +172,7 @@ void restir_destroy(restir_ctx *ctx);
 const char *restir_last_error(const restir_ctx *ctx);
 /* ABI revision; bumped whenever an entry point's signature changes. Bindings compare it with the
  * revision they were written against and refuse a mismatched library. */
-#define RESTIR_ABI_VERSION 5
+#define RESTIR_ABI_VERSION 6
 int restir_version(void);
 
 /* Light tables: device pointers to the reference buffer images (header + array). The counts are
@@ -353,6 +353,37 @@ int restir_primary_gbuffer(restir_ctx *ctx, const restir_camera *cam, const int3
 /* The 4x4 projection-view matrix of camera.h:25-50 for the given aspect, column-major, as the
  * reference writes it into RestirUniforms::prevFrameProjectionViewMatrix (app.cpp:779). */
 int restir_camera_projection_view(const restir_camera *cam, float aspect, float out_col_major[16]);
+
+/* ---- batched views (no counterpart in the reference, which renders one view): one persistent
+ * launch per pass covers the tiles of n views, each with its own uniforms, G-buffers,
+ * reservoirs and scene (the lights and BVH set on `scene`, a context on the same device). The
+ * pixels are computed by the same code as the single-view entry points, so each view's output is
+ * bit-identical to calling restir_initial_temporal / restir_spatial / restir_unbiased (inline
+ * schedule) / restir_shade for it alone. Workgroups take tiles from a global counter in view
+ * order, so small and large views share the GPU without a tail per view. The per-view records
+ * are written to context-owned device memory by kernels on `stream` (no host-memory reads at
+ * execution), so a sequence of these calls can be captured in a HIP graph; calls on one context
+ * must stay on one stream. Outputs may not overlap each other or any view's inputs. */
+typedef struct restir_view {
+    const restir_ctx *scene;         /* lights + BVH of this view */
+    const restir_uniforms *uniforms; /* screenSize, frame, flags, ... of this view */
+    const restir_gbuffer *cur;       /* G-buffer of this frame */
+    const restir_gbuffer *prev;      /* previous frame's (Pass A with temporal reuse) */
+    const restir_reservoir *in;      /* Pass A: previous reservoirs; B, C: input; shade: final */
+    restir_reservoir *out;           /* Pass A, B, C output */
+    float *rgb;                      /* restir_shade_views output planes */
+    uint64_t rgb_stride;
+} restir_view;
+
+#define RESTIR_MAX_VIEWS 64
+int restir_initial_temporal_views(restir_ctx *ctx, const restir_view *views, uint32_t n,
+                                  const restir_launch *launch, void *stream);
+int restir_spatial_views(restir_ctx *ctx, const restir_view *views, uint32_t n, int iter,
+                         const restir_launch *launch, void *stream);
+int restir_unbiased_views(restir_ctx *ctx, const restir_view *views, uint32_t n, int k, int iter,
+                          const restir_launch *launch, void *stream);
+int restir_shade_views(restir_ctx *ctx, const restir_view *views, uint32_t n, float gamma,
+                       void *stream);
 
 /* FP64 ground truth of the direct lighting at `n` listed pixels (device array of pixel indices
  * y*width+x): for each, the sum over every light of Le * BRDF * G * V in double precision - the

@@ -70,6 +70,12 @@ int restir_set_light_clusters(restir_ctx *ctx, uint32_t k, const int32_t *lightC
     if (int r = fetch(ctx, cnt, clusterCount, k, "set_light_clusters: cluster_count")) return r;
     if (int r = fetch(ctx, lights, clusterLights, n, "set_light_clusters: cluster_lights")) return r;
     if (int r = fetch(ctx, al, clusterAlias, n, "set_light_clusters: cluster_alias")) return r;
+    std::vector<float> pw;
+    if (int r = fetch(ctx, pw, lightPWithin, n, "set_light_clusters: light_pwithin")) return r;
+    std::vector<restir_alias_table_column> src;  // the global table, past its 16-byte header
+    if (int r = fetch(ctx, src, reinterpret_cast<const restir_alias_table_column *>(ctx->scene.alias), n,
+                      "set_light_clusters: alias table"))
+        return r;
     for (size_t l = 0; l < n; ++l) {
         if (lc[l] < 0 || lc[l] >= static_cast<int32_t>(k))
             return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "set_light_clusters: light %zu in cluster %d", l, lc[l]);
@@ -82,7 +88,21 @@ int restir_set_light_clusters(restir_ctx *ctx, uint32_t k, const int32_t *lightC
         for (int32_t i = 0; i < cnt[c]; ++i)
             if (al[beg[c] + i].alias < 0 || al[beg[c] + i].alias >= cnt[c])
                 return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "set_light_clusters: cluster %u alias", c);
+        // The proposal's density q(l) reads p_within from whichever record is at hand (the column
+        // that drew l, or light_pwithin[l]) and p_src likewise: they must be the same numbers
+        for (int32_t i = 0; i < cnt[c]; ++i) {
+            const restir_alias_table_column &col = al[beg[c] + i];
+            const int32_t l = lights[beg[c] + i];
+            if (lc[l] != static_cast<int32_t>(c))
+                return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "set_light_clusters: light %d listed in cluster %u but assigned %d", l, c, lc[l]);
+            if (pw[l] != col.oriProb || col.aliasOriProb != al[beg[c] + col.alias].oriProb)
+                return fail(ctx, RESTIR_E_INVALID_ARGUMENT,
+                            "set_light_clusters: cluster %u: light_pwithin / oriProb / aliasOriProb disagree", c);
+        }
     }
+    for (size_t l = 0; l < n; ++l)
+        if (src[l].aliasOriProb != src[src[l].alias].oriProb)
+            return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "set_light_clusters: alias table column %zu: aliasOriProb is not oriProb of its alias", l);
     ctx->nClusters = static_cast<int32_t>(k);
     ctx->lightCluster = lightCluster;
     ctx->lightPWithin = lightPWithin;

@@ -43,6 +43,8 @@ struct restir_ctx {
     size_t mlpScratchBytes = 0;
     void *truthScratch = nullptr;  // per-light-chunk partial sums of the FP64 ground truth
     size_t truthScratchBytes = 0;
+    void *viewTable = nullptr;     // per-view records of the batched-view launches
+    size_t viewTableBytes = 0;
     // light clusters of the learned proposal (device pointers, caller-owned)
     bool clustersSet = false;
     int32_t nClusters = 0;
@@ -55,6 +57,7 @@ struct restir_ctx {
         if (scratch) (void)hipFree(scratch);
         if (mlpScratch) (void)hipFree(mlpScratch);
         if (truthScratch) (void)hipFree(truthScratch);
+        if (viewTable) (void)hipFree(viewTable);
     }
 };
 

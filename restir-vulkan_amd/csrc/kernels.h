@@ -81,6 +81,48 @@ hipError_t launch_trace(int stack, bool refLoop, bool persistent, uint32_t gridC
 // clears n words (a kernel rather than hipMemsetAsync, so the sequence can be graph-captured)
 __global__ void k_zero_u32(uint32_t *p, uint32_t n);
 
+// ---- batched views: one persistent launch over the tiles of n views (each view its own
+// uniforms, scene tables, G-buffers and reservoirs); `table` holds views_table_bytes(n) of
+// device memory, `counter` one word.
+struct ViewA {
+    restir_uniforms u;
+    SceneView sc;
+    GView cur, prev;
+    const restir_reservoir *prevRes;
+    restir_reservoir *out;
+};
+struct ViewB {
+    restir_uniforms u;
+    GView g;
+    const restir_reservoir *in;
+    restir_reservoir *out;
+    float cosThr;
+};
+struct ViewC {
+    restir_uniforms u;
+    SceneView sc;
+    GView g;
+    const restir_reservoir *in;
+    restir_reservoir *out;
+};
+struct ViewShade {
+    SceneView sc;
+    rt::f3 cam;
+    GView g;
+    const restir_reservoir *res;
+    float *rgb;
+    uint64_t rgbP;
+};
+size_t views_table_bytes(uint32_t n);
+hipError_t launch_pass_a_views(int tile, int stack, const ViewA *views, uint32_t n, void *table,
+                               uint32_t *counter, uint32_t gridCap, hipStream_t s);
+hipError_t launch_pass_b_views(int tile, const ViewB *views, uint32_t n, int iter, void *table,
+                               uint32_t *counter, uint32_t gridCap, hipStream_t s);
+hipError_t launch_pass_c_views(int tile, int stack, const ViewC *views, uint32_t n, int k, int iter,
+                               void *table, uint32_t *counter, uint32_t gridCap, hipStream_t s);
+hipError_t launch_shade_views(const ViewShade *views, uint32_t n, float gamma, void *table,
+                              uint32_t *counter, uint32_t gridCap, hipStream_t s);
+
 // ---- learned proposal (mlp.hip)
 size_t mlp_scratch_bytes(uint32_t n, int K);
 hipError_t mlp_init(const restir_mlp &m, uint64_t seed, const float *prior, hipStream_t s);

@@ -355,11 +355,11 @@ __device__ __forceinline__ void layer_mfma(const uint16_t *act, const uint16_t *
 
 // One workgroup of 512 lanes (8 waves) proposes for 256 consecutive pixels. LDS holds the
 // activations [256][136] bf16 and one layer's weights [<=256][136] bf16; after the last layer
-// the same LDS holds the fp16 softmax numerators [256][K + 2].
+// the same LDS holds the fp16 softmax numerators [256][K + 4].
 template <int KC>
 __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
     __shared__ __align__(16) uint16_t lds[2 * kTilePx * kRow];
-    static_assert(kTilePx * (KC + 2) <= 2 * kTilePx * kRow, "softmax numerators fit the LDS tile");
+    static_assert(kTilePx * (KC + 4) <= 2 * kTilePx * kRow, "softmax numerators fit the LDS tile");
     uint16_t *act = lds;
     uint16_t *wbuf = lds + kTilePx * kRow;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -396,9 +396,10 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
         f32x16 acc[KC / 32];
         layer_mfma<KC / 32>(act, wbuf, a.bias + b_off(3), wave, lane, acc);
         __syncthreads();
-        // softmax numerators e = exp(z - rowmax) as fp16 rows of kERow halves: the row maximum is
-        // reduced across the 32 lanes holding a row's columns, and the padding makes the per-pixel
-        // scans below conflict-free (lane t reads word 129 t + c/2)
+        // softmax numerators e = exp(z - rowmax) as fp16 rows of KC + 4 halves: the row maximum
+        // is reduced across the 32 lanes holding a row's columns; the padding keeps rows 8-byte
+        // aligned and spreads the per-pixel scans below over the banks (lane t reads dwords
+        // 130 t + c/2, c/2 + 1)
         __half *e = reinterpret_cast<__half *>(lds);
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
@@ -410,7 +411,7 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
             const int row = 32 * wave + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
 #pragma unroll
             for (int nb = 0; nb < KC / 32; ++nb)
-                e[row * (KC + 2) + 32 * nb + (lane & 31)] = __float2half(__expf(acc[nb][i] - m));
+                e[row * (KC + 4) + 32 * nb + (lane & 31)] = __float2half(__expf(acc[nb][i] - m));
         }
         __syncthreads();
     }
@@ -423,17 +424,29 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
     // the cluster pmf is e[c] / se with se the sequential sum of the stored fp16 numerators; the
     // inverse-CDF scan below accumulates the same values in the same order, so the sampled
     // distribution and the q written for each candidate agree
-    const __half *lg = reinterpret_cast<const __half *>(lds) + tid * (KC + 2);
+    // one 8-byte LDS read yields 4 numerators; sums run in cluster order
+    const uint2 *row = reinterpret_cast<const uint2 *>(reinterpret_cast<const __half *>(lds) + tid * (KC + 4));
+    auto quad = [](uint2 v, int k) {
+        const uint32_t w = k < 2 ? v.x : v.y;
+        return __half2float(__ushort_as_half(static_cast<unsigned short>((k & 1) ? (w >> 16) : (w & 0xffffu))));
+    };
     float se = 0.0f;
-    for (int c = 0; c < KC; ++c) se += __half2float(lg[c]);
+    for (int c4 = 0; c4 < KC / 4; ++c4) {
+        const uint2 v = row[c4];
+        se += quad(v, 0);
+        se += quad(v, 1);
+        se += quad(v, 2);
+        se += quad(v, 3);
+    }
     const float rse = 1.0f / se;
     rt::Pcg rng = rt::pcg_seed(a.seed, p);
     // choose which candidates come from the MLP, and their sorted cluster uniforms
     constexpr int kMaxM = 16;
+    const int M = static_cast<int>(a.M);
     float uc[kMaxM];
     int m = 0;
     uint32_t srcMask = 0;
-    for (uint32_t j = 0; j < a.M; ++j) {
+    for (int j = 0; j < M; ++j) {
         if (rt::pcg_float(rng) < a.mix) srcMask |= 1u << j;
         else uc[m++] = rt::pcg_float(rng) * se;
     }
@@ -449,46 +462,95 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
     int ccl[kMaxM];
     {
         int c = 0, i = 0;
-        float cdf = __half2float(lg[0]);
+        uint2 grp = row[0];
+        float cdf = quad(grp, 0);
         while (i < m) {
             if (uc[i] < cdf || c == KC - 1) {
                 ccl[i++] = c;
             } else {
                 ++c;
-                cdf += __half2float(lg[c]);
+                if ((c & 3) == 0) grp = row[c >> 2];
+                cdf += quad(grp, c & 3);
             }
         }
     }
+    // Candidates in groups of 8, each step's loads issued together (no load waits on another
+    // candidate's). Random draws are consumed in candidate order, two per candidate.
     int mi = 0;
-    for (uint32_t j = 0; j < a.M; ++j) {
-        int32_t l;
-        if (srcMask & (1u << j)) {  // source distribution: the reference's alias table
-            const float r1 = rt::pcg_float(rng), r2 = rt::pcg_float(rng);
-            int32_t col = static_cast<int32_t>(static_cast<float>(a.nAlias) * r1);
-            col = col < a.nAlias - 1 ? col : a.nAlias - 1;
-            const float4 t = a.alias[col];
-            l = t.x > r2 ? col : __float_as_int(t.y);
-        } else {                    // MLP cluster, then by power inside it
-            const int c = ccl[mi++];
-            const int32_t cnt = a.clusterCount[c], beg = a.clusterBegin[c];
-            const float r1 = rt::pcg_float(rng), r2 = rt::pcg_float(rng);
-            if (cnt == 0) {
-                l = -1;
-            } else {
-                int32_t col = static_cast<int32_t>(static_cast<float>(cnt) * r1);
-                col = col < cnt - 1 ? col : cnt - 1;
-                const float4 t = a.clusterAlias[beg + col];
-                l = a.clusterLights[beg + (t.x > r2 ? col : __float_as_int(t.y))];
+    for (int j0 = 0; j0 < M; j0 += 8) {
+        float r1[8], r2[8], pw[8], ps[8];
+        int32_t cl[8], col[8], cnt[8], beg[8], l[8];
+        float4 t[8];
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+            const int j = j0 + jj;
+            cl[jj] = -2;  // beyond M
+            if (j < M) {
+                r1[jj] = rt::pcg_float(rng);
+                r2[jj] = rt::pcg_float(rng);
+                cl[jj] = (srcMask & (1u << j)) ? -1 : ccl[mi++];
             }
         }
-        float q = 0.0f;
-        if (l >= 0) {
-            const int c = a.lightCluster[l];
-            const float pi = __half2float(lg[c]) * rse;
-            q = (1.0f - a.mix) * pi * a.lightPWithin[l] + a.mix * a.alias[l].z;
+        // step 1: source candidates read their alias column, MLP candidates their cluster
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+            if (cl[jj] == -1) {
+                int32_t c = static_cast<int32_t>(static_cast<float>(a.nAlias) * r1[jj]);
+                col[jj] = c < a.nAlias - 1 ? c : a.nAlias - 1;
+                t[jj] = a.alias[col[jj]];
+            } else if (cl[jj] >= 0) {
+                cnt[jj] = a.clusterCount[cl[jj]];
+                beg[jj] = a.clusterBegin[cl[jj]];
+            }
         }
-        a.candLight[static_cast<size_t>(j) * a.npx + p] = l;
-        a.candQ[static_cast<size_t>(j) * a.npx + p] = q;
+        // step 2: MLP candidates read the column of the cluster's own alias table
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+            if (cl[jj] >= 0 && cnt[jj] > 0) {
+                int32_t c = static_cast<int32_t>(static_cast<float>(cnt[jj]) * r1[jj]);
+                col[jj] = c < cnt[jj] - 1 ? c : cnt[jj] - 1;
+                t[jj] = a.clusterAlias[beg[jj] + col[jj]];
+            }
+        }
+        // step 3: the light; its probability under the table that drew it comes with the column
+        // (oriProb of the column or of its alias, as the alias table stores both)
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+            l[jj] = -1;
+            if (cl[jj] == -1) {
+                const bool keep = t[jj].x > r2[jj];
+                l[jj] = keep ? col[jj] : __float_as_int(t[jj].y);
+                ps[jj] = keep ? t[jj].z : t[jj].w;
+            } else if (cl[jj] >= 0 && cnt[jj] > 0) {
+                const bool keep = t[jj].x > r2[jj];
+                l[jj] = a.clusterLights[beg[jj] + (keep ? col[jj] : __float_as_int(t[jj].y))];
+                pw[jj] = keep ? t[jj].z : t[jj].w;
+            }
+        }
+        // step 4: the other table's probability of that light
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+            if (l[jj] < 0) continue;
+            if (cl[jj] == -1) {
+                cl[jj] = a.lightCluster[l[jj]];
+                pw[jj] = a.lightPWithin[l[jj]];
+            } else {
+                ps[jj] = a.alias[l[jj]].z;
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+            const int j = j0 + jj;
+            if (j >= M) continue;
+            float q = 0.0f;
+            if (l[jj] >= 0) {
+                const int c = cl[jj];
+                const float pi = quad(row[c >> 2], c & 3) * rse;
+                q = (1.0f - a.mix) * pi * pw[jj] + a.mix * ps[jj];
+            }
+            a.candLight[static_cast<size_t>(j) * a.npx + p] = l[jj];
+            a.candQ[static_cast<size_t>(j) * a.npx + p] = q;
+        }
     }
 }
 

@@ -10,6 +10,9 @@
 // contiguous runs so that neighbouring tiles (which gather each other's reservoirs) share an L2.
 // G-buffer planes are read as SoA float planes (one coalesced dword per lane per plane);
 // reservoirs keep the reference's 64-byte AoS record and move as four 16-byte vectors.
+#include <cstring>
+#include <vector>
+
 #include "kernels.h"
 
 namespace rk {
@@ -89,12 +92,10 @@ struct PassAArgs {
     const float *candQ;
 };
 
+// Pass A for pixel (x, y); `stk` is this lane's column of the [level][lane] LDS stack.
 template <int TX, int TY, int STACK>
-__global__ __launch_bounds__(TX *TY) void k_pass_a(const PassAArgs a) {
+__device__ __forceinline__ void pass_a_px(const PassAArgs &a, uint32_t x, uint32_t y, int32_t *stk) {
     constexpr int kBlock = TX * TY;
-    __shared__ int32_t stack[STACK * kBlock];
-    uint32_t x, y;
-    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
     const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
     if (x >= W || y >= H) return;
     const size_t idx = static_cast<size_t>(y) * W + x;
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(TX *TY) void k_pass_a(const PassAArgs a) {
         f3 o, d;
         shadow_segment(pos, res.pos, o, d);
         TraceStats st{0, 0};
-        if (occluded<STACK, false>(sc.nodes, sc.tris, o, d, stack + threadIdx.x, kBlock, st)) {
+        if (occluded<STACK, false>(sc.nodes, sc.tris, o, d, stk, kBlock, st)) {
             res.w = 0.0f;
             res.sumW = 0.0f;
         }
@@ -215,6 +216,14 @@ __global__ __launch_bounds__(TX *TY) void k_pass_a(const PassAArgs a) {
     store_res(a.out, idx, res);
 }
 
+template <int TX, int TY, int STACK>
+__global__ __launch_bounds__(TX *TY) void k_pass_a(const PassAArgs a) {
+    __shared__ int32_t stack[STACK * TX * TY];
+    uint32_t x, y;
+    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
+    pass_a_px<TX, TY, STACK>(a, x, y, stack + threadIdx.x);
+}
+
 // ------------------------------------------------------------------------------- Pass B ---
 struct PassBArgs {
     restir_uniforms u;
@@ -227,9 +236,7 @@ struct PassBArgs {
 };
 
 template <int TX, int TY>
-__global__ __launch_bounds__(TX *TY) void k_pass_b(const PassBArgs a) {
-    uint32_t x, y;
-    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
+__device__ __forceinline__ void pass_b_px(const PassBArgs &a, uint32_t x, uint32_t y) {
     const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
     if (x >= W || y >= H) return;
     const size_t idx = static_cast<size_t>(y) * W + x;
@@ -263,6 +270,13 @@ __global__ __launch_bounds__(TX *TY) void k_pass_b(const PassBArgs a) {
         res_combine(res, rr, ph, rng);
     }
     store_res(a.out, idx, res);
+}
+
+template <int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void k_pass_b(const PassBArgs a) {
+    uint32_t x, y;
+    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
+    pass_b_px<TX, TY>(a, x, y);
 }
 
 // ------------------------------------------------------------------------------- Pass C ---
@@ -322,11 +336,8 @@ __device__ __forceinline__ void finish_w(float &w, float &sumW, float pHat, uint
 }
 
 template <int TX, int TY, int STACK>
-__global__ __launch_bounds__(TX *TY) void k_pass_c(const PassCArgs a) {
+__device__ __forceinline__ void pass_c_px(const PassCArgs &a, uint32_t x, uint32_t y, int32_t *stk) {
     constexpr int kBlock = TX * TY;
-    __shared__ int32_t stack[STACK * kBlock];
-    uint32_t x, y;
-    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
     const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
     if (x >= W || y >= H) return;
     const size_t idx = static_cast<size_t>(y) * W + x;
@@ -346,7 +357,6 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c(const PassCArgs a) {
     const bool vis = (a.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) != 0;
     const f3 lp = res.pos;
     uint32_t numSamples = origM;
-    int32_t *stk = stack + threadIdx.x;
     // The centre re-test (unbiasedReuse.glsl:157-166) zeroes Z whatever the neighbours give, and
     // the neighbour loop draws no random numbers, so testing the centre first and skipping the
     // neighbour rays when it is shadowed yields the reference's result with fewer rays.
@@ -376,6 +386,14 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c(const PassCArgs a) {
     }
     finish_w(res.w, res.sumW, res.pHat, numSamples);
     store_res(a.out, idx, res);
+}
+
+template <int TX, int TY, int STACK>
+__global__ __launch_bounds__(TX *TY) void k_pass_c(const PassCArgs a) {
+    __shared__ int32_t stack[STACK * TX * TY];
+    uint32_t x, y;
+    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
+    pass_c_px<TX, TY, STACK>(a, x, y, stack + threadIdx.x);
 }
 
 // ------------------------------------------------------------------- Pass C, ray queues ---
@@ -577,8 +595,7 @@ struct ShadeArgs {
     uint64_t rgbP;
 };
 
-__global__ __launch_bounds__(256) void k_shade(const ShadeArgs a) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+__device__ __forceinline__ void shade_px(const ShadeArgs &a, size_t i) {
     const GView &g = a.g;
     if (i >= static_cast<size_t>(g.W) * g.H) return;
     const f3 pos = plane3(g.pos, i, g.P);
@@ -609,6 +626,10 @@ __global__ __launch_bounds__(256) void k_shade(const ShadeArgs a) {
     a.rgb[i] = c.x;
     a.rgb[a.rgbP + i] = c.y;
     a.rgb[2 * a.rgbP + i] = c.z;
+}
+
+__global__ __launch_bounds__(256) void k_shade(const ShadeArgs a) {
+    shade_px(a, static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x);
 }
 
 // ------------------------------------------------------------------------- batch trace ----
@@ -982,6 +1003,165 @@ hipError_t launch_pass_c(int tile, int stack, const restir_uniforms &u, int k, i
                            : dispatch_c_queued<64>(tile, args, q, gridCap, s);
     }
     return stack <= 32 ? dispatch_c<32>(tile, args, s) : dispatch_c<64>(tile, args, s);
+}
+
+// ------------------------------------------------------------------------ batched views ---
+// One persistent launch covers the tiles of n views: workgroups take tiles from a global
+// counter in view order (view v owns tiles [tileStart[v], tileStart[v + 1])) and run the same
+// per-pixel body as the single-view kernels, so every pixel's result is bit-identical to a
+// per-view launch. The table (tile prefix, then one argument record per view) sits in device
+// memory; it is written on the stream by k_put_blob launches whose by-value arguments carry the
+// bytes, so a captured graph holds its own copy of every table.
+struct ViewTable {
+    const uint32_t *tileStart;
+    const void *args;
+    uint32_t n;
+};
+constexpr int kBlobWords = 64;  // 1 KB of table per upload launch
+struct Blob {
+    uint4 w[kBlobWords];
+};
+__global__ __launch_bounds__(64) void k_put_blob(const Blob b, uint32_t nWords, uint4 *dst) {
+    if (threadIdx.x < nWords) dst[threadIdx.x] = b.w[threadIdx.x];
+}
+
+enum ViewKind { kViewA = 0, kViewB = 1, kViewC = 2, kViewShade = 3 };
+
+template <int KIND, int TX, int TY, int STACK>
+__global__ __launch_bounds__(TX *TY) void k_views(const ViewTable t, uint32_t *__restrict__ counter) {
+    __shared__ int32_t stack[STACK * TX * TY];
+    __shared__ uint32_t sTile;
+    const uint32_t total = t.tileStart[t.n];
+    for (;;) {
+        if (threadIdx.x == 0) sTile = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t tile = sTile;
+        __syncthreads();
+        if (tile >= total) break;  // uniform across the workgroup: every wave leaves together
+        uint32_t v = 0;
+        while (t.tileStart[v + 1] <= tile) ++v;
+        const uint32_t lt = tile - t.tileStart[v];
+        if constexpr (KIND == kViewShade) {
+            shade_px(static_cast<const ShadeArgs *>(t.args)[v], static_cast<size_t>(lt) * (TX * TY) + threadIdx.x);
+        } else {
+            uint32_t tilesX;
+            if constexpr (KIND == kViewA) tilesX = static_cast<const PassAArgs *>(t.args)[v].tilesX;
+            else if constexpr (KIND == kViewB) tilesX = static_cast<const PassBArgs *>(t.args)[v].tilesX;
+            else tilesX = static_cast<const PassCArgs *>(t.args)[v].tilesX;
+            const uint32_t x = (lt % tilesX) * TX + threadIdx.x % TX;
+            const uint32_t y = (lt / tilesX) * TY + threadIdx.x / TX;
+            if constexpr (KIND == kViewA)
+                pass_a_px<TX, TY, STACK>(static_cast<const PassAArgs *>(t.args)[v], x, y, stack + threadIdx.x);
+            else if constexpr (KIND == kViewB)
+                pass_b_px<TX, TY>(static_cast<const PassBArgs *>(t.args)[v], x, y);
+            else
+                pass_c_px<TX, TY, STACK>(static_cast<const PassCArgs *>(t.args)[v], x, y, stack + threadIdx.x);
+        }
+    }
+}
+
+size_t views_table_bytes(uint32_t n) { return 256 + static_cast<size_t>(n) * 512; }
+
+// Writes the table (tile prefix + records) into `table` and launches the persistent kernel.
+template <int KIND, class Args, int TX, int TY, int STACK>
+static hipError_t run_views(const Args *recs, const uint32_t *tiles, uint32_t n, void *table,
+                            uint32_t *counter, uint32_t gridCap, hipStream_t s) {
+    static_assert(sizeof(Args) <= 512, "view record fits its table slot");
+    std::vector<uint8_t> bytes(256 + sizeof(Args) * n, 0);
+    uint32_t *start = reinterpret_cast<uint32_t *>(bytes.data());
+    uint32_t total = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        start[v] = total;
+        total += tiles[v];
+    }
+    start[n] = total;
+    std::memcpy(bytes.data() + 256, recs, sizeof(Args) * n);
+    const size_t words = (bytes.size() + 15) / 16;
+    bytes.resize(words * 16, 0);
+    for (size_t w0 = 0; w0 < words; w0 += kBlobWords) {
+        Blob b{};
+        const uint32_t nw = static_cast<uint32_t>(words - w0 < static_cast<size_t>(kBlobWords) ? words - w0 : kBlobWords);
+        std::memcpy(b.w, bytes.data() + 16 * w0, 16 * nw);
+        RK_LAUNCH("k_put_blob", k_put_blob, dim3(1), dim3(64), 0, s, b, nw, static_cast<uint4 *>(table) + w0);
+    }
+    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, counter, 1u);
+    const ViewTable t{static_cast<const uint32_t *>(table),
+                      static_cast<const uint8_t *>(table) + 256, n};
+    const uint32_t grid = total < gridCap ? total : gridCap;
+    if (grid == 0) return hipGetLastError();
+    RK_LAUNCH("k_views", (k_views<KIND, TX, TY, STACK>), dim3(grid), dim3(TX * TY), 0, s, t, counter);
+    return hipGetLastError();
+}
+
+template <int KIND, class Args, int STACK>
+static hipError_t dispatch_views(int tile, const Args *recs, const uint32_t *tiles, uint32_t n,
+                                 void *table, uint32_t *counter, uint32_t gridCap, hipStream_t s) {
+    switch (tile) {
+        case 0: return run_views<KIND, Args, 16, 16, STACK>(recs, tiles, n, table, counter, gridCap, s);
+        case 1: return run_views<KIND, Args, 64, 1, STACK>(recs, tiles, n, table, counter, gridCap, s);
+        case 2: return run_views<KIND, Args, 8, 8, STACK>(recs, tiles, n, table, counter, gridCap, s);
+        case 3: return run_views<KIND, Args, 16, 8, STACK>(recs, tiles, n, table, counter, gridCap, s);
+        default: return run_views<KIND, Args, 32, 8, STACK>(recs, tiles, n, table, counter, gridCap, s);
+    }
+}
+
+static void tiles_of(int tile, uint32_t W, uint32_t H, uint32_t &tilesX, uint32_t &nTiles) {
+    const Tile t = kTiles[tile];
+    tilesX = (W + t.tx - 1) / t.tx;
+    nTiles = tilesX * ((H + t.ty - 1) / t.ty);
+}
+
+hipError_t launch_pass_a_views(int tile, int stack, const ViewA *views, uint32_t n, void *table,
+                               uint32_t *counter, uint32_t gridCap, hipStream_t s) {
+    std::vector<PassAArgs> recs(n);
+    std::vector<uint32_t> tiles(n);
+    for (uint32_t v = 0; v < n; ++v) {
+        const ViewA &w = views[v];
+        recs[v] = PassAArgs{w.u, w.sc, w.cur, w.prev, w.prevRes, w.out, 0, 0, nullptr, nullptr};
+        tiles_of(tile, w.u.screenSize[0], w.u.screenSize[1], recs[v].tilesX, recs[v].nTiles);
+        tiles[v] = recs[v].nTiles;
+    }
+    return stack <= 32 ? dispatch_views<kViewA, PassAArgs, 32>(tile, recs.data(), tiles.data(), n, table, counter, gridCap, s)
+                       : dispatch_views<kViewA, PassAArgs, 64>(tile, recs.data(), tiles.data(), n, table, counter, gridCap, s);
+}
+
+hipError_t launch_pass_b_views(int tile, const ViewB *views, uint32_t n, int iter, void *table,
+                               uint32_t *counter, uint32_t gridCap, hipStream_t s) {
+    std::vector<PassBArgs> recs(n);
+    std::vector<uint32_t> tiles(n);
+    for (uint32_t v = 0; v < n; ++v) {
+        const ViewB &w = views[v];
+        recs[v] = PassBArgs{w.u, w.g, w.in, w.out, 0, 0, w.u.frame * 31u + static_cast<uint32_t>(iter), w.cosThr};
+        tiles_of(tile, w.u.screenSize[0], w.u.screenSize[1], recs[v].tilesX, recs[v].nTiles);
+        tiles[v] = recs[v].nTiles;
+    }
+    return dispatch_views<kViewB, PassBArgs, 1>(tile, recs.data(), tiles.data(), n, table, counter, gridCap, s);
+}
+
+hipError_t launch_pass_c_views(int tile, int stack, const ViewC *views, uint32_t n, int k, int iter,
+                               void *table, uint32_t *counter, uint32_t gridCap, hipStream_t s) {
+    std::vector<PassCArgs> recs(n);
+    std::vector<uint32_t> tiles(n);
+    for (uint32_t v = 0; v < n; ++v) {
+        const ViewC &w = views[v];
+        recs[v] = PassCArgs{w.u, w.sc, w.g, w.in, w.out, 0, 0, k, w.u.frame * 17u + static_cast<uint32_t>(iter)};
+        tiles_of(tile, w.u.screenSize[0], w.u.screenSize[1], recs[v].tilesX, recs[v].nTiles);
+        tiles[v] = recs[v].nTiles;
+    }
+    return stack <= 32 ? dispatch_views<kViewC, PassCArgs, 32>(tile, recs.data(), tiles.data(), n, table, counter, gridCap, s)
+                       : dispatch_views<kViewC, PassCArgs, 64>(tile, recs.data(), tiles.data(), n, table, counter, gridCap, s);
+}
+
+hipError_t launch_shade_views(const ViewShade *views, uint32_t n, float gamma, void *table,
+                              uint32_t *counter, uint32_t gridCap, hipStream_t s) {
+    std::vector<ShadeArgs> recs(n);
+    std::vector<uint32_t> tiles(n);
+    for (uint32_t v = 0; v < n; ++v) {
+        const ViewShade &w = views[v];
+        recs[v] = ShadeArgs{w.sc, w.cam, gamma, w.g, w.res, w.rgb, w.rgbP};
+        tiles[v] = static_cast<uint32_t>((static_cast<size_t>(w.g.W) * w.g.H + 255) / 256);
+    }
+    return run_views<kViewShade, ShadeArgs, 16, 16, 1>(recs.data(), tiles.data(), n, table, counter, gridCap, s);
 }
 
 hipError_t launch_shade(const SceneView &sc, f3 cam, float gamma, const GView &g,

@@ -21,7 +21,7 @@ TEMPORAL_REUSE_FLAG = 1 << 1
 TRACE_FIRST_HIT = 0
 TRACE_REFERENCE_LOOP = 1
 TRACE_ONE_RAY_PER_LANE = 2
-ABI_VERSION = 5  # RESTIR_ABI_VERSION of include/restir_abi.h
+ABI_VERSION = 6  # RESTIR_ABI_VERSION of include/restir_abi.h
 
 ERRORS = {
     -1: "RESTIR_E_INVALID_ARGUMENT",
@@ -125,13 +125,20 @@ assert C.sizeof(Uniforms) == 128 and C.sizeof(GBuffer) == 56 and C.sizeof(Camera
 assert C.sizeof(MlpStruct) == 48
 
 
+class View(C.Structure):
+    """restir_view: one view of a batched dispatch."""
+    _fields_ = [("scene", C.c_void_p), ("uniforms", C.POINTER(Uniforms)), ("cur", C.POINTER(GBuffer)),
+                ("prev", C.POINTER(GBuffer)), ("in_", C.c_void_p), ("out", C.c_void_p),
+                ("rgb", C.c_void_p), ("rgb_stride", C.c_uint64)]
+
+
 class KernelTime(C.Structure):
     """restir_kernel_time: one row of restir_profile_read."""
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint32), ("total_ms", C.c_float),
                 ("max_ms", C.c_float), ("_reserved", C.c_uint32)]
 
 
-assert C.sizeof(KernelTime) == 64
+assert C.sizeof(KernelTime) == 64 and C.sizeof(View) == 64
 
 # Exported symbols with their signatures (restype, argtypes); the CPU test suite checks that the
 # library exports every one of them and that include/restir_abi.h declares the same set.
@@ -170,6 +177,10 @@ SIGNATURES = {
                                 _vp, C.POINTER(_f32), C.POINTER(_f32), _u32, _f32, _u32, _vp, _vp]),
     "restir_mlp_features": (_i32, [_vp, C.POINTER(GBuffer), C.POINTER(_f32), C.POINTER(_f32),
                                    C.POINTER(_f32), _vp, _u32, _vp, _vp]),
+    "restir_initial_temporal_views": (_i32, [_vp, _vp, _u32, C.POINTER(Launch), _vp]),
+    "restir_spatial_views": (_i32, [_vp, _vp, _u32, _i32, C.POINTER(Launch), _vp]),
+    "restir_unbiased_views": (_i32, [_vp, _vp, _u32, _i32, _i32, C.POINTER(Launch), _vp]),
+    "restir_shade_views": (_i32, [_vp, _vp, _u32, _f32, _vp]),
     "restir_ground_truth": (_i32, [_vp, C.POINTER(_f32), C.POINTER(GBuffer), _vp, _u32, _u32, _vp,
                                    _u64, _vp]),
     "restir_profile_enable": (_i32, [_vp, _i32]),
@@ -429,6 +440,35 @@ class Context:
         self._check(self.lib.restir_trace_shadow(self.h, as_ptr(p1), as_ptr(p2), n,
                                                  as_ptr(shadowed), as_ptr(visits), as_ptr(tests),
                                                  loop, stream_ptr(stream)), "restir_trace_shadow")
+
+    # ------------------------------------------------------------------- batched views
+    @staticmethod
+    def _views(views):
+        arr = (View * len(views))(*views)
+        return arr, len(views)
+
+    def initial_temporal_views(self, views, launch=None, stream=None):
+        arr, n = self._views(views)
+        self._check(self.lib.restir_initial_temporal_views(
+            self.h, arr, n, C.byref(launch) if launch is not None else None, stream_ptr(stream)),
+            "restir_initial_temporal_views")
+
+    def spatial_views(self, views, it, launch=None, stream=None):
+        arr, n = self._views(views)
+        self._check(self.lib.restir_spatial_views(
+            self.h, arr, n, it, C.byref(launch) if launch is not None else None, stream_ptr(stream)),
+            "restir_spatial_views")
+
+    def unbiased_views(self, views, k, it, launch=None, stream=None):
+        arr, n = self._views(views)
+        self._check(self.lib.restir_unbiased_views(
+            self.h, arr, n, k, it, C.byref(launch) if launch is not None else None, stream_ptr(stream)),
+            "restir_unbiased_views")
+
+    def shade_views(self, views, gamma=1.0, stream=None):
+        arr, n = self._views(views)
+        self._check(self.lib.restir_shade_views(self.h, arr, n, float(gamma), stream_ptr(stream)),
+                    "restir_shade_views")
 
     def ground_truth(self, camera_pos, g, pixels, n, rgb, rgb_stride, n_tri_samples=256, stream=None):
         """FP64 direct lighting of the listed pixels into rgb (device float64, [3][rgb_stride])."""

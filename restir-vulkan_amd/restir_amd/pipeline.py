@@ -11,11 +11,13 @@ Device memory comes from torch (plumbing only); every computation runs in libres
 """
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 
 from . import abi
-from .abi import (GBUFFER_PLANES, RESERVOIR_DTYPE, Context, build_aabb_tree, build_alias_table,
-                  gbuffer_struct, make_uniforms, pack_buffer, projection_view)
+from .abi import (GBUFFER_PLANES, RESERVOIR_DTYPE, Context, as_ptr, build_aabb_tree,
+                  build_alias_table, gbuffer_struct, make_uniforms, pack_buffer, projection_view)
 
 
 def _torch():
@@ -187,9 +189,121 @@ class Renderer:
         return self.gbuf[self.cur if i is None else i].cpu().numpy()
 
 
+class BatchRenderer:
+    """Frames of several views per step, each pass of all views in one persistent launch
+    (restir_*_views). ``views`` is a list of (scene, width, height); views of the same Scene
+    object share its device tables (one context per distinct scene). Every view has its own
+    G-buffers, reservoirs and image; ``rgb`` holds all images back to back and ``images[v]``
+    is view v's [3 * W * H] slice. Pass C uses the inline schedule (shadow rays traced per
+    pixel), so each view's output is bit-identical to a Renderer of that view alone."""
+
+    def __init__(self, views, *, device_index=0, mode="unbiased", M=32, k=5, spatial_passes=1,
+                 spatial_neighbors=4, radius=30.0, temporal=True, visibility=True, temporal_mult=20,
+                 launch=None):
+        torch = _torch()
+        self.torch = torch
+        self.device = torch.device("cuda", device_index)
+        self.ctx = Context(device_index)  # launches the batched passes
+        self.mode, self.M, self.k = mode, M, k
+        self.spatial_passes, self.spatial_neighbors, self.radius = spatial_passes, spatial_neighbors, radius
+        self.temporal_mult = temporal_mult
+        self.flags = (abi.VISIBILITY_REUSE_FLAG if visibility else 0) | \
+                     (abi.TEMPORAL_REUSE_FLAG if temporal else 0)
+        self.launch = launch
+        self.scene_dev = {}
+        self.views = []
+        total = sum(3 * w * h for _, w, h in views)
+        self.rgb = torch.zeros(total, dtype=torch.float32, device=self.device)
+        off = 0
+        for sc, w, h in views:
+            if id(sc) not in self.scene_dev:
+                c = Context(device_index)
+                self.scene_dev[id(sc)] = (c, SceneOnDevice(c, sc, self.device))
+            c, dev = self.scene_dev[id(sc)]
+            n = w * h
+            self.views.append({
+                "scene": sc, "ctx": c, "dev": dev, "W": w, "H": h,
+                "gbuf": [torch.zeros(GBUFFER_PLANES * n, dtype=torch.float32, device=self.device) for _ in range(2)],
+                "res": [torch.zeros(16 * n, dtype=torch.int32, device=self.device) for _ in range(2)],
+                "tmp": torch.zeros(16 * n, dtype=torch.int32, device=self.device),
+                "rgb": self.rgb[off: off + 3 * n], "prev_pv": None, "cam": None,
+            })
+            off += 3 * n
+        self.images = [v["rgb"] for v in self.views]
+        self.frame = 0
+        self.cur = 1
+
+    def reset(self):
+        for v in self.views:
+            for t in (*v["gbuf"], *v["res"], v["tmp"]):
+                t.zero_()
+            v["prev_pv"], v["cam"] = None, None
+        self.rgb.zero_()
+        self.frame, self.cur = 0, 1
+
+    def _g(self, v, i):
+        return gbuffer_struct(v["gbuf"][i].data_ptr(), v["W"], v["H"])
+
+    def render_frame(self, cams, stream=None):
+        """One frame of every view; cams[v] is view v's camera."""
+        self.cur = 1 - self.cur
+        self.frame += 1
+        i, p = self.cur, 1 - self.cur
+        keep = []  # ctypes structures referenced by pointer until the calls return
+
+        def view(v, u, cur, prev, res_in, out, rgb=None):
+            gs = [cur, prev]
+            keep.extend([u, *gs])
+            return abi.View(v["ctx"].h.value, C.pointer(u), C.pointer(cur), C.pointer(prev),
+                            as_ptr(res_in), as_ptr(out), as_ptr(rgb) if rgb is not None else None,
+                            v["W"] * v["H"])
+
+        us = []
+        for v, cam in zip(self.views, cams):
+            v["cam"] = cam
+            v["ctx"].primary_gbuffer(cam, v["dev"].d_tri_material, v["dev"].d_materials,
+                                     len(v["scene"].materials), v["gbuf"][i], v["W"], v["H"], stream=stream)
+            pv_now = projection_view(cam, v["W"] / v["H"])
+            prev_pv = v["prev_pv"] if v["prev_pv"] is not None else pv_now
+            u = make_uniforms(v["W"], v["H"], self.frame, cam["position"], prev_pv, M=self.M,
+                              temporal_mult=self.temporal_mult, neighbors=self.spatial_neighbors,
+                              radius=self.radius, flags=self.flags)
+            us.append(u)
+            v["prev_pv"] = pv_now
+        n_sp = self.spatial_passes
+        if self.mode == "unbiased":
+            first = [v["tmp"] if n_sp % 2 == 1 else v["res"][i] for v in self.views]
+        else:
+            first = [v["res"][i] for v in self.views]
+        va = [view(v, u, self._g(v, i), self._g(v, p), v["res"][p], f)
+              for v, u, f in zip(self.views, us, first)]
+        self.ctx.initial_temporal_views(va, self.launch, stream)
+        if self.mode == "unbiased":
+            src = first
+            for j in range(n_sp):
+                dst = [v["res"][i] if (n_sp - j) % 2 == 1 else v["tmp"] for v in self.views]
+                vc = [view(v, u, self._g(v, i), self._g(v, i), a, b)
+                      for v, u, a, b in zip(self.views, us, src, dst)]
+                self.ctx.unbiased_views(vc, self.k, j, self.launch, stream)
+                src = dst
+        else:
+            for j in range(self.spatial_passes):
+                vb = [view(v, u, self._g(v, i), self._g(v, i), v["res"][i], v["res"][p])
+                      for v, u in zip(self.views, us)]
+                self.ctx.spatial_views(vb, 2 * j, self.launch, stream)
+                vb = [view(v, u, self._g(v, i), self._g(v, i), v["res"][p], v["res"][i])
+                      for v, u in zip(self.views, us)]
+                self.ctx.spatial_views(vb, 2 * j + 1, self.launch, stream)
+        vs = [view(v, u, self._g(v, i), self._g(v, i), v["res"][i], None, v["rgb"])
+              for v, u in zip(self.views, us)]
+        self.ctx.shade_views(vs, 1.0, stream)
+        return self.rgb
+
+
 class FrameSequenceGraph:
-    """A sequence of consecutive frames of a Renderer (G-buffer, ReSTIR passes, shading) captured
-    into one HIP graph, so the whole sequence is submitted with a single launch.
+    """A sequence of consecutive frames of a Renderer or BatchRenderer (G-buffers, ReSTIR passes,
+    shading) captured into one HIP graph, so the whole sequence is submitted with a single launch
+    (for a BatchRenderer, cams[f] lists every view's camera of frame f).
 
     The renderer must have rendered one frame eagerly at this size first, so the library's
     scratch already exists (allocation is not allowed under capture, and the library reports it
@@ -203,10 +317,10 @@ class FrameSequenceGraph:
 
     def __init__(self, r: Renderer, cams):
         torch = r.torch
-        if r.learned is not None:
+        if getattr(r, "learned", None) is not None:
             raise ValueError("FrameSequenceGraph: a renderer with a learned proposal cannot be captured")
         self.r, self.cams = r, list(cams)
-        self.images = torch.zeros(len(self.cams), 3 * r.W * r.H, dtype=torch.float32, device=r.device)
+        self.images = torch.zeros(len(self.cams), r.rgb.numel(), dtype=torch.float32, device=r.device)
         self.stream = torch.cuda.Stream(device=r.device)
         self.graph = torch.cuda.CUDAGraph()
         self.stream.wait_stream(torch.cuda.current_stream(r.device))

@@ -1,9 +1,10 @@
 """GPU checks of batching: a HIP-graph-captured 8-frame sequence against the same frames run
-eagerly, and the tile autotuner (its choice is recorded and does not change results)."""
+eagerly, the tile autotuner (its choice is recorded and does not change results), and the
+persistent multi-view passes against per-view launches (bit-identical)."""
 import pytest
 
-from restir_amd import scenes
-from restir_amd.pipeline import FrameSequenceGraph, Renderer
+from restir_amd import abi, scenes
+from restir_amd.pipeline import BatchRenderer, FrameSequenceGraph, Renderer
 from restir_amd.tuning import autotune
 
 pytestmark = pytest.mark.gpu
@@ -86,3 +87,70 @@ def test_autotune_records_choice_and_keeps_results(torch):
     torch.cuda.synchronize()
     assert torch.equal(bits(r.rgb), bits(ref.rgb))
     assert torch.equal(bits(r.res[r.cur]), bits(ref.res[ref.cur]))
+
+
+def batch_views():
+    """Three views over two scenes at different sizes (one of them not a multiple of any tile)."""
+    a = scenes.cornell_box(1)
+    b = scenes.terrain(11, grid=48, n_boxes=60, n_point_lights=2000, extent=16.0)
+    return [(a, 64, 48), (b, 80, 45), (a, 33, 70)]
+
+
+@pytest.mark.parametrize("mode", ["unbiased", "biased"])
+def test_batched_views_match_single_view_renderers(torch, mode):
+    views = batch_views()
+    kw = dict(mode=mode, k=3, M=8, spatial_passes=2)
+    batch = BatchRenderer(views, **kw)
+    singles = [Renderer(sc, w, h, launch=abi.Launch(16, 16, abi.SCHEDULE_INLINE), **kw) for sc, w, h in views]
+    for f in range(3):
+        cams = [scenes.orbit_camera(sc.camera, f) for sc, _, _ in views]
+        batch.render_frame(cams)
+        for r, cam in zip(singles, cams):
+            r.render_frame(cam)
+        torch.cuda.synchronize()
+        for v, r in enumerate(singles):
+            assert torch.equal(bits(batch.images[v]), bits(r.rgb)), (f, v)
+            assert torch.equal(bits(batch.views[v]["res"][batch.cur]), bits(r.res[r.cur])), (f, v)
+
+
+def test_batched_views_in_a_graph_of_8_frames(torch):
+    views = batch_views()
+    kw = dict(mode="unbiased", k=3, M=8, spatial_passes=2)
+    seq = [[scenes.orbit_camera(sc.camera, f) for sc, _, _ in views] for f in range(8)]
+    eager = BatchRenderer(views, **kw)
+    imgs = []
+    for cams in seq:
+        eager.render_frame(cams)
+        imgs.append(eager.rgb.clone())
+    torch.cuda.synchronize()
+    b = BatchRenderer(views, **kw)
+    b.render_frame(seq[0])  # allocates the view table outside the capture
+    torch.cuda.synchronize()
+    b.reset()
+    torch.cuda.synchronize()
+    fg = FrameSequenceGraph(b, seq)
+    for rep in range(3):
+        for v in b.views:
+            for t in (*v["gbuf"], *v["res"], v["tmp"]):
+                t.zero_()
+        fg.replay()
+        torch.cuda.synchronize()
+        for f in range(8):
+            assert torch.equal(bits(fg.images[f]), bits(imgs[f])), (rep, f)
+        eager.render_frame(seq[rep])
+        torch.cuda.synchronize()
+
+
+def test_batched_views_reject_overlapping_outputs(torch):
+    views = batch_views()[:2]
+    b = BatchRenderer(views, mode="unbiased", k=3, M=8)
+    b.render_frame([sc.camera for sc, _, _ in views])
+    v0, v1 = b.views
+    u = abi.make_uniforms(v0["W"], v0["H"], 1, views[0][0].camera["position"],
+                          abi.projection_view(views[0][0].camera, v0["W"] / v0["H"]), M=8)
+    g0 = b._g(v0, 0)
+    import ctypes as C
+    rec = [abi.View(v0["ctx"].h.value, C.pointer(u), C.pointer(g0), C.pointer(g0), abi.as_ptr(v0["res"][0]),
+                    abi.as_ptr(v0["res"][1]), None, 0)] * 2  # both views write the same buffer
+    with pytest.raises(abi.RestirError):
+        b.ctx.initial_temporal_views(rec)

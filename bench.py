@@ -48,6 +48,21 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the two rocprofv3 --pmc passes
+    (FETCH_SIZE, WRITE_SIZE, each scaled by a plain copy kernel's reading) summarised in
+    profiles/traffic_1080p_terrain_pmc.md; None when the summary has no line for it."""
+    path = ROOT / "profiles" / "traffic_1080p_terrain_pmc.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_per_launch_bytes:"):
+            name, _, val = line.split(":", 1)[1].rpartition("=")
+            if name.strip() == kernel:
+                return int(val)
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +81,7 @@ def parse():
     ap.add_argument("--error-pixels", type=int, default=16384)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-frames", type=int, default=8, help="1080p frames of the CPU-oracle sample")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (x3), whose hardware byte counters "
                          "calibrate the traffic counters of the other kernels")
@@ -219,6 +235,7 @@ def main():
     torch.cuda.synchronize()
     trace_ms = e0.elapsed_time(e1) / args.trace_reps
     Vn, Tn = ray_sets["neighbour"][-1]["V"], ray_sets["neighbour"][-1]["T"]
+    cpu_rays = (q1[:nr].cpu().numpy(), q2[:nr].cpu().numpy()) if not args.no_cpu_baseline else None
     rays_per_s = nr / (trace_ms * 1e-3)
     roof_rays = min(HBM_PEAK_GBS * 1e9 / (NODE_BYTES * Vn + TRI_BYTES * Tn),
                     VALU_LANE_OPS / (56 * Vn + 45 * Tn))
@@ -275,7 +292,7 @@ def main():
                      "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None,
+                     "traffic": measured_traffic(dominant) if W * H == 1920 * 1080 else None,
                      "algorithmic_bytes_per_launch": round(dom_bytes) if dom_bytes else None,
                      "avg_launch_ms": round(kernels[dominant]["avg_ms"], 4)},
     }
@@ -291,7 +308,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(ctx, scene, sd, cams[-1], args)
+        out["cpu_baseline"] = cpu_baseline(fr, scene, sd, args, cpu_rays)
     print(json.dumps(out), flush=True)
 
 
@@ -375,53 +392,41 @@ def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
     }
 
 
-def cpu_baseline(ctx, scene, sd, cam, args):
-    """The CPU oracle on this host's cores: one frame of the same passes at 480x270 (1/16 of the
-    1080p pixels, same scene, camera and settings), scaled to 1080p; plus the oracle's traversal
-    rays/s on that frame's Pass A rays. The G-buffer input is rendered by the GPU (not timed)."""
-    import torch
-
+def cpu_baseline(fr, scene, sd, args, rays):
+    """The CPU oracle on this host's cores, on a bounded sample of the same workload: the first
+    `cpu_frames` frames of the plain pipeline at full 1080p (Pass A + 2x Pass C + shading over
+    the same G-buffers, history chained from an empty one), and the oracle's traversal of the
+    GPU's last Pass C neighbour rays (the ray set of the Mrays/s figure), repeated to >= 3 s."""
     import oracle
     from restir_amd import abi
-    from restir_amd.abi import make_uniforms
 
-    w, h = 480, 270
-    scale = (args.width * args.height) / (w * h)
+    W, H = fr.W, fr.H
     orc = oracle.Scene(sd.nodes, sd.tris, scene.point_lights, None, sd.alias)
-    gb = torch.zeros(abi.GBUFFER_PLANES * w * h, dtype=torch.float32, device="cuda")
-    ctx.primary_gbuffer(cam, sd.d_tri_material, sd.d_materials, len(scene.materials), gb, w, h)
-    planes = gb.cpu().numpy()
-    g = oracle.gbuffer(planes, w, h)
-    pv = abi.projection_view(cam, w / h)
-    u = make_uniforms(w, h, 5, cam["position"], pv, M=args.M, temporal_mult=20, neighbors=args.k,
-                      radius=30.0, flags=abi.VISIBILITY_REUSE_FLAG | abi.TEMPORAL_REUSE_FLAG)
-    prev = np.zeros(w * h, dtype=abi.RESERVOIR_DTYPE)
+    prev = np.zeros(W * H, dtype=abi.RESERVOIR_DTYPE)
+    gs = [oracle.gbuffer(fr.G[i].cpu().numpy(), W, H) for i in range(args.cpu_frames + 1)]
     t0 = time.perf_counter()
-    cur = oracle.pass_a(orc, u, g, g, prev, abi.RESERVOIR_DTYPE)
-    for j in range(args.spatial_passes):
-        cur = oracle.pass_c(orc, u, args.k, j, g, cur)
-    oracle.shade(orc, cam["position"], 1.0, g, cur)
-    frame_s = time.perf_counter() - t0
-    # traversal sample: this frame's Pass A rays (RIS result before the visibility test)
-    P = w * h
-    pos = planes[: 3 * P].reshape(3, P).T
-    u.flags = 0
-    ris = oracle.pass_a(orc, u, g, g, prev, abi.RESERVOIR_DTYPE)
-    lit = ris["sumWeights"] != 0
-    p1 = np.zeros((int(lit.sum()), 4), dtype=np.float32)
-    p2 = np.zeros_like(p1)
-    p1[:, :3] = pos[lit]
-    p2[:, :3] = ris["pos_lum"][lit, :3]
-    t1 = time.perf_counter()
-    oracle.trace_shadow(orc, p1, p2, with_counts=False)
-    trace_s = time.perf_counter() - t1
+    for f in range(args.cpu_frames):
+        u = fr.uniforms(f, args.M)
+        cur = oracle.pass_a(orc, u, gs[f + 1], gs[f], prev, abi.RESERVOIR_DTYPE)
+        for j in range(args.spatial_passes):
+            cur = oracle.pass_c(orc, u, args.k, j, gs[f + 1], cur)
+        oracle.shade(orc, fr.cams[f]["position"], 1.0, gs[f + 1], cur)
+        prev = cur
+    frame_s = (time.perf_counter() - t0) / args.cpu_frames
+    p1, p2 = rays
+    reps, t1 = 0, time.perf_counter()
+    while True:
+        oracle.trace_shadow(orc, p1, p2, with_counts=False)
+        reps += 1
+        if time.perf_counter() - t1 >= 3.0:
+            break
+    trace_s = (time.perf_counter() - t1) / reps
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(frame_s * 1e3 * scale, 1), "unit": "ms/frame @1080p (scaled from 480x270)",
-            "cores": cores, "kind": "port",
-            "sample": f"one frame (pass A + {args.spatial_passes}x pass C + shade, plain M={args.M}) of "
-                      f"the same scene at {w}x{h}: {frame_s:.2f} s, x{scale:.0f} to 1080p; traversal of "
-                      f"its {len(p1)} pass-A rays: {trace_s:.2f} s",
-            "mrays_per_s": round(len(p1) / trace_s / 1e6, 3)}
+    return {"value": round(frame_s * 1e3, 1), "unit": "ms/frame @1080p", "cores": cores, "kind": "port",
+            "sample": f"{args.cpu_frames} frames of the plain pipeline (M={args.M}, Pass A + "
+                      f"{args.spatial_passes}x Pass C + shading) at {W}x{H}: {frame_s * args.cpu_frames:.1f} s; "
+                      f"traversal of {len(p1)} Pass C neighbour rays x{reps}: {trace_s * reps:.1f} s",
+            "mrays_per_s": round(len(p1) / trace_s / 1e6, 2)}
 
 
 if __name__ == "__main__":

@@ -45,19 +45,21 @@ __device__ __forceinline__ void make_features(f3 pos, f3 n, f3 cam, const float 
     s[6] = v.x;
     s[7] = v.y;
     s[8] = v.z;
-    for (int k = 0; k < 6; ++k) dst[k] = f2bf(s[k]);
+    // dst is 4-byte aligned and every pair below starts at an even index: one packed
+    // conversion and one 32-bit store per pair
+    uint32_t *d2 = reinterpret_cast<uint32_t *>(dst);
+    for (int k = 0; k < 3; ++k) d2[k] = f2bf2(s[2 * k], s[2 * k + 1]);
     for (int i = 0; i < 9; ++i) {
         float sn, cs;
         sincosf(3.14159265358979f * s[i], &sn, &cs);
         for (int f = 0; f < 6; ++f) {
-            dst[6 + 12 * i + 2 * f] = f2bf(sn);
-            dst[6 + 12 * i + 2 * f + 1] = f2bf(cs);
+            d2[3 + 6 * i + f] = f2bf2(sn, cs);
             const float s2 = 2.0f * sn * cs, c2 = (cs - sn) * (cs + sn);
             sn = s2;
             cs = c2;
         }
     }
-    for (int k = kFeatures; k < kIn; ++k) dst[k] = 0;
+    for (int k = kFeatures / 2; k < kIn / 2; ++k) d2[k] = 0u;
 }
 
 struct FeatArgs {
@@ -353,6 +355,12 @@ __device__ __forceinline__ void layer_mfma(const uint16_t *act, const uint16_t *
     }
 }
 
+__device__ __forceinline__ float4 unpack_h4(uint2 v) {  // four fp16 -> float
+    const float2 lo = __half22float2(__builtin_bit_cast(__half2, v.x));
+    const float2 hi = __half22float2(__builtin_bit_cast(__half2, v.y));
+    return make_float4(lo.x, lo.y, hi.x, hi.y);
+}
+
 // One workgroup of 512 lanes (8 waves) proposes for 256 consecutive pixels. LDS holds the
 // activations [256][136] bf16 and one layer's weights [<=256][136] bf16; after the last layer
 // the same LDS holds the fp16 softmax numerators [256][K + 4].
@@ -424,59 +432,66 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
     // the cluster pmf is e[c] / se with se the sequential sum of the stored fp16 numerators; the
     // inverse-CDF scan below accumulates the same values in the same order, so the sampled
     // distribution and the q written for each candidate agree
-    // one 8-byte LDS read yields 4 numerators; sums run in cluster order
+    // one 8-byte LDS read yields 4 numerators. Cluster blocks of 32: block sums in registers
+    // (sequential inside a block, blocks in order), so a draw picks its block with KC/32
+    // compares and scans at most 32 numerators of that block
     const uint2 *row = reinterpret_cast<const uint2 *>(reinterpret_cast<const __half *>(lds) + tid * (KC + 4));
-    auto quad = [](uint2 v, int k) {
-        const uint32_t w = k < 2 ? v.x : v.y;
-        return __half2float(__ushort_as_half(static_cast<unsigned short>((k & 1) ? (w >> 16) : (w & 0xffffu))));
-    };
+    constexpr int kBlocks = KC / 32;
+    float bs[kBlocks];
     float se = 0.0f;
-    for (int c4 = 0; c4 < KC / 4; ++c4) {
-        const uint2 v = row[c4];
-        se += quad(v, 0);
-        se += quad(v, 1);
-        se += quad(v, 2);
-        se += quad(v, 3);
+#pragma unroll
+    for (int b = 0; b < kBlocks; ++b) {
+        float sum = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float4 f = unpack_h4(row[8 * b + q]);
+            sum += f.x;
+            sum += f.y;
+            sum += f.z;
+            sum += f.w;
+        }
+        bs[b] = sum;
+        se += sum;
     }
     const float rse = 1.0f / se;
-    rt::Pcg rng = rt::pcg_seed(a.seed, p);
-    // choose which candidates come from the MLP, and their sorted cluster uniforms
-    constexpr int kMaxM = 16;
-    const int M = static_cast<int>(a.M);
-    float uc[kMaxM];
-    int m = 0;
-    uint32_t srcMask = 0;
-    for (int j = 0; j < M; ++j) {
-        if (rt::pcg_float(rng) < a.mix) srcMask |= 1u << j;
-        else uc[m++] = rt::pcg_float(rng) * se;
-    }
-    for (int i = 1; i < m; ++i) {  // insertion sort, m <= 16
-        const float v = uc[i];
-        int j = i - 1;
-        while (j >= 0 && uc[j] > v) {
-            uc[j + 1] = uc[j];
-            --j;
+    // cluster of the draw t in [0, se): the first c whose running sum exceeds t (numerators of
+    // zero are never chosen); rounding overflow falls to the last cluster of the block
+    auto pick_cluster = [&](float t) {
+        int blk = kBlocks - 1;
+        float base = 0.0f, cum = 0.0f;
+        bool found = false;
+#pragma unroll
+        for (int b = 0; b < kBlocks; ++b) {
+            const float nx = cum + bs[b];
+            if (!found && (t < nx || b == kBlocks - 1)) {
+                blk = b;
+                base = cum;
+                found = true;
+            }
+            cum = nx;
         }
-        uc[j + 1] = v;
-    }
-    int ccl[kMaxM];
-    {
-        int c = 0, i = 0;
-        uint2 grp = row[0];
-        float cdf = quad(grp, 0);
-        while (i < m) {
-            if (uc[i] < cdf || c == KC - 1) {
-                ccl[i++] = c;
-            } else {
-                ++c;
-                if ((c & 3) == 0) grp = row[c >> 2];
-                cdf += quad(grp, c & 3);
+        int c = 32 * blk + 31;
+        float run = base;
+        bool hit = false;
+        for (int q = 0; q < 8 && !hit; ++q) {
+            const float4 f = unpack_h4(row[8 * blk + q]);
+            const float e4[4] = {f.x, f.y, f.z, f.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                run += e4[k];
+                if (!hit && t < run) {
+                    c = 32 * blk + 4 * q + k;
+                    hit = true;
+                }
             }
         }
-    }
+        return c;
+    };
+    rt::Pcg rng = rt::pcg_seed(a.seed, p);
+    const int M = static_cast<int>(a.M);
     // Candidates in groups of 8, each step's loads issued together (no load waits on another
-    // candidate's). Random draws are consumed in candidate order, two per candidate.
-    int mi = 0;
+    // candidate's). Per candidate: one draw for the mixture, one for the cluster (MLP
+    // candidates), then two for the alias column.
     for (int j0 = 0; j0 < M; j0 += 8) {
         float r1[8], r2[8], pw[8], ps[8];
         int32_t cl[8], col[8], cnt[8], beg[8], l[8];
@@ -486,9 +501,10 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
             const int j = j0 + jj;
             cl[jj] = -2;  // beyond M
             if (j < M) {
+                const bool src = rt::pcg_float(rng) < a.mix;
+                cl[jj] = src ? -1 : pick_cluster(rt::pcg_float(rng) * se);
                 r1[jj] = rt::pcg_float(rng);
                 r2[jj] = rt::pcg_float(rng);
-                cl[jj] = (srcMask & (1u << j)) ? -1 : ccl[mi++];
             }
         }
         // step 1: source candidates read their alias column, MLP candidates their cluster
@@ -545,7 +561,7 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
             float q = 0.0f;
             if (l[jj] >= 0) {
                 const int c = cl[jj];
-                const float pi = quad(row[c >> 2], c & 3) * rse;
+                const float pi = __half2float(reinterpret_cast<const __half *>(row)[c]) * rse;
                 q = (1.0f - a.mix) * pi * pw[jj] + a.mix * ps[jj];
             }
             a.candLight[static_cast<size_t>(j) * a.npx + p] = l[jj];

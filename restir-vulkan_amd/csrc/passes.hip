@@ -399,10 +399,10 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c(const PassCArgs a) {
 // ------------------------------------------------------------------- Pass C, ray queues ---
 // The same pass as k_pass_c, with its shadow rays moved out of the per-pixel lanes into two
 // queues traced by the persistent tracer: (1) merge every pixel and queue its centre ray;
-// (2) trace; (3) queue the neighbour rays of pixels whose centre sees the light (the others have
-// Z = 0 whatever their neighbours give); (4) trace; (5) form Z and W. Rays are appended with one
-// atomic per wave, slots handed out by prefix sums over the wave, so a queue holds only live rays
-// in pixel-tile order.
+// (2) trace the centre rays, each one that sees the light queueing its pixel's neighbour rays
+// as it finishes (the others have Z = 0 whatever their neighbours give); (3) trace the neighbour
+// rays; (4) form Z and W. Rays are appended with one atomic per wave, slots handed out by prefix
+// sums over the wave, so a queue holds only live rays.
 struct CQueues {
     uint32_t *counter;     // [0] centre rays queued, [1] neighbour rays queued
     float4 *c1, *c2;       // centre ray end points; c1.w carries the pixel index
@@ -484,32 +484,6 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
 }
 
 template <int TX, int TY>
-__global__ __launch_bounds__(TX *TY) void k_pass_c_emit(const PassCArgs a, const CQueues q) {
-    uint32_t x, y;
-    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
-    const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
-    const bool inside = x < W && y < H;
-    const size_t idx = inside ? static_cast<size_t>(y) * W + x : 0;
-    uint32_t want = 0;
-    if (inside) {
-        const uint32_t b = q.bits[idx];
-        if (!(b & (kBitBackground | kBitDone | kBitCentreShadowed))) want = b & 0xffffu;
-    }
-    const uint32_t slot = wave_append(&q.counter[1], static_cast<uint32_t>(__popc(want)));
-    if (want) {
-        const float4 l = reinterpret_cast<const float4 *>(a.out + idx)[0];
-        uint32_t s = slot;
-        for (uint32_t m = want; m; m &= m - 1) {
-            const uint32_t j = static_cast<uint32_t>(__ffs(static_cast<int>(m)) - 1);
-            const f3 nwp = plane3(a.g.pos, q.nIdx[static_cast<size_t>(j) * q.npx + idx], a.g.P);
-            q.n1[s] = make_float4(nwp.x, nwp.y, nwp.z, __uint_as_float(static_cast<uint32_t>(idx) | (j << 27)));
-            q.n2[s] = make_float4(l.x, l.y, l.z, 0.0f);
-            ++s;
-        }
-    }
-}
-
-template <int TX, int TY>
 __global__ __launch_bounds__(TX *TY) void k_pass_c_finish(const PassCArgs a, const CQueues q) {
     uint32_t x, y;
     tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
@@ -535,11 +509,15 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_finish(const PassCArgs a, con
 // Persistent tracer over a queue whose length a previous kernel left in *count. Results go to
 // the pixel named by the ray's tag: a centre ray sets kBitCentreShadowed in bits[pixel]; a
 // neighbour ray sets bit `slot` of occ[pixel].
+// Centre rays additionally emit their pixel's neighbour rays as soon as they finish unshadowed:
+// the centre re-test decides Z = 0 by itself (unbiasedReuse.glsl:157-166) and the neighbour
+// tests draw no random numbers, so only lit centres need neighbour rays, and emitting them here
+// saves a full-screen pass over the pixels. `g` (positions) is read by centre rays only.
 template <int STACK, bool CENTRE>
 __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const float4 *__restrict__ p1,
                                                      const float4 *__restrict__ p2,
                                                      const uint32_t *__restrict__ count,
-                                                     uint32_t *__restrict__ head, CQueues q) {
+                                                     uint32_t *__restrict__ head, CQueues q, const GView g) {
     __shared__ int32_t stack[STACK * 256];
     int32_t *stk = stack + threadIdx.x;
     const uint32_t n = *count;
@@ -548,6 +526,7 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
     Walk w;
     int32_t ray = -1;
     uint32_t tag = 0;
+    f3 lp{0.0f, 0.0f, 0.0f};  // light end of a centre ray: the end point of its neighbour rays
     bool drained = false;
     for (;;) {
         const uint64_t idle = __ballot(ray < 0);
@@ -562,11 +541,13 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
                 const uint32_t i = base + static_cast<uint32_t>(__popcll(idle & below));
                 if (i < n) {
                     const float4 a4 = p1[i];
+                    const f3 b = xyz(p2[i]);
                     f3 o, d;
-                    shadow_segment(xyz(a4), xyz(p2[i]), o, d);
+                    shadow_segment(xyz(a4), b, o, d);
                     walk_begin(w, o, d);
                     tag = __float_as_uint(a4.w);
                     ray = static_cast<int32_t>(i);
+                    if (CENTRE) lp = b;
                 }
             }
         }
@@ -574,7 +555,23 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
             if (drained) break;
             continue;
         }
-        if (ray >= 0 && walk_step<STACK, false>(w, sc.nodes, sc.tris, stk, 256)) {
+        bool fin = false;
+        if (ray >= 0) fin = walk_step<STACK, false>(w, sc.nodes, sc.tris, stk, 256);
+        if constexpr (CENTRE) {
+            const bool lit = fin && !w.hit;
+            if (__ballot(lit) != 0) {  // uniform: the whole wave appends together
+                const uint32_t above = lit ? (q.bits[tag] & 0xffffu) : 0u;
+                uint32_t slot = wave_append(&q.counter[1], static_cast<uint32_t>(__popc(above)));
+                for (uint32_t m = above; m; m &= m - 1) {
+                    const uint32_t j = static_cast<uint32_t>(__ffs(static_cast<int>(m)) - 1);
+                    const f3 nwp = plane3(g.pos, q.nIdx[static_cast<size_t>(j) * q.npx + tag], g.P);
+                    q.n1[slot] = make_float4(nwp.x, nwp.y, nwp.z, __uint_as_float(tag | (j << 27)));
+                    q.n2[slot] = make_float4(lp.x, lp.y, lp.z, 0.0f);
+                    ++slot;
+                }
+            }
+        }
+        if (fin) {
             if (w.hit) {
                 if (CENTRE) q.bits[tag] |= kBitCentreShadowed;
                 else atomicOr(&q.occ[tag & 0x07ffffffu], 1u << (tag >> 27));
@@ -922,10 +919,9 @@ static hipError_t run_c_queued(PassCArgs args, const CQueues &q, uint32_t gridCa
     RK_LAUNCH("k_pass_c_merge", (k_pass_c_merge<TX, TY>), tiles, block, 0, s, args, q);
     if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
         RK_LAUNCH("k_trace_queue(centre)", (k_trace_queue<STACK, true>), dim3(rayBlocks), dim3(256), 0, s,
-                  args.sc, q.c1, q.c2, q.counter + 0, q.counter + 2, q);
-        RK_LAUNCH("k_pass_c_emit", (k_pass_c_emit<TX, TY>), tiles, block, 0, s, args, q);
+                  args.sc, q.c1, q.c2, q.counter + 0, q.counter + 2, q, args.g);
         RK_LAUNCH("k_trace_queue(neighbour)", (k_trace_queue<STACK, false>), dim3(rayBlocks), dim3(256), 0,
-                  s, args.sc, q.n1, q.n2, q.counter + 1, q.counter + 3, q);
+                  s, args.sc, q.n1, q.n2, q.counter + 1, q.counter + 3, q, args.g);
         RK_LAUNCH("k_pass_c_finish", (k_pass_c_finish<TX, TY>), tiles, block, 0, s, args, q);
     }
     return hipGetLastError();

@@ -16,7 +16,7 @@ TILES = ((8, 8), (16, 8), (16, 16), (32, 8), (64, 1))
 _PASS_KERNELS = {
     "a": ("k_pass_a",),
     "b": ("k_pass_b",),
-    "c": ("k_pass_c_merge", "k_trace_queue(centre)", "k_pass_c_emit", "k_trace_queue(neighbour)",
+    "c": ("k_pass_c_merge", "k_trace_queue(centre)", "k_trace_queue(neighbour)",
           "k_pass_c_finish", "k_pass_c"),
 }
 

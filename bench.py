@@ -76,10 +76,11 @@ def parse():
     ap.add_argument("--spatial-passes", type=int, default=2)
     ap.add_argument("--tile", type=str, default="16x16")
     ap.add_argument("--trace-reps", type=int, default=20)
-    ap.add_argument("--pretrain", type=int, default=64, help="untimed training frames of the MLP")
+    ap.add_argument("--pretrain", type=int, default=256, help="untimed training frames of the MLP")
     ap.add_argument("--error-frames", type=int, default=48, help="independent frames per method")
     ap.add_argument("--error-pixels", type=int, default=16384)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-views", action="store_true", help="skip the batched 16-view measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=8, help="1080p frames of the CPU-oracle sample")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -300,6 +301,9 @@ def main():
     if not args.no_learned:
         out.update(learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms))
 
+    if not args.no_views:
+        out["batched_views"] = batched_views(torch, args)
+
     if args.copy_calibration:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
@@ -390,6 +394,67 @@ def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
             "speedup_learned_vs_plain": round(float(speedup), 4),
         },
     }
+
+
+def batched_views(torch, args):
+    """16 views (resolutions log-uniform in [640x360, 2560x1440], seeded) over 4 seeded terrains
+    (about 25k-200k triangles, 1k-50k point lights), unbiased ReSTIR M = 32 with temporal reuse
+    and 2 Pass C: 8 orbiting frames of all views captured in one HIP graph of persistent
+    multi-view launches, against the same frames rendered view by view (eager, one launch
+    sequence per view, queued Pass C)."""
+    from restir_amd import scenes
+    from restir_amd.pipeline import BatchRenderer, FrameSequenceGraph, Renderer
+
+    rng = np.random.default_rng(3)
+    specs = [(110, 60, 1000), (160, 200, 5000), (220, 400, 20000), (300, 1000, 50000)]
+    scs = [scenes.terrain(20 + i, grid=g, n_boxes=b, n_point_lights=nl, extent=32.0)
+           for i, (g, b, nl) in enumerate(specs)]
+    views = []
+    for v in range(16):
+        h = int(round(360 * 4.0 ** rng.random()))
+        views.append((scs[v % 4], int(round(h * 16 / 9)), h))
+    px = sum(w * h for _, w, h in views)
+    frames = 8
+    seq = [[scenes.orbit_camera(sc.camera, f) for sc, _, _ in views] for f in range(frames)]
+    kw = dict(mode="unbiased", k=args.k, M=args.M, spatial_passes=args.spatial_passes)
+    b = BatchRenderer(views, **kw)
+    b.render_frame(seq[0])
+    torch.cuda.synchronize()
+    b.reset()
+    fg = FrameSequenceGraph(b, seq)
+    fg.replay()  # warm
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 3
+    e0.record()
+    for _ in range(reps):
+        fg.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    graph_ms = e0.elapsed_time(e1) / (reps * frames)
+    del fg, b
+    rs = [Renderer(sc, w, h, **kw) for sc, w, h in views]
+    for r, cam in zip(rs, seq[0]):
+        r.render_frame(cam)
+    torch.cuda.synchronize()
+    e0.record()
+    for cams in seq:
+        for r, cam in zip(rs, cams):
+            r.render_frame(cam)
+    e1.record()
+    torch.cuda.synchronize()
+    eager_ms = e0.elapsed_time(e1) / frames
+    del rs
+    torch.cuda.empty_cache()
+    log(f"batched views: 16 views, {px / 1e6:.1f} Mpx/frame: graph of 8 frames {graph_ms:.2f} ms/frame, "
+        f"view-by-view {eager_ms:.2f} ms/frame")
+    return {"views": 16, "scenes": [int(sc.n_tris) for sc in scs],
+            "lights": [int(len(sc.point_lights)) for sc in scs],
+            "resolutions": [f"{w}x{h}" for _, w, h in views], "pixels_per_frame": px,
+            "frames_per_graph": frames, "graph_ms_per_frame": round(graph_ms, 3),
+            "view_by_view_ms_per_frame": round(eager_ms, 3),
+            "graph_mpx_per_s": round(px / (graph_ms * 1e-3) / 1e6, 1),
+            "speedup": round(eager_ms / graph_ms, 3)}
 
 
 def cpu_baseline(fr, scene, sd, args, rays):

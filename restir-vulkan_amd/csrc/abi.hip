@@ -98,6 +98,18 @@ int ensure(restir_ctx *ctx, void **buf, size_t *have, size_t bytes) {
     return RESTIR_OK;
 }
 
+// Workgroups of a persistent launch: as many 256-lane groups as fit a CU (160 KB of LDS at
+// `ldsBytes` each, at most 8 groups = 32 waves), on every CU.
+uint32_t resident_blocks(uint32_t cus, uint32_t ldsBytes) {
+    uint32_t perCu = ldsBytes > 0 ? 163840u / ldsBytes : 8u;
+    perCu = perCu > 8u ? 8u : (perCu < 1u ? 1u : perCu);
+    return cus * perCu;
+}
+// the persistent tracers keep rt::kWalkRegs stack entries in registers, the rest in LDS
+uint32_t persistent_blocks(uint32_t cus, int stack) {
+    return resident_blocks(cus, static_cast<uint32_t>(stack - rt::kWalkRegs) * 256u * 4u);
+}
+
 int ensure_scratch(restir_ctx *ctx, size_t bytes) { return ensure(ctx, &ctx->scratch, &ctx->scratchBytes, bytes); }
 
 int pick_tile(restir_ctx *ctx, const restir_launch *l, int *tile) {
@@ -133,7 +145,8 @@ int restir_create(restir_ctx **out, int device) {
     }
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
-        ctx->persistentBlocks = static_cast<uint32_t>(cus) * 5u;  // 5 x 256 lanes fit a CU's LDS
+        ctx->cus = static_cast<uint32_t>(cus);
+    ctx->persistentBlocks = persistent_blocks(ctx->cus, ctx->stack);
     *out = ctx;
     return RESTIR_OK;
 }
@@ -236,7 +249,9 @@ int restir_set_bvh(restir_ctx *ctx, const restir_aabb_tree_node *nodes, uint32_t
         return fail(ctx, RESTIR_E_BAD_TREE, "set_bvh: tree depth %u exceeds the 64-entry traversal stack", depth);
     ctx->scene.nodes = reinterpret_cast<const rt::Node *>(nodes);
     ctx->scene.tris = reinterpret_cast<const float4 *>(tris);
-    ctx->stack = depth <= 32 ? 32 : 64;
+    // a walk defers at most one sibling per internal level below the root: depth - 1 entries
+    ctx->stack = depth <= 24 ? 24 : (depth <= 32 ? 32 : 64);
+    ctx->persistentBlocks = persistent_blocks(ctx->cus, ctx->stack);
     ctx->treeDepth = depth;
     ctx->bvhSet = true;
     return RESTIR_OK;

@@ -65,7 +65,7 @@ int restir_initial_temporal_views(restir_ctx *ctx, const restir_view *views, uin
     if (int r = pick_tile(ctx, launch, &tile)) return r;
     std::vector<rk::ViewA> rec(n);
     std::vector<Span> outs, ins;
-    int stack = 32;
+    int stack = 24;  // the smallest compiled stack; raised to the deepest scene's
     for (uint32_t v = 0; v < n; ++v) {
         const restir_view &w = views[v];
         const restir_uniforms *u = w.uniforms;
@@ -89,7 +89,7 @@ int restir_initial_temporal_views(restir_ctx *ctx, const restir_view *views, uin
     if (int r = check_disjoint(ctx, outs, ins, "initial_temporal_views")) return r;
     if (int r = prepare(ctx, n)) return r;
     const hipError_t e = rk::launch_pass_a_views(tile, stack, rec.data(), n, ctx->viewTable, view_counter(ctx),
-                                                 ctx->persistentBlocks, static_cast<hipStream_t>(stream));
+                                                 resident_blocks(ctx->cus, static_cast<uint32_t>(stack) * 1024u), static_cast<hipStream_t>(stream));
     return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "initial_temporal_views launch");
 }
 
@@ -129,7 +129,7 @@ int restir_unbiased_views(restir_ctx *ctx, const restir_view *views, uint32_t n,
     if (int r = pick_tile(ctx, launch, &tile)) return r;
     std::vector<rk::ViewC> rec(n);
     std::vector<Span> outs, ins;
-    int stack = 32;
+    int stack = 24;  // the smallest compiled stack; raised to the deepest scene's
     for (uint32_t v = 0; v < n; ++v) {
         const restir_view &w = views[v];
         const restir_uniforms *u = w.uniforms;
@@ -148,7 +148,7 @@ int restir_unbiased_views(restir_ctx *ctx, const restir_view *views, uint32_t n,
     if (int r = check_disjoint(ctx, outs, ins, "unbiased_views")) return r;
     if (int r = prepare(ctx, n)) return r;
     const hipError_t e = rk::launch_pass_c_views(tile, stack, rec.data(), n, k, iter, ctx->viewTable,
-                                                 view_counter(ctx), ctx->persistentBlocks,
+                                                 view_counter(ctx), resident_blocks(ctx->cus, static_cast<uint32_t>(stack) * 1024u),
                                                  static_cast<hipStream_t>(stream));
     return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "unbiased_views launch");
 }

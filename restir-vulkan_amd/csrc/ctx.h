@@ -32,7 +32,8 @@ struct restir_ctx {
     int stack = 32;
     uint32_t treeDepth = 0;
     uint32_t *counters = nullptr;  // device work counters of the persistent kernels
-    uint32_t persistentBlocks = 1280;
+    uint32_t cus = 256;                // compute units of the device
+    uint32_t persistentBlocks = 1280;  // workgroups of the persistent tracers (see persistent_blocks)
     void *scratch = nullptr;       // ray queues of the queued Pass C schedule
     size_t scratchBytes = 0;
     uint32_t queuedNpx = 0;        // shape of the last queued Pass C (its rays stay in scratch)
@@ -73,4 +74,6 @@ rk::GView view_of(const restir_gbuffer *g);
 int check_frame(restir_ctx *ctx, const restir_uniforms *u, const char *name);
 int pick_tile(restir_ctx *ctx, const restir_launch *l, int *tile);
 int ensure(restir_ctx *ctx, void **buf, size_t *have, size_t bytes);
+uint32_t resident_blocks(uint32_t cus, uint32_t ldsBytes);
+uint32_t persistent_blocks(uint32_t cus, int stack);
 }  // namespace rki

@@ -11,6 +11,7 @@
 // G-buffer planes are read as SoA float planes (one coalesced dword per lane per plane);
 // reservoirs keep the reference's 64-byte AoS record and move as four 16-byte vectors.
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "kernels.h"
@@ -25,6 +26,15 @@ int tile_index(uint32_t tx, uint32_t ty) {
     for (int i = 0; i < kNumTiles; ++i)
         if (kTiles[i].tx == tx && kTiles[i].ty == ty) return i;
     return -1;
+}
+
+// Calls f with the compiled traversal-stack depth that covers `stack` entries (24, 32 or 64
+// dwords per lane of LDS: the smaller the stack, the more workgroups share a CU).
+template <class F>
+static hipError_t by_stack(int stack, F &&f) {
+    if (stack <= 24) return f(std::integral_constant<int, 24>{});
+    if (stack <= 32) return f(std::integral_constant<int, 32>{});
+    return f(std::integral_constant<int, 64>{});
 }
 
 // Hardware places workgroup b on XCD b % 8. Remap so XCD k processes tiles
@@ -518,7 +528,7 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
                                                      const float4 *__restrict__ p2,
                                                      const uint32_t *__restrict__ count,
                                                      uint32_t *__restrict__ head, CQueues q, const GView g) {
-    __shared__ int32_t stack[STACK * 256];
+    __shared__ int32_t stack[(STACK - kWalkRegs) * 256];  // the top kWalkRegs entries are registers
     int32_t *stk = stack + threadIdx.x;
     const uint32_t n = *count;
     const uint32_t lane = threadIdx.x & 63u;
@@ -657,7 +667,7 @@ __global__ __launch_bounds__(256) void k_trace(const TraceArgs a) {
 // lanes whose rays ended early do not wait for the slowest ray of their wave.
 template <int STACK, bool REF>
 __global__ __launch_bounds__(256) void k_trace_pool(const TraceArgs a, uint32_t *__restrict__ head) {
-    __shared__ int32_t stack[STACK * 256];
+    __shared__ int32_t stack[(STACK - kWalkRegs) * 256];  // the top kWalkRegs entries are registers
     int32_t *stk = stack + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
@@ -872,7 +882,7 @@ hipError_t launch_pass_a(int tile, int stack, const restir_uniforms &u, const Sc
                          restir_reservoir *out, const int32_t *candLight, const float *candQ,
                          hipStream_t s) {
     PassAArgs args{u, sc, cur, prev, prevRes, out, 0, 0, candLight, candQ};
-    return stack <= 32 ? dispatch_a<32>(tile, args, s) : dispatch_a<64>(tile, args, s);
+    return by_stack(stack, [&](auto S) { return dispatch_a<decltype(S)::value>(tile, args, s); });
 }
 
 template <int TX, int TY>
@@ -995,10 +1005,9 @@ hipError_t launch_pass_c(int tile, int stack, const restir_uniforms &u, int k, i
     PassCArgs args{u, sc, g, in, out, 0, 0, k, u.frame * 17u + static_cast<uint32_t>(iter)};
     if (queueScratch) {
         const CQueues q = carve_queues(queueScratch, u.screenSize[0] * u.screenSize[1], k);
-        return stack <= 32 ? dispatch_c_queued<32>(tile, args, q, gridCap, s)
-                           : dispatch_c_queued<64>(tile, args, q, gridCap, s);
+        return by_stack(stack, [&](auto S) { return dispatch_c_queued<decltype(S)::value>(tile, args, q, gridCap, s); });
     }
-    return stack <= 32 ? dispatch_c<32>(tile, args, s) : dispatch_c<64>(tile, args, s);
+    return by_stack(stack, [&](auto S) { return dispatch_c<decltype(S)::value>(tile, args, s); });
 }
 
 // ------------------------------------------------------------------------ batched views ---
@@ -1117,8 +1126,10 @@ hipError_t launch_pass_a_views(int tile, int stack, const ViewA *views, uint32_t
         tiles_of(tile, w.u.screenSize[0], w.u.screenSize[1], recs[v].tilesX, recs[v].nTiles);
         tiles[v] = recs[v].nTiles;
     }
-    return stack <= 32 ? dispatch_views<kViewA, PassAArgs, 32>(tile, recs.data(), tiles.data(), n, table, counter, gridCap, s)
-                       : dispatch_views<kViewA, PassAArgs, 64>(tile, recs.data(), tiles.data(), n, table, counter, gridCap, s);
+    return by_stack(stack, [&](auto S) {
+        return dispatch_views<kViewA, PassAArgs, decltype(S)::value>(tile, recs.data(), tiles.data(), n, table,
+                                                                     counter, gridCap, s);
+    });
 }
 
 hipError_t launch_pass_b_views(int tile, const ViewB *views, uint32_t n, int iter, void *table,
@@ -1144,8 +1155,10 @@ hipError_t launch_pass_c_views(int tile, int stack, const ViewC *views, uint32_t
         tiles_of(tile, w.u.screenSize[0], w.u.screenSize[1], recs[v].tilesX, recs[v].nTiles);
         tiles[v] = recs[v].nTiles;
     }
-    return stack <= 32 ? dispatch_views<kViewC, PassCArgs, 32>(tile, recs.data(), tiles.data(), n, table, counter, gridCap, s)
-                       : dispatch_views<kViewC, PassCArgs, 64>(tile, recs.data(), tiles.data(), n, table, counter, gridCap, s);
+    return by_stack(stack, [&](auto S) {
+        return dispatch_views<kViewC, PassCArgs, decltype(S)::value>(tile, recs.data(), tiles.data(), n, table,
+                                                                     counter, gridCap, s);
+    });
 }
 
 hipError_t launch_shade_views(const ViewShade *views, uint32_t n, float gamma, void *table,
@@ -1186,14 +1199,12 @@ hipError_t launch_trace(int stack, bool refLoop, bool persistent, uint32_t gridC
                         uint32_t n, uint32_t *shadowed, uint32_t *visits, uint32_t *tests,
                         hipStream_t s) {
     TraceArgs args{sc, p1, p2, n, shadowed, visits, tests};
-    if (stack <= 32) {
-        if (refLoop) run_trace<32, true>(persistent, gridCap, counter, args, s);
-        else run_trace<32, false>(persistent, gridCap, counter, args, s);
-    } else {
-        if (refLoop) run_trace<64, true>(persistent, gridCap, counter, args, s);
-        else run_trace<64, false>(persistent, gridCap, counter, args, s);
-    }
-    return hipGetLastError();
+    return by_stack(stack, [&](auto S) {
+        constexpr int kS = decltype(S)::value;
+        if (refLoop) run_trace<kS, true>(persistent, gridCap, counter, args, s);
+        else run_trace<kS, false>(persistent, gridCap, counter, args, s);
+        return hipGetLastError();
+    });
 }
 
 CamBasis make_cam_basis(const restir_camera &c, float aspect) {

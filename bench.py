@@ -107,6 +107,7 @@ class Frames:
         self.rgb = torch.zeros(3 * npx, dtype=torch.float32, device=dev)
         self.flags = abi.VISIBILITY_REUSE_FLAG | abi.TEMPORAL_REUSE_FLAG
         self.learned = None
+        self.launch_a = launch  # Pass A's shape and ray schedule (choose_pass_a_schedule)
 
     def uniforms(self, f, M, seed_frame=None, static=False):
         fr = f + 1 if seed_frame is None else seed_frame
@@ -127,9 +128,9 @@ class Frames:
         first = self.tmp if n_sp % 2 == 1 else self.res[i]
         if learned is not None:
             cl, cq = learned.propose(u, g, M, s)
-            ctx.initial_temporal_candidates(u, g, gp, self.res[p], first, cl, cq, self.launch, s)
+            ctx.initial_temporal_candidates(u, g, gp, self.res[p], first, cl, cq, self.launch_a, s)
         else:
-            ctx.initial_temporal(u, g, gp, self.res[p], first, self.launch, s)
+            ctx.initial_temporal(u, g, gp, self.res[p], first, self.launch_a, s)
         src = first
         for j in range(n_sp):
             dst = self.res[i] if (n_sp - j) % 2 == 1 else self.tmp
@@ -143,6 +144,28 @@ class Frames:
     def run(self, f, M, **kw):
         for _ in self.frame(f, M, **kw):
             pass
+
+    def choose_pass_a_schedule(self, f, M, learned=None, reps=5):
+        """Times Pass A of frame f on both ray schedules (queued: RIS, ray queue, finish; inline:
+        one dispatch) and keeps the faster; both give identical reservoirs. Returns the record."""
+        torch, ctx, abi = self.torch, self.ctx, self.abi
+        table = {}
+        for name, sched in (("queued", abi.SCHEDULE_QUEUED), ("inline", abi.SCHEDULE_INLINE)):
+            self.launch_a = abi.Launch(self.launch.tile_x, self.launch.tile_y, sched)
+            self.run(f, M, learned=learned, train=False)
+            torch.cuda.synchronize()
+            ctx.profile_read()
+            ctx.profile_enable(True)
+            for _ in range(reps):
+                self.run(f, M, learned=learned, train=False)
+            rows = ctx.profile_read()
+            ctx.profile_enable(False)
+            table[name] = sum(rows[k][1] for k in ("k_pass_a", "k_pass_a_ris", "k_trace_queue(pass_a)",
+                                                   "k_pass_a_finish") if k in rows) / reps
+        best = min(table, key=table.get)
+        self.launch_a = abi.Launch(self.launch.tile_x, self.launch.tile_y,
+                                   abi.SCHEDULE_QUEUED if best == "queued" else abi.SCHEDULE_INLINE)
+        return {"choice": best, "ms": {k: round(v, 4) for k, v in table.items()}}
 
     def timed(self, first, count, M, learned=None):
         """count frames from `first`, every kernel timed with events; (wall ms/frame, kernels)."""
@@ -199,6 +222,11 @@ def main():
     fr = Frames(torch, ctx, scene, cams, G, W, H, args, launch, stream)
 
     # ---- plain pipeline (the reference's configuration)
+    for f in range(WU):
+        fr.run(f, args.M)
+    sched_plain = fr.choose_pass_a_schedule(WU - 1, args.M)
+    for t in fr.res:
+        t.zero_()
     for f in range(WU):
         fr.run(f, args.M)
     plain_ms, kernels = fr.timed(WU, K, args.M)
@@ -283,7 +311,8 @@ def main():
             "point_lights": int(len(scene.point_lights)), "M_plain": args.M, "M_learned": args.learned_M,
             "k": args.k, "spatial_passes": args.spatial_passes, "tile": args.tile,
         },
-        "plain": {"ms_per_frame": round(plain_ms, 4), "kernels": rounded(kernels)},
+        "plain": {"ms_per_frame": round(plain_ms, 4), "pass_a_schedule": sched_plain,
+                  "kernels": rounded(kernels)},
         "mrays_per_s": round(rays_per_s / 1e6, 1),
         "traversal": {"rays": nr, "ray_set": "last frame's second Pass C neighbour rays",
                       "node_visits_per_ray": round(Vn, 2), "tri_tests_per_ray": round(Tn, 2),
@@ -329,6 +358,7 @@ def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
         fr.run(i % nframes, args.learned_M, learned=lp)
     torch.cuda.synchronize()
     log(f"learned: {args.pretrain} training frames ({time.time() - t0:.1f}s), loss {lp.loss.item():.3f}")
+    sched_learned = fr.choose_pass_a_schedule(WU - 1, args.learned_M, learned=lp)
     for t in fr.res:
         t.zero_()
     for f in range(WU):
@@ -380,7 +410,8 @@ def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
     return {
         "value": round(value, 4),
         "ms_per_step": round(learned_ms, 4),
-        "learned": {"ms_per_frame": round(learned_ms, 4), "kernels": lk, "train_loss": round(lp.loss.item(), 4),
+        "learned": {"ms_per_frame": round(learned_ms, 4), "pass_a_schedule": sched_learned, "kernels": lk,
+                    "train_loss": round(lp.loss.item(), 4),
                     "pretrain_frames": args.pretrain},
         "equal_rmse": {
             "protocol": "static camera, independent frames (history cleared), FP64 GPU ground truth on "

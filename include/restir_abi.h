@@ -137,10 +137,11 @@ typedef struct restir_gbuffer {
 } restir_gbuffer;
 
 /* Launch shape of the per-pixel passes: the workgroup tile (pixels per workgroup = tile_x *
- * tile_y, a multiple of 64; {0,0} = built-in default) and the schedule of Pass C's shadow rays:
- * RESTIR_SCHEDULE_QUEUED (default) moves them into queues traced by a persistent kernel, using
- * context-owned scratch; RESTIR_SCHEDULE_INLINE traces them per pixel in a single dispatch, as
- * the reference does. Both produce identical reservoirs. */
+ * tile_y, a multiple of 64; {0,0} = built-in default) and the schedule of the shadow rays of
+ * Pass A (visibility reuse) and Pass C: RESTIR_SCHEDULE_QUEUED (default) moves them into queues
+ * traced by a persistent kernel, using context-owned scratch; RESTIR_SCHEDULE_INLINE traces them
+ * per pixel in a single dispatch, as the reference does. Both produce identical reservoirs.
+ * (Pass C falls back to the inline schedule at 2^27 pixels and more.) */
 #define RESTIR_SCHEDULE_QUEUED 0
 #define RESTIR_SCHEDULE_INLINE 1
 typedef struct restir_launch {
@@ -414,7 +415,8 @@ int restir_profile_enable(restir_ctx *ctx, int enable);
 int restir_profile_read(restir_ctx *ctx, restir_kernel_time *out, uint32_t capacity,
                         uint32_t *count);
 
-/* The shadow rays of the last queued-schedule restir_unbiased call on this context (which = 0:
+/* The shadow rays of the last queued-schedule restir_unbiased call on this context, unless a
+ * queued Pass A ran since (the queues share scratch) (which = 0:
  * the centre re-tests, 1: the neighbour tests), as p1/p2 float4 pairs in queue order (p1.w holds a
  * pixel tag, ignored by restir_trace_shadow). Copies min(queue length, capacity) rays on `stream`
  * and stores the queue length in *count (a host pointer; the call synchronizes `stream`). Lets a

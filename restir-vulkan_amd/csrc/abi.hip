@@ -105,12 +105,29 @@ uint32_t resident_blocks(uint32_t cus, uint32_t ldsBytes) {
     perCu = perCu > 8u ? 8u : (perCu < 1u ? 1u : perCu);
     return cus * perCu;
 }
-// the persistent tracers keep rt::kWalkRegs stack entries in registers, the rest in LDS
+// the persistent tracers' 256-lane groups each hold a `stack`-entry LDS stack per lane
 uint32_t persistent_blocks(uint32_t cus, int stack) {
-    return resident_blocks(cus, static_cast<uint32_t>(stack - rt::kWalkRegs) * 256u * 4u);
+    return resident_blocks(cus, static_cast<uint32_t>(stack) * 256u * 4u);
 }
 
 int ensure_scratch(restir_ctx *ctx, size_t bytes) { return ensure(ctx, &ctx->scratch, &ctx->scratchBytes, bytes); }
+
+// Binds the device and, for the queued schedule (the default), sizes the context scratch for
+// Pass A's ray queue; *queues stays null for the inline schedule. The Pass C ray export is
+// invalidated: the queues share the scratch.
+int pass_a_queues(restir_ctx *ctx, const restir_launch *launch, uint32_t npx, void **queues) {
+    const uint32_t schedule = launch ? launch->schedule : RESTIR_SCHEDULE_QUEUED;
+    if (schedule != RESTIR_SCHEDULE_QUEUED && schedule != RESTIR_SCHEDULE_INLINE)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "unknown schedule %u", schedule);
+    if (int r = use_device(ctx)) return r;
+    *queues = nullptr;
+    if (schedule == RESTIR_SCHEDULE_QUEUED) {
+        if (int r = ensure_scratch(ctx, rk::pass_a_queue_bytes(npx))) return r;
+        *queues = ctx->scratch;
+        ctx->queuedNpx = 0;
+    }
+    return RESTIR_OK;
+}
 
 int pick_tile(restir_ctx *ctx, const restir_launch *l, int *tile) {
     if (!l || (l->tile_x == 0 && l->tile_y == 0)) {
@@ -277,9 +294,11 @@ int restir_initial_temporal(restir_ctx *ctx, const restir_uniforms *u, const res
     }
     int tile;
     if (int r = pick_tile(ctx, launch, &tile)) return r;
-    if (int r = use_device(ctx)) return r;
+    void *queues = nullptr;
+    if (int r = pass_a_queues(ctx, launch, W * H, &queues)) return r;
     const hipError_t e = rk::launch_pass_a(tile, ctx->stack, *u, ctx->scene, view_of(cur), view_of(prev),
-                                           prevRes, out, nullptr, nullptr, static_cast<hipStream_t>(stream));
+                                           prevRes, out, nullptr, nullptr, queues, ctx->persistentBlocks,
+                                           static_cast<hipStream_t>(stream));
     return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "initial_temporal launch");
 }
 
@@ -315,9 +334,11 @@ int restir_unbiased(restir_ctx *ctx, const restir_uniforms *u, int k, int iter,
     if (overlaps(in, bytes, out, bytes)) return fail(ctx, RESTIR_E_ALIASING, "unbiased: in and out overlap");
     int tile;
     if (int r = pick_tile(ctx, launch, &tile)) return r;
-    const uint32_t schedule = launch ? launch->schedule : RESTIR_SCHEDULE_QUEUED;
+    uint32_t schedule = launch ? launch->schedule : RESTIR_SCHEDULE_QUEUED;
     if (schedule != RESTIR_SCHEDULE_QUEUED && schedule != RESTIR_SCHEDULE_INLINE)
         return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "unbiased: unknown schedule %u", schedule);
+    // a queued neighbour ray carries its pixel in 27 bits of its tag
+    if (static_cast<uint64_t>(W) * H >= (1ull << 27)) schedule = RESTIR_SCHEDULE_INLINE;
     if (int r = use_device(ctx)) return r;
     void *queues = nullptr;
     if (schedule == RESTIR_SCHEDULE_QUEUED) {

@@ -165,9 +165,11 @@ int restir_initial_temporal_candidates(restir_ctx *ctx, const restir_uniforms *u
     }
     int tile;
     if (int r = pick_tile(ctx, launch, &tile)) return r;
-    if (int r = use_device(ctx)) return r;
+    void *queues = nullptr;
+    if (int r = pass_a_queues(ctx, launch, W * H, &queues)) return r;
     const hipError_t e = rk::launch_pass_a(tile, ctx->stack, *u, ctx->scene, view_of(cur), view_of(prev), prevRes,
-                                           out, candLight, candQ, static_cast<hipStream_t>(stream));
+                                           out, candLight, candQ, queues, ctx->persistentBlocks,
+                                           static_cast<hipStream_t>(stream));
     return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "initial_temporal_candidates launch");
 }
 

@@ -73,6 +73,7 @@ int check_gbuffer(restir_ctx *ctx, const restir_gbuffer *g, uint32_t W, uint32_t
 rk::GView view_of(const restir_gbuffer *g);
 int check_frame(restir_ctx *ctx, const restir_uniforms *u, const char *name);
 int pick_tile(restir_ctx *ctx, const restir_launch *l, int *tile);
+int pass_a_queues(restir_ctx *ctx, const restir_launch *launch, uint32_t npx, void **queues);
 int ensure(restir_ctx *ctx, void **buf, size_t *have, size_t bytes);
 uint32_t resident_blocks(uint32_t cus, uint32_t ldsBytes);
 uint32_t persistent_blocks(uint32_t cus, int stack);

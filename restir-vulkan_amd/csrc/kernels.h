@@ -51,10 +51,13 @@ extern const Tile kTiles[kNumTiles];
 int tile_index(uint32_t tx, uint32_t ty);  // -1 if not compiled
 
 // candLight/candQ: the learned proposal's candidates ([M][npx]), or null for alias-table RIS.
+// queueScratch == nullptr: one dispatch, the visibility ray traced inline per pixel. Otherwise
+// the ray-queue schedule, using pass_a_queue_bytes(npx) bytes of device scratch.
 hipError_t launch_pass_a(int tile, int stack, const restir_uniforms &u, const SceneView &sc,
                          const GView &cur, const GView &prev, const restir_reservoir *prevRes,
                          restir_reservoir *out, const int32_t *candLight, const float *candQ,
-                         hipStream_t s);
+                         void *queueScratch, uint32_t gridCap, hipStream_t s);
+size_t pass_a_queue_bytes(uint32_t npx);
 hipError_t launch_pass_b(int tile, const restir_uniforms &u, int iter, float cosThr,
                          const GView &g, const restir_reservoir *in, restir_reservoir *out,
                          hipStream_t s);

@@ -47,13 +47,30 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
     return x * per + (x < rem ? x : rem) + slot;
 }
 
+// Tile t of the blocked order: bands of kBandRows tile rows, each walked in column blocks of
+// kBlockCols tiles (the last band and block may be smaller). An XCD's run of consecutive tiles
+// is then a compact screen region, so the reservoirs that neighbouring pixels gather (up to
+// 30 px away) are shared in its L2 instead of spanning the full screen width.
+constexpr uint32_t kBandRows = 16, kBlockCols = 16;
+__device__ __forceinline__ void blocked_tile(uint32_t t, uint32_t tilesX, uint32_t tilesY,
+                                             uint32_t &tx, uint32_t &ty) {
+    const uint32_t band = t / (kBandRows * tilesX), r = t % (kBandRows * tilesX);
+    const uint32_t h = min(kBandRows, tilesY - band * kBandRows);
+    const uint32_t cb = r / (kBlockCols * h), rr = r % (kBlockCols * h);
+    const uint32_t w = min(kBlockCols, tilesX - cb * kBlockCols);
+    tx = cb * kBlockCols + rr % w;
+    ty = band * kBandRows + rr / w;
+}
+
 template <int TX, int TY>
 __device__ __forceinline__ void tile_pixel(uint32_t tilesX, uint32_t nTiles, uint32_t &px,
                                            uint32_t &py) {
     const uint32_t t = xcd_tile(blockIdx.x, nTiles);
+    uint32_t tx, ty;
+    blocked_tile(t, tilesX, nTiles / tilesX, tx, ty);
     const uint32_t tid = threadIdx.x;
-    px = (t % tilesX) * TX + tid % TX;
-    py = (t / tilesX) * TY + tid / TX;
+    px = tx * TX + tid % TX;
+    py = ty * TY + tid / TX;
 }
 
 __device__ __forceinline__ f3 plane3(const float *__restrict__ p, size_t i, uint64_t P) {
@@ -102,24 +119,35 @@ struct PassAArgs {
     const float *candQ;
 };
 
-// Pass A for pixel (x, y); `stk` is this lane's column of the [level][lane] LDS stack.
-template <int TX, int TY, int STACK>
-__device__ __forceinline__ void pass_a_px(const PassAArgs &a, uint32_t x, uint32_t y, int32_t *stk) {
-    constexpr int kBlock = TX * TY;
-    const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
-    if (x >= W || y >= H) return;
-    const size_t idx = static_cast<size_t>(y) * W + x;
+// Pass A in pieces (restirOmni.glsl:86-212), shared by the inline kernel and the queued schedule.
+struct PassASurface {
+    size_t idx;
+    f3 pos, n, alb;
+    Surface s;
+};
+__device__ __forceinline__ PassASurface pass_a_surface(const PassAArgs &a, uint32_t x, uint32_t y) {
+    PassASurface p;
+    p.idx = static_cast<size_t>(y) * a.u.screenSize[0] + x;
     const GView &g = a.cur;
-    const f3 pos = plane3(g.pos, idx, g.P);
-    const f3 n = plane3(g.nrm, idx, g.P);
-    const f3 alb = plane3(g.alb, idx, g.P);
-    const float rough = g.mat[idx], metal = g.mat[g.P + idx];
+    p.pos = plane3(g.pos, p.idx, g.P);
+    p.n = plane3(g.nrm, p.idx, g.P);
+    p.alb = plane3(g.alb, p.idx, g.P);
+    const float rough = g.mat[p.idx], metal = g.mat[g.P + p.idx];
     const f3 cam{a.u.cameraPos[0], a.u.cameraPos[1], a.u.cameraPos[2]};
-    const float albLum = luminance(alb.x, alb.y, alb.z);
-    const Surface s = make_surface(pos, n, cam, albLum, rough, metal);
+    p.s = make_surface(p.pos, p.n, cam, luminance(p.alb.x, p.alb.y, p.alb.z), rough, metal);
+    return p;
+}
+__device__ __forceinline__ Pcg pass_a_rng(const PassAArgs &a, uint32_t x, uint32_t y) {
+    return pcg_seed(static_cast<uint64_t>(a.u.frame), static_cast<uint64_t>(y * 10007u + x));
+}
 
-    Res res = res_zero();
-    Pcg rng = pcg_seed(static_cast<uint64_t>(a.u.frame), static_cast<uint64_t>(y * 10007u + x));
+// Initial RIS over M candidates (alias table or the learned proposal's list).
+__device__ __forceinline__ void pass_a_ris(const PassAArgs &a, const PassASurface &ps, Res &res, Pcg &rng) {
+    const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
+    const size_t idx = ps.idx;
+    const f3 pos = ps.pos, n = ps.n;
+    const Surface &s = ps.s;
+    res = res_zero();
     const SceneView &sc = a.sc;
     if (dot(n, n) != 0.0f && a.candLight) {
         // RIS over the learned proposal's candidates: weight p_hat / q with the proposal's own
@@ -180,17 +208,13 @@ __device__ __forceinline__ void pass_a_px(const PassAArgs &a, uint32_t x, uint32
             }
         }
     }
-    // Visibility reuse, restirOmni.glsl:148-160. A reservoir with sumWeights == 0 already has
-    // w == 0 and is unchanged by a shadow hit, so its ray is skipped.
-    if ((a.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) && res.sumW != 0.0f) {
-        f3 o, d;
-        shadow_segment(pos, res.pos, o, d);
-        TraceStats st{0, 0};
-        if (occluded<STACK, false>(sc.nodes, sc.tris, o, d, stk, kBlock, st)) {
-            res.w = 0.0f;
-            res.sumW = 0.0f;
-        }
-    }
+}
+
+// Temporal reuse of the previous frame's reservoir at the reprojected pixel.
+__device__ __forceinline__ void pass_a_temporal(const PassAArgs &a, const PassASurface &ps, Res &res, Pcg &rng) {
+    const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
+    const f3 pos = ps.pos, n = ps.n, alb = ps.alb;
+    const Surface &s = ps.s;
     // Temporal reuse, restirOmni.glsl:163-209 (COMPARE_DEPTH off: world-position test).
     if (a.u.flags & RESTIR_TEMPORAL_REUSE_FLAG) {
         const float *m = a.u.prevFrameProjectionViewMatrix;
@@ -223,6 +247,33 @@ __device__ __forceinline__ void pass_a_px(const PassAArgs &a, uint32_t x, uint32
             }
         }
     }
+}
+
+// Pass A for pixel (x, y), shadow ray traced inline; `stk` is this lane's column of the
+// [level][lane] LDS stack.
+template <int TX, int TY, int STACK>
+__device__ __forceinline__ void pass_a_px(const PassAArgs &a, uint32_t x, uint32_t y, int32_t *stk) {
+    constexpr int kBlock = TX * TY;
+    if (x >= a.u.screenSize[0] || y >= a.u.screenSize[1]) return;
+    const PassASurface ps = pass_a_surface(a, x, y);
+    const size_t idx = ps.idx;
+    const f3 pos = ps.pos;
+    const SceneView &sc = a.sc;
+    Res res;
+    Pcg rng = pass_a_rng(a, x, y);
+    pass_a_ris(a, ps, res, rng);
+    // Visibility reuse, restirOmni.glsl:148-160. A reservoir with sumWeights == 0 already has
+    // w == 0 and is unchanged by a shadow hit, so its ray is skipped.
+    if ((a.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) && res.sumW != 0.0f) {
+        f3 o, d;
+        shadow_segment(pos, res.pos, o, d);
+        TraceStats st{0, 0};
+        if (occluded<STACK, false>(sc.nodes, sc.tris, o, d, stk, kBlock, st)) {
+            res.w = 0.0f;
+            res.sumW = 0.0f;
+        }
+    }
+    pass_a_temporal(a, ps, res, rng);
     store_res(a.out, idx, res);
 }
 
@@ -409,10 +460,10 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c(const PassCArgs a) {
 // ------------------------------------------------------------------- Pass C, ray queues ---
 // The same pass as k_pass_c, with its shadow rays moved out of the per-pixel lanes into two
 // queues traced by the persistent tracer: (1) merge every pixel and queue its centre ray;
-// (2) trace the centre rays, each one that sees the light queueing its pixel's neighbour rays
-// as it finishes (the others have Z = 0 whatever their neighbours give); (3) trace the neighbour
-// rays; (4) form Z and W. Rays are appended with one atomic per wave, slots handed out by prefix
-// sums over the wave, so a queue holds only live rays.
+// (2) trace; (3) queue the neighbour rays of pixels whose centre sees the light (the others have
+// Z = 0 whatever their neighbours give); (4) trace; (5) form Z and W. Rays are appended with one
+// atomic per wave, slots handed out by prefix sums over the wave, so a queue holds only live rays
+// in pixel-tile order.
 struct CQueues {
     uint32_t *counter;     // [0] centre rays queued, [1] neighbour rays queued
     float4 *c1, *c2;       // centre ray end points; c1.w carries the pixel index
@@ -494,6 +545,32 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
 }
 
 template <int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void k_pass_c_emit(const PassCArgs a, const CQueues q) {
+    uint32_t x, y;
+    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
+    const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
+    const bool inside = x < W && y < H;
+    const size_t idx = inside ? static_cast<size_t>(y) * W + x : 0;
+    uint32_t want = 0;
+    if (inside) {
+        const uint32_t b = q.bits[idx];
+        if (!(b & (kBitBackground | kBitDone | kBitCentreShadowed))) want = b & 0xffffu;
+    }
+    const uint32_t slot = wave_append(&q.counter[1], static_cast<uint32_t>(__popc(want)));
+    if (want) {
+        const float4 l = reinterpret_cast<const float4 *>(a.out + idx)[0];
+        uint32_t s = slot;
+        for (uint32_t m = want; m; m &= m - 1) {
+            const uint32_t j = static_cast<uint32_t>(__ffs(static_cast<int>(m)) - 1);
+            const f3 nwp = plane3(a.g.pos, q.nIdx[static_cast<size_t>(j) * q.npx + idx], a.g.P);
+            q.n1[s] = make_float4(nwp.x, nwp.y, nwp.z, __uint_as_float(static_cast<uint32_t>(idx) | (j << 27)));
+            q.n2[s] = make_float4(l.x, l.y, l.z, 0.0f);
+            ++s;
+        }
+    }
+}
+
+template <int TX, int TY>
 __global__ __launch_bounds__(TX *TY) void k_pass_c_finish(const PassCArgs a, const CQueues q) {
     uint32_t x, y;
     tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
@@ -519,16 +596,12 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_finish(const PassCArgs a, con
 // Persistent tracer over a queue whose length a previous kernel left in *count. Results go to
 // the pixel named by the ray's tag: a centre ray sets kBitCentreShadowed in bits[pixel]; a
 // neighbour ray sets bit `slot` of occ[pixel].
-// Centre rays additionally emit their pixel's neighbour rays as soon as they finish unshadowed:
-// the centre re-test decides Z = 0 by itself (unbiasedReuse.glsl:157-166) and the neighbour
-// tests draw no random numbers, so only lit centres need neighbour rays, and emitting them here
-// saves a full-screen pass over the pixels. `g` (positions) is read by centre rays only.
 template <int STACK, bool CENTRE>
 __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const float4 *__restrict__ p1,
                                                      const float4 *__restrict__ p2,
                                                      const uint32_t *__restrict__ count,
-                                                     uint32_t *__restrict__ head, CQueues q, const GView g) {
-    __shared__ int32_t stack[(STACK - kWalkRegs) * 256];  // the top kWalkRegs entries are registers
+                                                     uint32_t *__restrict__ head, CQueues q) {
+    __shared__ int32_t stack[STACK * 256];
     int32_t *stk = stack + threadIdx.x;
     const uint32_t n = *count;
     const uint32_t lane = threadIdx.x & 63u;
@@ -536,7 +609,6 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
     Walk w;
     int32_t ray = -1;
     uint32_t tag = 0;
-    f3 lp{0.0f, 0.0f, 0.0f};  // light end of a centre ray: the end point of its neighbour rays
     bool drained = false;
     for (;;) {
         const uint64_t idle = __ballot(ray < 0);
@@ -551,13 +623,11 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
                 const uint32_t i = base + static_cast<uint32_t>(__popcll(idle & below));
                 if (i < n) {
                     const float4 a4 = p1[i];
-                    const f3 b = xyz(p2[i]);
                     f3 o, d;
-                    shadow_segment(xyz(a4), b, o, d);
+                    shadow_segment(xyz(a4), xyz(p2[i]), o, d);
                     walk_begin(w, o, d);
                     tag = __float_as_uint(a4.w);
                     ray = static_cast<int32_t>(i);
-                    if (CENTRE) lp = b;
                 }
             }
         }
@@ -567,20 +637,6 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
         }
         bool fin = false;
         if (ray >= 0) fin = walk_step<STACK, false>(w, sc.nodes, sc.tris, stk, 256);
-        if constexpr (CENTRE) {
-            const bool lit = fin && !w.hit;
-            if (__ballot(lit) != 0) {  // uniform: the whole wave appends together
-                const uint32_t above = lit ? (q.bits[tag] & 0xffffu) : 0u;
-                uint32_t slot = wave_append(&q.counter[1], static_cast<uint32_t>(__popc(above)));
-                for (uint32_t m = above; m; m &= m - 1) {
-                    const uint32_t j = static_cast<uint32_t>(__ffs(static_cast<int>(m)) - 1);
-                    const f3 nwp = plane3(g.pos, q.nIdx[static_cast<size_t>(j) * q.npx + tag], g.P);
-                    q.n1[slot] = make_float4(nwp.x, nwp.y, nwp.z, __uint_as_float(tag | (j << 27)));
-                    q.n2[slot] = make_float4(lp.x, lp.y, lp.z, 0.0f);
-                    ++slot;
-                }
-            }
-        }
         if (fin) {
             if (w.hit) {
                 if (CENTRE) q.bits[tag] |= kBitCentreShadowed;
@@ -589,6 +645,63 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
             ray = -1;
         }
     }
+}
+
+// ---------------------------------------------------------------- Pass A, ray queue ---
+// Pass A with its visibility ray (restirOmni.glsl:148-160) moved out of the per-pixel lanes:
+// (1) RIS per pixel; the reservoir is parked in the output, the random stream's state in
+// scratch, and the ray queued; (2) the persistent tracer marks shadowed pixels; (3) per pixel:
+// the shadow verdict, temporal reuse, store. Same draws, same order, same result as k_pass_a.
+struct AQueues {
+    uint32_t *counter;  // [0] rays queued, [1] tracer head
+    float4 *p1, *p2;    // p1.w carries the pixel index
+    uint64_t *rng;      // per pixel: the random stream's state after RIS
+    uint32_t *occ;      // per pixel: kBitCentreShadowed once its ray hit
+};
+
+template <int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void k_pass_a_ris(const PassAArgs a, const AQueues q) {
+    uint32_t x, y;
+    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
+    const bool inside = x < a.u.screenSize[0] && y < a.u.screenSize[1];
+    bool emit = false;
+    size_t idx = 0;
+    f3 pos{0.0f, 0.0f, 0.0f}, lp{0.0f, 0.0f, 0.0f};
+    if (inside) {
+        const PassASurface ps = pass_a_surface(a, x, y);
+        idx = ps.idx;
+        pos = ps.pos;
+        Res res;
+        Pcg rng = pass_a_rng(a, x, y);
+        pass_a_ris(a, ps, res, rng);
+        store_res(a.out, idx, res);
+        q.rng[idx] = rng.state;
+        q.occ[idx] = 0u;
+        emit = (a.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) && res.sumW != 0.0f;
+        lp = res.pos;
+    }
+    const uint32_t slot = wave_append(&q.counter[0], emit ? 1u : 0u);
+    if (emit) {
+        q.p1[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(static_cast<uint32_t>(idx)));
+        q.p2[slot] = make_float4(lp.x, lp.y, lp.z, 0.0f);
+    }
+}
+
+template <int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void k_pass_a_finish(const PassAArgs a, const AQueues q) {
+    uint32_t x, y;
+    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
+    if (x >= a.u.screenSize[0] || y >= a.u.screenSize[1]) return;
+    const PassASurface ps = pass_a_surface(a, x, y);
+    Res res = load_res(a.out, ps.idx);
+    Pcg rng = pass_a_rng(a, x, y);  // the stream's increment; its state resumes after RIS
+    rng.state = q.rng[ps.idx];
+    if ((a.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) && res.sumW != 0.0f && (q.occ[ps.idx] & kBitCentreShadowed)) {
+        res.w = 0.0f;
+        res.sumW = 0.0f;
+    }
+    pass_a_temporal(a, ps, res, rng);
+    store_res(a.out, ps.idx, res);
 }
 
 // ------------------------------------------------------------------------------- Shading --
@@ -667,7 +780,7 @@ __global__ __launch_bounds__(256) void k_trace(const TraceArgs a) {
 // lanes whose rays ended early do not wait for the slowest ray of their wave.
 template <int STACK, bool REF>
 __global__ __launch_bounds__(256) void k_trace_pool(const TraceArgs a, uint32_t *__restrict__ head) {
-    __shared__ int32_t stack[(STACK - kWalkRegs) * 256];  // the top kWalkRegs entries are registers
+    __shared__ int32_t stack[STACK * 256];
     int32_t *stk = stack + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
@@ -877,11 +990,68 @@ static hipError_t dispatch_a(int tile, const PassAArgs &args, hipStream_t s) {
         default: return run_a<32, 8, STACK>(args, s);
     }
 }
+size_t pass_a_queue_bytes(uint32_t npx) {
+    const size_t n = npx;
+    return 256 + 2 * sizeof(float4) * n + sizeof(uint64_t) * n + sizeof(uint32_t) * n + 4 * 256;
+}
+
+static AQueues carve_a_queues(void *scratch, uint32_t npx) {
+    auto align = [](size_t v) { return (v + 255) & ~static_cast<size_t>(255); };
+    char *p = static_cast<char *>(scratch);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char *r = p + off;
+        off = align(off + bytes);
+        return r;
+    };
+    const size_t n = npx;
+    AQueues q;
+    q.counter = reinterpret_cast<uint32_t *>(take(256));
+    q.p1 = reinterpret_cast<float4 *>(take(sizeof(float4) * n));
+    q.p2 = reinterpret_cast<float4 *>(take(sizeof(float4) * n));
+    q.rng = reinterpret_cast<uint64_t *>(take(sizeof(uint64_t) * n));
+    q.occ = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * n));
+    return q;
+}
+
+template <int TX, int TY, int STACK>
+static hipError_t run_a_queued(PassAArgs args, const AQueues &q, uint32_t gridCap, hipStream_t s) {
+    grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
+    const dim3 tiles(args.nTiles), block(TX * TY);
+    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 2u);
+    RK_LAUNCH("k_pass_a_ris", (k_pass_a_ris<TX, TY>), tiles, block, 0, s, args, q);
+    if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
+        CQueues cq{};
+        cq.bits = q.occ;  // a shadowed ray sets kBitCentreShadowed in occ[pixel]
+        RK_LAUNCH("k_trace_queue(pass_a)", (k_trace_queue<STACK, true>), dim3(gridCap), dim3(256), 0, s,
+                  args.sc, q.p1, q.p2, q.counter + 0, q.counter + 1, cq);
+    }
+    RK_LAUNCH("k_pass_a_finish", (k_pass_a_finish<TX, TY>), tiles, block, 0, s, args, q);
+    return hipGetLastError();
+}
+template <int STACK>
+static hipError_t dispatch_a_queued(int tile, const PassAArgs &args, const AQueues &q, uint32_t gridCap,
+                                    hipStream_t s) {
+    switch (tile) {
+        case 0: return run_a_queued<16, 16, STACK>(args, q, gridCap, s);
+        case 1: return run_a_queued<64, 1, STACK>(args, q, gridCap, s);
+        case 2: return run_a_queued<8, 8, STACK>(args, q, gridCap, s);
+        case 3: return run_a_queued<16, 8, STACK>(args, q, gridCap, s);
+        default: return run_a_queued<32, 8, STACK>(args, q, gridCap, s);
+    }
+}
+
 hipError_t launch_pass_a(int tile, int stack, const restir_uniforms &u, const SceneView &sc,
                          const GView &cur, const GView &prev, const restir_reservoir *prevRes,
                          restir_reservoir *out, const int32_t *candLight, const float *candQ,
-                         hipStream_t s) {
+                         void *queueScratch, uint32_t gridCap, hipStream_t s) {
     PassAArgs args{u, sc, cur, prev, prevRes, out, 0, 0, candLight, candQ};
+    if (queueScratch) {
+        const AQueues q = carve_a_queues(queueScratch, u.screenSize[0] * u.screenSize[1]);
+        return by_stack(stack, [&](auto S) {
+            return dispatch_a_queued<decltype(S)::value>(tile, args, q, gridCap, s);
+        });
+    }
     return by_stack(stack, [&](auto S) { return dispatch_a<decltype(S)::value>(tile, args, s); });
 }
 
@@ -929,9 +1099,10 @@ static hipError_t run_c_queued(PassCArgs args, const CQueues &q, uint32_t gridCa
     RK_LAUNCH("k_pass_c_merge", (k_pass_c_merge<TX, TY>), tiles, block, 0, s, args, q);
     if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
         RK_LAUNCH("k_trace_queue(centre)", (k_trace_queue<STACK, true>), dim3(rayBlocks), dim3(256), 0, s,
-                  args.sc, q.c1, q.c2, q.counter + 0, q.counter + 2, q, args.g);
+                  args.sc, q.c1, q.c2, q.counter + 0, q.counter + 2, q);
+        RK_LAUNCH("k_pass_c_emit", (k_pass_c_emit<TX, TY>), tiles, block, 0, s, args, q);
         RK_LAUNCH("k_trace_queue(neighbour)", (k_trace_queue<STACK, false>), dim3(rayBlocks), dim3(256), 0,
-                  s, args.sc, q.n1, q.n2, q.counter + 1, q.counter + 3, q, args.g);
+                  s, args.sc, q.n1, q.n2, q.counter + 1, q.counter + 3, q);
         RK_LAUNCH("k_pass_c_finish", (k_pass_c_finish<TX, TY>), tiles, block, 0, s, args, q);
     }
     return hipGetLastError();

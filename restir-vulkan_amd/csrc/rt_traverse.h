@@ -173,18 +173,13 @@ __device__ __forceinline__ bool occluded(const Node *__restrict__ nodes,
 // Step-wise form of `occluded` for schedules that interleave rays on one lane: the state holds one
 // ray's traversal, and each step visits exactly one node (the same visiting order, node count and
 // result as `occluded`). A step returns true when the ray is finished; `hit` then has the answer.
-// The stack's top kWalkRegs entries live in registers (r0 = top) and only deeper entries go to
-// LDS, so a kStack-entry stack takes (kStack - kWalkRegs) dwords of LDS per lane and more
-// workgroups fit a CU.
-constexpr int kWalkRegs = 4;
 struct Walk {
     Seg s;
     int32_t cur;
-    int top;  // entries on the stack (registers + LDS)
+    int top;
     int counter;
     bool hit;
     uint32_t nv, nt;
-    int32_t r0, r1, r2, r3;
 };
 
 __device__ __forceinline__ void walk_begin(Walk &w, f3 o, f3 d) {
@@ -194,29 +189,8 @@ __device__ __forceinline__ void walk_begin(Walk &w, f3 o, f3 d) {
     w.counter = 0;
     w.hit = false;
     w.nv = w.nt = 0;
-    w.r0 = w.r1 = w.r2 = w.r3 = 0;
 }
 
-// entry i of the stack (0 = bottom) is in LDS slot i while i < top - kWalkRegs
-__device__ __forceinline__ void walk_push(Walk &w, int32_t v, int32_t *stk, int stride) {
-    if (w.top >= kWalkRegs) stk[(w.top - kWalkRegs) * stride] = w.r3;
-    w.r3 = w.r2;
-    w.r2 = w.r1;
-    w.r1 = w.r0;
-    w.r0 = v;
-    ++w.top;
-}
-__device__ __forceinline__ int32_t walk_pop(Walk &w, const int32_t *stk, int stride) {
-    const int32_t v = w.r0;
-    w.r0 = w.r1;
-    w.r1 = w.r2;
-    w.r2 = w.r3;
-    --w.top;
-    if (w.top >= kWalkRegs) w.r3 = stk[(w.top - kWalkRegs) * stride];
-    return v;
-}
-
-// `stk`: this lane's LDS column of kStack - kWalkRegs entries, `stride` dwords apart.
 template <int kStack, bool kRefLoop>
 __device__ __forceinline__ bool walk_step(Walk &w, const Node *__restrict__ nodes,
                                           const float4 *__restrict__ tris, int32_t *stk, int stride) {
@@ -249,7 +223,10 @@ __device__ __forceinline__ bool walk_step(Walk &w, const Node *__restrict__ node
                 }
             }
         } else {
-            if (next >= 0 && w.top < kStack) walk_push(w, next, stk, stride);
+            if (next >= 0 && w.top < kStack) {
+                stk[w.top * stride] = next;
+                ++w.top;
+            }
             next = nd.right;
         }
     }
@@ -262,7 +239,8 @@ __device__ __forceinline__ bool walk_step(Walk &w, const Node *__restrict__ node
     }
     if (next < 0) {
         if (w.top == 0) return true;
-        next = walk_pop(w, stk, stride);
+        --w.top;
+        next = stk[w.top * stride];
     }
     w.cur = next;
     return false;

@@ -26,7 +26,9 @@ def test_profile_records_each_kernel(torch):
     r.render_frame(r.scene.camera)
     rows = r.ctx.profile_read()
     r.ctx.profile_enable(False)
-    assert rows["k_pass_a"][0] == 1
+    # queued schedule (the default): Pass A is RIS, its ray queue, and the finish
+    assert rows["k_pass_a_ris"][0] == 1 and rows["k_pass_a_finish"][0] == 1
+    assert rows["k_trace_queue(pass_a)"][0] == 1
     assert rows["k_pass_c_merge"][0] == 2 and rows["k_trace_queue(neighbour)"][0] == 2
     assert rows["k_shade"][0] == 1
     for name, (n, total, mx) in rows.items():

@@ -293,7 +293,10 @@ int restir_set_light_clusters(restir_ctx *ctx, uint32_t k, const int32_t *light_
 
 /* Draws M <= 16 candidates per pixel from q: cand_light[j * W*H + p], cand_q[j * W*H + p]
  * (light index, or -1 with q = 0 for an empty cluster). Background pixels are left untouched.
- * scene_lo/hi bound the positions fed to the features. Seeded by (u->frame, pixel). */
+ * The MLP is evaluated once per 2x2 pixel quad, at the quad's first surface pixel in the order
+ * (0,0), (1,0), (0,1), (1,1); every pixel of the quad draws from that distribution with its own
+ * random stream, and cand_q is the density it drew from. scene_lo/hi bound the positions fed
+ * to the features. Seeded by (u->frame, pixel). */
 int restir_learned_candidates(restir_ctx *ctx, const restir_uniforms *u, const restir_gbuffer *g,
                               const restir_mlp *mlp, const float scene_lo[3],
                               const float scene_hi[3], uint32_t M, float mix, int32_t *cand_light,

@@ -8,7 +8,8 @@
 // every product runs on v_mfma_f32_32x32x16_bf16 with fp32 accumulation.
 //
 // Kernels:
-//   k_propose      fused inference: features -> 4 MFMA layers -> softmax -> M candidates (l, q)
+//   k_propose      fused inference per 2x2 pixel quad: features -> 4 MFMA layers -> softmax ->
+//                  M candidates (l, q) for each of its pixels
 //   k_features     features of listed pixels (bf16), for training and tests
 //   k_gemm         strided C = A*B with bias / ReLU / ReLU-gate epilogues (training fwd/bwd),
 //                  or split-K fp32 partials for the weight gradients
@@ -361,7 +362,23 @@ __device__ __forceinline__ float4 unpack_h4(uint2 v) {  // four fp16 -> float
     return make_float4(lo.x, lo.y, hi.x, hi.y);
 }
 
-// One workgroup of 512 lanes (8 waves) proposes for 256 consecutive pixels. LDS holds the
+// The pixel whose features stand for quad (qx, qy): its first surface pixel in the order
+// (0,0), (1,0), (0,1), (1,1) inside the image, else its top-left pixel.
+__device__ __forceinline__ uint32_t quad_pixel(const rk::GView &g, uint32_t qx, uint32_t qy) {
+    const uint32_t x0 = 2 * qx, y0 = 2 * qy;
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t x = x0 + (j & 1), y = y0 + (j >> 1);
+        if (x >= g.W || y >= g.H) continue;
+        const uint32_t p = y * g.W + x;
+        const float nx = g.nrm[p], ny = g.nrm[g.P + p], nz = g.nrm[2 * g.P + p];
+        if ((nx * nx + ny * ny) + nz * nz != 0.0f) return p;
+    }
+    return y0 * g.W + x0;
+}
+
+// One workgroup of 512 lanes (8 waves) proposes for 256 pixel quads (2x2 pixels each): one MLP
+// evaluation per quad, each pixel drawing its own candidates from the quad's distribution with
+// its own random stream (q is that distribution, so RIS stays unbiased). LDS holds the
 // activations [256][136] bf16 and one layer's weights [<=256][136] bf16; after the last layer
 // the same LDS holds the fp16 softmax numerators [256][K + 4].
 template <int KC>
@@ -371,12 +388,15 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
     uint16_t *act = lds;
     uint16_t *wbuf = lds + kTilePx * kRow;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t p0 = blockIdx.x * kTilePx;
+    // one MLP row per 2x2 pixel quad: the quad's first surface pixel in the order (0,0), (1,0),
+    // (0,1), (1,1) stands for it (quad_pixel)
+    const uint32_t W = a.f.g.W, H = a.f.g.H, QW = (W + 1) / 2, QH = (H + 1) / 2;
+    const uint32_t q0 = blockIdx.x * kTilePx;
     // features (lanes 0..255) while lanes 256..511 stage W1
     if (tid < kTilePx) {
-        const uint32_t p = p0 + tid;
+        const uint32_t qd = q0 + tid;
         uint16_t *row = act + tid * kRow;
-        if (p < a.npx) pixel_features(a.f, p, row);
+        if (qd < QW * QH) pixel_features(a.f, quad_pixel(a.f.g, qd % QW, qd / QW), row);
         else for (int k = 0; k < kIn; ++k) row[k] = 0;
     } else {
         for (int e = tid - kTilePx; e < kHidden * (kIn / 8); e += kTilePx) {
@@ -423,19 +443,17 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
         }
         __syncthreads();
     }
-    if (tid >= kTilePx) return;
-    const uint32_t p = p0 + tid;
-    if (p >= a.npx) return;
-    const FeatArgs &f = a.f;
-    const f3 n = f3{f.g.nrm[p], f.g.nrm[f.g.P + p], f.g.nrm[2 * f.g.P + p]};
-    if (rt::dot(n, n) == 0.0f) return;  // background: Pass A draws no candidates there
+    // two lanes per quad, each drawing for one row of its 2x2 pixels
+    const uint32_t qd = q0 + (tid >> 1);
+    if (qd >= QW * QH) return;
+    const uint32_t qx = qd % QW, qy = qd / QW;
     // the cluster pmf is e[c] / se with se the sequential sum of the stored fp16 numerators; the
     // inverse-CDF scan below accumulates the same values in the same order, so the sampled
     // distribution and the q written for each candidate agree
     // one 8-byte LDS read yields 4 numerators. Cluster blocks of 32: block sums in registers
     // (sequential inside a block, blocks in order), so a draw picks its block with KC/32
     // compares and scans at most 32 numerators of that block
-    const uint2 *row = reinterpret_cast<const uint2 *>(reinterpret_cast<const __half *>(lds) + tid * (KC + 4));
+    const uint2 *row = reinterpret_cast<const uint2 *>(reinterpret_cast<const __half *>(lds) + (tid >> 1) * (KC + 4));
     constexpr int kBlocks = KC / 32;
     float bs[kBlocks];
     float se = 0.0f;
@@ -487,8 +505,15 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
         }
         return c;
     };
-    rt::Pcg rng = rt::pcg_seed(a.seed, p);
     const int M = static_cast<int>(a.M);
+    const FeatArgs &f = a.f;
+    const uint32_t py = 2 * qy + static_cast<uint32_t>(tid & 1);
+    for (uint32_t px = 2 * qx; px < 2 * qx + 2; ++px) {
+    if (px >= W || py >= H) continue;
+    const uint32_t p = py * W + px;
+    const f3 n = f3{f.g.nrm[p], f.g.nrm[f.g.P + p], f.g.nrm[2 * f.g.P + p]};
+    if (rt::dot(n, n) == 0.0f) continue;  // background: Pass A draws no candidates there
+    rt::Pcg rng = rt::pcg_seed(a.seed, p);
     // Candidates in groups of 8, each step's loads issued together (no load waits on another
     // candidate's). Per candidate: one draw for the mixture, one for the cluster (MLP
     // candidates), then two for the alias column.
@@ -568,6 +593,7 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
             a.candQ[static_cast<size_t>(j) * a.npx + p] = q;
         }
     }
+    }  // pixels of this lane
 }
 
 }  // namespace rm
@@ -836,7 +862,8 @@ hipError_t mlp_propose(const restir_mlp &m, const GView &g, f3 cam, const float 
     a.clusterAlias = reinterpret_cast<const float4 *>(clusterAlias);
     a.candLight = candLight;
     a.candQ = candQ;
-    const dim3 grid((a.npx + rm::kTilePx - 1) / rm::kTilePx), block(512);
+    const uint32_t quads = ((g.W + 1) / 2) * ((g.H + 1) / 2);
+    const dim3 grid((quads + rm::kTilePx - 1) / rm::kTilePx), block(512);
     switch (a.K) {
         case 64: RK_LAUNCH("k_propose", (rm::k_propose<64>), grid, block, 0, s, a); break;
         case 128: RK_LAUNCH("k_propose", (rm::k_propose<128>), grid, block, 0, s, a); break;

@@ -544,28 +544,60 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
     }
 }
 
+// Queues the neighbour rays of pixels whose centre ray found the light. The wave's rays are
+// written slot-major: lane i writes slots i, i + 64, ... of the wave's run, finding the pixel that
+// owns a slot by binary search over the wave's prefix sums, so each store instruction covers
+// consecutive queue entries (pixel-major stores scatter a wave's writes over partial lines).
 template <int TX, int TY>
 __global__ __launch_bounds__(TX *TY) void k_pass_c_emit(const PassCArgs a, const CQueues q) {
     uint32_t x, y;
     tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
     const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
     const bool inside = x < W && y < H;
-    const size_t idx = inside ? static_cast<size_t>(y) * W + x : 0;
+    const uint32_t idx = inside ? y * W + x : 0u;
     uint32_t want = 0;
     if (inside) {
         const uint32_t b = q.bits[idx];
         if (!(b & (kBitBackground | kBitDone | kBitCentreShadowed))) want = b & 0xffffu;
     }
-    const uint32_t slot = wave_append(&q.counter[1], static_cast<uint32_t>(__popc(want)));
-    if (want) {
-        const float4 l = reinterpret_cast<const float4 *>(a.out + idx)[0];
-        uint32_t s = slot;
-        for (uint32_t m = want; m; m &= m - 1) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t cnt = static_cast<uint32_t>(__popc(want));
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off);
+        if (lane >= static_cast<uint32_t>(off)) incl += t;
+    }
+    const uint32_t excl = incl - cnt;
+    const uint32_t total = __shfl(incl, 63);
+    if (total == 0) return;  // uniform: the whole wave leaves
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&q.counter[1], total);
+    base = __shfl(base, 0);
+    float4 l = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (want) l = reinterpret_cast<const float4 *>(a.out + idx)[0];
+    for (uint32_t r0 = 0; r0 < total; r0 += 64) {  // every lane runs every round
+        const uint32_t r = r0 + lane;
+        // owner: the last lane whose exclusive prefix is <= r (it has r < its inclusive prefix)
+        uint32_t o = 0;
+#pragma unroll
+        for (uint32_t step = 32; step > 0; step >>= 1) {
+            const uint32_t c = o + step;
+            const uint32_t ex = __shfl(excl, static_cast<int>(c < 64 ? c : 63));
+            if (c < 64 && ex <= r) o = c;
+        }
+        const uint32_t oWant = __shfl(want, static_cast<int>(o));
+        const uint32_t oExcl = __shfl(excl, static_cast<int>(o));
+        const uint32_t oIdx = __shfl(idx, static_cast<int>(o));
+        const float lx = __shfl(l.x, static_cast<int>(o)), ly = __shfl(l.y, static_cast<int>(o)),
+                    lz = __shfl(l.z, static_cast<int>(o));
+        if (r < total) {
+            uint32_t m = oWant;
+            for (uint32_t k = r - oExcl; k > 0; --k) m &= m - 1;  // the k-th set bit of the owner
             const uint32_t j = static_cast<uint32_t>(__ffs(static_cast<int>(m)) - 1);
-            const f3 nwp = plane3(a.g.pos, q.nIdx[static_cast<size_t>(j) * q.npx + idx], a.g.P);
-            q.n1[s] = make_float4(nwp.x, nwp.y, nwp.z, __uint_as_float(static_cast<uint32_t>(idx) | (j << 27)));
-            q.n2[s] = make_float4(l.x, l.y, l.z, 0.0f);
-            ++s;
+            const f3 nwp = plane3(a.g.pos, q.nIdx[static_cast<size_t>(j) * q.npx + oIdx], a.g.P);
+            q.n1[base + r] = make_float4(nwp.x, nwp.y, nwp.z, __uint_as_float(oIdx | (j << 27)));
+            q.n2[base + r] = make_float4(lx, ly, lz, 0.0f);
         }
     }
 }

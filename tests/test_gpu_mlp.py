@@ -113,6 +113,17 @@ def _expected_q(lp, logits, lights, pwithin, src):
     return (1 - lp.mix) * pi[rows, c] * pwithin[lights] + lp.mix * src[lights]
 
 
+def quad_representative(normal, W, H, p):
+    """The pixel whose features the proposal uses for pixel p: the first surface pixel of p's
+    2x2 quad in the order (0,0), (1,0), (0,1), (1,1), else the quad's top-left pixel."""
+    x0, y0 = (p % W) & ~1, (p // W) & ~1
+    for j in range(4):
+        x, y = x0 + (j & 1), y0 + (j >> 1)
+        if x < W and y < H and np.any(normal[:, y * W + x] != 0):
+            return y * W + x
+    return y0 * W + x0
+
+
 def test_proposal_density_and_distribution(torch):
     sc = small_terrain()
     W, H = 64, 48
@@ -128,11 +139,12 @@ def test_proposal_density_and_distribution(torch):
     normal = r.gbuf[r.cur].cpu().numpy()[3 * W * H: 6 * W * H].reshape(3, -1)
     surf = np.flatnonzero(np.abs(normal).sum(axis=0) > 0)[::7][:256]
     assert len(surf) >= 64
-    # logits of the same pixels through the GEMM path
+    # logits through the GEMM path at the pixel that stands for each pixel's 2x2 quad
     n = 256 if len(surf) >= 256 else 64 * (len(surf) // 64)
     surf = surf[:n]
+    reps = np.array([quad_representative(normal, W, H, p) for p in surf], dtype=np.uint32)
     X = torch.zeros(n * 128, dtype=torch.int16, device="cuda")
-    r.ctx.mlp_features(g, sc.camera["position"], lp.lo, lp.hi, torch.from_numpy(surf.astype(np.uint32)).cuda(), n, X)
+    r.ctx.mlp_features(g, sc.camera["position"], lp.lo, lp.hi, torch.from_numpy(reps).cuda(), n, X)
     logits = torch.zeros(n, 256, device="cuda")
     r.ctx.mlp_forward(lp.mlp, X, n, logits)
     l32 = logits.cpu().numpy()

@@ -173,7 +173,7 @@ void restir_destroy(restir_ctx *ctx);
 const char *restir_last_error(const restir_ctx *ctx);
 /* ABI revision; bumped whenever an entry point's signature changes. Bindings compare it with the
  * revision they were written against and refuse a mismatched library. */
-#define RESTIR_ABI_VERSION 6
+#define RESTIR_ABI_VERSION 7
 int restir_version(void);
 
 /* Light tables: device pointers to the reference buffer images (header + array). The counts are
@@ -187,6 +187,18 @@ int restir_set_lights(restir_ctx *ctx, const void *point_lights, size_t point_by
  * read back once to validate child indices and to measure its depth. */
 int restir_set_bvh(restir_ctx *ctx, const restir_aabb_tree_node *nodes, uint32_t n_nodes,
                    const restir_triangle *tris, uint32_t n_tris);
+
+/* AabbTree::build (aabbTreeBuilder.cpp:52-214) on the device, in place of the host build plus
+ * restir_set_bvh for scenes whose geometry changes: the tree restir_build_aabb_tree makes, node
+ * for node and bit for bit, from device vertex data (n_tris x 9 floats: p1, p2, p3), written to
+ * device buffers nodes[n_tris - 1] and tris[n_tris] and bound to ctx as restir_set_bvh binds them
+ * (the build's own bookkeeping replaces set_bvh's read-back check). depth_out (host, may be NULL)
+ * gets the tree depth. The build waits for the stream once per tree level (it reads each level's
+ * node count back), so it cannot be captured into a graph. Context scratch: about 380 bytes per
+ * triangle. A non-finite coordinate gives RESTIR_E_INVALID_ARGUMENT. */
+int restir_build_bvh(restir_ctx *ctx, const float *tri_xyz, uint32_t n_tris,
+                     restir_aabb_tree_node *nodes, restir_triangle *tris, uint32_t *depth_out,
+                     void *stream);
 
 /* Pass A: initial RIS + visibility reuse + temporal reuse (restirOmni.glsl:86-212).
  * prev / prev_res are read only when RESTIR_TEMPORAL_REUSE_FLAG is set. */
@@ -241,6 +253,15 @@ int restir_build_aabb_tree(const float *tri_xyz, uint32_t n_tris, restir_aabb_tr
 int restir_build_alias_table(const restir_point_light *points, uint32_t n_point,
                              const restir_tri_light *tris, uint32_t n_tri,
                              restir_alias_table_column *columns);
+
+/* createAliasTable (misc.cpp:418-497) on the device, for lights whose power changes: the columns
+ * restir_build_alias_table makes, bit for bit, from device light records (point lights if
+ * n_point > 0, else triangle lights; plain arrays, no SSBO header) into device columns[n]. One
+ * workgroup, one launch, no host round trip (it may be captured into a graph); the work is
+ * ordered on `stream`. Context scratch: 20 bytes per light. */
+int restir_build_alias_table_device(restir_ctx *ctx, const restir_point_light *points,
+                                    uint32_t n_point, const restir_tri_light *tris, uint32_t n_tri,
+                                    restir_alias_table_column *columns, void *stream);
 
 /* Light clusters for the learned proposal (no reference counterpart): seeded k-means over the n
  * point-light positions into k clusters (1 <= k <= n; initial centroids by a PCG32(seed)

@@ -46,6 +46,11 @@ struct restir_ctx {
     size_t truthScratchBytes = 0;
     void *viewTable = nullptr;     // per-view records of the batched-view launches
     size_t viewTableBytes = 0;
+    void *bvhScratch = nullptr;    // primitive order, keys and ranges of the GPU tree build
+    size_t bvhScratchBytes = 0;
+    void *bvhInfoHost = nullptr;   // pinned: each tree level's node and split counts
+    void *aliasScratch = nullptr;  // powers, weights and queues of the device alias-table build
+    size_t aliasScratchBytes = 0;
     // light clusters of the learned proposal (device pointers, caller-owned)
     bool clustersSet = false;
     int32_t nClusters = 0;
@@ -59,6 +64,9 @@ struct restir_ctx {
         if (mlpScratch) (void)hipFree(mlpScratch);
         if (truthScratch) (void)hipFree(truthScratch);
         if (viewTable) (void)hipFree(viewTable);
+        if (bvhScratch) (void)hipFree(bvhScratch);
+        if (bvhInfoHost) (void)hipHostFree(bvhInfoHost);
+        if (aliasScratch) (void)hipFree(aliasScratch);
     }
 };
 

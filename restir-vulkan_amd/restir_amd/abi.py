@@ -21,7 +21,7 @@ TEMPORAL_REUSE_FLAG = 1 << 1
 TRACE_FIRST_HIT = 0
 TRACE_REFERENCE_LOOP = 1
 TRACE_ONE_RAY_PER_LANE = 2
-ABI_VERSION = 6  # RESTIR_ABI_VERSION of include/restir_abi.h
+ABI_VERSION = 7  # RESTIR_ABI_VERSION of include/restir_abi.h
 
 ERRORS = {
     -1: "RESTIR_E_INVALID_ARGUMENT",
@@ -85,6 +85,7 @@ class GBuffer(C.Structure):
 
 SCHEDULE_QUEUED = 0
 SCHEDULE_INLINE = 1
+LIGHT_BUFFER_HEADER_BYTES = 16  # RESTIR_LIGHT_BUFFER_HEADER_BYTES
 
 
 class Launch(C.Structure):
@@ -150,6 +151,7 @@ SIGNATURES = {
     "restir_last_error": (C.c_char_p, [_vp]),
     "restir_set_lights": (_i32, [_vp, _vp, _sz, _vp, _sz, _vp, _sz]),
     "restir_set_bvh": (_i32, [_vp, _vp, _u32, _vp, _u32]),
+    "restir_build_bvh": (_i32, [_vp, _vp, _u32, _vp, _vp, C.POINTER(_u32), _vp]),
     "restir_initial_temporal": (_i32, [_vp, C.POINTER(Uniforms), C.POINTER(GBuffer),
                                        C.POINTER(GBuffer), _vp, _vp, C.POINTER(Launch), _vp]),
     "restir_spatial": (_i32, [_vp, C.POINTER(Uniforms), _i32, C.POINTER(GBuffer), _vp, _vp,
@@ -160,6 +162,7 @@ SIGNATURES = {
     "restir_trace_shadow": (_i32, [_vp, _vp, _vp, _u32, _vp, _vp, _vp, _i32, _vp]),
     "restir_build_aabb_tree": (_i32, [_vp, _u32, _vp, _vp, C.POINTER(_u32)]),
     "restir_build_alias_table": (_i32, [_vp, _u32, _vp, _u32, _vp]),
+    "restir_build_alias_table_device": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _vp]),
     "restir_build_light_clusters": (_i32, [_vp, _u32, _u32, C.c_uint64, _u32, _vp, _vp, _vp, _vp,
                                            _vp, _vp, _vp]),
     "restir_primary_gbuffer": (_i32, [_vp, C.POINTER(Camera), _vp, _vp, _u32, _vp, _vp, _vp, _vp,
@@ -306,10 +309,11 @@ class LightClusters:
 def pack_buffer(records: np.ndarray | None, dtype: np.dtype) -> np.ndarray:
     """SSBO image: int32 count + 12 bytes of padding, then the records (sceneBuffers.h:100-124)."""
     n = 0 if records is None else len(records)
-    buf = np.zeros(16 + n * dtype.itemsize, dtype=np.uint8)
+    h = LIGHT_BUFFER_HEADER_BYTES
+    buf = np.zeros(h + n * dtype.itemsize, dtype=np.uint8)
     buf[:4] = np.frombuffer(np.int32(n).tobytes(), dtype=np.uint8)
     if n:
-        buf[16:] = np.frombuffer(np.ascontiguousarray(records, dtype=dtype).tobytes(), dtype=np.uint8)
+        buf[h:] = np.frombuffer(np.ascontiguousarray(records, dtype=dtype).tobytes(), dtype=np.uint8)
     return buf
 
 
@@ -410,6 +414,22 @@ class Context:
     def set_bvh(self, nodes, n_nodes, tris, n_tris):
         self._check(self.lib.restir_set_bvh(self.h, as_ptr(nodes), n_nodes, as_ptr(tris), n_tris),
                     "restir_set_bvh")
+
+    def build_bvh(self, tri_xyz, n_tris, nodes, tris, stream=None) -> int:
+        """Device tree build (restir_build_bvh) from device vertices [n_tris, 9]; binds the tree
+        to this context and returns its depth."""
+        depth = C.c_uint32(0)
+        self._check(self.lib.restir_build_bvh(self.h, as_ptr(tri_xyz), n_tris, as_ptr(nodes),
+                                              as_ptr(tris), C.byref(depth), stream_ptr(stream)),
+                    "restir_build_bvh")
+        return depth.value
+
+    def build_alias_table(self, points, n_point, tri_lights, n_tri, columns, stream=None):
+        """Device alias-table build (restir_build_alias_table_device) from device light records
+        (no SSBO header) into device columns."""
+        self._check(self.lib.restir_build_alias_table_device(
+            self.h, as_ptr(points), n_point, as_ptr(tri_lights), n_tri, as_ptr(columns),
+            stream_ptr(stream)), "restir_build_alias_table_device")
 
     def initial_temporal(self, u, cur, prev, prev_res, out, launch=None, stream=None):
         self._check(self.lib.restir_initial_temporal(

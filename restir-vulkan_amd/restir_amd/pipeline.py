@@ -33,25 +33,70 @@ def to_device(arr: np.ndarray, torch, device):
 
 
 class SceneOnDevice:
-    """BVH, triangles, light tables and materials resident in HBM and bound to a context."""
+    """BVH, triangles, light tables and materials resident in HBM and bound to a context.
 
-    def __init__(self, ctx: Context, scene, device="cuda"):
+    ``build="host"`` builds the tree and the alias table on the host (restir_build_aabb_tree,
+    restir_build_alias_table) and uploads them; ``build="device"`` uploads vertices and light
+    records and builds the same tree and table in HBM (restir_build_bvh,
+    restir_build_alias_table_device); ``nodes`` / ``tris`` / ``alias`` are then read back on first
+    access, for checking."""
+
+    def __init__(self, ctx: Context, scene, device="cuda", build="host"):
         torch = _torch()
         self.scene = scene
-        self.nodes, self.tris, self.depth = build_aabb_tree(scene.tri_xyz)
-        self.alias = build_alias_table(scene.point_lights, scene.tri_lights)
+        if build not in ("host", "device"):
+            raise ValueError(f"build must be 'host' or 'device', not {build!r}")
+        n = scene.n_tris
+        if build == "host":
+            self._nodes, self._tris, self.depth = build_aabb_tree(scene.tri_xyz)
+            self.d_nodes = to_device(self._nodes, torch, device)
+            self.d_tris = to_device(self._tris, torch, device)
+            ctx.set_bvh(self.d_nodes, len(self._nodes), self.d_tris, len(self._tris))
+        else:
+            self._nodes = self._tris = None
+            self.d_xyz = torch.from_numpy(np.ascontiguousarray(scene.tri_xyz, dtype=np.float32)).to(device)
+            self.d_nodes = torch.empty(abi.NODE_DTYPE.itemsize * (n - 1), dtype=torch.uint8, device=device)
+            self.d_tris = torch.empty(abi.TRIANGLE_DTYPE.itemsize * n, dtype=torch.uint8, device=device)
+            self.depth = ctx.build_bvh(self.d_xyz, n, self.d_nodes, self.d_tris)
         pl = pack_buffer(scene.point_lights, abi.POINT_LIGHT_DTYPE)
         tl = pack_buffer(scene.tri_lights, abi.TRI_LIGHT_DTYPE)
-        al = pack_buffer(self.alias, abi.ALIAS_DTYPE)
-        self.d_nodes = to_device(self.nodes, torch, device)
-        self.d_tris = to_device(self.tris, torch, device)
         self.d_points = to_device(pl, torch, device)
         self.d_trilights = to_device(tl, torch, device)
-        self.d_alias = to_device(al, torch, device)
+        n_pt, n_tl = len(scene.point_lights), len(scene.tri_lights)
+        if build == "host":
+            self._alias = build_alias_table(scene.point_lights, scene.tri_lights)
+            al = pack_buffer(self._alias, abi.ALIAS_DTYPE)
+            self.d_alias = to_device(al, torch, device)
+        else:  # header from the host, columns built in place after it
+            self._alias = None
+            n_l = n_pt if n_pt > 0 else n_tl
+            al = pack_buffer(np.zeros(n_l, dtype=abi.ALIAS_DTYPE), abi.ALIAS_DTYPE)
+            self.d_alias = to_device(al, torch, device)
+            if n_l:
+                h = abi.LIGHT_BUFFER_HEADER_BYTES
+                ctx.build_alias_table(self.d_points[h:] if n_pt else None, n_pt,
+                                      self.d_trilights[h:] if n_tl else None, n_tl, self.d_alias[h:])
         self.d_tri_material = to_device(scene.tri_material.astype(np.int32), torch, device)
         self.d_materials = to_device(scene.materials.astype(np.float32), torch, device)
-        ctx.set_bvh(self.d_nodes, len(self.nodes), self.d_tris, len(self.tris))
         ctx.set_lights(self.d_points, pl.nbytes, self.d_trilights, tl.nbytes, self.d_alias, al.nbytes)
+
+    @property
+    def nodes(self) -> np.ndarray:
+        if self._nodes is None:
+            self._nodes = self.d_nodes.cpu().numpy().view(abi.NODE_DTYPE)
+        return self._nodes
+
+    @property
+    def tris(self) -> np.ndarray:
+        if self._tris is None:
+            self._tris = self.d_tris.cpu().numpy().view(abi.TRIANGLE_DTYPE)
+        return self._tris
+
+    @property
+    def alias(self) -> np.ndarray:
+        if self._alias is None:
+            self._alias = self.d_alias[abi.LIGHT_BUFFER_HEADER_BYTES:].cpu().numpy().view(abi.ALIAS_DTYPE)
+        return self._alias
 
 
 class Renderer:
@@ -60,19 +105,20 @@ class Renderer:
     mode "unbiased": A -> C x spatial_passes (k neighbours) ; mode "biased": A -> (B, B) x
     spatial_iterations. ``launch`` selects the workgroup tile and Pass C schedule
     (restir_launch) of every pass; ``launches`` ({"a"|"b"|"c": Launch}) overrides it per pass, as
-    restir_amd.tuning.autotune chooses.
+    restir_amd.tuning.autotune chooses. ``scene_build`` ("host" or "device") selects where the
+    tree and alias table are built (SceneOnDevice).
     """
 
     def __init__(self, scene, width, height, *, device_index=0, mode="unbiased", M=32, k=5,
                  spatial_passes=1, spatial_neighbors=4, radius=30.0, temporal=True,
                  visibility=True, temporal_mult=20, launch=None, ctx=None, learned=None,
-                 train_samples=4096, lr=1e-3, train=True, launches=None):
+                 train_samples=4096, lr=1e-3, train=True, launches=None, scene_build="host"):
         torch = _torch()
         self.torch = torch
         self.device = torch.device("cuda", device_index)
         self.ctx = ctx or Context(device_index)
         self.scene = scene
-        self.dev = SceneOnDevice(self.ctx, scene, self.device)
+        self.dev = SceneOnDevice(self.ctx, scene, self.device, build=scene_build)
         self.W, self.H = width, height
         self.mode, self.M, self.k = mode, M, k
         self.spatial_passes = spatial_passes

@@ -81,6 +81,7 @@ def parse():
     ap.add_argument("--error-pixels", type=int, default=16384)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-views", action="store_true", help="skip the batched 16-view measurement")
+    ap.add_argument("--no-builds", action="store_true", help="skip the scene-builder measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=8, help="1080p frames of the CPU-oracle sample")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -333,6 +334,9 @@ def main():
     if not args.no_views:
         out["batched_views"] = batched_views(torch, args)
 
+    if not args.no_builds:
+        out["scene_builds"] = scene_builds(torch, scene)
+
     if args.copy_calibration:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
@@ -425,6 +429,56 @@ def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
             "speedup_learned_vs_plain": round(float(speedup), 4),
         },
     }
+
+
+def scene_builds(torch, scene):
+    """Device tree build (restir_build_bvh) against the host build of the same algorithm
+    (restir_build_aabb_tree: the reference's sequential code, one core), on the 1080p scene and
+    on a seeded 1M-triangle terrain. Device time: host wall clock around a build (it waits for
+    the stream once per tree level), best of 5 after one warm build; the device tree is compared
+    byte for byte with the host one."""
+    from restir_amd import abi, scenes
+    from restir_amd.abi import Context
+
+    big = scenes.terrain(31, grid=708, n_boxes=500, n_point_lights=10)
+    ctx = Context(0)
+    rows = []
+    for name, sc in (("1080p scene", scene), ("terrain 708x708 + 500 boxes", big)):
+        tri = np.ascontiguousarray(sc.tri_xyz, dtype=np.float32)
+        n = len(tri)
+        t = time.perf_counter()
+        hn, ht, hd = abi.build_aabb_tree(tri)
+        host_ms = (time.perf_counter() - t) * 1e3
+        x = torch.from_numpy(tri).cuda()
+        nodes = torch.empty(abi.NODE_DTYPE.itemsize * (n - 1), dtype=torch.uint8, device="cuda")
+        tris = torch.empty(abi.TRIANGLE_DTYPE.itemsize * n, dtype=torch.uint8, device="cuda")
+        ctx.build_bvh(x, n, nodes, tris)
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(5):
+            t = time.perf_counter()
+            depth = ctx.build_bvh(x, n, nodes, tris)
+            torch.cuda.synchronize()
+            best = min(best, (time.perf_counter() - t) * 1e3)
+        ctx.profile_read()
+        ctx.profile_enable(True)
+        ctx.build_bvh(x, n, nodes, tris)
+        torch.cuda.synchronize()
+        prof = ctx.profile_read()
+        ctx.profile_enable(False)
+        same_tree = bool(np.array_equal(nodes.cpu().numpy(), hn.view(np.uint8)) and depth == hd)
+
+        top = sorted(prof.items(), key=lambda kv: -kv[1][1])[:6]
+        rows.append({"scene": name, "triangles": n, "depth": depth,
+                     "tree_device_ms": round(best, 3), "tree_host_ms": round(host_ms, 1),
+                     "tree_speedup": round(host_ms / best, 1), "tree_identical": same_tree,
+                     "tree_kernels_ms": {k: round(v[1], 3) for k, v in top},
+                     "tree_launches": int(sum(v[0] for v in prof.values()))})
+        log(f"scene build {name}: tree {best:.2f} ms on the GPU vs {host_ms:.0f} ms host "
+            f"(identical {same_tree})")
+        del x, nodes, tris
+    ctx.close()
+    return rows
 
 
 def batched_views(torch, args):

@@ -26,6 +26,7 @@ using namespace rki;
 namespace {
 
 constexpr uint32_t kLanes = 1024;  // one workgroup of 16 waves
+constexpr bool kSwitchedOff = true;
 
 // exclusive scan of v over the workgroup; *total gets the sum
 __device__ inline uint32_t group_exclusive(uint32_t v, uint32_t *waveTotal, uint32_t *total) {
@@ -175,6 +176,14 @@ extern "C" int restir_build_alias_table_device(restir_ctx *ctx, const restir_poi
                                       : sizeof(restir_tri_light) * static_cast<size_t>(n);
     if (overlaps(columns, colBytes, in, inBytes))
         return fail(ctx, RESTIR_E_ALIASING, "build_alias_table_device: columns overlap the light records");
+    // Switched off: a test run faulted the GPU while this kernel's 5000-round case ran, after
+    // the same case had passed in an earlier run, and no cause has been identified (a CPU
+    // emulation of every index stays in bounds and matches the host table). Until one is, the
+    // entry point refuses instead of launching.
+    if (kSwitchedOff)
+        return fail(ctx, RESTIR_E_NOT_READY,
+                    "build_alias_table_device is switched off (unexplained GPU fault under investigation); "
+                    "use restir_build_alias_table on the host");
     if (int r = use_device(ctx)) return r;
     if (int r = ensure(ctx, &ctx->aliasScratch, &ctx->aliasScratchBytes, 20ull * n)) return r;
     float *power = static_cast<float *>(ctx->aliasScratch);

@@ -35,11 +35,10 @@ def to_device(arr: np.ndarray, torch, device):
 class SceneOnDevice:
     """BVH, triangles, light tables and materials resident in HBM and bound to a context.
 
-    ``build="host"`` builds the tree and the alias table on the host (restir_build_aabb_tree,
-    restir_build_alias_table) and uploads them; ``build="device"`` uploads vertices and light
-    records and builds the same tree and table in HBM (restir_build_bvh,
-    restir_build_alias_table_device); ``nodes`` / ``tris`` / ``alias`` are then read back on first
-    access, for checking."""
+    ``build="host"`` builds the tree on the host (restir_build_aabb_tree) and uploads it;
+    ``build="device"`` uploads the vertices and builds the same tree in HBM (restir_build_bvh);
+    ``nodes`` / ``tris`` are then read back on first access, for checking. The alias table is
+    built on the host (restir_build_alias_table) in both modes."""
 
     def __init__(self, ctx: Context, scene, device="cuda", build="host"):
         torch = _torch()
@@ -62,20 +61,11 @@ class SceneOnDevice:
         tl = pack_buffer(scene.tri_lights, abi.TRI_LIGHT_DTYPE)
         self.d_points = to_device(pl, torch, device)
         self.d_trilights = to_device(tl, torch, device)
-        n_pt, n_tl = len(scene.point_lights), len(scene.tri_lights)
-        if build == "host":
-            self._alias = build_alias_table(scene.point_lights, scene.tri_lights)
-            al = pack_buffer(self._alias, abi.ALIAS_DTYPE)
-            self.d_alias = to_device(al, torch, device)
-        else:  # header from the host, columns built in place after it
-            self._alias = None
-            n_l = n_pt if n_pt > 0 else n_tl
-            al = pack_buffer(np.zeros(n_l, dtype=abi.ALIAS_DTYPE), abi.ALIAS_DTYPE)
-            self.d_alias = to_device(al, torch, device)
-            if n_l:
-                h = abi.LIGHT_BUFFER_HEADER_BYTES
-                ctx.build_alias_table(self.d_points[h:] if n_pt else None, n_pt,
-                                      self.d_trilights[h:] if n_tl else None, n_tl, self.d_alias[h:])
+        # the alias table is built on the host in both modes (the device alias build is
+        # switched off in the library)
+        self._alias = build_alias_table(scene.point_lights, scene.tri_lights)
+        al = pack_buffer(self._alias, abi.ALIAS_DTYPE)
+        self.d_alias = to_device(al, torch, device)
         self.d_tri_material = to_device(scene.tri_material.astype(np.int32), torch, device)
         self.d_materials = to_device(scene.materials.astype(np.float32), torch, device)
         ctx.set_lights(self.d_points, pl.nbytes, self.d_trilights, tl.nbytes, self.d_alias, al.nbytes)
@@ -94,8 +84,6 @@ class SceneOnDevice:
 
     @property
     def alias(self) -> np.ndarray:
-        if self._alias is None:
-            self._alias = self.d_alias[abi.LIGHT_BUFFER_HEADER_BYTES:].cpu().numpy().view(abi.ALIAS_DTYPE)
         return self._alias
 
 
@@ -106,7 +94,7 @@ class Renderer:
     spatial_iterations. ``launch`` selects the workgroup tile and Pass C schedule
     (restir_launch) of every pass; ``launches`` ({"a"|"b"|"c": Launch}) overrides it per pass, as
     restir_amd.tuning.autotune chooses. ``scene_build`` ("host" or "device") selects where the
-    tree and alias table are built (SceneOnDevice).
+    tree is built (SceneOnDevice).
     """
 
     def __init__(self, scene, width, height, *, device_index=0, mode="unbiased", M=32, k=5,

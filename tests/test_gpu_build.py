@@ -118,8 +118,8 @@ def test_device_tree_rejects_bad_input(torch, ctx):
 
 
 def test_frame_on_device_built_scene_equals_host_built(torch):
-    """Renderer with the tree and the alias table built in HBM: every reservoir of two unbiased
-    frames equals the host-built scene's, bit for bit."""
+    """Renderer with the tree built in HBM: every reservoir of two unbiased frames equals the
+    host-built scene's, bit for bit."""
     sc = scenes.terrain(11, grid=48, n_boxes=60, n_point_lights=2000, extent=16.0)
     out = []
     for build in ("host", "device"):
@@ -131,7 +131,11 @@ def test_frame_on_device_built_scene_equals_host_built(torch):
     assert np.array_equal(out[0], out[1])
 
 
-# ---- alias table: restir_build_alias_table_device against createAliasTable's host restatement
+# ---- alias table: restir_build_alias_table_device against createAliasTable's host restatement.
+# Skipped: the library switches that entry point off (it refuses with RESTIR_E_NOT_READY) after a
+# test run faulted the GPU while its 5000-round "dominant" case ran, with no cause identified.
+ALIAS_SKIP = pytest.mark.skip(reason="device alias build switched off: unexplained GPU fault in its "
+                                     "'dominant' case, cause not identified")
 def _points(lum):
     pts = np.zeros(len(lum), dtype=abi.POINT_LIGHT_DTYPE)
     pts["color_lum"][:, 3] = np.asarray(lum, dtype=np.float32)
@@ -161,6 +165,7 @@ def _device_alias(torch, ctx, points=None, tri_lights=None):
     return cols.cpu().numpy()
 
 
+@ALIAS_SKIP
 @pytest.mark.parametrize("case", list(ALIAS_CASES))
 def test_device_alias_table_matches_host_build(torch, ctx, case):
     pts = ALIAS_CASES[case](np.random.default_rng(5))
@@ -175,9 +180,18 @@ def test_device_alias_table_matches_host_build(torch, ctx, case):
                              f"device {dev.view(abi.ALIAS_DTYPE)[i]} host {host[i]}")
 
 
+@ALIAS_SKIP
 def test_device_alias_table_triangle_lights(torch, ctx):
     rng = np.random.default_rng(8)
     tri = rng.standard_normal((700, 9)).astype(np.float32)
     tls = scenes.make_tri_lights(tri, rng.uniform(0.0, 3.0, (700, 3)))
     host = abi.build_alias_table(None, tls)
     assert np.array_equal(_device_alias(torch, ctx, tri_lights=tls), host.view(np.uint8))
+
+
+def test_device_alias_build_refuses_while_switched_off(torch, ctx):
+    """The switched-off entry point returns an error and launches nothing."""
+    pts = torch.zeros(32 * 4, dtype=torch.uint8, device="cuda")
+    cols = torch.zeros(16 * 4, dtype=torch.uint8, device="cuda")
+    with pytest.raises(abi.RestirError, match="switched off"):
+        ctx.build_alias_table(pts, 4, None, 0, cols)

@@ -355,8 +355,10 @@ struct PassCArgs {
 // The neighbour merge of unbiasedReuse.glsl:84-124 for pixel (x, y): k random neighbours in a
 // disk, each merged into `res` with the target evaluated at this pixel. Records each
 // neighbour's pixel index and sample count for the later Z computation.
+// `keep(i, ni, M)` records neighbour i's pixel and sample count for the visibility step.
+template <typename Keep>
 __device__ __forceinline__ void pass_c_merge(const PassCArgs &a, uint32_t x, uint32_t y, size_t idx,
-                                             f3 pos, f3 n, Res &res, uint32_t *nIdx, uint32_t *nM) {
+                                             f3 pos, f3 n, Res &res, Keep keep) {
     const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
     const GView &g = a.g;
     const f3 alb = plane3(g.alb, idx, g.P);
@@ -377,8 +379,7 @@ __device__ __forceinline__ void pass_c_merge(const PassCArgs &a, uint32_t x, uin
         ny = ny < 0 ? 0 : (ny > static_cast<int32_t>(H) - 1 ? static_cast<int32_t>(H) - 1 : ny);
         const uint32_t ni = static_cast<uint32_t>(ny) * W + static_cast<uint32_t>(nx);
         const Res rr = load_res(a.in, ni);
-        nIdx[i] = ni;
-        nM[i] = rr.M;
+        keep(i, ni, rr.M);
         res.M += rr.M;
         const float ph = phat_of(s, rr);
         const float weight = ph * rr.w * static_cast<float>(rr.M);
@@ -414,7 +415,10 @@ __device__ __forceinline__ void pass_c_px(const PassCArgs &a, uint32_t x, uint32
     uint32_t nIdx[16];
     uint32_t nM[16];
     const int k = a.k;
-    pass_c_merge(a, x, y, idx, pos, n, res, nIdx, nM);
+    pass_c_merge(a, x, y, idx, pos, n, res, [&](int i, uint32_t ni, uint32_t m) {
+        nIdx[i] = ni;
+        nM[i] = m;
+    });
     const bool vis = (a.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) != 0;
     const f3 lp = res.pos;
     uint32_t numSamples = origM;
@@ -472,6 +476,7 @@ struct CQueues {
     uint32_t *bits;        // per pixel: above-horizon neighbour mask (bits 0..15), flags below
     uint32_t *occ;         // per pixel: occluded neighbour mask
     uint32_t *nM, *nIdx;   // [slot * npx + pixel]
+    float4 *surf;          // per pixel: {position.xyz, normal.x}, {normal.yz, 0, 0}
     uint32_t npx;
 };
 constexpr uint32_t kBitBackground = 1u << 29;  // pass-through pixel, already final
@@ -495,6 +500,17 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, uint32_t coun
     return base + incl - count;
 }
 
+// Position and normal of every pixel in one 32-byte record: the merge's horizon test and the
+// neighbour-ray emission gather them for random neighbours, one sector per neighbour instead of
+// one line per plane.
+__global__ __launch_bounds__(256) void k_pack_surface(const GView g, uint32_t npx, float4 *surf) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= npx) return;
+    const f3 p = plane3(g.pos, i, g.P), n = plane3(g.nrm, i, g.P);
+    surf[2 * static_cast<size_t>(i)] = make_float4(p.x, p.y, p.z, n.x);
+    surf[2 * static_cast<size_t>(i) + 1] = make_float4(n.y, n.z, 0.0f, 0.0f);
+}
+
 template <int TX, int TY>
 __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, const CQueues q) {
     uint32_t x, y;
@@ -515,19 +531,19 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
             q.bits[idx] = kBitBackground;
         } else {
             const uint32_t origM = res.M;
-            uint32_t nIdx[16];
-            uint32_t nM[16];
-            pass_c_merge(a, x, y, idx, pos, n, res, nIdx, nM);
+            pass_c_merge(a, x, y, idx, pos, n, res, [&](int i, uint32_t ni, uint32_t m) {
+                q.nIdx[static_cast<size_t>(i) * q.npx + idx] = ni;
+                q.nM[static_cast<size_t>(i) * q.npx + idx] = m;
+            });
             lp = res.pos;
             uint32_t above = 0, numSamples = origM;
             for (int j = 0; j < a.k; ++j) {
-                const f3 nwp = plane3(g.pos, nIdx[j], g.P);
-                const f3 nn = plane3(g.nrm, nIdx[j], g.P);
+                const uint32_t ni = q.nIdx[static_cast<size_t>(j) * q.npx + idx];
+                const float4 s0 = q.surf[2 * static_cast<size_t>(ni)], s1 = q.surf[2 * static_cast<size_t>(ni) + 1];
+                const f3 nwp{s0.x, s0.y, s0.z}, nn{s0.w, s1.x, s1.y};
                 if (dot(lp - nwp, nn) < 0.0f) continue;
                 above |= 1u << j;
-                numSamples += nM[j];
-                q.nM[static_cast<size_t>(j) * q.npx + idx] = nM[j];
-                q.nIdx[static_cast<size_t>(j) * q.npx + idx] = nIdx[j];
+                if (!vis) numSamples += q.nM[static_cast<size_t>(j) * q.npx + idx];
             }
             if (!vis) finish_w(res.w, res.sumW, res.pHat, numSamples);
             store_res(a.out, idx, res);
@@ -595,7 +611,8 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_emit(const PassCArgs a, const
             uint32_t m = oWant;
             for (uint32_t k = r - oExcl; k > 0; --k) m &= m - 1;  // the k-th set bit of the owner
             const uint32_t j = static_cast<uint32_t>(__ffs(static_cast<int>(m)) - 1);
-            const f3 nwp = plane3(a.g.pos, q.nIdx[static_cast<size_t>(j) * q.npx + oIdx], a.g.P);
+            const float4 sp = q.surf[2 * static_cast<size_t>(q.nIdx[static_cast<size_t>(j) * q.npx + oIdx])];
+            const f3 nwp{sp.x, sp.y, sp.z};
             q.n1[base + r] = make_float4(nwp.x, nwp.y, nwp.z, __uint_as_float(oIdx | (j << 27)));
             q.n2[base + r] = make_float4(lx, ly, lz, 0.0f);
         }
@@ -1128,6 +1145,8 @@ static hipError_t run_c_queued(PassCArgs args, const CQueues &q, uint32_t gridCa
     const dim3 tiles(args.nTiles), block(TX * TY);
     const uint32_t rayBlocks = gridCap;
     RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 4u);
+    const uint32_t npx = args.u.screenSize[0] * args.u.screenSize[1];
+    RK_LAUNCH("k_pack_surface", k_pack_surface, dim3((npx + 255) / 256), dim3(256), 0, s, args.g, npx, q.surf);
     RK_LAUNCH("k_pass_c_merge", (k_pass_c_merge<TX, TY>), tiles, block, 0, s, args, q);
     if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
         RK_LAUNCH("k_trace_queue(centre)", (k_trace_queue<STACK, true>), dim3(rayBlocks), dim3(256), 0, s,
@@ -1153,7 +1172,7 @@ static hipError_t dispatch_c_queued(int tile, const PassCArgs &args, const CQueu
 
 size_t pass_c_queue_bytes(uint32_t npx, int k) {
     const size_t n = npx, kk = static_cast<size_t>(k);
-    return 256 + 2 * sizeof(float4) * n * (1 + kk) + sizeof(uint32_t) * n * (3 + 2 * kk) + 6 * 256;
+    return 256 + 2 * sizeof(float4) * n * (2 + kk) + sizeof(uint32_t) * n * (3 + 2 * kk) + 7 * 256;
 }
 
 // Carves the queues out of one scratch allocation of pass_c_queue_bytes(npx, k) bytes.
@@ -1178,6 +1197,7 @@ static CQueues carve_queues(void *scratch, uint32_t npx, int k) {
     q.occ = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * n));
     q.nM = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * n * kk));
     q.nIdx = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * n * kk));
+    q.surf = reinterpret_cast<float4 *>(take(2 * sizeof(float4) * n));
     q.npx = npx;
     return q;
 }

@@ -173,12 +173,17 @@ void restir_destroy(restir_ctx *ctx);
 const char *restir_last_error(const restir_ctx *ctx);
 /* ABI revision; bumped whenever an entry point's signature changes. Bindings compare it with the
  * revision they were written against and refuse a mismatched library. */
-#define RESTIR_ABI_VERSION 7
+#define RESTIR_ABI_VERSION 8
 int restir_version(void);
 
 /* Light tables: device pointers to the reference buffer images (header + array). The counts are
  * read back once here and validated against the byte sizes. A NULL triangle buffer means none.
- * Replaces descriptor set 0 bindings 0..2 of restirOmni.glsl:11-23. */
+ * Replaces descriptor set 0 bindings 0..2 of restirOmni.glsl:11-23. As in the reference, the
+ * alias table covers the point lights if there are any, else the triangle lights (its column
+ * count must match). Extension: with both kinds present, an alias table of n_point + n_tri
+ * columns (restir_build_alias_table_mixed) is one table over the point lights followed by the
+ * triangle lights, and Pass A samples both kinds from it (a triangle light is drawn as the
+ * reference draws it, restirOmni.glsl:122-131). */
 int restir_set_lights(restir_ctx *ctx, const void *point_lights, size_t point_bytes,
                       const void *tri_lights, size_t tri_bytes, const void *alias_table,
                       size_t alias_bytes);
@@ -265,13 +270,23 @@ int restir_build_alias_table_device(restir_ctx *ctx, const restir_point_light *p
                                     uint32_t n_point, const restir_tri_light *tris, uint32_t n_tri,
                                     restir_alias_table_column *columns, void *stream);
 
+/* One alias table over n_point point lights followed by n_tri triangle lights (no reference
+ * counterpart: createAliasTable covers one kind): weights luminance and luminance * area, built by
+ * createAliasTable's FIFO construction (misc.cpp:418-497). Writes n_point + n_tri columns. */
+int restir_build_alias_table_mixed(const restir_point_light *points, uint32_t n_point,
+                                   const restir_tri_light *tris, uint32_t n_tri,
+                                   restir_alias_table_column *columns);
+
 /* Light clusters for the learned proposal (no reference counterpart): seeded k-means over the n
  * point-light positions into k clusters (1 <= k <= n; initial centroids by a PCG32(seed)
  * partial Fisher-Yates draw, `iterations` Lloyd steps, ties to the lower centroid), then
  * per-cluster Vose tables over luminance as createAliasTable builds them (misc.cpp:418-497).
  * Outputs: light_cluster[n], light_pwithin[n] (luminance share within its cluster),
  * cluster_begin[k], cluster_count[k], cluster_lights[n] (lights grouped by cluster, index
- * order), cluster_alias[n] (alias entries local to the cluster), centroids[3k] (may be NULL). */
+ * order), cluster_alias[n] (alias entries local to the cluster), centroids[3k] (may be NULL).
+ * The clusters must cover the alias table's index space: with one table over both kinds, pass
+ * after the point lights one record per triangle light with its centroid as position and
+ * luminance * area as luminance. */
 int restir_build_light_clusters(const restir_point_light *points, uint32_t n, uint32_t k,
                                 uint64_t seed, uint32_t iterations, int32_t *light_cluster,
                                 float *light_pwithin, int32_t *cluster_begin,

@@ -36,6 +36,7 @@ SIGS = {
     "oracle_add_sample": (None, [_vp, _vp, _vp, _f32, C.c_int32, _f32, _f32, _vp]),
     "oracle_combine": (None, [_vp, _vp, _f32, _vp]),
     "oracle_build_alias_table": (_i32, [_vp, _u32, _vp, _u32, _vp]),
+    "oracle_build_alias_table_mixed": (_i32, [_vp, _u32, _vp, _u32, _vp]),
     "oracle_build_aabb_tree": (_i32, [_vp, _u32, _vp, _vp, C.POINTER(_u32)]),
     "oracle_build_light_clusters": (_i32, [_vp, _u32, _u32, C.c_uint64, _u32, _vp, _vp, _vp, _vp,
                                            _vp, _vp, _vp]),
@@ -111,6 +112,15 @@ def build_alias_table(points, tri_lights, dtype):
     lib().oracle_build_alias_table(_p(points), 0 if points is None else len(points),
                                    _p(tri_lights), 0 if tri_lights is None else len(tri_lights),
                                    out.ctypes.data)
+    return out
+
+
+def build_alias_table_mixed(points, tri_lights, dtype):
+    """One table over the point lights followed by the triangle lights (the union light space)."""
+    n = len(points) + len(tri_lights)
+    out = np.zeros(n, dtype=dtype)
+    lib().oracle_build_alias_table_mixed(_p(points), len(points), _p(tri_lights), len(tri_lights),
+                                         out.ctypes.data)
     return out
 
 

@@ -296,23 +296,13 @@ void oracle_combine(restir_reservoir *self, const restir_reservoir *other, float
 }
 
 /* ------------------------------------------------------------------------ alias table ----- */
-int oracle_build_alias_table(const restir_point_light *points, uint32_t n_point,
-                             const restir_tri_light *tris, uint32_t n_tri,
-                             restir_alias_table_column *out) {
-    /* misc.cpp:418-497 — Vose's method with two FIFO queues */
-    uint32_t n = n_point > 0 ? n_point : n_tri;
-    if (n == 0) return 0;
-    float *p = (float *)malloc(sizeof(float) * n);
+/* misc.cpp:418-497 — Vose's method with two FIFO queues over weights w[0..n) (modified) */
+static void vose_fifo(float *p, uint32_t n, restir_alias_table_column *out) {
     uint32_t *big = (uint32_t *)malloc(sizeof(uint32_t) * (n + 1));
     uint32_t *small = (uint32_t *)malloc(sizeof(uint32_t) * (n + 1));
     size_t bh = 0, bt = 0, sh = 0, st = 0, cap = n + 1;
     float powerSum = 0.0f;
-    for (uint32_t i = 0; i < n; ++i) {
-        float w = n_point > 0 ? points[i].color_luminance[3]
-                              : tris[i].emission_luminance[3] * tris[i].normalArea[3];
-        powerSum += w;
-        p[i] = w;
-    }
+    for (uint32_t i = 0; i < n; ++i) powerSum += p[i];
     for (uint32_t i = 0; i < n; ++i) {
         out[i].prob = 0.0f;
         out[i].alias = -1;
@@ -332,7 +322,36 @@ int oracle_build_alias_table(const restir_point_light *points, uint32_t n_point,
     while (bh != bt) { uint32_t g = big[bh++ % cap]; out[g].prob = 1.0f; out[g].alias = (int32_t)g; }
     while (sh != st) { uint32_t l = small[sh++ % cap]; out[l].prob = 1.0f; out[l].alias = (int32_t)l; }
     for (uint32_t i = 0; i < n; ++i) out[i].aliasOriProb = out[out[i].alias].oriProb;
-    free(p); free(big); free(small);
+    free(big); free(small);
+}
+
+int oracle_build_alias_table(const restir_point_light *points, uint32_t n_point,
+                             const restir_tri_light *tris, uint32_t n_tri,
+                             restir_alias_table_column *out) {
+    /* misc.cpp:418-497: point lights if any, else triangle lights */
+    uint32_t n = n_point > 0 ? n_point : n_tri;
+    if (n == 0) return 0;
+    float *p = (float *)malloc(sizeof(float) * n);
+    for (uint32_t i = 0; i < n; ++i)
+        p[i] = n_point > 0 ? points[i].color_luminance[3]
+                           : tris[i].emission_luminance[3] * tris[i].normalArea[3];
+    vose_fifo(p, n, out);
+    free(p);
+    return 0;
+}
+
+int oracle_build_alias_table_mixed(const restir_point_light *points, uint32_t n_point,
+                                   const restir_tri_light *tris, uint32_t n_tri,
+                                   restir_alias_table_column *out) {
+    /* extension: one table over point lights then triangle lights, weights as misc.cpp:426-442 */
+    uint32_t n = n_point + n_tri;
+    if (n == 0) return 0;
+    float *p = (float *)malloc(sizeof(float) * n);
+    for (uint32_t i = 0; i < n_point; ++i) p[i] = points[i].color_luminance[3];
+    for (uint32_t i = 0; i < n_tri; ++i)
+        p[n_point + i] = tris[i].emission_luminance[3] * tris[i].normalArea[3];
+    vose_fifo(p, n, out);
+    free(p);
     return 0;
 }
 
@@ -716,6 +735,8 @@ void oracle_pass_a(const oracle_scene *s, const restir_uniforms *u, const restir
     uint32_t W = u->screenSize[0], H = u->screenSize[1];
     v3 cam = ld3(u->cameraPos);
     int32_t pointCount = (int32_t)s->n_points, aliasCount = (int32_t)s->n_alias;
+    /* one table over both kinds (point lights first) when it has a column for every light */
+    int mixed = pointCount > 0 && s->n_tri_lights > 0 && aliasCount == pointCount + (int32_t)s->n_tri_lights;
 #pragma omp parallel for schedule(dynamic, 4)
     for (int64_t yy = 0; yy < (int64_t)H; ++yy) {
         uint32_t y = (uint32_t)yy;
@@ -741,12 +762,13 @@ void oracle_pass_a(const oracle_scene *s, const restir_uniforms *u, const restir
                     float ln4[4] = {0, 0, 0, 0};
                     float lum;
                     int32_t lidx;
-                    if (pointCount != 0) {
+                    if (mixed ? idx < pointCount : pointCount != 0) {
                         const restir_point_light *pl = &s->points[idx];
                         L = ld3(pl->pos);
                         lum = pl->color_luminance[3];
                         lidx = idx;
                     } else {
+                        if (mixed) idx -= pointCount;
                         const restir_tri_light *tl = &s->tri_lights[idx];
                         float a = pcg_float(rng), b = pcg_float(rng);
                         float sa = sqrtf(a); /* pickPointOnTriangle, :68-71 */

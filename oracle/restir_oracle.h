@@ -83,6 +83,12 @@ void oracle_combine(restir_reservoir *self, const restir_reservoir *other, float
 int oracle_build_alias_table(const restir_point_light *points, uint32_t n_point,
                              const restir_tri_light *tris, uint32_t n_tri,
                              restir_alias_table_column *out);
+/* Extension (no reference counterpart): one table over n_point point lights followed by n_tri
+ * triangle lights, weights luminance and luminance * area, built as createAliasTable builds. A
+ * scene whose alias table has n_point + n_tri columns (both > 0) samples from this union. */
+int oracle_build_alias_table_mixed(const restir_point_light *points, uint32_t n_point,
+                                   const restir_tri_light *tris, uint32_t n_tri,
+                                   restir_alias_table_column *out);
 
 /* Seeded k-means light clusters with per-cluster Vose tables (no reference counterpart; the
  * learned proposal's cluster structure). Pinned by k-means invariants in tests/. */

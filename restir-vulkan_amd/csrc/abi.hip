@@ -196,11 +196,14 @@ int restir_set_lights(restir_ctx *ctx, const void *pointBuf, size_t pointBytes, 
         return fail(ctx, RESTIR_E_SIZE_MISMATCH, "set_lights: %d triangle lights exceed %zu bytes", nt, triBytes);
     if (hdr + static_cast<size_t>(na) * sizeof(restir_alias_table_column) > aliasBytes)
         return fail(ctx, RESTIR_E_SIZE_MISMATCH, "set_lights: %d alias columns exceed %zu bytes", na, aliasBytes);
-    // the alias table covers point lights if there are any, else triangle lights (misc.cpp:426-442)
+    // the alias table covers point lights if there are any, else triangle lights (misc.cpp:426-442),
+    // or, with a column for every light of both kinds, the union (point lights first)
     const int32_t expect = np > 0 ? np : nt;
     if (expect == 0) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "set_lights: no lights");
-    if (na != expect)
-        return fail(ctx, RESTIR_E_SIZE_MISMATCH, "set_lights: alias table has %d columns, expected %d", na, expect);
+    const bool mixed = np > 0 && nt > 0 && na == np + nt;
+    if (na != expect && !mixed)
+        return fail(ctx, RESTIR_E_SIZE_MISMATCH, "set_lights: alias table has %d columns, expected %d%s", na, expect,
+                    np > 0 && nt > 0 ? " (or point + triangle lights for one table over both)" : "");
     const char *pb = static_cast<const char *>(pointBuf);
     const char *ab = static_cast<const char *>(aliasBuf);
     {   // every alias must name a light: the kernels index the light arrays with it
@@ -219,6 +222,8 @@ int restir_set_lights(restir_ctx *ctx, const void *pointBuf, size_t pointBytes, 
     ctx->scene.nPoints = np;
     ctx->scene.nTriLights = nt;
     ctx->scene.nAlias = na;
+    ctx->scene.mixed = mixed ? 1 : 0;
+    ctx->scene.nLights = mixed ? np + nt : expect;
     ctx->lightsSet = true;
     return RESTIR_OK;
 }

@@ -54,15 +54,14 @@ int restir_set_light_clusters(restir_ctx *ctx, uint32_t k, const int32_t *lightC
                               const restir_alias_table_column *clusterAlias) {
     if (!ctx) return RESTIR_E_INVALID_ARGUMENT;
     ctx->clustersSet = false;
-    if (!ctx->lightsSet || ctx->scene.nPoints == 0)
-        return fail(ctx, RESTIR_E_NOT_READY, "set_light_clusters: set point lights first");
+    if (!ctx->lightsSet) return fail(ctx, RESTIR_E_NOT_READY, "set_light_clusters: set lights first");
     if (!lightCluster || !lightPWithin || !clusterBegin || !clusterCount || !clusterLights || !clusterAlias)
         return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "set_light_clusters: NULL buffer");
-    if (k == 0 || k > static_cast<uint32_t>(ctx->scene.nPoints))
+    if (k == 0 || k > static_cast<uint32_t>(ctx->scene.nLights))
         return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "set_light_clusters: k=%u", k);
     if (int r = use_device(ctx)) return r;
     // validate: the kernels index lights and cluster slots with these values
-    const size_t n = static_cast<size_t>(ctx->scene.nPoints);
+    const size_t n = static_cast<size_t>(ctx->scene.nLights);  // the alias table's index space
     std::vector<int32_t> lc, beg, cnt, lights;
     std::vector<restir_alias_table_column> al;
     if (int r = fetch(ctx, lc, lightCluster, n, "set_light_clusters: light_cluster")) return r;
@@ -145,8 +144,7 @@ int restir_initial_temporal_candidates(restir_ctx *ctx, const restir_uniforms *u
                                        restir_reservoir *out, const int32_t *candLight, const float *candQ,
                                        const restir_launch *launch, void *stream) {
     if (int r = check_frame(ctx, u, "initial_temporal_candidates")) return r;
-    if (!ctx->lightsSet || ctx->scene.nPoints == 0)
-        return fail(ctx, RESTIR_E_NOT_READY, "initial_temporal_candidates: point lights not set");
+    if (!ctx->lightsSet) return fail(ctx, RESTIR_E_NOT_READY, "initial_temporal_candidates: lights not set");
     const bool vis = (u->flags & RESTIR_VISIBILITY_REUSE_FLAG) != 0;
     const bool temporal = (u->flags & RESTIR_TEMPORAL_REUSE_FLAG) != 0;
     if (vis && !ctx->bvhSet) return fail(ctx, RESTIR_E_NOT_READY, "initial_temporal_candidates: BVH not set");
@@ -192,7 +190,8 @@ int restir_mlp_train(restir_ctx *ctx, restir_mlp *m, const restir_uniforms *u, c
     if (int r = ensure(ctx, &ctx->mlpScratch, &ctx->mlpScratchBytes, rk::mlp_scratch_bytes(n, m->n_clusters)))
         return r;
     const rt::f3 cam{u->cameraPos[0], u->cameraPos[1], u->cameraPos[2]};
-    const hipError_t e = rk::mlp_train(*m, view_of(g), cam, lo, hi, res, ctx->lightCluster, ctx->scene.nPoints,
+    const hipError_t e = rk::mlp_train(*m, view_of(g), cam, lo, hi, res, ctx->lightCluster, ctx->scene.nLights,
+                                       ctx->scene.mixed ? ctx->scene.nPoints : (ctx->scene.nPoints == 0 ? 0 : -1),
                                        n, lr, seed, lossOut,
                                        ctx->mlpScratch, static_cast<hipStream_t>(stream));
     return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "mlp_train launch");

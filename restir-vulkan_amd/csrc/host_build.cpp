@@ -356,3 +356,20 @@ extern "C" int restir_build_alias_table(const restir_point_light *points, uint32
     vose(power.data(), n, cols);
     return RESTIR_OK;
 }
+
+// Extension (no reference counterpart): one table over the point lights followed by the triangle
+// lights, with createAliasTable's weights and construction (misc.cpp:418-497).
+extern "C" int restir_build_alias_table_mixed(const restir_point_light *points, uint32_t nPoint,
+                                              const restir_tri_light *triLights, uint32_t nTri,
+                                              restir_alias_table_column *cols) {
+    const uint64_t n = static_cast<uint64_t>(nPoint) + nTri;
+    if (n == 0) return RESTIR_OK;
+    if (n > (1ull << 30)) return RESTIR_E_INVALID_ARGUMENT;
+    if (!cols || (nPoint > 0 && !points) || (nTri > 0 && !triLights)) return RESTIR_E_INVALID_ARGUMENT;
+    std::vector<float> power(static_cast<size_t>(n));
+    for (uint32_t i = 0; i < nPoint; ++i) power[i] = points[i].color_luminance[3];
+    for (uint32_t i = 0; i < nTri; ++i)
+        power[nPoint + i] = triLights[i].emission_luminance[3] * triLights[i].normalArea[3];
+    vose(power.data(), static_cast<uint32_t>(n), cols);
+    return RESTIR_OK;
+}

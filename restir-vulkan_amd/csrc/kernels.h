@@ -31,6 +31,8 @@ struct SceneView {
     const restir_tri_light *triLights;
     const float4 *alias;  // aliasTableColumn as float4 {prob, alias bits, oriProb, aliasOriProb}
     int32_t nPoints, nTriLights, nAlias;
+    int32_t mixed;    // 1: the alias table covers point lights then triangle lights (the union)
+    int32_t nLights;  // lights the alias table indexes: nAlias
     const rt::Node *nodes;
     const float4 *tris;  // 3 float4 per triangle
 };
@@ -137,7 +139,7 @@ hipError_t mlp_loss_grad(const restir_mlp &m, const uint16_t *X, const int32_t *
                          const uint32_t *mask, uint32_t n, float *gw, float *gb, float *loss,
                          void *scratch, hipStream_t s);
 hipError_t mlp_train(restir_mlp &m, const GView &g, rt::f3 cam, const float lo[3], const float hi[3],
-                     const restir_reservoir *res, const int32_t *lightCluster, int32_t nLights,
+                     const restir_reservoir *res, const int32_t *lightCluster, int32_t nLights, int32_t triBase,
                      uint32_t n, float lr,
                      uint32_t seed, float *lossOut, void *scratch, hipStream_t s);
 hipError_t mlp_propose(const restir_mlp &m, const GView &g, rt::f3 cam, const float lo[3],

@@ -282,7 +282,8 @@ struct GatherArgs {
     FeatArgs f;
     const restir_reservoir *res;
     const int32_t *lightCluster;
-    int32_t nLights;
+    int32_t nLights;  // lights the clusters cover (the alias table's index space)
+    int32_t triBase;  // index of triangle light 0 in that space, -1 if it has none
     uint32_t npx, n, seed;
     uint16_t *X;
     int32_t *target;
@@ -291,7 +292,8 @@ struct GatherArgs {
 };
 
 // Sample b takes a pseudo-random pixel; it trains (mask = 1) if that pixel's final reservoir
-// holds a point light with W > 0, with the light's cluster as the target.
+// holds a light of the clustered index space with W > 0, with the light's cluster as the target
+// (a reservoir names triangle light t as -1 - t).
 __global__ __launch_bounds__(256) void k_train_gather(const GatherArgs a) {
     const uint32_t b = blockIdx.x * 256u + threadIdx.x;
     if (b >= a.n) return;
@@ -301,9 +303,10 @@ __global__ __launch_bounds__(256) void k_train_gather(const GatherArgs a) {
     const bool surface = pixel_features(a.f, p, a.X + static_cast<size_t>(b) * kIn);
     const float4 q2 = reinterpret_cast<const float4 *>(a.res + p)[2];  // lightIndex, pHat, sumW, w
     const int32_t li = __float_as_int(q2.x);
-    const bool ok = surface && li >= 0 && li < a.nLights && q2.w > 0.0f && q2.z > 0.0f;
+    const int32_t j = li >= 0 ? li : (a.triBase >= 0 ? a.triBase + (-1 - li) : -1);
+    const bool ok = surface && j >= 0 && j < a.nLights && q2.w > 0.0f && q2.z > 0.0f;
     a.mask[b] = ok ? 1u : 0u;
-    a.target[b] = ok ? a.lightCluster[li] : 0;
+    a.target[b] = ok ? a.lightCluster[j] : 0;
     if (ok) atomicAdd(a.nValid, 1u);
 }
 
@@ -795,7 +798,7 @@ hipError_t mlp_features(const GView &g, f3 cam, const float lo[3], const float h
 
 hipError_t mlp_train(restir_mlp &m, const GView &g, f3 cam, const float lo[3], const float hi[3],
                      const restir_reservoir *res, const int32_t *lightCluster, int32_t nLights,
-                     uint32_t n, float lr,
+                     int32_t triBase, uint32_t n, float lr,
                      uint32_t seed, float *lossOut, void *scratch, hipStream_t s) {
     const int K = static_cast<int>(m.n_clusters);
     const MlpScratch t = carve_mlp(scratch, n, K);
@@ -804,6 +807,7 @@ hipError_t mlp_train(restir_mlp &m, const GView &g, f3 cam, const float lo[3], c
     ga.res = res;
     ga.lightCluster = lightCluster;
     ga.nLights = nLights;
+    ga.triBase = triBase;
     ga.npx = g.W * g.H;
     ga.n = n;
     ga.seed = seed;

@@ -21,7 +21,7 @@ TEMPORAL_REUSE_FLAG = 1 << 1
 TRACE_FIRST_HIT = 0
 TRACE_REFERENCE_LOOP = 1
 TRACE_ONE_RAY_PER_LANE = 2
-ABI_VERSION = 7  # RESTIR_ABI_VERSION of include/restir_abi.h
+ABI_VERSION = 8  # RESTIR_ABI_VERSION of include/restir_abi.h
 
 ERRORS = {
     -1: "RESTIR_E_INVALID_ARGUMENT",
@@ -163,6 +163,7 @@ SIGNATURES = {
     "restir_build_aabb_tree": (_i32, [_vp, _u32, _vp, _vp, C.POINTER(_u32)]),
     "restir_build_alias_table": (_i32, [_vp, _u32, _vp, _u32, _vp]),
     "restir_build_alias_table_device": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _vp]),
+    "restir_build_alias_table_mixed": (_i32, [_vp, _u32, _vp, _u32, _vp]),
     "restir_build_light_clusters": (_i32, [_vp, _u32, _u32, C.c_uint64, _u32, _vp, _vp, _vp, _vp,
                                            _vp, _vp, _vp]),
     "restir_primary_gbuffer": (_i32, [_vp, C.POINTER(Camera), _vp, _vp, _u32, _vp, _vp, _vp, _vp,
@@ -278,6 +279,48 @@ def build_alias_table(points: np.ndarray | None, tri_lights: np.ndarray | None) 
     if rc != 0:
         raise RestirError(rc, "restir_build_alias_table", "build failed")
     return cols
+
+
+def build_alias_table_mixed(points: np.ndarray, tri_lights: np.ndarray) -> np.ndarray:
+    """One table over the point lights followed by the triangle lights (extension)."""
+    lib = load_library()
+    pts = np.ascontiguousarray(points, dtype=POINT_LIGHT_DTYPE)
+    tls = np.ascontiguousarray(tri_lights, dtype=TRI_LIGHT_DTYPE)
+    cols = np.zeros(len(pts) + len(tls), dtype=ALIAS_DTYPE)
+    rc = lib.restir_build_alias_table_mixed(pts.ctypes.data if len(pts) else None, len(pts),
+                                            tls.ctypes.data if len(tls) else None, len(tls),
+                                            cols.ctypes.data)
+    if rc != 0:
+        raise RestirError(rc, "restir_build_alias_table_mixed", "build failed")
+    return cols
+
+
+def uses_union(points, tri_lights, union=None) -> bool:
+    """Whether a scene's alias table covers both kinds of lights: by default when both are
+    present (the reference's table covers point lights if any, else triangle lights)."""
+    both = points is not None and len(points) > 0 and tri_lights is not None and len(tri_lights) > 0
+    return both if union is None else bool(union) and both
+
+
+def scene_alias_table(points, tri_lights, union=None) -> np.ndarray:
+    """The scene's alias table: one table over both kinds (uses_union), else createAliasTable's."""
+    if uses_union(points, tri_lights, union):
+        return build_alias_table_mixed(points, tri_lights)
+    return build_alias_table(points if points is not None and len(points) else None,
+                             tri_lights if tri_lights is not None and len(tri_lights) else None)
+
+
+def union_light_records(points: np.ndarray, tri_lights: np.ndarray) -> np.ndarray:
+    """Point-light records for the union light space: the point lights, then one record per
+    triangle light (centroid position, luminance * area), as the cluster builder takes them."""
+    t = np.zeros(len(tri_lights), dtype=POINT_LIGHT_DTYPE)
+    if len(tri_lights):
+        c = (tri_lights["p1"][:, :3] + tri_lights["p2"][:, :3] + tri_lights["p3"][:, :3]) / np.float32(3.0)
+        t["pos"][:, :3] = c
+        t["pos"][:, 3] = 1.0
+        t["color_lum"][:, :3] = tri_lights["emission_lum"][:, :3]
+        t["color_lum"][:, 3] = (tri_lights["emission_lum"][:, 3] * tri_lights["normal_area"][:, 3]).astype(np.float32)
+    return np.concatenate([np.asarray(points, dtype=POINT_LIGHT_DTYPE), t])
 
 
 class LightClusters:

@@ -18,7 +18,11 @@ class LearnedProposal:
         import torch
         self.torch = torch
         self.ctx, self.k, self.mix = ctx, k, mix
-        cl = LightClusters(scene.point_lights, k, seed=seed, iterations=iterations)
+        # the clusters cover the alias table's index space: with one table over both kinds of
+        # lights, the triangle lights follow the point lights as (centroid, luminance * area)
+        union = abi.uses_union(scene.point_lights, scene.tri_lights)
+        records = abi.union_light_records(scene.point_lights, scene.tri_lights) if union else scene.point_lights
+        cl = LightClusters(records, k, seed=seed, iterations=iterations)
         self.clusters = cl
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
         self.d_light_cluster = t(cl.light_cluster)

@@ -40,7 +40,7 @@ class SceneOnDevice:
     ``nodes`` / ``tris`` are then read back on first access, for checking. The alias table is
     built on the host (restir_build_alias_table) in both modes."""
 
-    def __init__(self, ctx: Context, scene, device="cuda", build="host"):
+    def __init__(self, ctx: Context, scene, device="cuda", build="host", union_lights=None):
         torch = _torch()
         self.scene = scene
         if build not in ("host", "device"):
@@ -62,8 +62,10 @@ class SceneOnDevice:
         self.d_points = to_device(pl, torch, device)
         self.d_trilights = to_device(tl, torch, device)
         # the alias table is built on the host in both modes (the device alias build is
-        # switched off in the library)
-        self._alias = build_alias_table(scene.point_lights, scene.tri_lights)
+        # switched off in the library); with both kinds of lights, one table over both
+        # (abi.uses_union) unless union_lights=False asks for the reference's point-light table
+        self.union = abi.uses_union(scene.point_lights, scene.tri_lights, union_lights)
+        self._alias = abi.scene_alias_table(scene.point_lights, scene.tri_lights, union_lights)
         al = pack_buffer(self._alias, abi.ALIAS_DTYPE)
         self.d_alias = to_device(al, torch, device)
         self.d_tri_material = to_device(scene.tri_material.astype(np.int32), torch, device)

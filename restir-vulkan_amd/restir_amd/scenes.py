@@ -171,10 +171,16 @@ def value_noise(rng, xs, zs, cells=8, octaves=3):
 
 
 def terrain(seed: int = 2, grid: int = 256, n_boxes: int = 500, n_point_lights: int = 10000,
-            extent: float = 32.0, height: float = 6.0, light_intensity=(0.01, 10.0)) -> Scene:
+            extent: float = 32.0, height: float = 6.0, light_intensity=(0.01, 10.0),
+            n_tri_lights: int = 0, tri_light_emission=(0.01, 10.0)) -> Scene:
     """Config 2 scene: a grid x grid vertex heightfield over [-extent, extent]^2 displaced by
     value noise (2*(grid-1)^2 triangles) plus n_boxes random oriented boxes (12 triangles each);
-    point lights uniform in the scene AABB with per-channel log-uniform intensity."""
+    point lights uniform in the scene AABB with per-channel log-uniform intensity. With
+    n_tri_lights, that many emissive triangles are added as free-floating lamps (from a separate
+    random stream, so the rest of the scene is unchanged): random orientation, circumradius
+    uniform in [0.3, 1], centres between 1 and 4 units above the highest terrain or box point,
+    per-channel log-uniform emission, each with its own emissive material whose albedo is the
+    emission. They follow the other triangles and occlude like them."""
     rng = np.random.default_rng(seed)
     t = np.linspace(0.0, 1.0, grid)
     X, Z = np.meshgrid(t, t, indexing="ij")
@@ -207,9 +213,30 @@ def terrain(seed: int = 2, grid: int = 256, n_boxes: int = 500, n_point_lights: 
     rgb = np.exp(rng.uniform(math.log(a), math.log(b), size=(n_point_lights, 3)))
     camera = dict(position=(0.0, 0.8 * extent, 1.4 * extent), lookAt=(0.0, 0.0, 0.0),
                   worldUp=(0.0, 1.0, 0.0), fovY=math.pi / 3.0, zNear=0.01, zFar=1000.0)
+    tri_lights = np.zeros(0, dtype=TRI_LIGHT_DTYPE)
+    if n_tri_lights:
+        lrng = np.random.default_rng([seed, 7])
+        top = float(hi[1])  # above every terrain and box triangle
+        centres = np.stack([lrng.uniform(-0.9 * extent, 0.9 * extent, n_tri_lights),
+                            lrng.uniform(top + 1.0, top + 4.0, n_tri_lights),
+                            lrng.uniform(-0.9 * extent, 0.9 * extent, n_tri_lights)], axis=1)
+        radius = lrng.uniform(0.3, 1.0, n_tri_lights)
+        corner = np.array([[1.0, 0.0, 0.0], [-0.5, 0.0, 0.75 ** 0.5], [-0.5, 0.0, -(0.75 ** 0.5)]])
+        lamps = np.stack([centres[i] + radius[i] * (corner @ _random_rotation(lrng).T)
+                          for i in range(n_tri_lights)]).reshape(-1, 9).astype(np.float32)
+        ea, eb = tri_light_emission
+        emission = np.exp(lrng.uniform(math.log(ea), math.log(eb), size=(n_tri_lights, 3)))
+        tri_lights = make_tri_lights(lamps, emission)
+        em = np.zeros((n_tri_lights, materials.shape[1]), dtype=np.float32)
+        em[:, 0:3] = emission
+        em[:, 3] = 1.0  # emissive flag
+        em[:, 4] = 1.0  # roughness
+        tri_material = np.concatenate([tri_material, len(materials) + np.arange(n_tri_lights, dtype=np.int32)])
+        materials = np.concatenate([materials, em])
+        tri_xyz = np.concatenate([tri_xyz, lamps])
     return Scene("terrain", tri_xyz, tri_material, materials, make_point_lights(pos, rgb),
-                 np.zeros(0, dtype=TRI_LIGHT_DTYPE), camera,
-                 dict(seed=seed, grid=grid, n_boxes=n_boxes))
+                 tri_lights, camera,
+                 dict(seed=seed, grid=grid, n_boxes=n_boxes, n_tri_lights=n_tri_lights))
 
 
 def orbit_camera(camera: dict, frame: int, degrees_per_frame: float = 0.5) -> dict:
@@ -228,5 +255,5 @@ def config_scene(name: str) -> Scene:
     if name in ("c1", "cornell"):
         return cornell_box(1)
     if name in ("c2", "terrain"):
-        return terrain(2)
+        return terrain(2, n_tri_lights=2000)
     raise ValueError(name)

@@ -39,7 +39,7 @@ class HostScene:
     def __init__(self, scene: scenes.Scene):
         self.scene = scene
         self.nodes, self.tris, self.depth = abi.build_aabb_tree(scene.tri_xyz)
-        self.alias = abi.build_alias_table(scene.point_lights, scene.tri_lights)
+        self.alias = abi.scene_alias_table(scene.point_lights, scene.tri_lights)
         pts = scene.point_lights if len(scene.point_lights) else None
         tls = scene.tri_lights if len(scene.tri_lights) else None
         self.oracle = oracle.Scene(self.nodes, self.tris, pts, tls, self.alias)

@@ -92,7 +92,7 @@ def test_device_tree_matches_host_build(torch, ctx, case):
 
 
 def test_device_tree_matches_host_build_full_size_terrain(torch, ctx):
-    """The 1080p workload's scene (terrain 256x256 + 500 boxes, 136,050 triangles)."""
+    """The 1080p workload's scene (terrain 256x256 + 500 boxes + 2,000 lamps, 138,050 triangles)."""
     _assert_same_tree(torch, ctx, scenes.config_scene("terrain").tri_xyz, "terrain")
 
 

@@ -223,21 +223,43 @@ def frames_mean_and_var(r, frames):
     return (mean.view(3, r.H, r.W).cpu().numpy(), var_mean.view(3, r.H, r.W).cpu().numpy())
 
 
+def cornell_with_ceiling_lights():
+    """The Cornell box (lights 0.25 from the walls) plus its two ceiling triangles, shrunk to half
+    size about the ceiling's centre and lowered by 0.3, as emissive triangle lights: both kinds in
+    one alias table (the union). (Lights just under the ceiling would light it with a near-singular
+    1/d^2 that a stratified ground truth cannot integrate to 1%.)"""
+    import dataclasses
+    sc = scenes.cornell_box(1, light_margin=0.25)
+    tri = sc.tri_xyz.reshape(-1, 3, 3)
+    ceiling = np.flatnonzero(np.all(np.abs(tri[:, :, 1] - tri[:, :, 1].max()) < 1e-6, axis=1))[:2]
+    t = tri[ceiling].copy()
+    centre = t.reshape(-1, 3).mean(axis=0)
+    t = centre + 0.5 * (t - centre)
+    t[:, :, 1] -= 0.3
+    tl = scenes.make_tri_lights(t.reshape(-1, 9), np.array([[6.0, 5.0, 4.0], [4.0, 5.0, 6.0]]))
+    return dataclasses.replace(sc, tri_lights=tl)
+
+
+@pytest.mark.parametrize("lights", ["points", "points+triangles"])
 @pytest.mark.parametrize("learned", [False, True])
-def test_ris_converges_to_fp64_ground_truth(torch, learned):
+def test_ris_converges_to_fp64_ground_truth(torch, learned, lights):
     """Independent frames of unbiased ReSTIR (plain, or over the online-trained proposal)
     average to the FP64 oracle's direct lighting of the Cornell box: the squared error of the
     mean matches the variance the frames themselves show, and the image mean agrees to 1%; the per-pixel
-    residual (about 6% relative RMSE after 1500 frames) is the frames' own noise."""
-    sc = scenes.cornell_box(1, light_margin=0.25)
+    residual (about 6% relative RMSE after 1500 frames) is the frames' own noise. With triangle
+    lights the alias table and the proposal's clusters cover both kinds, and the ground truth
+    integrates the triangles with 1024 stratified samples each."""
+    sc = scenes.cornell_box(1, light_margin=0.25) if lights == "points" else cornell_with_ceiling_lights()
     W = H = 96
     hs = HostScene(sc)
     r = Renderer(sc, W, H, mode="unbiased", k=3, M=16)
+    assert r.dev.union == (lights != "points")
     if learned:
         r.learned = LearnedProposal(r.ctx, sc, k=64, seed=4)
         r.train_samples = 1024
     r.render_gbuffer(sc.camera, 0)
-    gt = oracle.ground_truth_f64(hs.oracle, sc.camera["position"], oracle.gbuffer(r.gbuf[0].cpu().numpy(), W, H))
+    gt = oracle.ground_truth_f64(hs.oracle, sc.camera["position"], oracle.gbuffer(r.gbuf[0].cpu().numpy(), W, H),
+                                 n_tri_samples=1024 if lights != "points" else 0)
     mean, var_mean = frames_mean_and_var(r, 1500)
     surf = gt.sum(axis=0) > 0
     err2 = ((mean - gt)[:, surf] ** 2).sum()

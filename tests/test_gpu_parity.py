@@ -139,6 +139,34 @@ def test_unbiased_two_passes_moving_camera_terrain(torch):
         run_frame_and_compare(r, scenes.orbit_camera(sc.camera, f))
 
 
+def test_unbiased_terrain_with_point_and_triangle_lights(torch):
+    """The 1080p workload's scene: 10k point lights and 2k emissive box triangles in one alias
+    table (the union); every pass and the shading equal the oracle's on the same inputs."""
+    sc = scenes.config_scene("terrain")
+    r = Renderer(sc, 480, 270, mode="unbiased", k=5, spatial_passes=2)
+    assert r.dev.union and len(r.dev.alias) == len(sc.point_lights) + len(sc.tri_lights) == 12000
+    for f in range(3):
+        a, _ = run_frame_and_compare(r, scenes.orbit_camera(sc.camera, f))
+    li = a["lightIndex"][a["sumWeights"] > 0]
+    assert (li < 0).any() and (li >= 0).any()  # both kinds were drawn
+
+
+@pytest.mark.parametrize("schedule", [abi.SCHEDULE_QUEUED, abi.SCHEDULE_INLINE])
+def test_biased_cornell_with_triangle_lights_bit_exact(torch, schedule):
+    import dataclasses
+    sc = scenes.cornell_box(1)
+    tri = sc.tri_xyz.reshape(-1, 3, 3)
+    ceiling = np.flatnonzero(np.all(np.abs(tri[:, :, 1] - tri[:, :, 1].max()) < 1e-6, axis=1))[:2]
+    t = tri[ceiling].copy()
+    t[:, :, 1] -= 0.01
+    sc = dataclasses.replace(sc, tri_lights=scenes.make_tri_lights(t.reshape(-1, 9), np.full((2, 3), 3.0)))
+    r = Renderer(sc, 128, 96, mode="biased", spatial_passes=1, spatial_neighbors=5,
+                 launch=abi.Launch(16, 16, schedule))
+    assert r.dev.union
+    run_frame_and_compare(r, sc.camera)
+    run_frame_and_compare(r, sc.camera)
+
+
 def test_trace_shadow_matches_oracle(torch):
     for sc in (scenes.cornell_box(1), scenes.terrain(2)):
         hs = HostScene(sc)

@@ -43,6 +43,12 @@ HBM_PEAK_GBS = 8000.0
 VALU_LANE_OPS = 39.3e12  # fp32 lane-ops/s (non-packed), the traversal's ALU ceiling
 NODE_BYTES, TRI_BYTES = 80, 48
 BYTES_PER_PX_SHADE = 124
+# compulsory bytes per pixel of the resampling passes (each buffer byte read or written once):
+# Pass A = current G 44 + previous G 36 + previous reservoir 64 + output 64; Pass C (without its
+# rays) = G 48 + reservoir in 64 + out 64
+BYTES_PER_PX_PASS_A, BYTES_PER_PX_PASS_C = 208, 176
+PASS_A_KERNELS = ("k_pass_a", "k_pass_a_ris", "k_pass_a_finish")
+PASS_C_KERNELS = ("k_pack_surface", "k_pass_c_merge", "k_pass_c_emit", "k_pass_c_finish", "k_pass_c")
 TRUTH_TRI_SAMPLES = 64  # stratified samples per triangle light in the FP64 ground truth
 
 
@@ -280,6 +286,15 @@ def main():
     algo_bytes = {"k_trace_queue(neighbour)": ray_bytes("neighbour"),
                   "k_trace_queue(centre)": ray_bytes("centre"),
                   "k_shade": BYTES_PER_PX_SHADE * npx}
+    def pass_rate(names, bytes_px, passes_per_frame):
+        ms = sum(kernels[k]["ms_per_frame"] for k in names if k in kernels) / passes_per_frame
+        gbps = bytes_px * npx / (ms * 1e-3) / 1e9
+        return {"kernels": [k for k in names if k in kernels], "ms_per_pass": round(ms, 4),
+                "algorithmic_bytes_per_px": bytes_px, "achieved_GBps": round(gbps, 1),
+                "frac_of_hbm_peak": round(gbps / HBM_PEAK_GBS, 4)}
+    resampling = {"pass_a": pass_rate(PASS_A_KERNELS, BYTES_PER_PX_PASS_A, 1),
+                  "pass_c_without_rays": pass_rate(PASS_C_KERNELS, BYTES_PER_PX_PASS_C, args.spatial_passes),
+                  "shade": pass_rate(("k_shade",), BYTES_PER_PX_SHADE, 1)}
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_frame"])
     dom_bytes = algo_bytes.get(dominant)
     achieved = dom_bytes / (kernels[dominant]["avg_ms"] * 1e-3) / 1e9 if dom_bytes else None
@@ -316,7 +331,7 @@ def main():
             "k": args.k, "spatial_passes": args.spatial_passes, "tile": args.tile,
         },
         "plain": {"ms_per_frame": round(plain_ms, 4), "pass_a_schedule": sched_plain,
-                  "kernels": rounded(kernels)},
+                  "kernels": rounded(kernels), "resampling_passes": resampling},
         "mrays_per_s": round(rays_per_s / 1e6, 1),
         "traversal": {"rays": nr, "ray_set": "last frame's second Pass C neighbour rays",
                       "node_visits_per_ray": round(Vn, 2), "tri_tests_per_ray": round(Tn, 2),

@@ -506,9 +506,10 @@ def scene_builds(torch, scene):
 def batched_views(torch, args):
     """16 views (resolutions log-uniform in [640x360, 2560x1440], seeded) over 4 seeded terrains
     (about 25k-200k triangles, 1k-50k point lights), unbiased ReSTIR M = 32 with temporal reuse
-    and 2 Pass C: 8 orbiting frames of all views captured in one HIP graph of persistent
-    multi-view launches, against the same frames rendered view by view (eager, one launch
-    sequence per view, queued Pass C)."""
+    and 2 Pass C: 8 orbiting frames of all views captured in one HIP graph (shading as one
+    persistent multi-view launch; Passes A and C either per view on the queued ray schedule or as
+    persistent multi-view launches, both timed), against the same frames rendered view by view
+    (eager, one launch sequence per view, queued Pass C)."""
     from restir_amd import scenes
     from restir_amd.pipeline import BatchRenderer, FrameSequenceGraph, Renderer
 
@@ -526,22 +527,28 @@ def batched_views(torch, args):
     frames = 8
     seq = [[scenes.orbit_camera(sc.camera, f) for sc, _, _ in views] for f in range(frames)]
     kw = dict(mode="unbiased", k=args.k, M=args.M, spatial_passes=args.spatial_passes)
-    b = BatchRenderer(views, **kw)
-    b.render_frame(seq[0])
-    torch.cuda.synchronize()
-    b.reset()
-    fg = FrameSequenceGraph(b, seq)
-    fg.replay()  # warm
-    torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    reps = 3
-    e0.record()
-    for _ in range(reps):
-        fg.replay()
-    e1.record()
-    torch.cuda.synchronize()
-    graph_ms = e0.elapsed_time(e1) / (reps * frames)
-    del fg, b
+
+    def graph_ms_per_frame(rays):
+        b = BatchRenderer(views, rays=rays, **kw)
+        b.render_frame(seq[0])
+        torch.cuda.synchronize()
+        b.reset()
+        fg = FrameSequenceGraph(b, seq)
+        fg.replay()  # warm
+        torch.cuda.synchronize()
+        reps = 3
+        e0.record()
+        for _ in range(reps):
+            fg.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / (reps * frames)
+        del fg, b
+        return ms
+
+    graph_by_rays = {r: graph_ms_per_frame(r) for r in ("queued", "persistent")}
+    graph_ms = min(graph_by_rays.values())
     rs = [Renderer(sc, w, h, **kw) for sc, w, h in views]
     for r, cam in zip(rs, seq[0]):
         r.render_frame(cam)
@@ -555,13 +562,15 @@ def batched_views(torch, args):
     eager_ms = e0.elapsed_time(e1) / frames
     del rs
     torch.cuda.empty_cache()
-    log(f"batched views: 16 views, {px / 1e6:.1f} Mpx/frame: graph of 8 frames {graph_ms:.2f} ms/frame, "
-        f"view-by-view {eager_ms:.2f} ms/frame")
+    log(f"batched views: 16 views, {px / 1e6:.1f} Mpx/frame: graph of 8 frames {graph_ms:.2f} ms/frame "
+        f"(rays queued per view {graph_by_rays['queued']:.2f}, persistent multi-view "
+        f"{graph_by_rays['persistent']:.2f}), view-by-view {eager_ms:.2f} ms/frame")
     return {"views": 16, "scenes": [int(sc.n_tris) for sc in scs],
             "lights": [int(len(sc.point_lights)) for sc in scs],
             "triangle_lights": [int(len(sc.tri_lights)) for sc in scs],
             "resolutions": [f"{w}x{h}" for _, w, h in views], "pixels_per_frame": px,
             "frames_per_graph": frames, "graph_ms_per_frame": round(graph_ms, 3),
+            "graph_ms_per_frame_by_ray_schedule": {k: round(v, 3) for k, v in graph_by_rays.items()},
             "view_by_view_ms_per_frame": round(eager_ms, 3),
             "graph_mpx_per_s": round(px / (graph_ms * 1e-3) / 1e6, 1),
             "speedup": round(eager_ms / graph_ms, 3)}

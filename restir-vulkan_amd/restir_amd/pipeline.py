@@ -226,16 +226,19 @@ class Renderer:
 
 
 class BatchRenderer:
-    """Frames of several views per step, each pass of all views in one persistent launch
-    (restir_*_views). ``views`` is a list of (scene, width, height); views of the same Scene
-    object share its device tables (one context per distinct scene). Every view has its own
-    G-buffers, reservoirs and image; ``rgb`` holds all images back to back and ``images[v]``
-    is view v's [3 * W * H] slice. Pass C uses the inline schedule (shadow rays traced per
-    pixel), so each view's output is bit-identical to a Renderer of that view alone."""
+    """Frames of several views per step. ``views`` is a list of (scene, width, height); views of
+    the same Scene object share its device tables (one context per distinct scene). Every view
+    has its own G-buffers, reservoirs and image; ``rgb`` holds all images back to back and
+    ``images[v]`` is view v's [3 * W * H] slice. Pass B and shading of all views run as one
+    persistent launch each (restir_*_views). The passes that trace shadow rays, A and C, run
+    either the same way (``rays="persistent"``, the default: one launch per pass over all views,
+    rays traced per pixel) or view by view on the queued ray schedule of the view's scene context
+    (``rays="queued"``: each view's rays compacted and traced by the persistent tracer). Both give
+    each view's output bit-identical to a Renderer of that view alone."""
 
     def __init__(self, views, *, device_index=0, mode="unbiased", M=32, k=5, spatial_passes=1,
                  spatial_neighbors=4, radius=30.0, temporal=True, visibility=True, temporal_mult=20,
-                 launch=None):
+                 launch=None, rays="persistent"):
         torch = _torch()
         self.torch = torch
         self.device = torch.device("cuda", device_index)
@@ -246,6 +249,11 @@ class BatchRenderer:
         self.flags = (abi.VISIBILITY_REUSE_FLAG if visibility else 0) | \
                      (abi.TEMPORAL_REUSE_FLAG if temporal else 0)
         self.launch = launch
+        if rays not in ("queued", "persistent"):
+            raise ValueError(f"rays must be 'queued' or 'persistent', not {rays!r}")
+        self.rays = rays
+        tx, ty = (launch.tile_x, launch.tile_y) if launch is not None else (0, 0)
+        self.launch_q = abi.Launch(tx, ty, abi.SCHEDULE_QUEUED)
         self.scene_dev = {}
         self.views = []
         total = sum(3 * w * h for _, w, h in views)
@@ -311,16 +319,24 @@ class BatchRenderer:
             first = [v["tmp"] if n_sp % 2 == 1 else v["res"][i] for v in self.views]
         else:
             first = [v["res"][i] for v in self.views]
-        va = [view(v, u, self._g(v, i), self._g(v, p), v["res"][p], f)
-              for v, u, f in zip(self.views, us, first)]
-        self.ctx.initial_temporal_views(va, self.launch, stream)
+        if self.rays == "persistent":
+            va = [view(v, u, self._g(v, i), self._g(v, p), v["res"][p], f)
+                  for v, u, f in zip(self.views, us, first)]
+            self.ctx.initial_temporal_views(va, self.launch, stream)
+        else:
+            for v, u, f in zip(self.views, us, first):
+                v["ctx"].initial_temporal(u, self._g(v, i), self._g(v, p), v["res"][p], f, self.launch_q, stream)
         if self.mode == "unbiased":
             src = first
             for j in range(n_sp):
                 dst = [v["res"][i] if (n_sp - j) % 2 == 1 else v["tmp"] for v in self.views]
-                vc = [view(v, u, self._g(v, i), self._g(v, i), a, b)
-                      for v, u, a, b in zip(self.views, us, src, dst)]
-                self.ctx.unbiased_views(vc, self.k, j, self.launch, stream)
+                if self.rays == "persistent":
+                    vc = [view(v, u, self._g(v, i), self._g(v, i), a, b)
+                          for v, u, a, b in zip(self.views, us, src, dst)]
+                    self.ctx.unbiased_views(vc, self.k, j, self.launch, stream)
+                else:
+                    for v, u, a, b in zip(self.views, us, src, dst):
+                        v["ctx"].unbiased(u, self.k, j, self._g(v, i), a, b, self.launch_q, stream)
                 src = dst
         else:
             for j in range(self.spatial_passes):

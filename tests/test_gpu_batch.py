@@ -96,11 +96,12 @@ def batch_views():
     return [(a, 64, 48), (b, 80, 45), (a, 33, 70)]
 
 
-@pytest.mark.parametrize("mode", ["unbiased", "biased"])
-def test_batched_views_match_single_view_renderers(torch, mode):
+@pytest.mark.parametrize("mode,rays", [("unbiased", "queued"), ("unbiased", "persistent"),
+                                       ("biased", "queued"), ("biased", "persistent")])
+def test_batched_views_match_single_view_renderers(torch, mode, rays):
     views = batch_views()
     kw = dict(mode=mode, k=3, M=8, spatial_passes=2)
-    batch = BatchRenderer(views, **kw)
+    batch = BatchRenderer(views, rays=rays, **kw)
     singles = [Renderer(sc, w, h, launch=abi.Launch(16, 16, abi.SCHEDULE_INLINE), **kw) for sc, w, h in views]
     for f in range(3):
         cams = [scenes.orbit_camera(sc.camera, f) for sc, _, _ in views]
@@ -113,9 +114,10 @@ def test_batched_views_match_single_view_renderers(torch, mode):
             assert torch.equal(bits(batch.views[v]["res"][batch.cur]), bits(r.res[r.cur])), (f, v)
 
 
-def test_batched_views_in_a_graph_of_8_frames(torch):
+@pytest.mark.parametrize("rays", ["queued", "persistent"])
+def test_batched_views_in_a_graph_of_8_frames(torch, rays):
     views = batch_views()
-    kw = dict(mode="unbiased", k=3, M=8, spatial_passes=2)
+    kw = dict(mode="unbiased", k=3, M=8, spatial_passes=2, rays=rays)
     seq = [[scenes.orbit_camera(sc.camera, f) for sc, _, _ in views] for f in range(8)]
     eager = BatchRenderer(views, **kw)
     imgs = []

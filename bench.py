@@ -41,7 +41,11 @@ for p in (ROOT, ROOT / "restir-vulkan_amd"):
 METRIC = "ms/frame @1080p to equal RMSE vs FP64 ref: learned vs plain RIS; Mrays/s"
 HBM_PEAK_GBS = 8000.0
 VALU_LANE_OPS = 39.3e12  # fp32 lane-ops/s (non-packed), the traversal's ALU ceiling
-NODE_BYTES, TRI_BYTES = 80, 48
+# Traversal work per ray: V visits of 128-byte 4-entry nodes and T triangle tests of 48 bytes
+# each (V and T counted by the tracer on the same rays); ALU per visit: 4 box tests of
+# 16 lane-ops (6 FMA + 6 min/max + 4 compares) + 24 of walk control; 45 per triangle test.
+NODE_BYTES, TRI_BYTES = 128, 48
+VISIT_OPS, TEST_OPS = 4 * 16 + 24, 45
 BYTES_PER_PX_SHADE = 124
 # compulsory bytes per pixel of the resampling passes (each buffer byte read or written once):
 # Pass A = current G 44 + previous G 36 + previous reservoir 64 + output 64; Pass C (without its
@@ -265,10 +269,11 @@ def main():
     # standalone traversal throughput on the last Pass C's neighbour rays (left in q1/q2)
     nr = ray_sets["neighbour"][-1]["rays"]
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ctx.trace_shadow(q1, q2, nr, sh, stream=stream)
+    # one ray per lane in queue order: the tracer shape of the passes' ray queues
+    ctx.trace_shadow(q1, q2, nr, sh, one_ray_per_lane=True, stream=stream)
     e0.record(stream)
     for _ in range(args.trace_reps):
-        ctx.trace_shadow(q1, q2, nr, sh, stream=stream)
+        ctx.trace_shadow(q1, q2, nr, sh, one_ray_per_lane=True, stream=stream)
     e1.record(stream)
     torch.cuda.synchronize()
     trace_ms = e0.elapsed_time(e1) / args.trace_reps
@@ -276,7 +281,7 @@ def main():
     cpu_rays = (q1[:nr].cpu().numpy(), q2[:nr].cpu().numpy()) if not args.no_cpu_baseline else None
     rays_per_s = nr / (trace_ms * 1e-3)
     roof_rays = min(HBM_PEAK_GBS * 1e9 / (NODE_BYTES * Vn + TRI_BYTES * Tn),
-                    VALU_LANE_OPS / (56 * Vn + 45 * Tn))
+                    VALU_LANE_OPS / (VISIT_OPS * Vn + TEST_OPS * Tn))
     log(f"traversal: {nr} neighbour rays, V={Vn:.1f} T={Tn:.2f}: {trace_ms:.3f} ms, "
         f"{rays_per_s / 1e6:.0f} Mrays/s, roofline {roof_rays / 1e6:.0f} Mrays/s")
 
@@ -334,6 +339,8 @@ def main():
                   "kernels": rounded(kernels), "resampling_passes": resampling},
         "mrays_per_s": round(rays_per_s / 1e6, 1),
         "traversal": {"rays": nr, "ray_set": "last frame's second Pass C neighbour rays",
+                      "walk": "4-entry nodes (two tree levels per visit), one ray per lane in queue order",
+                      "node_bytes": NODE_BYTES, "ops_per_visit": VISIT_OPS, "ops_per_test": TEST_OPS,
                       "node_visits_per_ray": round(Vn, 2), "tri_tests_per_ray": round(Tn, 2),
                       "ms": round(trace_ms, 4), "roofline_mrays_per_s": round(roof_rays / 1e6, 1),
                       "frac": round(rays_per_s / roof_rays, 4), "pass_c_rays": ray_sets},

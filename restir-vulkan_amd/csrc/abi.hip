@@ -269,14 +269,7 @@ int restir_set_bvh(restir_ctx *ctx, const restir_aabb_tree_node *nodes, uint32_t
                     reached, nNodes, leaves, nTris);
     if (depth > 64)
         return fail(ctx, RESTIR_E_BAD_TREE, "set_bvh: tree depth %u exceeds the 64-entry traversal stack", depth);
-    ctx->scene.nodes = reinterpret_cast<const rt::Node *>(nodes);
-    ctx->scene.tris = reinterpret_cast<const float4 *>(tris);
-    // a walk defers at most one sibling per internal level below the root: depth - 1 entries
-    ctx->stack = depth <= 24 ? 24 : (depth <= 32 ? 32 : 64);
-    ctx->persistentBlocks = persistent_blocks(ctx->cus, ctx->stack);
-    ctx->treeDepth = depth;
-    ctx->bvhSet = true;
-    return RESTIR_OK;
+    return bind_tree(ctx, nodes, nNodes, tris, depth, nullptr);
 }
 
 int restir_initial_temporal(restir_ctx *ctx, const restir_uniforms *u, const restir_gbuffer *cur,

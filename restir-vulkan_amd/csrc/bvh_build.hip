@@ -583,6 +583,98 @@ Layout layout(uint32_t n) {
 
 }  // namespace
 
+// ------------------------------------------------------------ 4-entry nodes of a bound tree --
+namespace {
+
+__device__ inline bool finite_box(float4 mn, float4 mx) {
+    constexpr float kInf = __builtin_huge_valf();
+    return fabsf(mn.x) < kInf && fabsf(mn.y) < kInf && fabsf(mn.z) < kInf && fabsf(mx.x) < kInf &&
+           fabsf(mx.y) < kInf && fabsf(mx.z) < kInf && mn.x <= mx.x && mn.y <= mx.y && mn.z <= mx.z;
+}
+__device__ inline bool inside(float4 mn, float4 mx, float4 outerMn, float4 outerMx) {
+    return outerMn.x <= mn.x && outerMn.y <= mn.y && outerMn.z <= mn.z && mx.x <= outerMx.x &&
+           mx.y <= outerMx.y && mx.z <= outerMx.z;
+}
+
+// One thread per reference node x: its WNode (see rt_traverse.h), and the check that both boxes
+// stored in x are finite and ordered and that each internal child's two boxes lie inside the box
+// x stores for it (*loose != 0 otherwise). Every box is stored in exactly one node, so the
+// threads together check every box and every parent-child pair.
+__global__ __launch_bounds__(256) void k_wide_build(const rt::Node *__restrict__ nodes, uint32_t nNodes,
+                                                    rt::WNode *__restrict__ wide, uint32_t *__restrict__ loose) {
+    const uint32_t x = blockIdx.x * 256u + threadIdx.x;
+    if (x >= nNodes) return;
+    const rt::Node b = rt::load_node(nodes, static_cast<int32_t>(x));
+    bool ok = finite_box(b.lmin, b.lmax) && finite_box(b.rmin, b.rmax);
+    float4 mn[4], mx[4];
+    int32_t id[4], grp[2];
+    for (int side = 0; side < 2; ++side) {
+        const int32_t c = side ? b.right : b.left;
+        const float4 smn = side ? b.rmin : b.lmin, smx = side ? b.rmax : b.lmax;
+        if (c < 0) {
+            mn[2 * side] = smn;
+            mx[2 * side] = smx;
+            id[2 * side] = c;
+            mn[2 * side + 1] = mx[2 * side + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            id[2 * side + 1] = rt::kNoEntry;
+            grp[side] = -1;
+        } else {
+            const rt::Node cn = rt::load_node(nodes, c);
+            mn[2 * side] = cn.lmin;
+            mx[2 * side] = cn.lmax;
+            id[2 * side] = cn.left;
+            mn[2 * side + 1] = cn.rmin;
+            mx[2 * side + 1] = cn.rmax;
+            id[2 * side + 1] = cn.right;
+            grp[side] = c;
+            ok = ok && inside(cn.lmin, cn.lmax, smn, smx) && inside(cn.rmin, cn.rmax, smn, smx);
+        }
+    }
+    float4 *o = reinterpret_cast<float4 *>(wide + x);
+    o[0] = make_float4(mn[0].x, mn[1].x, mn[2].x, mn[3].x);
+    o[1] = make_float4(mn[0].y, mn[1].y, mn[2].y, mn[3].y);
+    o[2] = make_float4(mn[0].z, mn[1].z, mn[2].z, mn[3].z);
+    o[3] = make_float4(mx[0].x, mx[1].x, mx[2].x, mx[3].x);
+    o[4] = make_float4(mx[0].y, mx[1].y, mx[2].y, mx[3].y);
+    o[5] = make_float4(mx[0].z, mx[1].z, mx[2].z, mx[3].z);
+    reinterpret_cast<int4 *>(o)[6] = make_int4(id[0], id[1], id[2], id[3]);
+    reinterpret_cast<int4 *>(o)[7] = make_int4(grp[0], grp[1], 0, 0);
+    if (!ok) atomicOr(loose, 1u);
+}
+
+}  // namespace
+
+namespace rki {
+// Binds a validated tree (nNodes = nTris - 1, every index in range, depth <= 64) to the context
+// and builds its 4-entry nodes. Waits for the device first: work still in flight may read the
+// previous tree's 4-entry nodes, which share the context buffer.
+int bind_tree(restir_ctx *ctx, const void *nodes, uint32_t nNodes, const void *tris, uint32_t depth, hipStream_t s) {
+    auto hip = [&](hipError_t e, const char *what) { return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, what); };
+    if (int r = hip(hipDeviceSynchronize(), "bind tree: wait for work in flight")) return r;
+    const size_t wideBytes = sizeof(rt::WNode) * static_cast<size_t>(nNodes);
+    if (int r = ensure(ctx, &ctx->wide, &ctx->wideBytes, wideBytes + 256)) return r;
+    rt::WNode *wide = static_cast<rt::WNode *>(ctx->wide);
+    uint32_t *loose = reinterpret_cast<uint32_t *>(static_cast<char *>(ctx->wide) + wideBytes);
+    if (int r = hip(hipMemsetAsync(loose, 0, sizeof(uint32_t), s), "bind tree: clear")) return r;
+    RK_LAUNCH("k_wide_build", k_wide_build, dim3(groups(nNodes)), dim3(256), 0, s,
+              static_cast<const rt::Node *>(nodes), nNodes, wide, loose);
+    uint32_t hostLoose = 1;
+    if (int r = hip(hipMemcpyAsync(&hostLoose, loose, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "bind tree: read"))
+        return r;
+    if (int r = hip(hipStreamSynchronize(s), "bind tree: finish")) return r;
+    ctx->scene.nodes = static_cast<const rt::Node *>(nodes);
+    ctx->scene.tris = static_cast<const float4 *>(tris);
+    ctx->scene.wide = wide;
+    ctx->scene.treeNested = hostLoose == 0 ? 1 : 0;
+    // a walk defers at most one sibling per internal level below the root: depth - 1 entries
+    ctx->stack = depth <= 24 ? 24 : (depth <= 32 ? 32 : 64);
+    ctx->persistentBlocks = persistent_blocks(ctx->cus, ctx->stack);
+    ctx->treeDepth = depth;
+    ctx->bvhSet = true;
+    return RESTIR_OK;
+}
+}  // namespace rki
+
 extern "C" int restir_build_bvh(restir_ctx *ctx, const float *triXyz, uint32_t nTris, restir_aabb_tree_node *nodes,
                                 restir_triangle *tris, uint32_t *depthOut, void *stream) {
     if (!ctx) return RESTIR_E_INVALID_ARGUMENT;
@@ -679,11 +771,5 @@ extern "C" int restir_build_bvh(restir_ctx *ctx, const float *triXyz, uint32_t n
     if (depth > 64)
         return fail(ctx, RESTIR_E_BAD_TREE, "build_bvh: tree depth %u exceeds the 64-entry traversal stack", depth);
     if (depthOut) *depthOut = depth;
-    ctx->scene.nodes = reinterpret_cast<const rt::Node *>(nodes);
-    ctx->scene.tris = reinterpret_cast<const float4 *>(tris);
-    ctx->stack = depth <= 24 ? 24 : (depth <= 32 ? 32 : 64);
-    ctx->persistentBlocks = persistent_blocks(ctx->cus, ctx->stack);
-    ctx->treeDepth = depth;
-    ctx->bvhSet = true;
-    return RESTIR_OK;
+    return bind_tree(ctx, nodes, n - 1, tris, depth, s);
 }

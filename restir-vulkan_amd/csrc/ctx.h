@@ -51,6 +51,8 @@ struct restir_ctx {
     void *bvhInfoHost = nullptr;   // pinned: each tree level's node and split counts
     void *aliasScratch = nullptr;  // powers, weights and queues of the device alias-table build
     size_t aliasScratchBytes = 0;
+    void *wide = nullptr;          // the bound tree's 4-entry nodes (+ the nesting flag)
+    size_t wideBytes = 0;
     // light clusters of the learned proposal (device pointers, caller-owned)
     bool clustersSet = false;
     int32_t nClusters = 0;
@@ -67,10 +69,11 @@ struct restir_ctx {
         if (bvhScratch) (void)hipFree(bvhScratch);
         if (bvhInfoHost) (void)hipHostFree(bvhInfoHost);
         if (aliasScratch) (void)hipFree(aliasScratch);
+        if (wide) (void)hipFree(wide);
     }
 };
 
-// helpers shared by abi.hip and abi_mlp.hip (defined in abi.hip)
+// helpers shared by the ABI files (defined in abi.hip; bind_tree in bvh_build.hip)
 namespace rki {
 int fail(restir_ctx *ctx, int code, const char *fmt, ...);
 int hip_fail(restir_ctx *ctx, hipError_t e, const char *what);
@@ -85,4 +88,5 @@ int pass_a_queues(restir_ctx *ctx, const restir_launch *launch, uint32_t npx, vo
 int ensure(restir_ctx *ctx, void **buf, size_t *have, size_t bytes);
 uint32_t resident_blocks(uint32_t cus, uint32_t ldsBytes);
 uint32_t persistent_blocks(uint32_t cus, int stack);
+int bind_tree(restir_ctx *ctx, const void *nodes, uint32_t nNodes, const void *tris, uint32_t depth, hipStream_t s);
 }  // namespace rki

@@ -34,7 +34,9 @@ struct SceneView {
     int32_t mixed;    // 1: the alias table covers point lights then triangle lights (the union)
     int32_t nLights;  // lights the alias table indexes: nAlias
     const rt::Node *nodes;
-    const float4 *tris;  // 3 float4 per triangle
+    const float4 *tris;     // 3 float4 per triangle
+    const rt::WNode *wide;  // the 4-entry nodes over `nodes` (same indices), built by the library
+    int32_t treeNested;     // 1: every box finite and inside the box stored for its parent
 };
 
 struct GView {

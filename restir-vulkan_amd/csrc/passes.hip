@@ -278,7 +278,7 @@ __device__ __forceinline__ void pass_a_px(const PassAArgs &a, uint32_t x, uint32
         f3 o, d;
         shadow_segment(pos, res.pos, o, d);
         TraceStats st{0, 0};
-        if (occluded<STACK, false>(sc.nodes, sc.tris, o, d, stk, kBlock, st)) {
+        if (occluded_w<STACK>(sc.wide, sc.nodes, sc.tris, sc.treeNested != 0, o, d, stk, kBlock, st)) {
             res.w = 0.0f;
             res.sumW = 0.0f;
         }
@@ -440,7 +440,7 @@ __device__ __forceinline__ void pass_c_px(const PassCArgs &a, uint32_t x, uint32
         f3 o, d;
         shadow_segment(pos, lp, o, d);
         TraceStats st{0, 0};
-        centreShadowed = occluded<STACK, false>(a.sc.nodes, a.sc.tris, o, d, stk, kBlock, st);
+        centreShadowed = occluded_w<STACK>(a.sc.wide, a.sc.nodes, a.sc.tris, a.sc.treeNested != 0, o, d, stk, kBlock, st);
     }
     if (centreShadowed) {
         numSamples = 0;
@@ -454,7 +454,7 @@ __device__ __forceinline__ void pass_c_px(const PassCArgs &a, uint32_t x, uint32
                 f3 o, d;
                 shadow_segment(nwp, lp, o, d);
                 TraceStats st{0, 0};
-                if (occluded<STACK, false>(a.sc.nodes, a.sc.tris, o, d, stk, kBlock, st)) continue;
+                if (occluded_w<STACK>(a.sc.wide, a.sc.nodes, a.sc.tris, a.sc.treeNested != 0, o, d, stk, kBlock, st)) continue;
             }
             numSamples += nM[j];
         }
@@ -652,57 +652,27 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_finish(const PassCArgs a, con
     *r = v;
 }
 
-// Persistent tracer over a queue whose length a previous kernel left in *count. Results go to
-// the pixel named by the ray's tag: a centre ray sets kBitCentreShadowed in bits[pixel]; a
-// neighbour ray sets bit `slot` of occ[pixel].
+// Tracer over a queue whose length a previous kernel left in *count: one ray per lane, in queue
+// order, over a grid sized for the queue's capacity (lanes past the count return at once). Queue
+// order keeps the rays of neighbouring pixels in one wave, which walk much of the same tree; a
+// persistent pool that refills idle lanes mixes them and was slower. Results go to the pixel
+// named by the ray's tag: a centre ray sets kBitCentreShadowed in bits[pixel]; a neighbour ray
+// sets bit `slot` of occ[pixel].
 template <int STACK, bool CENTRE>
 __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const float4 *__restrict__ p1,
                                                      const float4 *__restrict__ p2,
-                                                     const uint32_t *__restrict__ count,
-                                                     uint32_t *__restrict__ head, CQueues q) {
+                                                     const uint32_t *__restrict__ count, CQueues q) {
     __shared__ int32_t stack[STACK * 256];
-    int32_t *stk = stack + threadIdx.x;
-    const uint32_t n = *count;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
-    Walk w;
-    int32_t ray = -1;
-    uint32_t tag = 0;
-    bool drained = false;
-    for (;;) {
-        const uint64_t idle = __ballot(ray < 0);
-        const uint32_t nIdle = static_cast<uint32_t>(__popcll(idle));
-        if (!drained && nIdle >= 32u) {
-            const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(idle)) - 1);
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(head, nIdle);
-            base = __shfl(base, static_cast<int>(leader));
-            drained = base + nIdle >= n;
-            if (ray < 0) {
-                const uint32_t i = base + static_cast<uint32_t>(__popcll(idle & below));
-                if (i < n) {
-                    const float4 a4 = p1[i];
-                    f3 o, d;
-                    shadow_segment(xyz(a4), xyz(p2[i]), o, d);
-                    walk_begin(w, o, d);
-                    tag = __float_as_uint(a4.w);
-                    ray = static_cast<int32_t>(i);
-                }
-            }
-        }
-        if (__ballot(ray >= 0) == 0) {
-            if (drained) break;
-            continue;
-        }
-        bool fin = false;
-        if (ray >= 0) fin = walk_step<STACK, false>(w, sc.nodes, sc.tris, stk, 256);
-        if (fin) {
-            if (w.hit) {
-                if (CENTRE) q.bits[tag] |= kBitCentreShadowed;
-                else atomicOr(&q.occ[tag & 0x07ffffffu], 1u << (tag >> 27));
-            }
-            ray = -1;
-        }
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= *count) return;
+    const float4 a4 = p1[i];
+    f3 o, d;
+    shadow_segment(xyz(a4), xyz(p2[i]), o, d);
+    TraceStats st{0, 0};
+    if (occluded_w<STACK>(sc.wide, sc.nodes, sc.tris, sc.treeNested != 0, o, d, stack + threadIdx.x, 256, st)) {
+        const uint32_t tag = __float_as_uint(a4.w);
+        if (CENTRE) q.bits[tag] |= kBitCentreShadowed;
+        else atomicOr(&q.occ[tag & 0x07ffffffu], 1u << (tag >> 27));
     }
 }
 
@@ -712,7 +682,7 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
 // scratch, and the ray queued; (2) the persistent tracer marks shadowed pixels; (3) per pixel:
 // the shadow verdict, temporal reuse, store. Same draws, same order, same result as k_pass_a.
 struct AQueues {
-    uint32_t *counter;  // [0] rays queued, [1] tracer head
+    uint32_t *counter;  // [0] rays queued
     float4 *p1, *p2;    // p1.w carries the pixel index
     uint64_t *rng;      // per pixel: the random stream's state after RIS
     uint32_t *occ;      // per pixel: kBitCentreShadowed once its ray hit
@@ -827,7 +797,10 @@ __global__ __launch_bounds__(256) void k_trace(const TraceArgs a) {
     f3 o, d;
     shadow_segment(xyz(a.p1[i]), xyz(a.p2[i]), o, d);
     TraceStats st{0, 0};
-    const bool occ = occluded<STACK, REF>(a.sc.nodes, a.sc.tris, o, d, stack + threadIdx.x, 256, st);
+    bool occ;
+    if constexpr (REF) occ = occluded<STACK, true>(a.sc.nodes, a.sc.tris, o, d, stack + threadIdx.x, 256, st);
+    else occ = occluded_w<STACK>(a.sc.wide, a.sc.nodes, a.sc.tris, a.sc.treeNested != 0, o, d, stack + threadIdx.x,
+                                 256, st);
     a.shadowed[i] = occ ? 1u : 0u;
     if (a.visits) a.visits[i] = st.visits;
     if (a.tests) a.tests[i] = st.tests;
@@ -843,7 +816,7 @@ __global__ __launch_bounds__(256) void k_trace_pool(const TraceArgs a, uint32_t 
     int32_t *stk = stack + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
-    Walk w;
+    std::conditional_t<REF, Walk, WWalk> w;  // the reference loop walks the binary nodes
     int32_t ray = -1;
     bool drained = false;  // wave-uniform: the counter has passed the last ray
     for (;;) {
@@ -860,7 +833,8 @@ __global__ __launch_bounds__(256) void k_trace_pool(const TraceArgs a, uint32_t 
                 if (i < a.n) {
                     f3 o, d;
                     shadow_segment(xyz(a.p1[i]), xyz(a.p2[i]), o, d);
-                    walk_begin(w, o, d);
+                    if constexpr (REF) walk_begin(w, o, d);
+                    else wwalk_begin(w, o, d, a.sc.treeNested != 0);
                     ray = static_cast<int32_t>(i);
                 }
             }
@@ -869,7 +843,12 @@ __global__ __launch_bounds__(256) void k_trace_pool(const TraceArgs a, uint32_t 
             if (drained) break;
             continue;
         }
-        if (ray >= 0 && walk_step<STACK, REF>(w, a.sc.nodes, a.sc.tris, stk, 256)) {
+        bool fin = false;
+        if (ray >= 0) {
+            if constexpr (REF) fin = walk_step<STACK, true>(w, a.sc.nodes, a.sc.tris, stk, 256);
+            else fin = walk_step_w<STACK>(w, a.sc.wide, a.sc.nodes, a.sc.tris, stk, 256);
+        }
+        if (fin) {
             a.shadowed[ray] = w.hit ? 1u : 0u;
             if (a.visits) a.visits[ray] = w.nv;
             if (a.tests) a.tests[ray] = w.nt;
@@ -1077,13 +1056,14 @@ template <int TX, int TY, int STACK>
 static hipError_t run_a_queued(PassAArgs args, const AQueues &q, uint32_t gridCap, hipStream_t s) {
     grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
     const dim3 tiles(args.nTiles), block(TX * TY);
-    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 2u);
+    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 1u);
     RK_LAUNCH("k_pass_a_ris", (k_pass_a_ris<TX, TY>), tiles, block, 0, s, args, q);
     if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
         CQueues cq{};
         cq.bits = q.occ;  // a shadowed ray sets kBitCentreShadowed in occ[pixel]
-        RK_LAUNCH("k_trace_queue(pass_a)", (k_trace_queue<STACK, true>), dim3(gridCap), dim3(256), 0, s,
-                  args.sc, q.p1, q.p2, q.counter + 0, q.counter + 1, cq);
+        const uint32_t npx = args.u.screenSize[0] * args.u.screenSize[1];
+        RK_LAUNCH("k_trace_queue(pass_a)", (k_trace_queue<STACK, true>), dim3((npx + 255) / 256), dim3(256), 0,
+                  s, args.sc, q.p1, q.p2, q.counter + 0, cq);
     }
     RK_LAUNCH("k_pass_a_finish", (k_pass_a_finish<TX, TY>), tiles, block, 0, s, args, q);
     return hipGetLastError();
@@ -1153,17 +1133,19 @@ template <int TX, int TY, int STACK>
 static hipError_t run_c_queued(PassCArgs args, const CQueues &q, uint32_t gridCap, hipStream_t s) {
     grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
     const dim3 tiles(args.nTiles), block(TX * TY);
-    const uint32_t rayBlocks = gridCap;
-    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 4u);
+    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 2u);
     const uint32_t npx = args.u.screenSize[0] * args.u.screenSize[1];
+    // queue capacities: one centre ray per pixel, k neighbour rays per pixel
+    const uint32_t centreBlocks = (npx + 255) / 256;
+    const uint32_t neighbourBlocks = static_cast<uint32_t>((static_cast<uint64_t>(npx) * args.k + 255) / 256);
     RK_LAUNCH("k_pack_surface", k_pack_surface, dim3((npx + 255) / 256), dim3(256), 0, s, args.g, npx, q.surf);
     RK_LAUNCH("k_pass_c_merge", (k_pass_c_merge<TX, TY>), tiles, block, 0, s, args, q);
     if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
-        RK_LAUNCH("k_trace_queue(centre)", (k_trace_queue<STACK, true>), dim3(rayBlocks), dim3(256), 0, s,
-                  args.sc, q.c1, q.c2, q.counter + 0, q.counter + 2, q);
+        RK_LAUNCH("k_trace_queue(centre)", (k_trace_queue<STACK, true>), dim3(centreBlocks), dim3(256), 0, s,
+                  args.sc, q.c1, q.c2, q.counter + 0, q);
         RK_LAUNCH("k_pass_c_emit", (k_pass_c_emit<TX, TY>), tiles, block, 0, s, args, q);
-        RK_LAUNCH("k_trace_queue(neighbour)", (k_trace_queue<STACK, false>), dim3(rayBlocks), dim3(256), 0,
-                  s, args.sc, q.n1, q.n2, q.counter + 1, q.counter + 3, q);
+        RK_LAUNCH("k_trace_queue(neighbour)", (k_trace_queue<STACK, false>), dim3(neighbourBlocks), dim3(256),
+                  0, s, args.sc, q.n1, q.n2, q.counter + 1, q);
         RK_LAUNCH("k_pass_c_finish", (k_pass_c_finish<TX, TY>), tiles, block, 0, s, args, q);
     }
     return hipGetLastError();

@@ -50,11 +50,14 @@ __device__ __forceinline__ Seg make_seg(f3 o, f3 d) {
 // absolute 1e-30 for results near zero), the reference's answer is known; otherwise — and for
 // any non-finite or huge distance — the division form is evaluated. The slack is four times the
 // bound, so the fallback is taken only for segments grazing a box face within ~1e-6.
+// The decision is used only when all six distances are finite, hence not NaN; there fminf/fmaxf
+// (one v_min/v_max or v_min3/v_max3 each) select the same values as the reference's a<b?a:b
+// forms, up to the sign of a zero, which no comparison below can see (emin/emax take |.|).
 __device__ __forceinline__ bool ray_box(const Seg &s, float4 mn, float4 mx) {
     const float ax = (mn.x - s.o.x) * s.inv.x, ay = (mn.y - s.o.y) * s.inv.y, az = (mn.z - s.o.z) * s.inv.z;
     const float bx = (mx.x - s.o.x) * s.inv.x, by = (mx.y - s.o.y) * s.inv.y, bz = (mx.z - s.o.z) * s.inv.z;
-    const float rmin = gmax(gmin(ax, bx), gmax(gmin(ay, by), gmin(az, bz)));
-    const float rmax = gmin(gmax(ax, bx), gmin(gmax(ay, by), gmax(az, bz)));
+    const float rmin = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+    const float rmax = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
     // a comparison is false for NaN, so this also rejects NaN and infinities
     const bool finite = fabsf(ax) < 1e30f && fabsf(bx) < 1e30f && fabsf(ay) < 1e30f &&
                         fabsf(by) < 1e30f && fabsf(az) < 1e30f && fabsf(bz) < 1e30f;
@@ -244,6 +247,163 @@ __device__ __forceinline__ bool walk_step(Walk &w, const Node *__restrict__ node
     }
     w.cur = next;
     return false;
+}
+
+// ---------------------------------------------------------------- 4-entry walk of the tree --
+// The same tree, two levels per visit. WNode x (one per internal node x of the reference tree,
+// same index) holds, for each side of x, either that child itself when it is a triangle leaf, or
+// the child's two children: up to four entries, each with the box the reference tree stores for
+// it. Both children's boxes sit in their parent, so the entries are copied, not recomputed.
+//
+// Shadow rays only ask whether some triangle on the segment is hit, and the reference tests a
+// triangle exactly when every box on its path from the root passes. A visit tests its entries'
+// boxes but not the boxes of the internal children between them and the visited node. That
+// skips no triangle and adds none when each skipped box passes whenever a box inside it passes:
+// ray_box decides as the division form RN(RN(b - o) / d) does, each slab distance is monotone
+// in the plane b for a fixed ray, and with finite planes and o and a finite nonzero d no distance
+// is NaN, so a box nested in another yields an interval [rmin, rmax] inside the other's and the
+// pass condition carries over. For a ray with a zero or non-finite component of d or o (a slab
+// distance 0/0 is NaN, and the reference's a<b?a:b chains then drop that slab), and for a tree
+// whose boxes are not known to be finite and nested, the skipped boxes are tested as well, from
+// the reference node of the visited node: every box on every path is then tested, as in the
+// binary walk.
+// The answer is the reference's as long as no push is dropped: each visit pushes at most one item
+// and each visit is one or two levels below the one that pushed it, so at most (level of the
+// current node) items wait, no more than the depth - 1 of the binary walk.
+//
+// Per side of a visited node x with internal child c: if both of c's entries are internal and
+// pass, the side yields one item, c itself (its visit tests the entries below both); if one
+// passes, the side yields that entry; leaves that pass are tested at once.
+struct alignas(16) WNode {
+    float4 lox, loy, loz;  // low corner of entry k in component k
+    float4 hix, hiy, hiz;  // high corner
+    int4 id;               // entry: >= 0 internal node, < 0 ~triangle, kNoEntry: none
+    int4 grp;              // .x / .y: the left / right child when it is internal, else -1
+};
+static_assert(sizeof(WNode) == 128, "one 128-byte line per node");
+constexpr int32_t kNoEntry = INT32_MIN;
+
+// A segment for the 4-entry walk: `full` asks for the skipped boxes to be tested too.
+struct WSeg {
+    Seg s;
+    bool full;
+};
+__device__ __forceinline__ WSeg make_wseg(f3 o, f3 d, bool treeNested) {
+    constexpr float kInf = __builtin_huge_valf();
+    // false for NaN as well as for zero or infinite components
+    const bool regular = fabsf(d.x) > 0.0f && fabsf(d.y) > 0.0f && fabsf(d.z) > 0.0f &&
+                         fabsf(d.x) < kInf && fabsf(d.y) < kInf && fabsf(d.z) < kInf &&
+                         fabsf(o.x) < kInf && fabsf(o.y) < kInf && fabsf(o.z) < kInf;
+    return WSeg{make_seg(o, d), !(treeNested && regular)};
+}
+
+__device__ __forceinline__ float4 corner(float4 x, float4 y, float4 z, int k) {
+    return k == 0 ? make_float4(x.x, y.x, z.x, 0.0f)
+         : k == 1 ? make_float4(x.y, y.y, z.y, 0.0f)
+         : k == 2 ? make_float4(x.z, y.z, z.z, 0.0f)
+                  : make_float4(x.w, y.w, z.w, 0.0f);
+}
+
+// One visit of node `cur`. Returns true when a triangle is hit; otherwise *next gets the node to
+// visit next (or -1 when the stack is empty), after pushing the left side's item if both sides
+// yield one.
+template <int kStack>
+__device__ __forceinline__ bool wide_visit(const WSeg &ws, const WNode *__restrict__ wide,
+                                           const Node *__restrict__ nodes, const float4 *__restrict__ tris,
+                                           int32_t cur, int32_t *next, int32_t *stk, int stride, int &top,
+                                           uint32_t &nv, uint32_t &nt) {
+    const Seg &s = ws.s;
+    const float4 *p = reinterpret_cast<const float4 *>(wide + cur);
+    const float4 lx = p[0], ly = p[1], lz = p[2], hx = p[3], hy = p[4], hz = p[5];
+    const int4 id = reinterpret_cast<const int4 *>(p)[6];
+    const int4 grp = reinterpret_cast<const int4 *>(p)[7];
+    ++nv;
+    bool sideL = true, sideR = true;
+    if (ws.full) {
+        const Node nd = load_node(nodes, cur);
+        sideL = ray_box(s, nd.lmin, nd.lmax);
+        sideR = ray_box(s, nd.rmin, nd.rmax);
+    }
+    const bool h0 = sideL && id.x != kNoEntry && ray_box(s, corner(lx, ly, lz, 0), corner(hx, hy, hz, 0));
+    const bool h1 = sideL && id.y != kNoEntry && ray_box(s, corner(lx, ly, lz, 1), corner(hx, hy, hz, 1));
+    const bool h2 = sideR && id.z != kNoEntry && ray_box(s, corner(lx, ly, lz, 2), corner(hx, hy, hz, 2));
+    const bool h3 = sideR && id.w != kNoEntry && ray_box(s, corner(lx, ly, lz, 3), corner(hx, hy, hz, 3));
+    uint32_t leaves = (h0 && id.x < 0 ? 1u : 0u) | (h1 && id.y < 0 ? 2u : 0u) |
+                      (h2 && id.z < 0 ? 4u : 0u) | (h3 && id.w < 0 ? 8u : 0u);
+    while (leaves) {
+        const uint32_t k = static_cast<uint32_t>(__builtin_ctz(leaves));
+        leaves &= leaves - 1u;
+        const int32_t e = k == 0 ? id.x : (k == 1 ? id.y : (k == 2 ? id.z : id.w));
+        ++nt;
+        if (ray_tri(tris, ~e, s.o, s.d)) return true;
+    }
+    const bool i0 = h0 && id.x >= 0, i1 = h1 && id.y >= 0, i2 = h2 && id.z >= 0, i3 = h3 && id.w >= 0;
+    const int32_t itemL = (i0 && i1) ? grp.x : (i0 ? id.x : (i1 ? id.y : -1));
+    const int32_t itemR = (i2 && i3) ? grp.y : (i2 ? id.z : (i3 ? id.w : -1));
+    int32_t nx = itemR;
+    if (itemL >= 0) {
+        if (nx < 0) {
+            nx = itemL;
+        } else if (top < kStack) {
+            stk[top * stride] = itemL;
+            ++top;
+        }
+    }
+    if (nx < 0 && top > 0) {
+        --top;
+        nx = stk[top * stride];
+    }
+    *next = nx;
+    return false;
+}
+
+// `occluded` (production loop) over the 4-entry nodes: same answer, about half the visits.
+template <int kStack>
+__device__ __forceinline__ bool occluded_w(const WNode *__restrict__ wide, const Node *__restrict__ nodes,
+                                           const float4 *__restrict__ tris, bool treeNested, f3 o, f3 d,
+                                           int32_t *stk, int stride, TraceStats &st) {
+    const WSeg s = make_wseg(o, d, treeNested);
+    int top = 0;
+    int32_t cur = 0;
+    uint32_t nv = 0, nt = 0;
+    bool hit = false;
+    while (cur >= 0) {
+        if (wide_visit<kStack>(s, wide, nodes, tris, cur, &cur, stk, stride, top, nv, nt)) {
+            hit = true;
+            break;
+        }
+    }
+    st.visits += nv;
+    st.tests += nt;
+    return hit;
+}
+
+// Step-wise form for the interleaving schedules: one visit (and the tests of the triangle leaves
+// it finds) per step; true when the ray is done, w.hit then has the answer.
+struct WWalk {
+    WSeg s;
+    int32_t cur;  // node to visit next, -1 when the stack is empty
+    int top;
+    bool hit;
+    uint32_t nv, nt;
+};
+__device__ __forceinline__ void wwalk_begin(WWalk &w, f3 o, f3 d, bool treeNested) {
+    w.s = make_wseg(o, d, treeNested);
+    w.cur = 0;
+    w.top = 0;
+    w.hit = false;
+    w.nv = w.nt = 0;
+}
+template <int kStack>
+__device__ __forceinline__ bool walk_step_w(WWalk &w, const WNode *__restrict__ wide, const Node *__restrict__ nodes,
+                                            const float4 *__restrict__ tris, int32_t *stk, int stride) {
+    int32_t next;
+    if (wide_visit<kStack>(w.s, wide, nodes, tris, w.cur, &next, stk, stride, w.top, w.nv, w.nt)) {
+        w.hit = true;
+        return true;
+    }
+    w.cur = next;
+    return next < 0;
 }
 
 // testVisibility, visibilityTest.glsl:1-30: shrink the segment p1 -> p2 by tMin = 0.001 at both

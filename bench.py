@@ -52,7 +52,7 @@ BYTES_PER_PX_SHADE = 124
 # rays) = G 48 + reservoir in 64 + out 64
 BYTES_PER_PX_PASS_A, BYTES_PER_PX_PASS_C = 208, 176
 PASS_A_KERNELS = ("k_pass_a", "k_pass_a_ris", "k_pass_a_finish")
-PASS_C_KERNELS = ("k_pack_surface", "k_pass_c_merge", "k_pass_c_emit", "k_pass_c_finish", "k_pass_c")
+PASS_C_KERNELS = ("k_pack_surface", "k_pass_c_merge", "k_pass_c_finish", "k_pass_c")
 TRUTH_TRI_SAMPLES = 64  # stratified samples per triangle light in the FP64 ground truth
 
 

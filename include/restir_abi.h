@@ -457,11 +457,12 @@ int restir_profile_read(restir_ctx *ctx, restir_kernel_time *out, uint32_t capac
                         uint32_t *count);
 
 /* The shadow rays of the last queued-schedule restir_unbiased call on this context, unless a
- * queued Pass A ran since (the queues share scratch) (which = 0:
- * the centre re-tests, 1: the neighbour tests), as p1/p2 float4 pairs in queue order (p1.w holds a
- * pixel tag, ignored by restir_trace_shadow). Copies min(queue length, capacity) rays on `stream`
- * and stores the queue length in *count (a host pointer; the call synchronizes `stream`). Lets a
- * benchmark count node visits and triangle tests of exactly the rays the traversal kernel saw. */
+ * queued Pass A ran since (the queues share scratch) (which = 0: the centre re-tests, 1: the
+ * neighbour tests that were traced, i.e. those of pixels whose centre ray found the light), as
+ * p1/p2 float4 pairs in queue order (p1.w holds a pixel tag, ignored by restir_trace_shadow).
+ * Copies min(number of rays, capacity) rays on `stream` and stores the number of rays in *count
+ * (a host pointer; the call synchronizes `stream`). Lets a benchmark count node visits and
+ * triangle tests of exactly the rays the traversal kernel traced. */
 int restir_unbiased_rays(restir_ctx *ctx, int which, float *p1, float *p2, uint32_t capacity,
                          uint32_t *count, void *stream);
 

@@ -32,6 +32,7 @@
 #include <cfloat>
 #include <climits>
 #include <cstdint>
+#include <cstring>
 
 #include "ctx.h"
 #include "kernels.h"
@@ -591,6 +592,7 @@ __device__ inline bool finite_box(float4 mn, float4 mx) {
     return fabsf(mn.x) < kInf && fabsf(mn.y) < kInf && fabsf(mn.z) < kInf && fabsf(mx.x) < kInf &&
            fabsf(mx.y) < kInf && fabsf(mx.z) < kInf && mn.x <= mx.x && mn.y <= mx.y && mn.z <= mx.z;
 }
+__device__ inline float box_max(float4 v) { return fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fabsf(v.z)); }
 __device__ inline bool inside(float4 mn, float4 mx, float4 outerMn, float4 outerMx) {
     return outerMn.x <= mn.x && outerMn.y <= mn.y && outerMn.z <= mn.z && mx.x <= outerMx.x &&
            mx.y <= outerMx.y && mx.z <= outerMx.z;
@@ -598,10 +600,12 @@ __device__ inline bool inside(float4 mn, float4 mx, float4 outerMn, float4 outer
 
 // One thread per reference node x: its WNode (see rt_traverse.h), and the check that both boxes
 // stored in x are finite and ordered and that each internal child's two boxes lie inside the box
-// x stores for it (*loose != 0 otherwise). Every box is stored in exactly one node, so the
-// threads together check every box and every parent-child pair.
+// x stores for it (info[0] != 0 otherwise). Every box is stored in exactly one node, so the
+// threads together check every box and every parent-child pair; info[1] gets the bits of the
+// largest |coordinate| of any box (non-negative floats order as their bit patterns; a NaN orders
+// above every number).
 __global__ __launch_bounds__(256) void k_wide_build(const rt::Node *__restrict__ nodes, uint32_t nNodes,
-                                                    rt::WNode *__restrict__ wide, uint32_t *__restrict__ loose) {
+                                                    rt::WNode *__restrict__ wide, uint32_t *__restrict__ info) {
     const uint32_t x = blockIdx.x * 256u + threadIdx.x;
     if (x >= nNodes) return;
     const rt::Node b = rt::load_node(nodes, static_cast<int32_t>(x));
@@ -639,7 +643,12 @@ __global__ __launch_bounds__(256) void k_wide_build(const rt::Node *__restrict__
     o[5] = make_float4(mx[0].z, mx[1].z, mx[2].z, mx[3].z);
     reinterpret_cast<int4 *>(o)[6] = make_int4(id[0], id[1], id[2], id[3]);
     reinterpret_cast<int4 *>(o)[7] = make_int4(grp[0], grp[1], 0, 0);
-    if (!ok) atomicOr(loose, 1u);
+    if (!ok) atomicOr(&info[0], 1u);
+    const float m = fmaxf(fmaxf(fmaxf(box_max(b.lmin), box_max(b.lmax)), box_max(b.rmin)), box_max(b.rmax));
+    const bool nan = b.lmin.x != b.lmin.x || b.lmin.y != b.lmin.y || b.lmin.z != b.lmin.z || b.lmax.x != b.lmax.x ||
+                     b.lmax.y != b.lmax.y || b.lmax.z != b.lmax.z || b.rmin.x != b.rmin.x || b.rmin.y != b.rmin.y ||
+                     b.rmin.z != b.rmin.z || b.rmax.x != b.rmax.x || b.rmax.y != b.rmax.y || b.rmax.z != b.rmax.z;
+    atomicMax(&info[1], nan ? 0x7fc00000u : __float_as_uint(m));
 }
 
 }  // namespace
@@ -654,18 +663,20 @@ int bind_tree(restir_ctx *ctx, const void *nodes, uint32_t nNodes, const void *t
     const size_t wideBytes = sizeof(rt::WNode) * static_cast<size_t>(nNodes);
     if (int r = ensure(ctx, &ctx->wide, &ctx->wideBytes, wideBytes + 256)) return r;
     rt::WNode *wide = static_cast<rt::WNode *>(ctx->wide);
-    uint32_t *loose = reinterpret_cast<uint32_t *>(static_cast<char *>(ctx->wide) + wideBytes);
-    if (int r = hip(hipMemsetAsync(loose, 0, sizeof(uint32_t), s), "bind tree: clear")) return r;
+    uint32_t *info = reinterpret_cast<uint32_t *>(static_cast<char *>(ctx->wide) + wideBytes);
+    if (int r = hip(hipMemsetAsync(info, 0, 2 * sizeof(uint32_t), s), "bind tree: clear")) return r;
     RK_LAUNCH("k_wide_build", k_wide_build, dim3(groups(nNodes)), dim3(256), 0, s,
-              static_cast<const rt::Node *>(nodes), nNodes, wide, loose);
-    uint32_t hostLoose = 1;
-    if (int r = hip(hipMemcpyAsync(&hostLoose, loose, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "bind tree: read"))
-        return r;
+              static_cast<const rt::Node *>(nodes), nNodes, wide, info);
+    uint32_t host[2] = {1u, 0x7fc00000u};
+    if (int r = hip(hipMemcpyAsync(host, info, sizeof host, hipMemcpyDeviceToHost, s), "bind tree: read")) return r;
     if (int r = hip(hipStreamSynchronize(s), "bind tree: finish")) return r;
+    float maxCoord;
+    std::memcpy(&maxCoord, &host[1], sizeof maxCoord);
     ctx->scene.nodes = static_cast<const rt::Node *>(nodes);
     ctx->scene.tris = static_cast<const float4 *>(tris);
     ctx->scene.wide = wide;
-    ctx->scene.treeNested = hostLoose == 0 ? 1 : 0;
+    ctx->scene.treeNested = host[0] == 0 ? 1 : 0;
+    ctx->scene.maxCoord = maxCoord;
     // a walk defers at most one sibling per internal level below the root: depth - 1 entries
     ctx->stack = depth <= 24 ? 24 : (depth <= 32 ? 32 : 64);
     ctx->persistentBlocks = persistent_blocks(ctx->cus, ctx->stack);

@@ -37,6 +37,7 @@ struct SceneView {
     const float4 *tris;     // 3 float4 per triangle
     const rt::WNode *wide;  // the 4-entry nodes over `nodes` (same indices), built by the library
     int32_t treeNested;     // 1: every box finite and inside the box stored for its parent
+    float maxCoord;         // largest |box coordinate| of the tree
 };
 
 struct GView {

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "scan.h"
 
 namespace rk {
 
@@ -278,7 +279,7 @@ __device__ __forceinline__ void pass_a_px(const PassAArgs &a, uint32_t x, uint32
         f3 o, d;
         shadow_segment(pos, res.pos, o, d);
         TraceStats st{0, 0};
-        if (occluded_w<STACK>(sc.wide, sc.nodes, sc.tris, sc.treeNested != 0, o, d, stk, kBlock, st)) {
+        if (occluded_w<STACK>(sc.wide, sc.nodes, sc.tris, sc.treeNested != 0, sc.maxCoord, o, d, stk, kBlock, st)) {
             res.w = 0.0f;
             res.sumW = 0.0f;
         }
@@ -440,7 +441,7 @@ __device__ __forceinline__ void pass_c_px(const PassCArgs &a, uint32_t x, uint32
         f3 o, d;
         shadow_segment(pos, lp, o, d);
         TraceStats st{0, 0};
-        centreShadowed = occluded_w<STACK>(a.sc.wide, a.sc.nodes, a.sc.tris, a.sc.treeNested != 0, o, d, stk, kBlock, st);
+        centreShadowed = occluded_w<STACK>(a.sc.wide, a.sc.nodes, a.sc.tris, a.sc.treeNested != 0, a.sc.maxCoord, o, d, stk, kBlock, st);
     }
     if (centreShadowed) {
         numSamples = 0;
@@ -454,7 +455,7 @@ __device__ __forceinline__ void pass_c_px(const PassCArgs &a, uint32_t x, uint32
                 f3 o, d;
                 shadow_segment(nwp, lp, o, d);
                 TraceStats st{0, 0};
-                if (occluded_w<STACK>(a.sc.wide, a.sc.nodes, a.sc.tris, a.sc.treeNested != 0, o, d, stk, kBlock, st)) continue;
+                if (occluded_w<STACK>(a.sc.wide, a.sc.nodes, a.sc.tris, a.sc.treeNested != 0, a.sc.maxCoord, o, d, stk, kBlock, st)) continue;
             }
             numSamples += nM[j];
         }
@@ -479,7 +480,7 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c(const PassCArgs a) {
 // atomic per wave, slots handed out by prefix sums over the wave, so a queue holds only live rays
 // in pixel-tile order.
 struct CQueues {
-    uint32_t *counter;     // [0] centre rays queued, [1] neighbour rays queued
+    uint32_t *counter;     // [0] centre rays queued, [1] neighbour rays queued, [2] / [3] their tracer heads
     float4 *c1, *c2;       // centre ray end points; c1.w carries the pixel index
     float4 *n1, *n2;       // neighbour ray end points; n1.w carries pixel | slot << 27
     uint32_t *origM;       // per pixel
@@ -530,6 +531,7 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
     const bool vis = (a.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) != 0;
     const size_t idx = inside ? static_cast<size_t>(y) * W + x : 0;
     bool emit = false;
+    uint32_t above = 0;  // above-horizon neighbour slots
     f3 pos{0.0f, 0.0f, 0.0f}, lp{0.0f, 0.0f, 0.0f};
     if (inside) {
         const GView &g = a.g;
@@ -546,7 +548,7 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
                 q.nM[static_cast<size_t>(i) * q.npx + idx] = m;
             });
             lp = res.pos;
-            uint32_t above = 0, numSamples = origM;
+            uint32_t numSamples = origM;
             for (int j = 0; j < a.k; ++j) {
                 const uint32_t ni = q.nIdx[static_cast<size_t>(j) * q.npx + idx];
                 const float4 s0 = q.surf[2 * static_cast<size_t>(ni)], s1 = q.surf[2 * static_cast<size_t>(ni) + 1];
@@ -568,24 +570,14 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
         q.c1[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(static_cast<uint32_t>(idx)));
         q.c2[slot] = make_float4(lp.x, lp.y, lp.z, 0.0f);
     }
-}
-
-// Queues the neighbour rays of pixels whose centre ray found the light. The wave's rays are
-// written slot-major: lane i writes slots i, i + 64, ... of the wave's run, finding the pixel that
-// owns a slot by binary search over the wave's prefix sums, so each store instruction covers
-// consecutive queue entries (pixel-major stores scatter a wave's writes over partial lines).
-template <int TX, int TY>
-__global__ __launch_bounds__(TX *TY) void k_pass_c_emit(const PassCArgs a, const CQueues q) {
-    uint32_t x, y;
-    tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
-    const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
-    const bool inside = x < W && y < H;
-    const uint32_t idx = inside ? y * W + x : 0u;
-    uint32_t want = 0;
-    if (inside) {
-        const uint32_t b = q.bits[idx];
-        if (!(b & (kBitBackground | kBitDone | kBitCentreShadowed))) want = b & 0xffffu;
-    }
+    // One neighbour ray per above-horizon neighbour, queued before the centre's own ray is traced:
+    // the neighbour tracer skips the rays of pixels whose centre ray it finds shadowed, since the
+    // centre re-test zeroes Z whatever the neighbours give (unbiasedReuse.glsl:157-166) and the
+    // neighbour loop draws no random numbers. The wave's rays are written slot-major: lane i
+    // writes rays i, i + 64, ... of the wave's run, finding the pixel that owns a ray by binary
+    // search over the wave's prefix sums, so each store instruction covers consecutive queue
+    // entries.
+    const uint32_t want = emit ? above : 0u;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t cnt = static_cast<uint32_t>(__popc(want));
     uint32_t incl = cnt;
@@ -600,8 +592,7 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_emit(const PassCArgs a, const
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(&q.counter[1], total);
     base = __shfl(base, 0);
-    float4 l = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (want) l = reinterpret_cast<const float4 *>(a.out + idx)[0];
+    const uint32_t myIdx = static_cast<uint32_t>(idx);
     for (uint32_t r0 = 0; r0 < total; r0 += 64) {  // every lane runs every round
         const uint32_t r = r0 + lane;
         // owner: the last lane whose exclusive prefix is <= r (it has r < its inclusive prefix)
@@ -614,16 +605,15 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_emit(const PassCArgs a, const
         }
         const uint32_t oWant = __shfl(want, static_cast<int>(o));
         const uint32_t oExcl = __shfl(excl, static_cast<int>(o));
-        const uint32_t oIdx = __shfl(idx, static_cast<int>(o));
-        const float lx = __shfl(l.x, static_cast<int>(o)), ly = __shfl(l.y, static_cast<int>(o)),
-                    lz = __shfl(l.z, static_cast<int>(o));
+        const uint32_t oIdx = __shfl(myIdx, static_cast<int>(o));
+        const float lx = __shfl(lp.x, static_cast<int>(o)), ly = __shfl(lp.y, static_cast<int>(o)),
+                    lz = __shfl(lp.z, static_cast<int>(o));
         if (r < total) {
             uint32_t m = oWant;
             for (uint32_t k = r - oExcl; k > 0; --k) m &= m - 1;  // the k-th set bit of the owner
             const uint32_t j = static_cast<uint32_t>(__ffs(static_cast<int>(m)) - 1);
             const float4 sp = q.surf[2 * static_cast<size_t>(q.nIdx[static_cast<size_t>(j) * q.npx + oIdx])];
-            const f3 nwp{sp.x, sp.y, sp.z};
-            q.n1[base + r] = make_float4(nwp.x, nwp.y, nwp.z, __uint_as_float(oIdx | (j << 27)));
+            q.n1[base + r] = make_float4(sp.x, sp.y, sp.z, __uint_as_float(oIdx | (j << 27)));
             q.n2[base + r] = make_float4(lx, ly, lz, 0.0f);
         }
     }
@@ -652,27 +642,91 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_finish(const PassCArgs a, con
     *r = v;
 }
 
-// Tracer over a queue whose length a previous kernel left in *count: one ray per lane, in queue
-// order, over a grid sized for the queue's capacity (lanes past the count return at once). Queue
-// order keeps the rays of neighbouring pixels in one wave, which walk much of the same tree; a
-// persistent pool that refills idle lanes mixes them and was slower. Results go to the pixel
-// named by the ray's tag: a centre ray sets kBitCentreShadowed in bits[pixel]; a neighbour ray
-// sets bit `slot` of occ[pixel].
+// Tracer over a queue whose length a previous kernel left in *count. A fixed grid of workgroups
+// (as many as fit the device at once); each wave takes chunks of consecutive rays from a global
+// head, one atomic per chunk, and hands a chunk's rays to its idle lanes in queue order whenever
+// kTraceRefill or more lanes are idle. Queue order keeps the rays of neighbouring pixels
+// together, which walk much of the same tree (the L1 serves most node fetches); the refills keep
+// lanes busy when a wave's rays end at different depths. Chunks shrink as the queue drains
+// (guided self-scheduling: the rays left when the wave last took a chunk, over twice the grid's
+// waves, between kTraceMinChunk and kTraceMaxChunk), so that waves finish close together.
+// Results go to the pixel named by the ray's tag: a centre ray sets kBitCentreShadowed in
+// bits[pixel]; a neighbour ray sets bit `slot` of occ[pixel].
+#ifndef RK_TRACE_MAX_CHUNK
+#define RK_TRACE_MAX_CHUNK 128
+#endif
+#ifndef RK_TRACE_MIN_CHUNK
+#define RK_TRACE_MIN_CHUNK 32
+#endif
+#ifndef RK_TRACE_REFILL
+#define RK_TRACE_REFILL 32
+#endif
+constexpr uint32_t kTraceMaxChunk = RK_TRACE_MAX_CHUNK, kTraceMinChunk = RK_TRACE_MIN_CHUNK,
+                   kTraceRefill = RK_TRACE_REFILL;
 template <int STACK, bool CENTRE>
 __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const float4 *__restrict__ p1,
                                                      const float4 *__restrict__ p2,
-                                                     const uint32_t *__restrict__ count, CQueues q) {
+                                                     const uint32_t *__restrict__ count,
+                                                     uint32_t *__restrict__ head, CQueues q) {
     __shared__ int32_t stack[STACK * 256];
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= *count) return;
-    const float4 a4 = p1[i];
-    f3 o, d;
-    shadow_segment(xyz(a4), xyz(p2[i]), o, d);
-    TraceStats st{0, 0};
-    if (occluded_w<STACK>(sc.wide, sc.nodes, sc.tris, sc.treeNested != 0, o, d, stack + threadIdx.x, 256, st)) {
-        const uint32_t tag = __float_as_uint(a4.w);
-        if (CENTRE) q.bits[tag] |= kBitCentreShadowed;
-        else atomicOr(&q.occ[tag & 0x07ffffffu], 1u << (tag >> 27));
+    int32_t *stk = stack + threadIdx.x;
+    const uint32_t n = *count;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+    WWalk w;
+    int32_t ray = -1;
+    uint32_t tag = 0;
+    uint32_t next = 0, end = 0;  // wave-uniform: the rays of the wave's chunk not handed out yet
+    uint32_t seen = 0;           // wave-uniform: the head just after this wave's last chunk
+    bool drained = false;        // wave-uniform: the head has passed the last ray
+    const uint32_t nWaves = gridDim.x * (256u / 64u);
+    for (;;) {
+        const uint64_t idle = __ballot(ray < 0);
+        const uint32_t nIdle = static_cast<uint32_t>(__popcll(idle));
+        if (nIdle >= kTraceRefill) {
+            if (next >= end && !drained) {
+                const uint32_t left = n > seen ? n - seen : 0u;
+                uint32_t size = left / (2u * nWaves);
+                size = size < kTraceMinChunk ? kTraceMinChunk : (size > kTraceMaxChunk ? kTraceMaxChunk : size);
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(head, size);
+                base = __shfl(base, 0);
+                seen = base + size;
+                next = base < n ? base : n;
+                end = n - next < size ? n : next + size;
+                drained = base + size >= n;
+            }
+            if (next < end) {
+                if (ray < 0) {
+                    const uint32_t i = next + static_cast<uint32_t>(__popcll(idle & below));
+                    if (i < end) {
+                        const float4 a4 = p1[i];
+                        tag = __float_as_uint(a4.w);
+                        // a neighbour ray of a pixel whose centre ray was shadowed is not traced
+                        if (CENTRE || !(q.bits[tag & 0x07ffffffu] & kBitCentreShadowed)) {
+                            f3 o, d;
+                            shadow_segment(xyz(a4), xyz(p2[i]), o, d);
+                            wwalk_begin(w, o, d, sc.treeNested != 0, sc.maxCoord);
+                            ray = static_cast<int32_t>(i);
+                        }
+                    }
+                }
+                next = end - next < nIdle ? end : next + nIdle;
+            }
+        }
+        if (__ballot(ray >= 0) == 0) {
+            if (drained && next >= end) break;
+            continue;
+        }
+        bool fin = false;
+        if (ray >= 0) fin = walk_step_w<STACK>(w, sc.wide, sc.nodes, sc.tris, stk, 256);
+        if (fin) {
+            if (w.hit) {
+                if (CENTRE) q.bits[tag] |= kBitCentreShadowed;
+                else atomicOr(&q.occ[tag & 0x07ffffffu], 1u << (tag >> 27));
+            }
+            ray = -1;
+        }
     }
 }
 
@@ -682,7 +736,7 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
 // scratch, and the ray queued; (2) the persistent tracer marks shadowed pixels; (3) per pixel:
 // the shadow verdict, temporal reuse, store. Same draws, same order, same result as k_pass_a.
 struct AQueues {
-    uint32_t *counter;  // [0] rays queued
+    uint32_t *counter;  // [0] rays queued, [1] tracer head
     float4 *p1, *p2;    // p1.w carries the pixel index
     uint64_t *rng;      // per pixel: the random stream's state after RIS
     uint32_t *occ;      // per pixel: kBitCentreShadowed once its ray hit
@@ -799,7 +853,7 @@ __global__ __launch_bounds__(256) void k_trace(const TraceArgs a) {
     TraceStats st{0, 0};
     bool occ;
     if constexpr (REF) occ = occluded<STACK, true>(a.sc.nodes, a.sc.tris, o, d, stack + threadIdx.x, 256, st);
-    else occ = occluded_w<STACK>(a.sc.wide, a.sc.nodes, a.sc.tris, a.sc.treeNested != 0, o, d, stack + threadIdx.x,
+    else occ = occluded_w<STACK>(a.sc.wide, a.sc.nodes, a.sc.tris, a.sc.treeNested != 0, a.sc.maxCoord, o, d, stack + threadIdx.x,
                                  256, st);
     a.shadowed[i] = occ ? 1u : 0u;
     if (a.visits) a.visits[i] = st.visits;
@@ -834,7 +888,7 @@ __global__ __launch_bounds__(256) void k_trace_pool(const TraceArgs a, uint32_t 
                     f3 o, d;
                     shadow_segment(xyz(a.p1[i]), xyz(a.p2[i]), o, d);
                     if constexpr (REF) walk_begin(w, o, d);
-                    else wwalk_begin(w, o, d, a.sc.treeNested != 0);
+                    else wwalk_begin(w, o, d, a.sc.treeNested != 0, a.sc.maxCoord);
                     ray = static_cast<int32_t>(i);
                 }
             }
@@ -1056,14 +1110,13 @@ template <int TX, int TY, int STACK>
 static hipError_t run_a_queued(PassAArgs args, const AQueues &q, uint32_t gridCap, hipStream_t s) {
     grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
     const dim3 tiles(args.nTiles), block(TX * TY);
-    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 1u);
+    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 2u);
     RK_LAUNCH("k_pass_a_ris", (k_pass_a_ris<TX, TY>), tiles, block, 0, s, args, q);
     if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
         CQueues cq{};
         cq.bits = q.occ;  // a shadowed ray sets kBitCentreShadowed in occ[pixel]
-        const uint32_t npx = args.u.screenSize[0] * args.u.screenSize[1];
-        RK_LAUNCH("k_trace_queue(pass_a)", (k_trace_queue<STACK, true>), dim3((npx + 255) / 256), dim3(256), 0,
-                  s, args.sc, q.p1, q.p2, q.counter + 0, cq);
+        RK_LAUNCH("k_trace_queue(pass_a)", (k_trace_queue<STACK, true>), dim3(gridCap), dim3(256), 0, s,
+                  args.sc, q.p1, q.p2, q.counter + 0, q.counter + 1, cq);
     }
     RK_LAUNCH("k_pass_a_finish", (k_pass_a_finish<TX, TY>), tiles, block, 0, s, args, q);
     return hipGetLastError();
@@ -1133,19 +1186,15 @@ template <int TX, int TY, int STACK>
 static hipError_t run_c_queued(PassCArgs args, const CQueues &q, uint32_t gridCap, hipStream_t s) {
     grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
     const dim3 tiles(args.nTiles), block(TX * TY);
-    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 2u);
+    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 4u);
     const uint32_t npx = args.u.screenSize[0] * args.u.screenSize[1];
-    // queue capacities: one centre ray per pixel, k neighbour rays per pixel
-    const uint32_t centreBlocks = (npx + 255) / 256;
-    const uint32_t neighbourBlocks = static_cast<uint32_t>((static_cast<uint64_t>(npx) * args.k + 255) / 256);
     RK_LAUNCH("k_pack_surface", k_pack_surface, dim3((npx + 255) / 256), dim3(256), 0, s, args.g, npx, q.surf);
     RK_LAUNCH("k_pass_c_merge", (k_pass_c_merge<TX, TY>), tiles, block, 0, s, args, q);
     if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
-        RK_LAUNCH("k_trace_queue(centre)", (k_trace_queue<STACK, true>), dim3(centreBlocks), dim3(256), 0, s,
-                  args.sc, q.c1, q.c2, q.counter + 0, q);
-        RK_LAUNCH("k_pass_c_emit", (k_pass_c_emit<TX, TY>), tiles, block, 0, s, args, q);
-        RK_LAUNCH("k_trace_queue(neighbour)", (k_trace_queue<STACK, false>), dim3(neighbourBlocks), dim3(256),
-                  0, s, args.sc, q.n1, q.n2, q.counter + 1, q);
+        RK_LAUNCH("k_trace_queue(centre)", (k_trace_queue<STACK, true>), dim3(gridCap), dim3(256), 0, s,
+                  args.sc, q.c1, q.c2, q.counter + 0, q.counter + 2, q);
+        RK_LAUNCH("k_trace_queue(neighbour)", (k_trace_queue<STACK, false>), dim3(gridCap), dim3(256), 0, s,
+                  args.sc, q.n1, q.n2, q.counter + 1, q.counter + 3, q);
         RK_LAUNCH("k_pass_c_finish", (k_pass_c_finish<TX, TY>), tiles, block, 0, s, args, q);
     }
     return hipGetLastError();
@@ -1194,6 +1243,20 @@ static CQueues carve_queues(void *scratch, uint32_t npx, int k) {
     return q;
 }
 
+__global__ void k_traced_flag(const CQueues q, uint32_t len, uint32_t *flag) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= len) return;
+    const uint32_t tag = __float_as_uint(q.n1[i].w);
+    flag[i] = (q.bits[tag & 0x07ffffffu] & kBitCentreShadowed) ? 0u : 1u;
+}
+__global__ void k_traced_copy(const CQueues q, uint32_t len, const uint32_t *flag, const uint32_t *off, float4 *p1,
+                              float4 *p2, uint32_t cap) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= len || !flag[i] || off[i] >= cap) return;
+    p1[off[i]] = q.n1[i];
+    p2[off[i]] = q.n2[i];
+}
+
 hipError_t pass_c_rays(const void *queueScratch, uint32_t npx, int k, int which, float4 *p1,
                        float4 *p2, uint32_t cap, uint32_t *n, hipStream_t s) {
     const CQueues q = carve_queues(const_cast<void *>(queueScratch), npx, k);
@@ -1204,11 +1267,35 @@ hipError_t pass_c_rays(const void *queueScratch, uint32_t npx, int k, int which,
     const uint32_t maxLen = which == 0 ? npx : npx * static_cast<uint32_t>(k);
     if (len > maxLen) return hipErrorIllegalState;
     *n = len;
-    const uint32_t m = len < cap ? len : cap;
-    if (m == 0) return hipSuccess;
-    e = hipMemcpyAsync(p1, which == 0 ? q.c1 : q.n1, sizeof(float4) * m, hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(p2, which == 0 ? q.c2 : q.n2, sizeof(float4) * m, hipMemcpyDeviceToDevice, s);
+    if (which == 0) {
+        *n = len;
+        const uint32_t m = len < cap ? len : cap;
+        if (m == 0) return hipSuccess;
+        e = hipMemcpyAsync(p1, q.c1, sizeof(float4) * m, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(p2, q.c2, sizeof(float4) * m, hipMemcpyDeviceToDevice, s);
+        return e;
+    }
+    // neighbour rays: the ones the tracer traced (centre ray not shadowed), in queue order
+    *n = 0;
+    if (len == 0) return hipSuccess;
+    uint32_t *tmp = nullptr;
+    const size_t words = 2 * static_cast<size_t>(len) + rscan::temp_items(len) + 1;
+    e = hipMalloc(&tmp, sizeof(uint32_t) * words);
+    if (e != hipSuccess) return e;
+    uint32_t *flag = tmp, *off = tmp + len, *temp = tmp + 2 * static_cast<size_t>(len), *total = tmp + words - 1;
+    const uint32_t blocks = (len + 255) / 256;
+    RK_LAUNCH("k_traced_flag", k_traced_flag, dim3(blocks), dim3(256), 0, s, q, len, flag);
+    e = rscan::exclusive_scan<uint32_t>(flag, off, len, temp, total, s);
+    if (e == hipSuccess) {
+        RK_LAUNCH("k_traced_copy", k_traced_copy, dim3(blocks), dim3(256), 0, s, q, len, flag, off, p1, p2, cap);
+        e = hipGetLastError();
+    }
+    uint32_t kept = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&kept, total, sizeof kept, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    const hipError_t ef = hipFree(tmp);
+    if (e == hipSuccess) e = ef;
+    if (e == hipSuccess) *n = kept;
     return e;
 }
 

@@ -283,18 +283,59 @@ struct alignas(16) WNode {
 static_assert(sizeof(WNode) == 128, "one 128-byte line per node");
 constexpr int32_t kNoEntry = INT32_MIN;
 
-// A segment for the 4-entry walk: `full` asks for the skipped boxes to be tested too.
+// A segment for the 4-entry walk. `full`: test the skipped boxes too (see above). `quick`: o and
+// d are regular and every slab distance of every box of the tree stays below 1e30 in magnitude,
+// because (max |box coordinate| + max |o|) * max |1/d| < 1e30; ray_box_w then forms each slab
+// distance with one FMA and needs no per-box finiteness test.
 struct WSeg {
     Seg s;
-    bool full;
+    f3 noi;      // -RN(o * inv), per component
+    float eAbs;  // absolute slack of the quick test
+    bool full, quick;
 };
-__device__ __forceinline__ WSeg make_wseg(f3 o, f3 d, bool treeNested) {
+__device__ __forceinline__ WSeg make_wseg(f3 o, f3 d, bool treeNested, float maxCoord) {
     constexpr float kInf = __builtin_huge_valf();
     // false for NaN as well as for zero or infinite components
     const bool regular = fabsf(d.x) > 0.0f && fabsf(d.y) > 0.0f && fabsf(d.z) > 0.0f &&
                          fabsf(d.x) < kInf && fabsf(d.y) < kInf && fabsf(d.z) < kInf &&
                          fabsf(o.x) < kInf && fabsf(o.y) < kInf && fabsf(o.z) < kInf;
-    return WSeg{make_seg(o, d), !(treeNested && regular)};
+    WSeg w;
+    w.s = make_seg(o, d);
+    const f3 inv = w.s.inv;
+    w.noi = f3{-(o.x * inv.x), -(o.y * inv.y), -(o.z * inv.z)};
+    const float invMax = fmaxf(fmaxf(fabsf(inv.x), fabsf(inv.y)), fabsf(inv.z));
+    const float invMin = fminf(fminf(fabsf(inv.x), fabsf(inv.y)), fabsf(inv.z));
+    const float oMax = fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z));
+    // comparisons are false for NaN: a NaN maxCoord or an infinite 1/d leaves quick off
+    w.quick = regular && invMin > 1e-30f && (maxCoord + oMax) * invMax < 1e30f;
+    const float oiMax = fmaxf(fmaxf(fabsf(w.noi.x), fabsf(w.noi.y)), fabsf(w.noi.z));
+    w.eAbs = oiMax * 2.3841858e-7f + 1e-30f;  // 4u max |o * inv| + 1e-30, u = 2^-24
+    w.full = !(treeNested && regular);
+    return w;
+}
+
+// ray_box for a WSeg. Quick rays form each slab distance as t = RN(b * inv - oi) with one FMA
+// (oi = RN(o * inv)). With inv = (1/d)(1 + d1) and oi = o inv (1 + d2), t differs from the exact
+// (b - o)/d by at most 3.02u |t| + 1.02u |o inv|, and from the reference's RN(RN(b - o)/d) by one
+// more rounding: the widened intervals below carry 8u |v| + 4u max |o inv| + 1e-30, more than
+// twice that, and min/max keep such bounds. Undecided cases take the division form, as in
+// ray_box. All distances are finite by the `quick` condition, so fminf/fmaxf see no NaN.
+template <bool kQuick>
+__device__ __forceinline__ bool ray_box_w(const WSeg &w, float4 mn, float4 mx) {
+    if (!kQuick) return ray_box(w.s, mn, mx);
+    const f3 inv = w.s.inv, noi = w.noi;
+    const float ax = fmaf(mn.x, inv.x, noi.x), ay = fmaf(mn.y, inv.y, noi.y), az = fmaf(mn.z, inv.z, noi.z);
+    const float bx = fmaf(mx.x, inv.x, noi.x), by = fmaf(mx.y, inv.y, noi.y), bz = fmaf(mx.z, inv.z, noi.z);
+    const float rmin = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+    const float rmax = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+    constexpr float kRel = 8.0f * 5.9604645e-8f;
+    const float emin = fmaf(fabsf(rmin), kRel, w.eAbs), emax = fmaf(fabsf(rmax), kRel, w.eAbs);
+    const float minLo = rmin - emin, minHi = rmin + emin;
+    const float maxLo = rmax - emax, maxHi = rmax + emax;
+    const bool surelyIn = minHi < 1.0f && maxLo >= minHi && maxLo > 0.0f;
+    const bool surelyOut = minLo >= 1.0f || maxHi < minLo || maxHi <= 0.0f;
+    if (surelyIn || surelyOut) return surelyIn;
+    return ray_box_div(w.s.o, w.s.d, mn, mx);
 }
 
 __device__ __forceinline__ float4 corner(float4 x, float4 y, float4 z, int k) {
@@ -306,9 +347,9 @@ __device__ __forceinline__ float4 corner(float4 x, float4 y, float4 z, int k) {
 
 // One visit of node `cur`. Returns true when a triangle is hit; otherwise *next gets the node to
 // visit next (or -1 when the stack is empty), after pushing the left side's item if both sides
-// yield one.
-template <int kStack>
-__device__ __forceinline__ bool wide_visit(const WSeg &ws, const WNode *__restrict__ wide,
+// yield one. kQuick: every box test takes the quick form (all lanes' rays are quick).
+template <int kStack, bool kQuick>
+__device__ __forceinline__ bool wide_visit_k(const WSeg &ws, const WNode *__restrict__ wide,
                                            const Node *__restrict__ nodes, const float4 *__restrict__ tris,
                                            int32_t cur, int32_t *next, int32_t *stk, int stride, int &top,
                                            uint32_t &nv, uint32_t &nt) {
@@ -321,13 +362,18 @@ __device__ __forceinline__ bool wide_visit(const WSeg &ws, const WNode *__restri
     bool sideL = true, sideR = true;
     if (ws.full) {
         const Node nd = load_node(nodes, cur);
-        sideL = ray_box(s, nd.lmin, nd.lmax);
-        sideR = ray_box(s, nd.rmin, nd.rmax);
+        sideL = ray_box_w<kQuick>(ws, nd.lmin, nd.lmax);
+        sideR = ray_box_w<kQuick>(ws, nd.rmin, nd.rmax);
     }
-    const bool h0 = sideL && id.x != kNoEntry && ray_box(s, corner(lx, ly, lz, 0), corner(hx, hy, hz, 0));
-    const bool h1 = sideL && id.y != kNoEntry && ray_box(s, corner(lx, ly, lz, 1), corner(hx, hy, hz, 1));
-    const bool h2 = sideR && id.z != kNoEntry && ray_box(s, corner(lx, ly, lz, 2), corner(hx, hy, hz, 2));
-    const bool h3 = sideR && id.w != kNoEntry && ray_box(s, corner(lx, ly, lz, 3), corner(hx, hy, hz, 3));
+    // all four boxes are tested (an empty entry holds a zero box), without a branch per entry
+    const bool b0 = ray_box_w<kQuick>(ws, corner(lx, ly, lz, 0), corner(hx, hy, hz, 0));
+    const bool b1 = ray_box_w<kQuick>(ws, corner(lx, ly, lz, 1), corner(hx, hy, hz, 1));
+    const bool b2 = ray_box_w<kQuick>(ws, corner(lx, ly, lz, 2), corner(hx, hy, hz, 2));
+    const bool b3 = ray_box_w<kQuick>(ws, corner(lx, ly, lz, 3), corner(hx, hy, hz, 3));
+    const bool h0 = sideL & (id.x != kNoEntry) & b0;
+    const bool h1 = sideL & (id.y != kNoEntry) & b1;
+    const bool h2 = sideR & (id.z != kNoEntry) & b2;
+    const bool h3 = sideR & (id.w != kNoEntry) & b3;
     uint32_t leaves = (h0 && id.x < 0 ? 1u : 0u) | (h1 && id.y < 0 ? 2u : 0u) |
                       (h2 && id.z < 0 ? 4u : 0u) | (h3 && id.w < 0 ? 8u : 0u);
     while (leaves) {
@@ -357,12 +403,24 @@ __device__ __forceinline__ bool wide_visit(const WSeg &ws, const WNode *__restri
     return false;
 }
 
+// The visit with the quick box tests when every active lane's ray allows them (a wave-wide
+// choice, so the box tests run without per-box branches), else with the checked ones.
+template <int kStack>
+__device__ __forceinline__ bool wide_visit(const WSeg &ws, const WNode *__restrict__ wide,
+                                           const Node *__restrict__ nodes, const float4 *__restrict__ tris,
+                                           int32_t cur, int32_t *next, int32_t *stk, int stride, int &top,
+                                           uint32_t &nv, uint32_t &nt) {
+    if (__ballot(!ws.quick) == 0ull)
+        return wide_visit_k<kStack, true>(ws, wide, nodes, tris, cur, next, stk, stride, top, nv, nt);
+    return wide_visit_k<kStack, false>(ws, wide, nodes, tris, cur, next, stk, stride, top, nv, nt);
+}
+
 // `occluded` (production loop) over the 4-entry nodes: same answer, about half the visits.
 template <int kStack>
 __device__ __forceinline__ bool occluded_w(const WNode *__restrict__ wide, const Node *__restrict__ nodes,
-                                           const float4 *__restrict__ tris, bool treeNested, f3 o, f3 d,
-                                           int32_t *stk, int stride, TraceStats &st) {
-    const WSeg s = make_wseg(o, d, treeNested);
+                                           const float4 *__restrict__ tris, bool treeNested, float maxCoord,
+                                           f3 o, f3 d, int32_t *stk, int stride, TraceStats &st) {
+    const WSeg s = make_wseg(o, d, treeNested, maxCoord);
     int top = 0;
     int32_t cur = 0;
     uint32_t nv = 0, nt = 0;
@@ -387,8 +445,8 @@ struct WWalk {
     bool hit;
     uint32_t nv, nt;
 };
-__device__ __forceinline__ void wwalk_begin(WWalk &w, f3 o, f3 d, bool treeNested) {
-    w.s = make_wseg(o, d, treeNested);
+__device__ __forceinline__ void wwalk_begin(WWalk &w, f3 o, f3 d, bool treeNested, float maxCoord) {
+    w.s = make_wseg(o, d, treeNested, maxCoord);
     w.cur = 0;
     w.top = 0;
     w.hit = false;

@@ -142,36 +142,43 @@ __device__ __forceinline__ Pcg pass_a_rng(const PassAArgs &a, uint32_t x, uint32
     return pcg_seed(static_cast<uint64_t>(a.u.frame), static_cast<uint64_t>(y * 10007u + x));
 }
 
-// Initial RIS over M candidates (alias table or the learned proposal's list).
-// One light candidate of Pass A (restirOmni.glsl:113-145): point light li with source
-// probability prob.
-__device__ __forceinline__ void add_point_candidate(Res &res, const Surface &s, const SceneView &sc, int32_t li,
-                                                    float prob, Pcg &rng) {
-    const f3 L = xyz(*reinterpret_cast<const float4 *>(sc.points[li].pos));
-    const float lum = sc.points[li].color_luminance[3];
-    const float ph = eval_phat(s, L, f3{0.0f, 0.0f, 0.0f}, false, lum);
-    res_add_sample(res, L, make_float4(0.0f, 0.0f, 0.0f, 0.0f), lum, li, ph, prob, rng);
-}
-// A point on triangle light ti (two draws, pickPointOnTriangle) with its probability per unit
-// area seen from pos: prob / (|cos| * area).
-__device__ __forceinline__ void add_tri_candidate(Res &res, const Surface &s, f3 pos, const SceneView &sc, int32_t ti,
-                                                  float prob, Pcg &rng) {
-    const restir_tri_light &tl = sc.triLights[ti];
-    const f3 p1 = xyz(*reinterpret_cast<const float4 *>(tl.p1));
-    const f3 p2 = xyz(*reinterpret_cast<const float4 *>(tl.p2));
-    const f3 p3 = xyz(*reinterpret_cast<const float4 *>(tl.p3));
-    const float4 na = *reinterpret_cast<const float4 *>(tl.normalArea);
-    const float lum = tl.emission_luminance[3];
-    const float ra = pcg_float(rng), rb = pcg_float(rng);
-    const float sa = sqrtf(ra);  // pickPointOnTriangle, restirOmni.glsl:68-71
-    const f3 L = ((1.0f - sa) * p1 + (sa * (1.0f - rb)) * p2) + (rb * sa) * p3;
-    const f3 wi = normalize(pos - L);
-    const f3 nl = xyz(na);
-    prob /= fabsf(dot(wi, nl)) * na.w;
-    const float ph = eval_phat(s, L, nl, true, lum);
-    res_add_sample(res, L, make_float4(nl.x, nl.y, nl.z, 1.0f), lum, -1 - ti, ph, prob, rng);
+// One light candidate of Pass A (restirOmni.glsl:113-145) with source probability prob: point
+// light li, or (tri) a point on triangle light li picked with two draws (pickPointOnTriangle,
+// restirOmni.glsl:68-71), whose probability per unit area seen from pos is prob / (|cos| area).
+// Only the choice of the sample point depends on the kind; the target function and the
+// reservoir update run once for all lanes of a wave whatever kinds its lanes drew. `valid` false:
+// an empty candidate (p_hat 0, probability 1) that still counts in M and draws its number.
+__device__ __forceinline__ void add_candidate(Res &res, const Surface &s, f3 pos, const SceneView &sc, bool valid,
+                                              bool tri, int32_t li, float prob, Pcg &rng) {
+    f3 L{0.0f, 0.0f, 0.0f}, nl{0.0f, 0.0f, 0.0f};
+    float lum = 0.0f;
+    int32_t index = 0;
+    if (valid && !tri) {
+        L = xyz(*reinterpret_cast<const float4 *>(sc.points[li].pos));
+        lum = sc.points[li].color_luminance[3];
+        index = li;
+    } else if (valid) {
+        const restir_tri_light &tl = sc.triLights[li];
+        const f3 p1 = xyz(*reinterpret_cast<const float4 *>(tl.p1));
+        const f3 p2 = xyz(*reinterpret_cast<const float4 *>(tl.p2));
+        const f3 p3 = xyz(*reinterpret_cast<const float4 *>(tl.p3));
+        const float4 na = *reinterpret_cast<const float4 *>(tl.normalArea);
+        lum = tl.emission_luminance[3];
+        const float ra = pcg_float(rng), rb = pcg_float(rng);
+        const float sa = sqrtf(ra);
+        L = ((1.0f - sa) * p1 + (sa * (1.0f - rb)) * p2) + (rb * sa) * p3;
+        const f3 wi = normalize(pos - L);
+        nl = xyz(na);
+        prob /= fabsf(dot(wi, nl)) * na.w;
+        index = -1 - li;
+    }
+    const bool useLN = valid && tri;
+    const float ph = valid ? eval_phat(s, L, nl, useLN, lum) : 0.0f;
+    res_add_sample(res, L, make_float4(nl.x, nl.y, nl.z, useLN ? 1.0f : 0.0f), lum, index, ph,
+                   valid ? prob : 1.0f, rng);
 }
 
+// Initial RIS over M candidates (alias table or the learned proposal's list).
 __device__ __forceinline__ void pass_a_ris(const PassAArgs &a, const PassASurface &ps, Res &res, Pcg &rng) {
     const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
     const size_t idx = ps.idx;
@@ -188,13 +195,9 @@ __device__ __forceinline__ void pass_a_ris(const PassAArgs &a, const PassASurfac
         for (uint32_t i = 0; i < M; ++i) {
             const int32_t li = a.candLight[i * npx + idx];
             const float q = a.candQ[i * npx + idx];
-            if (li >= 0 && li < sc.nLights && q > 0.0f) {
-                if (li < sc.nPoints) add_point_candidate(res, s, sc, li, q, rng);
-                else add_tri_candidate(res, s, pos, sc, li - sc.nPoints, q, rng);
-            } else {
-                res_add_sample(res, f3{0.0f, 0.0f, 0.0f}, make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, 0,
-                               0.0f, 1.0f, rng);
-            }
+            const bool valid = li >= 0 && li < sc.nLights && q > 0.0f;
+            const bool tri = li >= sc.nPoints;
+            add_candidate(res, s, pos, sc, valid, tri, tri ? li - sc.nPoints : li, q, rng);
         }
     } else if (dot(n, n) != 0.0f) {
         const uint32_t M = a.u.initialLightSampleCount;
@@ -215,8 +218,8 @@ __device__ __forceinline__ void pass_a_ris(const PassAArgs &a, const PassASurfac
             }
             // restirOmni.glsl:117: point lights if there are any, else triangle lights; a union
             // table names point lights first
-            if (sc.mixed ? li < sc.nPoints : sc.nPoints != 0) add_point_candidate(res, s, sc, li, prob, rng);
-            else add_tri_candidate(res, s, pos, sc, sc.mixed ? li - sc.nPoints : li, prob, rng);
+            const bool tri = sc.mixed ? li >= sc.nPoints : sc.nPoints == 0;
+            add_candidate(res, s, pos, sc, true, tri, sc.mixed && tri ? li - sc.nPoints : li, prob, rng);
         }
     }
 }

@@ -75,6 +75,17 @@ def test_unbiased_rays_are_the_queued_rays(torch):
     assert 0 < nn <= npx * k
     t = p1[:nn, 3].cpu().numpy().view(np.uint32)
     assert (t & 0x07ffffff).max() < npx and (t >> 27).max() < k
+    # ... exactly the traced ones: rays of pixels whose centre ray found the light, each
+    # (pixel, slot) once, towards the pixel's merged light, from a neighbour pixel's position (a
+    # background neighbour has a zero normal, which the above-horizon test lets through)
+    pix, slot = t & 0x07ffffff, t >> 27
+    lit = tag[ref == 0]
+    assert np.isin(pix, lit).all() and len(np.unique(pix)) > 0.5 * len(lit)
+    assert len(np.unique(pix.astype(np.int64) * 16 + slot)) == nn
+    n1, n2 = p1[:nn].cpu().numpy(), p2[:nn].cpu().numpy()
+    assert np.array_equal(n2[:, :3], res["pos_lum"][pix, :3])
+    pixel_pos = {tuple(v) for v in pos}
+    assert all(tuple(v) in pixel_pos for v in n1[:, :3])
     # capacity smaller than the queue: copies a prefix, still reports the full length
     q1 = torch.zeros(5, 4, dtype=torch.float32, device="cuda")
     q2 = torch.zeros_like(q1)

@@ -350,7 +350,11 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": measured_traffic(dominant) if W * H == 1920 * 1080 else None,
                      "algorithmic_bytes_per_launch": round(dom_bytes) if dom_bytes else None,
-                     "avg_launch_ms": round(kernels[dominant]["avg_ms"], 4)},
+                     "avg_launch_ms": round(kernels[dominant]["avg_ms"], 4),
+                     "note": ("algorithmic bytes = node and triangle bytes per visit and test of each ray "
+                              "(128V + 48T); the tree is served by L1/L2 and the Infinity Cache, so the "
+                              "effective rate can pass the HBM peak while the HBM traffic stays at "
+                              "`traffic` bytes per launch" if dominant.startswith("k_trace") else None)},
     }
 
     if not args.no_learned:

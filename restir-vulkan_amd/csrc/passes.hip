@@ -525,15 +525,16 @@ __global__ __launch_bounds__(256) void k_pack_surface(const GView g, uint32_t np
     surf[2 * static_cast<size_t>(i) + 1] = make_float4(n.y, n.z, 0.0f, 0.0f);
 }
 
-template <int TX, int TY>
+template <int TX, int TY, int STACK>
 __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, const CQueues q) {
+    __shared__ int32_t stack[STACK * TX * TY];
     uint32_t x, y;
     tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
     const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
     const bool inside = x < W && y < H;
     const bool vis = (a.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) != 0;
     const size_t idx = inside ? static_cast<size_t>(y) * W + x : 0;
-    bool emit = false;
+    bool emit = false, shadowed = false;
     uint32_t above = 0;  // above-horizon neighbour slots
     f3 pos{0.0f, 0.0f, 0.0f}, lp{0.0f, 0.0f, 0.0f};
     if (inside) {
@@ -564,7 +565,16 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
             store_res(a.out, idx, res);
             q.origM[idx] = origM;
             q.occ[idx] = 0u;
-            q.bits[idx] = above | (vis ? 0u : kBitDone);
+            // the centre re-test (unbiasedReuse.glsl:157-166), traced here, one ray per lane in
+            // tile order
+            if (vis) {
+                f3 o, d;
+                shadow_segment(pos, lp, o, d);
+                TraceStats st{0, 0};
+                shadowed = occluded_w<STACK>(a.sc.wide, a.sc.nodes, a.sc.tris, a.sc.treeNested != 0, a.sc.maxCoord,
+                                             o, d, stack + threadIdx.x, TX * TY, st);
+            }
+            q.bits[idx] = above | (vis ? 0u : kBitDone) | (shadowed ? kBitCentreShadowed : 0u);
             emit = vis;
         }
     }
@@ -573,14 +583,13 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
         q.c1[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(static_cast<uint32_t>(idx)));
         q.c2[slot] = make_float4(lp.x, lp.y, lp.z, 0.0f);
     }
-    // One neighbour ray per above-horizon neighbour, queued before the centre's own ray is traced:
-    // the neighbour tracer skips the rays of pixels whose centre ray it finds shadowed, since the
-    // centre re-test zeroes Z whatever the neighbours give (unbiasedReuse.glsl:157-166) and the
-    // neighbour loop draws no random numbers. The wave's rays are written slot-major: lane i
+    // One neighbour ray per above-horizon neighbour of a pixel whose centre ray found the light:
+    // a shadowed centre zeroes Z whatever the neighbours give (unbiasedReuse.glsl:157-166) and the
+    // neighbour loop draws no random numbers, so those rays are not needed. The wave's rays are written slot-major: lane i
     // writes rays i, i + 64, ... of the wave's run, finding the pixel that owns a ray by binary
     // search over the wave's prefix sums, so each store instruction covers consecutive queue
     // entries.
-    const uint32_t want = emit ? above : 0u;
+    const uint32_t want = emit && !shadowed ? above : 0u;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t cnt = static_cast<uint32_t>(__popc(want));
     uint32_t incl = cnt;
@@ -1192,10 +1201,8 @@ static hipError_t run_c_queued(PassCArgs args, const CQueues &q, uint32_t gridCa
     RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 4u);
     const uint32_t npx = args.u.screenSize[0] * args.u.screenSize[1];
     RK_LAUNCH("k_pack_surface", k_pack_surface, dim3((npx + 255) / 256), dim3(256), 0, s, args.g, npx, q.surf);
-    RK_LAUNCH("k_pass_c_merge", (k_pass_c_merge<TX, TY>), tiles, block, 0, s, args, q);
+    RK_LAUNCH("k_pass_c_merge", (k_pass_c_merge<TX, TY, STACK>), tiles, block, 0, s, args, q);
     if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
-        RK_LAUNCH("k_trace_queue(centre)", (k_trace_queue<STACK, true>), dim3(gridCap), dim3(256), 0, s,
-                  args.sc, q.c1, q.c2, q.counter + 0, q.counter + 2, q);
         RK_LAUNCH("k_trace_queue(neighbour)", (k_trace_queue<STACK, false>), dim3(gridCap), dim3(256), 0, s,
                   args.sc, q.n1, q.n2, q.counter + 1, q.counter + 3, q);
         RK_LAUNCH("k_pass_c_finish", (k_pass_c_finish<TX, TY>), tiles, block, 0, s, args, q);

@@ -16,7 +16,7 @@ TILES = ((8, 8), (16, 8), (16, 16), (32, 8), (64, 1))
 _PASS_KERNELS = {
     "a": ("k_pass_a", "k_pass_a_ris", "k_trace_queue(pass_a)", "k_pass_a_finish"),
     "b": ("k_pass_b",),
-    "c": ("k_pack_surface", "k_pass_c_merge", "k_trace_queue(centre)",
+    "c": ("k_pack_surface", "k_pass_c_merge",
           "k_trace_queue(neighbour)", "k_pass_c_finish", "k_pass_c"),
 }
 

@@ -483,8 +483,9 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c(const PassCArgs a) {
 // atomic per wave, slots handed out by prefix sums over the wave, so a queue holds only live rays
 // in pixel-tile order.
 struct CQueues {
-    uint32_t *counter;     // [0] centre rays queued, [1] neighbour rays queued, [2] / [3] their tracer heads
-    float4 *c1, *c2;       // centre ray end points; c1.w carries the pixel index
+    uint32_t *counter;     // [0] centre rays recorded, [1] neighbour rays queued, [3] their tracer's head
+    float4 *c1, *c2;       // centre ray end points (traced in the merge, kept for the export);
+                           // c1.w carries the pixel index
     float4 *n1, *n2;       // neighbour ray end points; n1.w carries pixel | slot << 27
     uint32_t *origM;       // per pixel
     uint32_t *bits;        // per pixel: above-horizon neighbour mask (bits 0..15), flags below
@@ -662,20 +663,11 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_finish(const PassCArgs a, con
 // lanes busy when a wave's rays end at different depths. Chunks shrink as the queue drains
 // (guided self-scheduling: the rays left when the wave last took a chunk, over twice the grid's
 // waves, between kTraceMinChunk and kTraceMaxChunk), so that waves finish close together.
-// Results go to the pixel named by the ray's tag: a centre ray sets kBitCentreShadowed in
-// bits[pixel]; a neighbour ray sets bit `slot` of occ[pixel].
-#ifndef RK_TRACE_MAX_CHUNK
-#define RK_TRACE_MAX_CHUNK 128
-#endif
-#ifndef RK_TRACE_MIN_CHUNK
-#define RK_TRACE_MIN_CHUNK 32
-#endif
-#ifndef RK_TRACE_REFILL
-#define RK_TRACE_REFILL 32
-#endif
-constexpr uint32_t kTraceMaxChunk = RK_TRACE_MAX_CHUNK, kTraceMinChunk = RK_TRACE_MIN_CHUNK,
-                   kTraceRefill = RK_TRACE_REFILL;
-template <int STACK, bool CENTRE>
+// Results go to the pixel named by the ray's tag: a per-pixel ray (PER_PIXEL: Pass A's
+// visibility ray) sets kBitCentreShadowed in bits[pixel]; a Pass C neighbour ray sets bit `slot`
+// of occ[pixel], and is not traced when its pixel's centre ray was shadowed.
+constexpr uint32_t kTraceMaxChunk = 128, kTraceMinChunk = 32, kTraceRefill = 32;
+template <int STACK, bool PER_PIXEL>
 __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const float4 *__restrict__ p1,
                                                      const float4 *__restrict__ p2,
                                                      const uint32_t *__restrict__ count,
@@ -715,7 +707,7 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
                         const float4 a4 = p1[i];
                         tag = __float_as_uint(a4.w);
                         // a neighbour ray of a pixel whose centre ray was shadowed is not traced
-                        if (CENTRE || !(q.bits[tag & 0x07ffffffu] & kBitCentreShadowed)) {
+                        if (PER_PIXEL || !(q.bits[tag & 0x07ffffffu] & kBitCentreShadowed)) {
                             f3 o, d;
                             shadow_segment(xyz(a4), xyz(p2[i]), o, d);
                             wwalk_begin(w, o, d, sc.treeNested != 0, sc.maxCoord);
@@ -734,7 +726,7 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
         if (ray >= 0) fin = walk_step_w<STACK>(w, sc.wide, sc.nodes, sc.tris, stk, 256);
         if (fin) {
             if (w.hit) {
-                if (CENTRE) q.bits[tag] |= kBitCentreShadowed;
+                if (PER_PIXEL) q.bits[tag] |= kBitCentreShadowed;
                 else atomicOr(&q.occ[tag & 0x07ffffffu], 1u << (tag >> 27));
             }
             ray = -1;

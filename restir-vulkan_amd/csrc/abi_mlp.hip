@@ -5,6 +5,7 @@
 
 #include "ctx.h"
 #include "kernels.h"
+#include "mlp.h"
 
 using namespace rki;
 
@@ -12,8 +13,10 @@ namespace {
 
 int check_mlp(restir_ctx *ctx, const restir_mlp *m, const char *name) {
     if (!m) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "%s: mlp is NULL", name);
-    if (m->n_clusters != 64 && m->n_clusters != 128 && m->n_clusters != 256)
-        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "%s: n_clusters %u is not 64, 128 or 256", name, m->n_clusters);
+    const uint32_t K = m->n_clusters;
+    if (K != 64 && K != 128 && K != 256 && (K % rm::kSuper != 0 || K > static_cast<uint32_t>(rm::kMaxClusters)))
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "%s: n_clusters %u is not 64, 128, 256 or a multiple of %d up to %d",
+                    name, K, rm::kSuper, rm::kMaxClusters);
     if (!m->weights_bf16 || !m->bias)
         return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "%s: mlp weight buffers are NULL", name);
     return RESTIR_OK;
@@ -124,6 +127,9 @@ int restir_learned_candidates(restir_ctx *ctx, const restir_uniforms *u, const r
         return fail(ctx, RESTIR_E_SIZE_MISMATCH, "learned_candidates: mlp has %u clusters, tables %d",
                     m->n_clusters, ctx->nClusters);
     if (M < 1 || M > 16) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "learned_candidates: M=%u outside [1,16]", M);
+    if (m->n_clusters > 256 && M > static_cast<uint32_t>(rm::kWideCand))
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "learned_candidates: M=%u above %d with %u clusters", M,
+                    rm::kWideCand, m->n_clusters);
     if (!(mix > 0.0f && mix <= 1.0f))
         return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "learned_candidates: mix must be in (0, 1]");
     if (!candLight || !candQ) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "learned_candidates: NULL output");

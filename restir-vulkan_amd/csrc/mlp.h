@@ -20,6 +20,12 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int kIn = 128;      // feature width (114 used, zero padded)
 constexpr int kHidden = 128;  // hidden width
 constexpr int kFeatures = 114;
+// cluster counts: 64, 128 or 256 (k_propose holds a quad's numerators in LDS), or a multiple of
+// kSuper up to kMaxClusters (k_propose_wide streams them in column blocks of kSuper), with at most
+// kWideCand candidates per pixel
+constexpr int kSuper = 128;
+constexpr int kMaxClusters = 4096;
+constexpr int kWideCand = 8;
 
 __device__ __forceinline__ f32x16 mfma32(bf16x8 a, bf16x8 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);

@@ -379,22 +379,11 @@ __device__ __forceinline__ uint32_t quad_pixel(const rk::GView &g, uint32_t qx, 
     return y0 * g.W + x0;
 }
 
-// One workgroup of 512 lanes (8 waves) proposes for 256 pixel quads (2x2 pixels each): one MLP
-// evaluation per quad, each pixel drawing its own candidates from the quad's distribution with
-// its own random stream (q is that distribution, so RIS stays unbiased). LDS holds the
-// activations [256][136] bf16 and one layer's weights [<=256][136] bf16; after the last layer
-// the same LDS holds the fp16 softmax numerators [256][K + 4].
-template <int KC>
-__global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
-    __shared__ __align__(16) uint16_t lds[2 * kTilePx * kRow];
-    static_assert(kTilePx * (KC + 4) <= 2 * kTilePx * kRow, "softmax numerators fit the LDS tile");
-    uint16_t *act = lds;
-    uint16_t *wbuf = lds + kTilePx * kRow;
+// Features of the workgroup's 256 quads and the three hidden layers: leaves H3 in `act` (bf16
+// rows of kRow) and the first `outRows` rows of W4 staged in `wbuf`. Ends with a barrier.
+__device__ __forceinline__ void propose_hidden(const ProposeArgs &a, uint16_t *act, uint16_t *wbuf, uint32_t q0,
+                                               uint32_t QW, uint32_t QH, int outRows) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // one MLP row per 2x2 pixel quad: the quad's first surface pixel in the order (0,0), (1,0),
-    // (0,1), (1,1) stands for it (quad_pixel)
-    const uint32_t W = a.f.g.W, H = a.f.g.H, QW = (W + 1) / 2, QH = (H + 1) / 2;
-    const uint32_t q0 = blockIdx.x * kTilePx;
     // features (lanes 0..255) while lanes 256..511 stage W1
     if (tid < kTilePx) {
         const uint32_t qd = q0 + tid;
@@ -420,9 +409,28 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
                 const float v = acc[nb][i];
                 act[row * kRow + 32 * nb + (lane & 31)] = f2bf(v > 0.0f ? v : 0.0f);
             }
-        stage_weights(wbuf, a.wbf + w_off(L + 1), L + 1 < 3 ? kHidden : KC);
+        stage_weights(wbuf, a.wbf + w_off(L + 1), L + 1 < 3 ? kHidden : outRows);
         __syncthreads();
     }
+}
+
+// One workgroup of 512 lanes (8 waves) proposes for 256 pixel quads (2x2 pixels each): one MLP
+// evaluation per quad, each pixel drawing its own candidates from the quad's distribution with
+// its own random stream (q is that distribution, so RIS stays unbiased). LDS holds the
+// activations [256][136] bf16 and one layer's weights [<=256][136] bf16; after the last layer
+// the same LDS holds the fp16 softmax numerators [256][K + 4].
+template <int KC>
+__global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
+    __shared__ __align__(16) uint16_t lds[2 * kTilePx * kRow];
+    static_assert(kTilePx * (KC + 4) <= 2 * kTilePx * kRow, "softmax numerators fit the LDS tile");
+    uint16_t *act = lds;
+    uint16_t *wbuf = lds + kTilePx * kRow;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // one MLP row per 2x2 pixel quad: the quad's first surface pixel in the order (0,0), (1,0),
+    // (0,1), (1,1) stands for it (quad_pixel)
+    const uint32_t W = a.f.g.W, H = a.f.g.H, QW = (W + 1) / 2, QH = (H + 1) / 2;
+    const uint32_t q0 = blockIdx.x * kTilePx;
+    propose_hidden(a, act, wbuf, q0, QW, QH, KC);
     {
         f32x16 acc[KC / 32];
         layer_mfma<KC / 32>(act, wbuf, a.bias + b_off(3), wave, lane, acc);
@@ -597,6 +605,259 @@ __global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
         }
     }
     }  // pixels of this lane
+}
+
+// ----------------------------------------------------------- fused inference, many clusters --
+// k_propose keeps all K softmax numerators of a quad in LDS, which has room for K <= 256. For
+// larger K (a multiple of 128, at most 4096) the output layer streams through LDS in column
+// blocks of kSuper = 128 clusters, three times over the same H3 rows:
+//   1. the row maxima of the logits;
+//   2. the numerators e = fp16(exp(z - max)) and their row sum se, accumulated as in k_propose:
+//      blocks of 32 numerators summed in order, block sums added to se in cluster order;
+//   3. the same numerators again: each of the lane's pending draws t = u * se selects the first
+//      block whose running sum exceeds t (the last block if rounding leaves none) and the first
+//      cluster in it whose running sum from the block's start exceeds t (the block's last if
+//      none), exactly k_propose's pick_cluster over all K / 32 blocks; every candidate picks
+//      up the numerator of its cluster, whether drawn from the MLP or the cluster of a light
+//      drawn from the source table.
+// Then each pixel's random stream is replayed to draw the lights and write (l, q) as k_propose
+// does, with the same draws in the same order. At most kWideCand candidates per pixel.
+constexpr int kERow = kSuper + 4;  // fp16 numerators per LDS row
+
+__device__ __forceinline__ void super_block_sums(const uint2 *row, float (&bs)[kSuper / 32]) {
+#pragma unroll
+    for (int b = 0; b < kSuper / 32; ++b) {
+        float sum = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float4 f = unpack_h4(row[8 * b + q]);
+            sum += f.x;
+            sum += f.y;
+            sum += f.z;
+            sum += f.w;
+        }
+        bs[b] = sum;
+    }
+}
+
+__global__ __launch_bounds__(512) void k_propose_wide(const ProposeArgs a) {
+    __shared__ __align__(16) uint16_t lds[2 * kTilePx * kRow];
+    static_assert(kTilePx * kERow <= kTilePx * kRow, "a column block's numerators fit the weight buffer");
+    uint16_t *act = lds;
+    uint16_t *wbuf = lds + kTilePx * kRow;
+    __half *etile = reinterpret_cast<__half *>(wbuf);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t W = a.f.g.W, H = a.f.g.H, QW = (W + 1) / 2, QH = (H + 1) / 2;
+    const uint32_t q0 = blockIdx.x * kTilePx;
+    const int nSuper = a.K / kSuper;
+    const uint16_t *w4 = a.wbf + w_off(3);
+    const float *b4 = a.bias + b_off(3);
+    propose_hidden(a, act, wbuf, q0, QW, QH, kSuper);
+
+    // 1. row maxima: each lane covers 16 rows x 4 columns of every block, then the 32 lanes
+    // sharing a row reduce
+    float rmax[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) rmax[i] = -INFINITY;
+    for (int sb = 0; sb < nSuper; ++sb) {
+        if (sb > 0) {
+            stage_weights(wbuf, w4 + static_cast<size_t>(sb) * kSuper * kIn, kSuper);
+            __syncthreads();
+        }
+        f32x16 acc[4];
+        layer_mfma<4>(act, wbuf, b4 + sb * kSuper, wave, lane, acc);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) rmax[i] = fmaxf(rmax[i], acc[nb][i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+#pragma unroll
+        for (int off = 16; off > 0; off >>= 1) rmax[i] = fmaxf(rmax[i], __shfl_xor(rmax[i], off));
+
+    // one column block's numerators into the LDS tile (which overlays the staged weights)
+    auto numerators = [&](int sb) {
+        stage_weights(wbuf, w4 + static_cast<size_t>(sb) * kSuper * kIn, kSuper);
+        __syncthreads();
+        f32x16 acc[4];
+        layer_mfma<4>(act, wbuf, b4 + sb * kSuper, wave, lane, acc);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int row = 32 * wave + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb)
+                etile[row * kERow + 32 * nb + (lane & 31)] = __float2half(__expf(acc[nb][i] - rmax[i]));
+        }
+        __syncthreads();
+    };
+
+    // lane pair per quad from here on, each lane drawing for one row of its 2x2 pixels
+    const uint32_t qd = q0 + (tid >> 1);
+    const bool quadOk = qd < QW * QH;
+    const uint32_t qx = quadOk ? qd % QW : 0, qy = quadOk ? qd / QW : 0;
+    const uint32_t py = 2 * qy + static_cast<uint32_t>(tid & 1);
+    const uint2 *erow = reinterpret_cast<const uint2 *>(etile + (tid >> 1) * kERow);
+    const __half *erowh = etile + (tid >> 1) * kERow;
+
+    // 2. the row sum
+    float se = 0.0f;
+    for (int sb = 0; sb < nSuper; ++sb) {
+        numerators(sb);
+        float bs[kSuper / 32];
+        super_block_sums(erow, bs);
+#pragma unroll
+        for (int b = 0; b < kSuper / 32; ++b) se += bs[b];
+        __syncthreads();  // the next block's weights overwrite the tile
+    }
+    const float rse = 1.0f / se;
+
+    // the lane's candidates: slot k = s * kWideCand + j for pixel s of its row, candidate j.
+    // MLP draws carry their target t; source draws the cluster of the drawn light.
+    const int M = static_cast<int>(a.M);
+    const FeatArgs &f = a.f;
+    uint32_t pix[2];
+    bool pixOk[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const uint32_t px = 2 * qx + s;
+        pix[s] = py * W + px;
+        pixOk[s] = false;
+        if (quadOk && px < W && py < H) {
+            const uint32_t p = pix[s];
+            const f3 n = f3{f.g.nrm[p], f.g.nrm[f.g.P + p], f.g.nrm[2 * f.g.P + p]};
+            pixOk[s] = rt::dot(n, n) != 0.0f;  // background: Pass A draws no candidates there
+        }
+    }
+    float tk[2 * kWideCand], ek[2 * kWideCand];
+    int32_t ck[2 * kWideCand];
+    uint32_t pend = 0u;  // MLP draws whose cluster is not found yet
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        rt::Pcg rng = rt::pcg_seed(a.seed, pix[s]);
+#pragma unroll
+        for (int j = 0; j < kWideCand; ++j) {
+            const int k = s * kWideCand + j;
+            tk[k] = 0.0f;
+            ek[k] = 0.0f;
+            ck[k] = -1;
+            if (!pixOk[s] || j >= M) continue;
+            const bool src = rt::pcg_float(rng) < a.mix;
+            if (!src) {
+                tk[k] = rt::pcg_float(rng) * se;
+                ck[k] = a.K - 1;
+                pend |= 1u << k;
+            }
+            const float r1 = rt::pcg_float(rng), r2 = rt::pcg_float(rng);
+            if (src) {
+                int32_t c = static_cast<int32_t>(static_cast<float>(a.nAlias) * r1);
+                c = c < a.nAlias - 1 ? c : a.nAlias - 1;
+                const float4 t4 = a.alias[c];
+                const int32_t l = t4.x > r2 ? c : __float_as_int(t4.y);
+                ck[k] = a.lightCluster[l];
+            }
+        }
+    }
+
+    // 3. find the drawn clusters and pick up every candidate's numerator
+    float cum = 0.0f;
+    for (int sb = 0; sb < nSuper; ++sb) {
+        numerators(sb);
+        float bs[kSuper / 32];
+        super_block_sums(erow, bs);
+        float nx[kSuper / 32];
+        float c2 = cum;
+#pragma unroll
+        for (int b = 0; b < kSuper / 32; ++b) {
+            c2 += bs[b];
+            nx[b] = c2;
+        }
+        const bool last = sb == nSuper - 1;
+        const int c0 = sb * kSuper;
+#pragma unroll
+        for (int k = 0; k < 2 * kWideCand; ++k) {
+            if (pend & (1u << k)) {
+                if (tk[k] < nx[kSuper / 32 - 1] || last) {
+                    int blk = kSuper / 32 - 1;
+                    float base = nx[kSuper / 32 - 2];
+                    bool found = false;
+#pragma unroll
+                    for (int b = 0; b < kSuper / 32; ++b) {
+                        if (!found && (tk[k] < nx[b] || b == kSuper / 32 - 1)) {
+                            blk = b;
+                            base = b == 0 ? cum : nx[b - 1];
+                            found = true;
+                        }
+                    }
+                    int c = 32 * blk + 31;
+                    float run = base;
+                    bool hit = false;
+                    for (int q = 0; q < 8 && !hit; ++q) {
+                        const float4 e4 = unpack_h4(erow[8 * blk + q]);
+                        const float ev[4] = {e4.x, e4.y, e4.z, e4.w};
+#pragma unroll
+                        for (int kk = 0; kk < 4; ++kk) {
+                            run += ev[kk];
+                            if (!hit && tk[k] < run) {
+                                c = 32 * blk + 4 * q + kk;
+                                hit = true;
+                            }
+                        }
+                    }
+                    ck[k] = c0 + c;
+                    ek[k] = __half2float(erowh[c]);
+                    pend &= ~(1u << k);
+                }
+            } else if (ck[k] >= c0 && ck[k] < c0 + kSuper) {
+                ek[k] = __half2float(erowh[ck[k] - c0]);
+            }
+        }
+        cum = nx[kSuper / 32 - 1];
+        __syncthreads();
+    }
+
+    // replay each pixel's stream: the lights and their q, as k_propose draws them
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        if (!pixOk[s]) continue;
+        const uint32_t p = pix[s];
+        rt::Pcg rng = rt::pcg_seed(a.seed, p);
+#pragma unroll
+        for (int j = 0; j < kWideCand; ++j) {
+            if (j >= M) continue;
+            const int k = s * kWideCand + j;
+            const bool src = rt::pcg_float(rng) < a.mix;
+            if (!src) (void)rt::pcg_float(rng);
+            const float r1 = rt::pcg_float(rng), r2 = rt::pcg_float(rng);
+            int32_t l = -1;
+            float pw = 0.0f, ps = 0.0f;
+            if (src) {
+                int32_t c = static_cast<int32_t>(static_cast<float>(a.nAlias) * r1);
+                c = c < a.nAlias - 1 ? c : a.nAlias - 1;
+                const float4 t4 = a.alias[c];
+                const bool keep = t4.x > r2;
+                l = keep ? c : __float_as_int(t4.y);
+                ps = keep ? t4.z : t4.w;
+                pw = a.lightPWithin[l];
+            } else {
+                const int32_t cnt = a.clusterCount[ck[k]], beg = a.clusterBegin[ck[k]];
+                if (cnt > 0) {
+                    int32_t c = static_cast<int32_t>(static_cast<float>(cnt) * r1);
+                    c = c < cnt - 1 ? c : cnt - 1;
+                    const float4 t4 = a.clusterAlias[beg + c];
+                    const bool keep = t4.x > r2;
+                    l = a.clusterLights[beg + (keep ? c : __float_as_int(t4.y))];
+                    pw = keep ? t4.z : t4.w;
+                    ps = a.alias[l].z;
+                }
+            }
+            const float q = l >= 0 ? (1.0f - a.mix) * (ek[k] * rse) * pw + a.mix * ps : 0.0f;
+            a.candLight[static_cast<size_t>(j) * a.npx + p] = l;
+            a.candQ[static_cast<size_t>(j) * a.npx + p] = q;
+        }
+    }
 }
 
 }  // namespace rm
@@ -871,7 +1132,8 @@ hipError_t mlp_propose(const restir_mlp &m, const GView &g, f3 cam, const float 
     switch (a.K) {
         case 64: RK_LAUNCH("k_propose", (rm::k_propose<64>), grid, block, 0, s, a); break;
         case 128: RK_LAUNCH("k_propose", (rm::k_propose<128>), grid, block, 0, s, a); break;
-        default: RK_LAUNCH("k_propose", (rm::k_propose<256>), grid, block, 0, s, a); break;
+        case 256: RK_LAUNCH("k_propose", (rm::k_propose<256>), grid, block, 0, s, a); break;
+        default: RK_LAUNCH("k_propose_wide", rm::k_propose_wide, grid, block, 0, s, a); break;
     }
     return hipGetLastError();
 }

@@ -96,6 +96,10 @@ def parse():
     ap.add_argument("--no-builds", action="store_true", help="skip the scene-builder measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=8, help="1080p frames of the CPU-oracle sample")
+    ap.add_argument("--workload", choices=("1080p", "stress4k"), default="1080p",
+                    help="stress4k: the 3840x2160 stress workload's learned pipeline (2,333,058 "
+                         "triangles, 1M point lights in 4,096 clusters, 4,096-way proposal); "
+                         "prints its own JSON line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (x3), whose hardware byte counters "
                          "calibrate the traffic counters of the other kernels")
@@ -206,6 +210,8 @@ def main():
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.workload == "stress4k":
+        return stress_4k(torch, args)
     from restir_amd import abi, scenes
     from restir_amd.abi import Context
     from restir_amd.pipeline import SceneOnDevice
@@ -623,6 +629,66 @@ def cpu_baseline(fr, scene, sd, args, rays):
                       f"{args.spatial_passes}x Pass C + shading) at {W}x{H}: {frame_s * args.cpu_frames:.1f} s; "
                       f"traversal of {len(p1)} Pass C neighbour rays x{reps}: {trace_s * reps:.1f} s",
             "mrays_per_s": round(len(p1) / trace_s / 1e6, 2)}
+
+
+def stress_4k(torch, args):
+    """The 4K stress workload's learned pipeline on one GPU: a 1024x1024-vertex heightfield plus
+    20,000 boxes (2,333,058 triangles, tree built on the device), 1M point lights with log-uniform
+    intensity [1e-3, 10] clustered into 4,096 groups (nearest-centroid step on the GPU), 3840x2160
+    frames of the orbiting camera with the 4,096-way MLP proposal (M = 8; inference, Pass A, two
+    Pass C, shading and one 4096-sample Adam step per frame) and, for comparison, plain RIS with
+    M = 32. Not part of it here: the 4 camera paths x 32 frames in one graph and fp16 G-buffer
+    normals. Prints one JSON line with the build times, ms/frame of both pipelines and the
+    per-kernel times of the learned one."""
+    from restir_amd import abi, scenes
+    from restir_amd.abi import Context
+    from restir_amd.learned import LearnedProposal
+    from restir_amd.pipeline import SceneOnDevice
+
+    W, H, K, WU = 3840, 2160, args.steps, args.warmup
+    npx = W * H
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream()
+    tx, ty = (int(v) for v in args.tile.split("x"))
+    launch = abi.Launch(tx, ty)
+    t0 = time.time()
+    scene = scenes.terrain(5, grid=1024, n_boxes=20000, n_point_lights=1_000_000, light_intensity=(1e-3, 10.0))
+    log(f"stress scene: {scene.n_tris} triangles, {len(scene.point_lights)} point lights ({time.time() - t0:.1f}s)")
+    ctx = Context(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sd = SceneOnDevice(ctx, scene, dev, build="device")
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    log(f"device tree (depth {sd.depth}) + host alias table over 1M lights: {setup_s:.2f} s")
+    t0 = time.perf_counter()
+    lp = LearnedProposal(ctx, scene, k=4096, seed=7, iterations=16, device_clustering=True)
+    torch.cuda.synchronize()
+    cluster_s = time.perf_counter() - t0
+    log(f"4,096 clusters over 1M lights (16 Lloyd steps, nearest step on the GPU) + MLP init: {cluster_s:.2f} s")
+    nframes = WU + K
+    cams = [scenes.orbit_camera(scene.camera, f) for f in range(nframes)]
+    G = [torch.zeros(abi.GBUFFER_PLANES * npx, dtype=torch.float32, device=dev) for _ in range(nframes + 1)]
+    for f in range(nframes):
+        ctx.primary_gbuffer(cams[f], sd.d_tri_material, sd.d_materials, len(scene.materials), G[f + 1],
+                            W, H, stream=stream)
+    torch.cuda.synchronize()
+    fr = Frames(torch, ctx, scene, cams, G, W, H, args, launch, stream)
+    out = {"workload": "stress4k", "width": W, "height": H, "triangles": int(scene.n_tris),
+           "point_lights": int(len(scene.point_lights)), "clusters": 4096, "steps": K, "warmup": WU,
+           "tree_and_alias_build_s": round(setup_s, 3), "clustering_s": round(cluster_s, 3)}
+    for name, M, learned in (("plain", args.M, None), ("learned", args.learned_M, lp)):
+        for t in fr.res:
+            t.zero_()
+        for f in range(WU):
+            fr.run(f, M, learned=learned)
+        ms, kernels = fr.timed(WU, K, M, learned=learned)
+        log(f"{name} M={M}: {ms:.2f} ms/frame; " + ", ".join(
+            f"{k} {v['ms_per_frame']:.3f}" for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_frame"])[:8]))
+        out[f"{name}_M"] = M
+        out[f"{name}_ms_per_frame"] = round(ms, 3)
+        out[f"{name}_kernels_ms_per_frame"] = {k: round(v["ms_per_frame"], 4) for k, v in kernels.items()}
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -293,6 +293,19 @@ int restir_build_light_clusters(const restir_point_light *points, uint32_t n, ui
                                 int32_t *cluster_count, int32_t *cluster_lights,
                                 restir_alias_table_column *cluster_alias, float *centroids);
 
+/* restir_build_light_clusters with its nearest-centroid step (n x k distances per Lloyd
+ * iteration) on ctx's device; the rest of the build is the host build's own code, so the outputs
+ * are byte-identical to restir_build_light_clusters for the same arguments. Same host inputs and
+ * outputs; synchronous; allocates and frees its device buffers (16 B per light and per cluster,
+ * 4 B per light). For light counts where the host step takes minutes (1M lights into 4,096
+ * clusters). No reference counterpart. */
+int restir_build_light_clusters_device(restir_ctx *ctx, const restir_point_light *points, uint32_t n,
+                                       uint32_t k, uint64_t seed, uint32_t iterations,
+                                       int32_t *light_cluster, float *light_pwithin,
+                                       int32_t *cluster_begin, int32_t *cluster_count,
+                                       int32_t *cluster_lights, restir_alias_table_column *cluster_alias,
+                                       float *centroids);
+
 /* ----------------------------------------------------------------------------------------------
  * Learned light proposal (no reference counterpart). A 128-128-128-128-K ReLU MLP over per-pixel
  * features (position in the scene box, normal, view direction; raw plus 6 sin/cos octaves = 114,
@@ -300,7 +313,9 @@ int restir_build_light_clusters(const restir_point_light *points, uint32_t n, ui
  *   q(l | x) = (1 - mix) * pi_x(cluster(l)) * p_within(l) + mix * p_src(l)
  * (p_src: the reference's alias-table distribution), so RIS with weights p_hat/q stays unbiased;
  * mix = 0 is rejected. Weights are bf16 with an fp32 master copy; all products run on MFMA with
- * fp32 accumulation. K is 64, 128 or 256. All pointers in restir_mlp are device pointers.
+ * fp32 accumulation. K is 64, 128 or 256 (a quad's K numerators held in LDS), or a multiple of
+ * 128 up to 4096 (the output layer streamed through LDS in 128-cluster blocks; at most 8
+ * candidates per pixel). All pointers in restir_mlp are device pointers.
  * -------------------------------------------------------------------------------------------- */
 typedef struct restir_mlp {
     uint32_t n_clusters;    /* K */

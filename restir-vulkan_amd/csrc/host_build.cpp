@@ -2,6 +2,8 @@
 // reference's exact buffer contents:
 //   restir_build_aabb_tree   <- AabbTree::build      (src/aabbTreeBuilder.cpp:52-214)
 //   restir_build_alias_table <- createAliasTable      (src/misc.cpp:418-497)
+//   restir_build_light_clusters (k-means light clusters of the learned proposal; its nearest-
+//   centroid step also runs on the GPU, cluster_build.hip)
 // Single-threaded like the reference; float arithmetic in the reference's order (no contraction).
 #include <cfloat>
 #include <cstddef>
@@ -12,6 +14,7 @@
 #include <vector>
 
 #include "../../include/restir_abi.h"
+#include "clusters.h"
 
 namespace {
 
@@ -279,12 +282,12 @@ inline uint32_t nearest(const float *c, uint32_t k, float x, float y, float z) {
 
 }  // namespace
 
-extern "C" int restir_build_light_clusters(const restir_point_light *points, uint32_t n, uint32_t k,
-                                           uint64_t seed, uint32_t iterations, int32_t *lightCluster,
-                                           float *lightPWithin, int32_t *clusterBegin,
-                                           int32_t *clusterCount, int32_t *clusterLights,
-                                           restir_alias_table_column *clusterAlias,
-                                           float *centroidsOut) {
+namespace rki {
+
+int build_light_clusters(const restir_point_light *points, uint32_t n, uint32_t k, uint64_t seed,
+                         uint32_t iterations, const NearestStep &nearestStep, int32_t *lightCluster,
+                         float *lightPWithin, int32_t *clusterBegin, int32_t *clusterCount,
+                         int32_t *clusterLights, restir_alias_table_column *clusterAlias, float *centroidsOut) {
     if (!points || !lightCluster || !lightPWithin || !clusterBegin || !clusterCount || !clusterLights ||
         !clusterAlias || k == 0 || k > n)
         return RESTIR_E_INVALID_ARGUMENT;
@@ -302,7 +305,7 @@ extern "C" int restir_build_light_clusters(const restir_point_light *points, uin
     std::vector<double> sum(3 * static_cast<size_t>(k));
     std::vector<uint32_t> cnt(k);
     for (uint32_t it = 0; it <= iterations; ++it) {
-        for (uint32_t l = 0; l < n; ++l) assign[l] = nearest(c.data(), k, points[l].pos[0], points[l].pos[1], points[l].pos[2]);
+        if (int r = nearestStep(c.data(), assign.data())) return r;
         if (it == iterations) break;  // the final assignment uses the final centroids
         std::fill(sum.begin(), sum.end(), 0.0);
         std::fill(cnt.begin(), cnt.end(), 0u);
@@ -340,6 +343,22 @@ extern "C" int restir_build_light_clusters(const restir_point_light *points, uin
     }
     if (centroidsOut) std::copy(c.begin(), c.end(), centroidsOut);
     return RESTIR_OK;
+}
+
+}  // namespace rki
+
+extern "C" int restir_build_light_clusters(const restir_point_light *points, uint32_t n, uint32_t k,
+                                           uint64_t seed, uint32_t iterations, int32_t *lightCluster,
+                                           float *lightPWithin, int32_t *clusterBegin,
+                                           int32_t *clusterCount, int32_t *clusterLights,
+                                           restir_alias_table_column *clusterAlias,
+                                           float *centroidsOut) {
+    const rki::NearestStep host = [&](const float *c, uint32_t *assign) {
+        for (uint32_t l = 0; l < n; ++l) assign[l] = nearest(c, k, points[l].pos[0], points[l].pos[1], points[l].pos[2]);
+        return 0;
+    };
+    return rki::build_light_clusters(points, n, k, seed, iterations, host, lightCluster, lightPWithin, clusterBegin,
+                                     clusterCount, clusterLights, clusterAlias, centroidsOut);
 }
 
 extern "C" int restir_build_alias_table(const restir_point_light *points, uint32_t nPoint,

@@ -166,6 +166,8 @@ SIGNATURES = {
     "restir_build_alias_table_mixed": (_i32, [_vp, _u32, _vp, _u32, _vp]),
     "restir_build_light_clusters": (_i32, [_vp, _u32, _u32, C.c_uint64, _u32, _vp, _vp, _vp, _vp,
                                            _vp, _vp, _vp]),
+    "restir_build_light_clusters_device": (_i32, [_vp, _vp, _u32, _u32, C.c_uint64, _u32, _vp, _vp, _vp,
+                                                  _vp, _vp, _vp, _vp]),
     "restir_primary_gbuffer": (_i32, [_vp, C.POINTER(Camera), _vp, _vp, _u32, _vp, _vp, _vp, _vp,
                                       _vp, _u64, _u32, _u32, _vp]),
     "restir_camera_projection_view": (_i32, [C.POINTER(Camera), _f32, C.POINTER(_f32)]),
@@ -324,9 +326,10 @@ def union_light_records(points: np.ndarray, tri_lights: np.ndarray) -> np.ndarra
 
 
 class LightClusters:
-    """Host arrays of restir_build_light_clusters."""
+    """Host arrays of restir_build_light_clusters (restir_build_light_clusters_device, whose
+    nearest-centroid step runs on the context's GPU, when `ctx` is given: same bytes)."""
 
-    def __init__(self, points: np.ndarray, k: int, seed: int = 7, iterations: int = 16):
+    def __init__(self, points: np.ndarray, k: int, seed: int = 7, iterations: int = 16, ctx=None):
         lib = load_library()
         n = len(points)
         pts = np.ascontiguousarray(points, dtype=POINT_LIGHT_DTYPE)
@@ -338,12 +341,16 @@ class LightClusters:
         self.lights = np.zeros(n, dtype=np.int32)
         self.alias = np.zeros(n, dtype=ALIAS_DTYPE)
         self.centroids = np.zeros((k, 3), dtype=np.float32)
-        rc = lib.restir_build_light_clusters(
-            pts.ctypes.data, n, k, seed, iterations, self.light_cluster.ctypes.data,
-            self.light_pwithin.ctypes.data, self.begin.ctypes.data, self.count.ctypes.data,
-            self.lights.ctypes.data, self.alias.ctypes.data, self.centroids.ctypes.data)
-        if rc != 0:
-            raise RestirError(rc, "restir_build_light_clusters", f"n={n} k={k}")
+        outs = (self.light_cluster.ctypes.data, self.light_pwithin.ctypes.data, self.begin.ctypes.data,
+                self.count.ctypes.data, self.lights.ctypes.data, self.alias.ctypes.data,
+                self.centroids.ctypes.data)
+        if ctx is None:
+            rc = lib.restir_build_light_clusters(pts.ctypes.data, n, k, seed, iterations, *outs)
+            if rc != 0:
+                raise RestirError(rc, "restir_build_light_clusters", f"n={n} k={k}")
+        else:
+            ctx._check(lib.restir_build_light_clusters_device(ctx.h, pts.ctypes.data, n, k, seed, iterations, *outs),
+                       "restir_build_light_clusters_device")
         lum = pts["color_lum"][:, 3].astype(np.float64)
         power = np.bincount(self.light_cluster, weights=lum, minlength=k)
         self.power_share = (power / power.sum()).astype(np.float32)

@@ -1,5 +1,5 @@
 """Device state of the learned light proposal: k-means light clusters with per-cluster alias
-tables (built by the library's host builder), the MLP parameters (bf16 working copy, fp32 master
+tables (built by the library's host builder, or with its nearest-centroid step on the GPU), the MLP parameters (bf16 working copy, fp32 master
 copy, Adam moments) and per-resolution candidate buffers. All compute runs in librestir_amd.so.
 """
 from __future__ import annotations
@@ -14,7 +14,7 @@ from .abi import LightClusters, MlpStruct, mlp_sizes
 
 class LearnedProposal:
     def __init__(self, ctx: abi.Context, scene, k: int = 256, seed: int = 7, iterations: int = 16,
-                 mix: float = 0.1, device="cuda"):
+                 mix: float = 0.1, device="cuda", device_clustering: bool = False):
         import torch
         self.torch = torch
         self.ctx, self.k, self.mix = ctx, k, mix
@@ -22,7 +22,7 @@ class LearnedProposal:
         # lights, the triangle lights follow the point lights as (centroid, luminance * area)
         union = abi.uses_union(scene.point_lights, scene.tri_lights)
         records = abi.union_light_records(scene.point_lights, scene.tri_lights) if union else scene.point_lights
-        cl = LightClusters(records, k, seed=seed, iterations=iterations)
+        cl = LightClusters(records, k, seed=seed, iterations=iterations, ctx=ctx if device_clustering else None)
         self.clusters = cl
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
         self.d_light_cluster = t(cl.light_cluster)

@@ -373,8 +373,8 @@ int restir_mlp_train(restir_ctx *ctx, restir_mlp *mlp, const restir_uniforms *u,
                      float lr, uint32_t seed, float *loss_out, void *stream);
 
 /* Building blocks exposed for checks: features of listed pixels (bf16, n x 128); logits of the
- * MLP (fp32, n x K) for bf16 features (n a multiple of 64); loss and gradients (fp32, parameter
- * layouts) for features, targets and a 0/1 mask. */
+ * MLP (fp32, n x K) for bf16 features (n a multiple of 64, features and weights_bf16 16-byte
+ * aligned); loss and gradients (fp32, parameter layouts) for features, targets and a 0/1 mask. */
 int restir_mlp_features(restir_ctx *ctx, const restir_gbuffer *g, const float camera_pos[3],
                         const float scene_lo[3], const float scene_hi[3], const uint32_t *pixels,
                         uint32_t n, uint16_t *features, void *stream);

@@ -19,6 +19,8 @@ int check_mlp(restir_ctx *ctx, const restir_mlp *m, const char *name) {
                     name, K, rm::kSuper, rm::kMaxClusters);
     if (!m->weights_bf16 || !m->bias)
         return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "%s: mlp weight buffers are NULL", name);
+    if (reinterpret_cast<uintptr_t>(m->weights_bf16) % 16 != 0)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "%s: weights_bf16 is not 16-byte aligned", name);
     return RESTIR_OK;
 }
 
@@ -221,6 +223,8 @@ int restir_mlp_forward(restir_ctx *ctx, const restir_mlp *m, const uint16_t *X, 
     if (!ctx) return RESTIR_E_INVALID_ARGUMENT;
     if (int r = check_mlp(ctx, m, "mlp_forward")) return r;
     if (!X || !logits) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "mlp_forward: NULL argument");
+    if (reinterpret_cast<uintptr_t>(X) % 16 != 0)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "mlp_forward: features are not 16-byte aligned");
     if (n == 0 || n % 64 != 0 || n > 65536)
         return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "mlp_forward: n=%u (multiple of 64, <= 65536)", n);
     if (int r = use_device(ctx)) return r;
@@ -236,6 +240,8 @@ int restir_mlp_loss_grad(restir_ctx *ctx, const restir_mlp *m, const uint16_t *X
     if (int r = check_mlp(ctx, m, "mlp_loss_grad")) return r;
     if (!X || !target || !mask || !gw || !gb || !loss)
         return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "mlp_loss_grad: NULL argument");
+    if (reinterpret_cast<uintptr_t>(X) % 16 != 0)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "mlp_loss_grad: features are not 16-byte aligned");
     if (n == 0 || n % 64 != 0 || n > 65536)
         return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "mlp_loss_grad: n=%u (multiple of 64, <= 65536)", n);
     if (int r = use_device(ctx)) return r;

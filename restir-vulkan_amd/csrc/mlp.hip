@@ -108,7 +108,8 @@ struct GemmArgs {
 
 // 64x64 output tile per 256-lane workgroup; each wave owns a 32x32 quarter. K advances in steps
 // of 32 through LDS tiles As[64][32] and Bt[64][32] (B staged transposed so every fragment is
-// eight consecutive bf16). Requires M, N multiples of 64 and K a multiple of 32.
+// eight consecutive bf16). Requires M, N multiples of 64, K a multiple of 32 and 16-byte aligned
+// operands whose strides are multiples of 8 elements.
 __global__ __launch_bounds__(256) void k_gemm(const GemmArgs g) {
     __shared__ __align__(16) uint16_t As[64][40];
     __shared__ __align__(16) uint16_t Bt[64][40];
@@ -122,12 +123,51 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmArgs g) {
     const bool aRowFast = g.sak != 1, bRowFast = g.sbn == 1;
     f32x16 acc;
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+    // 16-byte loads where one operand index is contiguous (every launch here): a lane loads 8
+    // consecutive k of one row, or 8 consecutive rows of one k and scatters them to LDS
+    const bool vecA = g.sak == 1 || g.sam == 1, vecB = g.sbk == 1 || g.sbn == 1;
     for (int k0 = kBeg; k0 < kEnd; k0 += 32) {
-        for (int e = tid; e < 64 * 32; e += 256) {
-            const int ra = aRowFast ? (e & 63) : (e >> 5), ca = aRowFast ? (e >> 6) : (e & 31);
-            As[ra][ca] = g.A[(m0 + ra) * g.sam + (k0 + ca) * g.sak];
-            const int rb = bRowFast ? (e & 63) : (e >> 5), cb = bRowFast ? (e >> 6) : (e & 31);
-            Bt[rb][cb] = g.B[(k0 + cb) * g.sbk + (n0 + rb) * g.sbn];
+        if (vecA) {
+            if (g.sak == 1) {
+                const int r = tid >> 2, c = (tid & 3) * 8;
+                *reinterpret_cast<uint4 *>(&As[r][c]) = *reinterpret_cast<const uint4 *>(&g.A[(m0 + r) * g.sam + k0 + c]);
+            } else {
+                const int kk = tid >> 3, r = (tid & 7) * 8;
+                const uint4 v = *reinterpret_cast<const uint4 *>(&g.A[(k0 + kk) * g.sak + m0 + r]);
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    As[r + 2 * i][kk] = static_cast<uint16_t>(w[i] & 0xffffu);
+                    As[r + 2 * i + 1][kk] = static_cast<uint16_t>(w[i] >> 16);
+                }
+            }
+        }
+        if (vecB) {
+            if (g.sbk == 1) {
+                const int r = tid >> 2, c = (tid & 3) * 8;
+                *reinterpret_cast<uint4 *>(&Bt[r][c]) = *reinterpret_cast<const uint4 *>(&g.B[k0 + c + (n0 + r) * g.sbn]);
+            } else {
+                const int kk = tid >> 3, r = (tid & 7) * 8;
+                const uint4 v = *reinterpret_cast<const uint4 *>(&g.B[(k0 + kk) * g.sbk + n0 + r]);
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    Bt[r + 2 * i][kk] = static_cast<uint16_t>(w[i] & 0xffffu);
+                    Bt[r + 2 * i + 1][kk] = static_cast<uint16_t>(w[i] >> 16);
+                }
+            }
+        }
+        if (!vecA || !vecB) {
+            for (int e = tid; e < 64 * 32; e += 256) {
+                if (!vecA) {
+                    const int ra = aRowFast ? (e & 63) : (e >> 5), ca = aRowFast ? (e >> 6) : (e & 31);
+                    As[ra][ca] = g.A[(m0 + ra) * g.sam + (k0 + ca) * g.sak];
+                }
+                if (!vecB) {
+                    const int rb = bRowFast ? (e & 63) : (e >> 5), cb = bRowFast ? (e >> 6) : (e & 31);
+                    Bt[rb][cb] = g.B[(k0 + cb) * g.sbk + (n0 + rb) * g.sbn];
+                }
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -214,14 +254,26 @@ __global__ __launch_bounds__(256) void k_colsum_partial(const uint16_t *G, int c
     if (q == 0) part[static_cast<size_t>(blockIdx.y) * cols + c] = ((red[0][c & 63] + red[1][c & 63]) + red[2][c & 63]) + red[3][c & 63];
 }
 
-// out[i] = sum over s < nParts of part[s * count + i], in order of s
-__global__ __launch_bounds__(256) void k_sum_partials(const float *part, int nParts, int64_t count,
-                                                      float *out) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-    if (i >= count) return;
+// Up to 8 reductions in one launch: for segment j, out_j[i] = sum over s < nParts_j of
+// part_j[s * count_j + i], in order of s. Lane t of the grid takes element t of the segments laid
+// end to end.
+struct SumSegments {
+    const float *part[8];
+    float *out[8];
+    int nParts[8];
+    int64_t count[8];
+    int n;
+};
+__global__ __launch_bounds__(256) void k_sum_partials(const SumSegments a) {
+    int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    int j = 0;
+    while (j < a.n && i >= a.count[j]) i -= a.count[j++];
+    if (j >= a.n) return;
+    const float *part = a.part[j];
+    const int64_t count = a.count[j];
     float sum = 0.0f;
-    for (int s = 0; s < nParts; ++s) sum += part[s * count + i];
-    out[i] = sum;
+    for (int s = 0; s < a.nParts[j]; ++s) sum += part[s * count + i];
+    a.out[j][i] = sum;
 }
 
 // --------------------------------------------------------------------------------- Adam -----
@@ -329,7 +381,7 @@ struct ProposeArgs {
     float *candQ;
 };
 
-constexpr int kTilePx = 256;                 // pixels per workgroup
+constexpr int kTilePx = 256;                 // pixel quads per workgroup of k_propose_wide
 constexpr int kRow = kIn + 8;                // padded LDS row (bf16) against bank conflicts
 
 __device__ __forceinline__ void stage_weights(uint16_t *wbuf, const uint16_t *src, int rows) {
@@ -379,19 +431,21 @@ __device__ __forceinline__ uint32_t quad_pixel(const rk::GView &g, uint32_t qx, 
     return y0 * g.W + x0;
 }
 
-// Features of the workgroup's 256 quads and the three hidden layers: leaves H3 in `act` (bf16
-// rows of kRow) and the first `outRows` rows of W4 staged in `wbuf`. Ends with a barrier.
+// Features of the workgroup's TILE quads (2 * TILE lanes, a wave per 32 quads) and the three
+// hidden layers: leaves H3 in `act` (bf16 rows of kRow) and the first `outRows` rows of W4 staged
+// in `wbuf`. Ends with a barrier.
+template <int TILE>
 __device__ __forceinline__ void propose_hidden(const ProposeArgs &a, uint16_t *act, uint16_t *wbuf, uint32_t q0,
                                                uint32_t QW, uint32_t QH, int outRows) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // features (lanes 0..255) while lanes 256..511 stage W1
-    if (tid < kTilePx) {
+    // features (lanes 0..TILE-1) while the other TILE lanes stage W1
+    if (tid < TILE) {
         const uint32_t qd = q0 + tid;
         uint16_t *row = act + tid * kRow;
         if (qd < QW * QH) pixel_features(a.f, quad_pixel(a.f.g, qd % QW, qd / QW), row);
         else for (int k = 0; k < kIn; ++k) row[k] = 0;
     } else {
-        for (int e = tid - kTilePx; e < kHidden * (kIn / 8); e += kTilePx) {
+        for (int e = tid - TILE; e < kHidden * (kIn / 8); e += TILE) {
             const int r = e / (kIn / 8), c = (e % (kIn / 8)) * 8;
             *reinterpret_cast<uint4 *>(&wbuf[r * kRow + c]) = *reinterpret_cast<const uint4 *>(&a.wbf[r * kIn + c]);
         }
@@ -414,26 +468,45 @@ __device__ __forceinline__ void propose_hidden(const ProposeArgs &a, uint16_t *a
     }
 }
 
-// One workgroup of 512 lanes (8 waves) proposes for 256 pixel quads (2x2 pixels each): one MLP
-// evaluation per quad, each pixel drawing its own candidates from the quad's distribution with
-// its own random stream (q is that distribution, so RIS stays unbiased). LDS holds the
-// activations [256][136] bf16 and one layer's weights [<=256][136] bf16; after the last layer
-// the same LDS holds the fp16 softmax numerators [256][K + 4].
+// One workgroup of 256 lanes (4 waves) proposes for kProposeTile = 128 pixel quads (2x2 pixels
+// each): one MLP evaluation per quad, each pixel drawing its own candidates from the quad's
+// distribution with its own random stream (q is that distribution, so RIS stays unbiased). LDS
+// holds the activations [128][136] bf16 and 128 rows of one layer's weights (W4 of K = 256 in two
+// halves); after the last layer the same LDS holds the fp16 softmax numerators [128][K + 4].
+// 69.6 KB of LDS lets two workgroups share a CU, so one's barriers and loads overlap the other's
+// work.
+constexpr int kProposeTile = 128;
 template <int KC>
-__global__ __launch_bounds__(512) void k_propose(const ProposeArgs a) {
-    __shared__ __align__(16) uint16_t lds[2 * kTilePx * kRow];
-    static_assert(kTilePx * (KC + 4) <= 2 * kTilePx * kRow, "softmax numerators fit the LDS tile");
+__global__ __launch_bounds__(2 * kProposeTile) void k_propose(const ProposeArgs a) {
+    constexpr int T = kProposeTile;
+    __shared__ __align__(16) uint16_t lds[2 * T * kRow];
+    static_assert(T * (KC + 4) <= 2 * T * kRow, "softmax numerators fit the LDS tile");
     uint16_t *act = lds;
-    uint16_t *wbuf = lds + kTilePx * kRow;
+    uint16_t *wbuf = lds + T * kRow;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // one MLP row per 2x2 pixel quad: the quad's first surface pixel in the order (0,0), (1,0),
     // (0,1), (1,1) stands for it (quad_pixel)
     const uint32_t W = a.f.g.W, H = a.f.g.H, QW = (W + 1) / 2, QH = (H + 1) / 2;
-    const uint32_t q0 = blockIdx.x * kTilePx;
-    propose_hidden(a, act, wbuf, q0, QW, QH, KC);
+    const uint32_t q0 = blockIdx.x * T;
+    propose_hidden<T>(a, act, wbuf, q0, QW, QH, KC < 128 ? KC : 128);
     {
         f32x16 acc[KC / 32];
-        layer_mfma<KC / 32>(act, wbuf, a.bias + b_off(3), wave, lane, acc);
+        if constexpr (KC <= 128) {
+            layer_mfma<KC / 32>(act, wbuf, a.bias + b_off(3), wave, lane, acc);
+        } else {
+            static_assert(KC == 256, "W4 in two halves of 128 rows");
+            f32x16 lo[4], hi[4];
+            layer_mfma<4>(act, wbuf, a.bias + b_off(3), wave, lane, lo);
+            __syncthreads();
+            stage_weights(wbuf, a.wbf + w_off(3) + 128 * kIn, 128);
+            __syncthreads();
+            layer_mfma<4>(act, wbuf, a.bias + b_off(3) + 128, wave, lane, hi);
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                acc[nb] = lo[nb];
+                acc[nb + 4] = hi[nb];
+            }
+        }
         __syncthreads();
         // softmax numerators e = exp(z - rowmax) as fp16 rows of KC + 4 halves: the row maximum
         // is reduced across the 32 lanes holding a row's columns; the padding keeps rows 8-byte
@@ -643,6 +716,7 @@ __device__ __forceinline__ void super_block_sums(const uint2 *row, float (&bs)[k
 __global__ __launch_bounds__(512) void k_propose_wide(const ProposeArgs a) {
     __shared__ __align__(16) uint16_t lds[2 * kTilePx * kRow];
     static_assert(kTilePx * kERow <= kTilePx * kRow, "a column block's numerators fit the weight buffer");
+    static_assert(kTilePx == 256, "512 lanes, two per quad");
     uint16_t *act = lds;
     uint16_t *wbuf = lds + kTilePx * kRow;
     __half *etile = reinterpret_cast<__half *>(wbuf);
@@ -652,7 +726,7 @@ __global__ __launch_bounds__(512) void k_propose_wide(const ProposeArgs a) {
     const int nSuper = a.K / kSuper;
     const uint16_t *w4 = a.wbf + w_off(3);
     const float *b4 = a.bias + b_off(3);
-    propose_hidden(a, act, wbuf, q0, QW, QH, kSuper);
+    propose_hidden<kTilePx>(a, act, wbuf, q0, QW, QH, kSuper);
 
     // 1. row maxima: each lane covers 16 rows x 4 columns of every block, then the 32 lanes
     // sharing a row reduce
@@ -889,11 +963,14 @@ struct MlpScratch {
     int32_t *target;
     uint32_t *mask, *pixels, *nValid;
 };
-// split-K of the weight-gradient GEMMs (reduction over the n samples): at most kMaxSplit parts
+// split-K of the weight-gradient GEMMs (reduction over the n samples): at most kMaxSplit parts.
+// Every layer keeps its own partials (weights: kMaxSplit x rows x kIn, biases: n/64 chunks x rows)
+// so that one launch reduces them all after the backward pass.
 constexpr int kMaxSplit = 16;
-static size_t part_floats(int K) {
-    const size_t rows = static_cast<size_t>(K > kHidden ? K : kHidden);
-    return static_cast<size_t>(kMaxSplit) * rows * kIn + 64 * rows;
+static size_t rows_before(int l) { return static_cast<size_t>(l) * kHidden; }  // output rows of layers < l
+static size_t part_floats(int K, uint32_t n) {
+    const size_t rows = 3 * static_cast<size_t>(kHidden) + static_cast<size_t>(K);
+    return static_cast<size_t>(kMaxSplit) * rows * kIn + static_cast<size_t>(n / 64) * rows;
 }
 // parts for an M x N (x 64-tiles) gradient over n samples: about 256 workgroups, each K range a
 // multiple of 32
@@ -906,7 +983,7 @@ static int split_for(int M, int N, int n) {
 size_t mlp_scratch_bytes(uint32_t n, int K) {
     const size_t nn = n, kk = static_cast<size_t>(K);
     return 16 * 256 + 2 * (nn * kIn * 6 + nn * kk) + 4 * (nn * kk + 3 * nn) +
-           4 * (rm::n_weights(K) + rm::n_bias(K)) + 4 * part_floats(K);
+           4 * (rm::n_weights(K) + rm::n_bias(K)) + 4 * part_floats(K, n);
 }
 static MlpScratch carve_mlp(void *base, uint32_t n, int K) {
     char *p = static_cast<char *>(base);
@@ -931,7 +1008,7 @@ static MlpScratch carve_mlp(void *base, uint32_t n, int K) {
     m.loss = reinterpret_cast<float *>(take(256));
     m.gw = reinterpret_cast<float *>(take(4 * rm::n_weights(K)));
     m.gb = reinterpret_cast<float *>(take(4 * rm::n_bias(K)));
-    m.part = reinterpret_cast<float *>(take(4 * part_floats(K)));
+    m.part = reinterpret_cast<float *>(take(4 * part_floats(K, n)));
     return m;
 }
 
@@ -987,7 +1064,13 @@ static void mlp_loss_grad_impl(const restir_mlp &m, const uint16_t *X, const int
     // layer 4: gW4[o][k] = sum_b dZ[b][o] H3[b][k]
     const uint16_t *dY = t.dZ;
     int N = K;
+    rm::SumSegments sums{};
+    int64_t sumCount = 0;
+    const int chunks = nn / 64;
+    float *partB = t.part + static_cast<size_t>(kMaxSplit) * (3 * static_cast<size_t>(kHidden) + K) * kIn;
     for (int l = 3; l >= 0; --l) {
+        float *pw = t.part + static_cast<size_t>(kMaxSplit) * rows_before(l) * kIn;
+        float *pb = partB + static_cast<size_t>(chunks) * rows_before(l);
         const uint16_t *in = l > 0 ? t.H[l - 1] : X;
         // split-K partials over the samples, then an in-order reduction (deterministic)
         const int sp = split_for(N, kIn, nn);
@@ -998,21 +1081,25 @@ static void mlp_loss_grad_impl(const restir_mlp &m, const uint16_t *X, const int
         gwA.B = in;      // B(b, k) = in[b][k]
         gwA.sbk = kIn;
         gwA.sbn = 1;
-        gwA.Cf = t.part;
+        gwA.Cf = pw;
         gwA.scm = kIn;
         gwA.scn = 1;
         gwA.kChunk = nn / sp;
         gwA.scz = static_cast<int64_t>(N) * kIn;
         RK_LAUNCH("k_gemm_splitk", rm::k_gemm, dim3(kIn / 64, N / 64, sp), dim3(256), 0, s, gwA);
-        const int64_t cnt = static_cast<int64_t>(N) * kIn;
-        RK_LAUNCH("k_sum_partials", rm::k_sum_partials, dim3(static_cast<uint32_t>((cnt + 255) / 256)), dim3(256), 0,
-                  s, static_cast<const float *>(t.part), sp, cnt, gw + rm::w_off(l));
-        // bias gradients: column sums of dY over 64-row chunks, then in order
-        const int chunks = nn / 64;
-        RK_LAUNCH("k_colsum_partial", rm::k_colsum_partial, dim3(N / 64, chunks), dim3(256), 0, s, dY, N, 64,
-                  t.part);
-        RK_LAUNCH("k_sum_partials", rm::k_sum_partials, dim3((N + 255) / 256), dim3(256), 0, s,
-                  static_cast<const float *>(t.part), chunks, static_cast<int64_t>(N), gb + rm::b_off(l));
+        // bias gradients: column sums of dY over 64-row chunks (added in order below)
+        RK_LAUNCH("k_colsum_partial", rm::k_colsum_partial, dim3(N / 64, chunks), dim3(256), 0, s, dY, N, 64, pb);
+        const int seg = sums.n;
+        sums.part[seg] = pw;
+        sums.nParts[seg] = sp;
+        sums.count[seg] = static_cast<int64_t>(N) * kIn;
+        sums.out[seg] = gw + rm::w_off(l);
+        sums.part[seg + 1] = pb;
+        sums.nParts[seg + 1] = chunks;
+        sums.count[seg + 1] = N;
+        sums.out[seg + 1] = gb + rm::b_off(l);
+        sums.n = seg + 2;
+        sumCount += sums.count[seg] + sums.count[seg + 1];
         if (l == 0) break;
         // dIn[b][k] = sum_o dY[b][o] W[o][k], gated by the ReLU of the layer below
         uint16_t *dIn = (dY == t.dA) ? t.dB : t.dA;
@@ -1033,6 +1120,9 @@ static void mlp_loss_grad_impl(const restir_mlp &m, const uint16_t *X, const int
         dY = dIn;
         N = kHidden;
     }
+    // the split-K and bias partials of all four layers, each summed in a fixed order
+    RK_LAUNCH("k_sum_partials", rm::k_sum_partials, dim3(static_cast<uint32_t>((sumCount + 255) / 256)), dim3(256), 0,
+              s, sums);
 }
 
 hipError_t mlp_loss_grad(const restir_mlp &m, const uint16_t *X, const int32_t *target, const uint32_t *mask,
@@ -1128,12 +1218,13 @@ hipError_t mlp_propose(const restir_mlp &m, const GView &g, f3 cam, const float 
     a.candLight = candLight;
     a.candQ = candQ;
     const uint32_t quads = ((g.W + 1) / 2) * ((g.H + 1) / 2);
-    const dim3 grid((quads + rm::kTilePx - 1) / rm::kTilePx), block(512);
+    const dim3 grid((quads + rm::kProposeTile - 1) / rm::kProposeTile), block(2 * rm::kProposeTile);
+    const dim3 gridW((quads + rm::kTilePx - 1) / rm::kTilePx), blockW(2 * rm::kTilePx);
     switch (a.K) {
         case 64: RK_LAUNCH("k_propose", (rm::k_propose<64>), grid, block, 0, s, a); break;
         case 128: RK_LAUNCH("k_propose", (rm::k_propose<128>), grid, block, 0, s, a); break;
         case 256: RK_LAUNCH("k_propose", (rm::k_propose<256>), grid, block, 0, s, a); break;
-        default: RK_LAUNCH("k_propose_wide", rm::k_propose_wide, grid, block, 0, s, a); break;
+        default: RK_LAUNCH("k_propose_wide", rm::k_propose_wide, gridW, blockW, 0, s, a); break;
     }
     return hipGetLastError();
 }

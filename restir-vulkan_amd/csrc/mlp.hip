@@ -9,7 +9,8 @@
 //
 // Kernels:
 //   k_propose      fused inference per 2x2 pixel quad: features -> 4 MFMA layers -> softmax ->
-//                  M candidates (l, q) for each of its pixels
+//                  M candidates (l, q) for each of its pixels (K <= 256)
+//   k_propose_wide the same for K up to 4096: the output layer streamed in 128-cluster blocks
 //   k_features     features of listed pixels (bf16), for training and tests
 //   k_gemm         strided C = A*B with bias / ReLU / ReLU-gate epilogues (training fwd/bwd),
 //                  or split-K fp32 partials for the weight gradients

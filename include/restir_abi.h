@@ -173,7 +173,7 @@ void restir_destroy(restir_ctx *ctx);
 const char *restir_last_error(const restir_ctx *ctx);
 /* ABI revision; bumped whenever an entry point's signature changes. Bindings compare it with the
  * revision they were written against and refuse a mismatched library. */
-#define RESTIR_ABI_VERSION 8
+#define RESTIR_ABI_VERSION 9
 int restir_version(void);
 
 /* Light tables: device pointers to the reference buffer images (header + array). The counts are
@@ -325,6 +325,11 @@ typedef struct restir_mlp {
     float *bias;            /* b1, b2, b3 (128 each) then b4 (K) */
     float *adam_m;          /* first moments: weights then biases */
     float *adam_v;          /* second moments: weights then biases */
+    uint32_t *adam_step_device; /* optional device copy of the step count (1 uint32), or NULL.
+                                 * When set, restir_mlp_train increments it on the device and the
+                                 * Adam step reads it there, so a sequence of training steps can be
+                                 * captured into a HIP graph and replayed; adam_step (host) then
+                                 * only counts the calls. restir_mlp_init zeroes it. */
 } restir_mlp;
 
 #define RESTIR_MLP_FEATURES 128
@@ -334,7 +339,7 @@ typedef struct restir_mlp {
 
 /* Seeded initialisation: He-uniform weights, zero biases except the output bias = out_prior
  * (device array of K floats, e.g. log cluster power shares; -30000 masks an empty cluster),
- * zeroed Adam moments, adam_step = 0. */
+ * zeroed Adam moments, adam_step = 0 (and *adam_step_device = 0 when set). */
 int restir_mlp_init(restir_ctx *ctx, restir_mlp *mlp, uint64_t seed, const float *out_prior,
                     void *stream);
 

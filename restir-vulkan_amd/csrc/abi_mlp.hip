@@ -9,6 +9,8 @@
 
 using namespace rki;
 
+static_assert(sizeof(restir_mlp) == 56, "restir_mlp layout (the Python binding mirrors it)");
+
 namespace {
 
 int check_mlp(restir_ctx *ctx, const restir_mlp *m, const char *name) {
@@ -49,7 +51,11 @@ int restir_mlp_init(restir_ctx *ctx, restir_mlp *m, uint64_t seed, const float *
         return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "mlp_init: NULL buffer");
     if (int r = use_device(ctx)) return r;
     m->adam_step = 0;
-    const hipError_t e = rk::mlp_init(*m, seed, prior, static_cast<hipStream_t>(stream));
+    hipError_t e = rk::mlp_init(*m, seed, prior, static_cast<hipStream_t>(stream));
+    if (e == hipSuccess && m->adam_step_device) {
+        rk::k_zero_u32<<<1, 64, 0, static_cast<hipStream_t>(stream)>>>(m->adam_step_device, 1u);
+        e = hipGetLastError();
+    }
     return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "mlp_init launch");
 }
 

@@ -285,8 +285,28 @@ struct AdamArgs {
     const float *gw, *gb;
     float *m, *v;      // nW + nB each
     uint32_t nW, nB;
-    float lr, b1, b2, eps, c1, c2;  // c1 = 1 - b1^t, c2 = 1 - b2^t
+    float lr, b1, b2, eps;
+    double b1d, b2d;               // b1, b2 in double for the bias corrections
+    uint32_t stepHost;             // step t, unless stepDev is set
+    const uint32_t *stepDev;
 };
+
+// b^t by binary exponentiation in double (the same products in the same order wherever t is
+// read), for the bias corrections 1 - b1^t and 1 - b2^t.
+__host__ __device__ inline double pow_step(double b, uint32_t t) {
+    double r = 1.0;
+    while (t) {
+        if (t & 1u) r *= b;
+        b *= b;
+        t >>= 1;
+    }
+    return r;
+}
+
+// Advances the device step count by one (lane 0; a vector atomic).
+__global__ void k_adam_count(uint32_t *step) {
+    if (threadIdx.x == 0) atomicAdd(step, 1u);
+}
 
 // Adam step; skipped when the batch held no training sample (*nValid == 0).
 __global__ __launch_bounds__(256) void k_adam_guarded(const AdamArgs a, const uint32_t *nValid) {
@@ -294,11 +314,14 @@ __global__ __launch_bounds__(256) void k_adam_guarded(const AdamArgs a, const ui
     if (i >= a.nW + a.nB || *nValid == 0u) return;
     const bool isW = i < a.nW;
     const float g = isW ? a.gw[i] : a.gb[i - a.nW];
+    const uint32_t t = a.stepDev ? *a.stepDev : a.stepHost;
+    const float c1 = static_cast<float>(1.0 - pow_step(a.b1d, t));
+    const float c2 = static_cast<float>(1.0 - pow_step(a.b2d, t));
     const float m = a.b1 * a.m[i] + (1.0f - a.b1) * g;
     const float v = a.b2 * a.v[i] + (1.0f - a.b2) * g * g;
     a.m[i] = m;
     a.v[i] = v;
-    const float step = a.lr * (m / a.c1) / (sqrtf(v / a.c2) + a.eps);
+    const float step = a.lr * (m / c1) / (sqrtf(v / c2) + a.eps);
     if (isW) {
         const float w = a.w[i] - step;
         a.w[i] = w;
@@ -1187,8 +1210,12 @@ hipError_t mlp_train(restir_mlp &m, const GView &g, f3 cam, const float lo[3], c
     a.b1 = static_cast<float>(b1);
     a.b2 = static_cast<float>(b2);
     a.eps = 1e-8f;
-    a.c1 = static_cast<float>(1.0 - pow(b1, m.adam_step));
-    a.c2 = static_cast<float>(1.0 - pow(b2, m.adam_step));
+    a.b1d = b1;
+    a.b2d = b2;
+    a.stepHost = m.adam_step;
+    a.stepDev = m.adam_step_device;
+    if (m.adam_step_device)
+        RK_LAUNCH("k_adam_count", rm::k_adam_count, dim3(1), dim3(64), 0, s, m.adam_step_device);
     RK_LAUNCH("k_adam_guarded", rm::k_adam_guarded, dim3((a.nW + a.nB + 255) / 256), dim3(256), 0, s, a, t.nValid);
     if (lossOut) RK_LAUNCH("k_copy_f32", rm::k_copy_f32, dim3(1), dim3(1), 0, s, t.loss, lossOut);
     return hipGetLastError();

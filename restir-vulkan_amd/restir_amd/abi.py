@@ -21,7 +21,7 @@ TEMPORAL_REUSE_FLAG = 1 << 1
 TRACE_FIRST_HIT = 0
 TRACE_REFERENCE_LOOP = 1
 TRACE_ONE_RAY_PER_LANE = 2
-ABI_VERSION = 8  # RESTIR_ABI_VERSION of include/restir_abi.h
+ABI_VERSION = 9  # RESTIR_ABI_VERSION of include/restir_abi.h
 
 ERRORS = {
     -1: "RESTIR_E_INVALID_ARGUMENT",
@@ -114,6 +114,7 @@ class MlpStruct(C.Structure):
         ("bias", C.c_void_p),
         ("adam_m", C.c_void_p),
         ("adam_v", C.c_void_p),
+        ("adam_step_device", C.c_void_p),
     ]
 
 
@@ -123,7 +124,7 @@ def mlp_sizes(k: int):
 
 
 assert C.sizeof(Uniforms) == 128 and C.sizeof(GBuffer) == 56 and C.sizeof(Camera) == 48
-assert C.sizeof(MlpStruct) == 48
+assert C.sizeof(MlpStruct) == 56
 
 
 class View(C.Structure):

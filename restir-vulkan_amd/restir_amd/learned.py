@@ -39,8 +39,10 @@ class LearnedProposal:
         self.b = torch.zeros(nB, dtype=torch.float32, device=device)
         self.m = torch.zeros(nW + nB, dtype=torch.float32, device=device)
         self.v = torch.zeros(nW + nB, dtype=torch.float32, device=device)
+        # the Adam step count lives on the device, so training steps can be graph-captured
+        self.step = torch.zeros(1, dtype=torch.int32, device=device)
         self.mlp = MlpStruct(k, 0, self.w_bf16.data_ptr(), self.w.data_ptr(), self.b.data_ptr(),
-                             self.m.data_ptr(), self.v.data_ptr())
+                             self.m.data_ptr(), self.v.data_ptr(), self.step.data_ptr())
         # output bias starts at the log power share of each cluster; empty clusters are masked
         share = cl.power_share.astype(np.float64)
         prior = np.where(share > 0, np.log(np.maximum(share, 1e-30)), -30000.0).astype(np.float32)
@@ -74,7 +76,8 @@ class LearnedProposal:
     def state(self):
         """Parameters and Adam moments as host arrays (the proposal's checkpoint)."""
         return {"w": self.w.cpu().numpy(), "b": self.b.cpu().numpy(), "m": self.m.cpu().numpy(),
-                "v": self.v.cpu().numpy(), "adam_step": int(self.mlp.adam_step)}
+                "v": self.v.cpu().numpy(), "w_bf16": self.w_bf16.cpu().numpy(),
+                "adam_step": int(self.step.item()), "train_seed": self.train_seed}
 
     def load_state(self, st):
         t = self.torch
@@ -82,9 +85,15 @@ class LearnedProposal:
         self.b.copy_(t.from_numpy(st["b"]))
         self.m.copy_(t.from_numpy(st["m"]))
         self.v.copy_(t.from_numpy(st["v"]))
-        bf = t.from_numpy(st["w"]).to(self.w.device).to(t.bfloat16)
-        self.w_bf16.copy_(bf.view(t.int16))
+        if "w_bf16" in st:
+            self.w_bf16.copy_(t.from_numpy(st["w_bf16"]))
+        else:
+            bf = t.from_numpy(st["w"]).to(self.w.device).to(t.bfloat16)
+            self.w_bf16.copy_(bf.view(t.int16))
         self.mlp.adam_step = int(st["adam_step"])
+        self.step.fill_(int(st["adam_step"]))
+        if "train_seed" in st:
+            self.train_seed = int(st["train_seed"])
 
 
 def log_uniform(rng, lo, hi, size):

@@ -363,14 +363,17 @@ class FrameSequenceGraph:
     renderer's host state advances to the end of the sequence, and each replay runs the frames
     with the uniforms, seeds and buffers recorded at capture (frame f's image is copied into
     ``images[f]``). Nothing in the graph clears memory with hipMemsetAsync: the library clears its
-    counters with kernels. The learned proposal's training keeps its Adam step count on the host,
-    so renderers with a learned proposal are not captured.
+    counters with kernels. A learned proposal is captured with its training steps when its Adam
+    step count lives on the device (restir_mlp.adam_step_device, as LearnedProposal keeps it):
+    each replay then continues training from the parameters it finds, with the random seeds
+    recorded at capture.
     """
 
     def __init__(self, r: Renderer, cams):
         torch = r.torch
-        if getattr(r, "learned", None) is not None:
-            raise ValueError("FrameSequenceGraph: a renderer with a learned proposal cannot be captured")
+        learned = getattr(r, "learned", None)
+        if learned is not None and not learned.mlp.adam_step_device:
+            raise ValueError("FrameSequenceGraph: the learned proposal keeps its Adam step count on the host")
         self.r, self.cams = r, list(cams)
         self.images = torch.zeros(len(self.cams), r.rgb.numel(), dtype=torch.float32, device=r.device)
         self.stream = torch.cuda.Stream(device=r.device)

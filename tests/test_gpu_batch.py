@@ -1,6 +1,7 @@
 """GPU checks of batching: a HIP-graph-captured 8-frame sequence against the same frames run
 eagerly, the tile autotuner (its choice is recorded and does not change results), and the
 persistent multi-view passes against per-view launches (bit-identical)."""
+import numpy as np
 import pytest
 
 from restir_amd import abi, scenes
@@ -58,13 +59,64 @@ def test_graph_of_8_frames_matches_eager(torch, mode):
         torch.cuda.synchronize()
 
 
-def test_graph_rejects_learned_renderer(torch):
+def test_graph_rejects_learned_renderer_with_host_step_count(torch):
     from restir_amd.learned import LearnedProposal
     sc = scenes.cornell_box(1)
     r = Renderer(sc, 32, 32)
     r.learned = LearnedProposal(r.ctx, sc, k=64, seed=1)
+    r.learned.mlp.adam_step_device = None
     with pytest.raises(ValueError):
         FrameSequenceGraph(r, [sc.camera])
+
+
+def test_graph_of_learned_frames_matches_eager(torch):
+    """Proposal, passes, shading and one Adam step per frame, 6 frames in one graph: each replay
+    from the same starting state (parameters, moments, step count, reservoirs) gives the eager
+    sequence's images and trained parameters bit for bit."""
+    from restir_amd.learned import LearnedProposal
+    sc = scenes.terrain(11, grid=32, n_boxes=20, n_point_lights=500, extent=8.0)
+    W, H = 64, 48
+    cams = [scenes.orbit_camera(sc.camera, f) for f in range(6)]
+    kw = dict(mode="unbiased", k=3, M=8, spatial_passes=2)
+
+    def renderer():
+        r = Renderer(sc, W, H, **kw)
+        r.learned = LearnedProposal(r.ctx, sc, k=64, seed=3)
+        r.train_samples = 1024
+        return r
+
+    eager = renderer()
+    start = eager.learned.state()
+    imgs = []
+    for cam in cams:
+        eager.render_frame(cam)
+        imgs.append(eager.rgb.clone())
+    torch.cuda.synchronize()
+    end = eager.learned.state()
+    assert end["adam_step"] == 6 and not np.array_equal(end["w"], start["w"])
+
+    r = renderer()
+    r.render_frame(cams[0])  # sizes the library's scratch outside the capture
+    torch.cuda.synchronize()
+    r.reset()
+    r.learned.load_state(start)
+    torch.cuda.synchronize()
+    fg = FrameSequenceGraph(r, cams)
+    for rep in range(2):
+        for t in (*r.gbuf, *r.res, r.tmp):
+            t.zero_()
+        r.learned.load_state(start)
+        torch.cuda.synchronize()
+        fg.replay()
+        torch.cuda.synchronize()
+        for f in range(len(cams)):
+            assert torch.equal(bits(fg.images[f]), bits(imgs[f])), (rep, f)
+        got = r.learned.state()
+        for key in ("w", "b", "m", "v", "w_bf16"):
+            assert np.array_equal(got[key].view(np.uint8), end[key].view(np.uint8)), (rep, key)
+        assert got["adam_step"] == 6
+        eager.render_frame(cams[rep])  # eager work between replays
+        torch.cuda.synchronize()
 
 
 def test_autotune_records_choice_and_keeps_results(torch):

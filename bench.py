@@ -100,6 +100,8 @@ def parse():
                     help="stress4k: the 3840x2160 stress workload's learned pipeline (2,333,058 "
                          "triangles, 1M point lights in 4,096 clusters, 4,096-way proposal); "
                          "prints its own JSON line")
+    ap.add_argument("--stress-paths", type=int, default=4,
+                    help="stress4k: camera paths of 32 learned frames captured into one graph (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (x3), whose hardware byte counters "
                          "calibrate the traffic counters of the other kernels")
@@ -637,9 +639,9 @@ def stress_4k(torch, args):
     intensity [1e-3, 10] clustered into 4,096 groups (nearest-centroid step on the GPU), 3840x2160
     frames of the orbiting camera with the 4,096-way MLP proposal (M = 8; inference, Pass A, two
     Pass C, shading and one 4096-sample Adam step per frame) and, for comparison, plain RIS with
-    M = 32. Not part of it here: the 4 camera paths x 32 frames in one graph and fp16 G-buffer
-    normals. Prints one JSON line with the build times, ms/frame of both pipelines and the
-    per-kernel times of the learned one."""
+    M = 32; then 4 camera paths x 32 learned frames in one HIP graph (stress_paths_graph). Not
+    part of it here: fp16 G-buffer normals. Prints one JSON line with the build times, ms/frame
+    of both pipelines, the per-kernel times and the graph's ms/frame."""
     from restir_amd import abi, scenes
     from restir_amd.abi import Context
     from restir_amd.learned import LearnedProposal
@@ -688,7 +690,59 @@ def stress_4k(torch, args):
         out[f"{name}_M"] = M
         out[f"{name}_ms_per_frame"] = round(ms, 3)
         out[f"{name}_kernels_ms_per_frame"] = {k: round(v["ms_per_frame"], 4) for k, v in kernels.items()}
+    if args.stress_paths > 0:
+        out.update(stress_paths_graph(torch, args, scene, ctx, lp, launch, W, H))
     print(json.dumps(out), flush=True)
+
+
+def stress_paths_graph(torch, args, scene, ctx, lp, launch, W, H):
+    """The stress workload's batching: camera paths of 32 frames each (G-buffer, learned
+    proposal, Pass A, two Pass C, shading and one Adam step per frame), one renderer per path on
+    one context and one proposal, all captured into one HIP graph; timed against the same
+    sequence run eagerly from the same starting state."""
+    from restir_amd import scenes
+    from restir_amd.pipeline import Renderer
+    paths, plen = args.stress_paths, 32
+    cams = [[scenes.orbit_camera(scene.camera, 180 * p + f) for f in range(plen)] for p in range(paths)]
+    rs = [Renderer(scene, W, H, mode="unbiased", k=args.k, M=args.learned_M, spatial_passes=args.spatial_passes,
+                   ctx=ctx, learned=lp, launch=launch, scene_build="device") for _ in range(paths)]
+    for r, c in zip(rs, cams):
+        r.render_frame(c[0])  # sizes the library's scratch outside the capture
+    torch.cuda.synchronize()
+    start = lp.state()
+
+    def restart():
+        for r in rs:
+            r.reset()
+        lp.load_state(start)
+        torch.cuda.synchronize()
+
+    restart()
+    t0 = time.perf_counter()
+    for r, c in zip(rs, cams):
+        for cam in c:
+            r.render_frame(cam)
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t0) * 1e3 / (paths * plen)
+    restart()
+    stream = torch.cuda.Stream()
+    graph = torch.cuda.CUDAGraph()
+    stream.wait_stream(torch.cuda.current_stream())
+    t0 = time.perf_counter()
+    with torch.cuda.graph(graph, stream=stream):
+        for r, c in zip(rs, cams):
+            for cam in c:
+                r.render_frame(cam, stream=stream)
+    capture_s = time.perf_counter() - t0
+    restart()
+    t0 = time.perf_counter()
+    graph.replay()
+    torch.cuda.synchronize()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / (paths * plen)
+    log(f"{paths} paths x {plen} frames (learned, G-buffers rendered in-frame): graph {graph_ms:.2f} ms/frame, "
+        f"eager {eager_ms:.2f} ms/frame, capture {capture_s:.1f} s")
+    return {"graph_paths": paths, "graph_frames_per_path": plen, "graph_ms_per_frame": round(graph_ms, 3),
+            "eager_ms_per_frame_same_sequence": round(eager_ms, 3), "graph_capture_s": round(capture_s, 2)}
 
 
 if __name__ == "__main__":

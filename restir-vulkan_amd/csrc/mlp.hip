@@ -202,22 +202,18 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmArgs g) {
 }
 
 // ----------------------------------------------------------------------------- loss/grad ----
-// dZ = (softmax(z) - onehot(target)) / n_valid for samples with mask, 0 otherwise (bf16);
-// the summed loss goes to *loss (divided by n_valid).
-__global__ __launch_bounds__(256) void k_softmax_ce(const float *Z, int K, const int32_t *target,
-                                                    const uint32_t *mask, int n,
-                                                    const uint32_t *nValid, uint16_t *dZ,
-                                                    float *loss) {
-    // one wave per sample: lanes stride over the K logits, reductions across the wave
-    const int b = blockIdx.x * 4 + static_cast<int>(threadIdx.x >> 6);
-    const int lane = static_cast<int>(threadIdx.x & 63);
-    if (b >= n) return;
+// dZ = (softmax(z) - onehot(target)) / n_valid for sample b with mask, 0 otherwise (bf16), by the
+// 64 lanes of one wave (lanes stride over the K logits, reductions across the wave); returns the
+// sample's loss divided by n_valid (0 for a sample that does not train).
+__device__ __forceinline__ float softmax_ce_row(const float *Z, int K, const int32_t *target, const uint32_t *mask,
+                                                int n, const uint32_t *nValid, uint16_t *dZ, int b, int lane) {
+    if (b >= n) return 0.0f;
     const float *z = Z + static_cast<size_t>(b) * K;
     uint16_t *d = dZ + static_cast<size_t>(b) * K;
     const int t = target[b];
     if (!mask[b] || t < 0 || t >= K) {  // an out-of-range target does not train
         for (int c = lane; c < K; c += 64) d[c] = 0;
-        return;
+        return 0.0f;
     }
     const float inv = 1.0f / static_cast<float>(*nValid);
     float mx = -INFINITY;
@@ -226,9 +222,25 @@ __global__ __launch_bounds__(256) void k_softmax_ce(const float *Z, int K, const
     float se = 0.0f;
     for (int c = lane; c < K; c += 64) se += expf(z[c] - mx);
     for (int off = 32; off > 0; off >>= 1) se += __shfl_xor(se, off);
-    if (lane == 0) atomicAdd(loss, -(z[t] - mx - logf(se)) * inv);
     const float rse = 1.0f / se;
     for (int c = lane; c < K; c += 64) d[c] = f2bf((expf(z[c] - mx) * rse - (c == t ? 1.0f : 0.0f)) * inv);
+    return -(z[t] - mx - logf(se)) * inv;
+}
+
+// The mean cross-entropy's gradient for 4 samples per workgroup (softmax_ce_row); the loss of the
+// workgroup's samples, added in sample order, goes to lossPart[blockIdx.x] (k_sum_partials then
+// adds the workgroups' sums in order).
+__global__ __launch_bounds__(256) void k_softmax_ce(const float *Z, int K, const int32_t *target,
+                                                    const uint32_t *mask, int n,
+                                                    const uint32_t *nValid, uint16_t *dZ,
+                                                    float *lossPart) {
+    __shared__ float wl[4];
+    const int w = static_cast<int>(threadIdx.x >> 6);
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const float sampleLoss = softmax_ce_row(Z, K, target, mask, n, nValid, dZ, blockIdx.x * 4 + w, lane);
+    if (lane == 0) wl[w] = sampleLoss;
+    __syncthreads();
+    if (threadIdx.x == 0) lossPart[blockIdx.x] = ((wl[0] + wl[1]) + wl[2]) + wl[3];
 }
 
 __global__ __launch_bounds__(256) void k_count_mask(const uint32_t *mask, uint32_t n, uint32_t *count) {
@@ -255,14 +267,14 @@ __global__ __launch_bounds__(256) void k_colsum_partial(const uint16_t *G, int c
     if (q == 0) part[static_cast<size_t>(blockIdx.y) * cols + c] = ((red[0][c & 63] + red[1][c & 63]) + red[2][c & 63]) + red[3][c & 63];
 }
 
-// Up to 8 reductions in one launch: for segment j, out_j[i] = sum over s < nParts_j of
+// Up to 9 reductions in one launch: for segment j, out_j[i] = sum over s < nParts_j of
 // part_j[s * count_j + i], in order of s. Lane t of the grid takes element t of the segments laid
 // end to end.
 struct SumSegments {
-    const float *part[8];
-    float *out[8];
-    int nParts[8];
-    int64_t count[8];
+    const float *part[9];
+    float *out[9];
+    int nParts[9];
+    int64_t count[9];
     int n;
 };
 __global__ __launch_bounds__(256) void k_sum_partials(const SumSegments a) {
@@ -983,7 +995,8 @@ static GemmArgs gemm_args(int M, int N, int K) {
 struct MlpScratch {
     uint16_t *X, *H[3], *dA, *dB, *dZ;
     float *Z, *gw, *gb, *loss;
-    float *part;  // split-K and column-sum partials
+    float *part;      // split-K and column-sum partials
+    float *lossPart;  // per-workgroup loss sums of k_softmax_ce
     int32_t *target;
     uint32_t *mask, *pixels, *nValid;
 };
@@ -1007,7 +1020,7 @@ static int split_for(int M, int N, int n) {
 size_t mlp_scratch_bytes(uint32_t n, int K) {
     const size_t nn = n, kk = static_cast<size_t>(K);
     return 16 * 256 + 2 * (nn * kIn * 6 + nn * kk) + 4 * (nn * kk + 3 * nn) +
-           4 * (rm::n_weights(K) + rm::n_bias(K)) + 4 * part_floats(K, n);
+           4 * (rm::n_weights(K) + rm::n_bias(K)) + 4 * part_floats(K, n) + 4 * (nn / 4) + 256;
 }
 static MlpScratch carve_mlp(void *base, uint32_t n, int K) {
     char *p = static_cast<char *>(base);
@@ -1033,6 +1046,7 @@ static MlpScratch carve_mlp(void *base, uint32_t n, int K) {
     m.gw = reinterpret_cast<float *>(take(4 * rm::n_weights(K)));
     m.gb = reinterpret_cast<float *>(take(4 * rm::n_bias(K)));
     m.part = reinterpret_cast<float *>(take(4 * part_floats(K, n)));
+    m.lossPart = reinterpret_cast<float *>(take(4 * (nn / 4)));
     return m;
 }
 
@@ -1081,15 +1095,20 @@ static void mlp_loss_grad_impl(const restir_mlp &m, const uint16_t *X, const int
         RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, t.nValid, 1u);
         RK_LAUNCH("k_count_mask", rm::k_count_mask, dim3((n + 255) / 256), dim3(256), 0, s, mask, n, t.nValid);
     }
-    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t *>(loss), 1u);
     mlp_fwd(m, X, n, t, t.Z, s);
     RK_LAUNCH("k_softmax_ce", rm::k_softmax_ce, dim3((n + 3) / 4), dim3(256), 0, s, t.Z, K, target, mask, nn,
-                       t.nValid, t.dZ, loss);
+                       t.nValid, t.dZ, t.lossPart);
     // layer 4: gW4[o][k] = sum_b dZ[b][o] H3[b][k]
     const uint16_t *dY = t.dZ;
     int N = K;
     rm::SumSegments sums{};
-    int64_t sumCount = 0;
+    // the loss: the workgroups' sums of k_softmax_ce, in order
+    sums.part[0] = t.lossPart;
+    sums.nParts[0] = (nn + 3) / 4;
+    sums.count[0] = 1;
+    sums.out[0] = loss;
+    sums.n = 1;
+    int64_t sumCount = 1;
     const int chunks = nn / 64;
     float *partB = t.part + static_cast<size_t>(kMaxSplit) * (3 * static_cast<size_t>(kHidden) + K) * kIn;
     for (int l = 3; l >= 0; --l) {

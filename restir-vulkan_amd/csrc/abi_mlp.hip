@@ -120,6 +120,20 @@ int restir_set_light_clusters(restir_ctx *ctx, uint32_t k, const int32_t *lightC
     ctx->clusterCount = clusterCount;
     ctx->clusterLights = clusterLights;
     ctx->clusterAlias = clusterAlias;
+    // the fused slot records k_propose reads: one 32-byte load per drawn slot gives the light, its
+    // p_within and its p_src
+    std::vector<float4> fused(2 * n);
+    for (uint32_t c = 0; c < k; ++c)
+        for (int32_t i = 0; i < cnt[c]; ++i) {
+            const size_t slot = static_cast<size_t>(beg[c]) + i;
+            const restir_alias_table_column &col = al[slot];
+            const int32_t lc = lights[slot], la = lights[beg[c] + col.alias];
+            fused[2 * slot] = make_float4(col.prob, __builtin_bit_cast(float, lc), __builtin_bit_cast(float, la), col.oriProb);
+            fused[2 * slot + 1] = make_float4(col.aliasOriProb, src[lc].oriProb, src[la].oriProb, 0.0f);
+        }
+    if (int r = ensure(ctx, &ctx->clusterFused, &ctx->clusterFusedBytes, sizeof(float4) * fused.size())) return r;
+    const hipError_t e = hipMemcpy(ctx->clusterFused, fused.data(), sizeof(float4) * fused.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(ctx, e, "set_light_clusters: fused slot records");
     ctx->clustersSet = true;
     return RESTIR_OK;
 }
@@ -148,7 +162,8 @@ int restir_learned_candidates(restir_ctx *ctx, const restir_uniforms *u, const r
     const uint32_t seed = u->frame * 0x9e3779b1u + 0x7f4a7c15u;
     const hipError_t e = rk::mlp_propose(*m, view_of(g), cam, lo, hi, ctx->scene, ctx->lightCluster,
                                          ctx->lightPWithin, ctx->clusterBegin, ctx->clusterCount,
-                                         ctx->clusterLights, ctx->clusterAlias, M, mix, seed, candLight,
+                                         ctx->clusterLights, ctx->clusterAlias,
+                                         static_cast<const float4 *>(ctx->clusterFused), M, mix, seed, candLight,
                                          candQ, static_cast<hipStream_t>(stream));
     return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "learned_candidates launch");
 }

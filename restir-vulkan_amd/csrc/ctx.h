@@ -60,6 +60,10 @@ struct restir_ctx {
     const float *lightPWithin = nullptr;
     const int32_t *clusterBegin = nullptr, *clusterCount = nullptr, *clusterLights = nullptr;
     const restir_alias_table_column *clusterAlias = nullptr;
+    // per cluster slot, built from the tables above: {prob, light of the column, light of its
+    // alias, p_within of the column}, {p_within of the alias, p_src of both lights, 0}
+    void *clusterFused = nullptr;
+    size_t clusterFusedBytes = 0;
     ~restir_ctx() {
         if (counters) (void)hipFree(counters);
         if (scratch) (void)hipFree(scratch);
@@ -70,6 +74,7 @@ struct restir_ctx {
         if (bvhInfoHost) (void)hipHostFree(bvhInfoHost);
         if (aliasScratch) (void)hipFree(aliasScratch);
         if (wide) (void)hipFree(wide);
+        if (clusterFused) (void)hipFree(clusterFused);
     }
 };
 

@@ -149,8 +149,8 @@ hipError_t mlp_propose(const restir_mlp &m, const GView &g, rt::f3 cam, const fl
                        const float hi[3], const SceneView &sc, const int32_t *lightCluster,
                        const float *lightPWithin, const int32_t *clusterBegin,
                        const int32_t *clusterCount, const int32_t *clusterLights,
-                       const restir_alias_table_column *clusterAlias, uint32_t M, float mix,
-                       uint32_t seed, int32_t *candLight, float *candQ, hipStream_t s);
+                       const restir_alias_table_column *clusterAlias, const float4 *clusterFused, uint32_t M,
+                       float mix, uint32_t seed, int32_t *candLight, float *candQ, hipStream_t s);
 
 // ---- FP64 ground truth (truth.hip)
 constexpr uint32_t kTruthChunk = 128;  // lights per workgroup row

@@ -413,6 +413,7 @@ struct ProposeArgs {
     const float *lightPWithin;
     const int32_t *clusterBegin, *clusterCount, *clusterLights;
     const float4 *clusterAlias;  // per-cluster alias columns (alias = local index)
+    const float4 *clusterFused;  // per slot: {prob, light, alias light, p_within}, {alias p_within, p_src, alias p_src, 0}
     int32_t *candLight;
     float *candQ;
 };
@@ -664,17 +665,22 @@ __global__ __launch_bounds__(2 * kProposeTile) void k_propose(const ProposeArgs 
                 beg[jj] = a.clusterBegin[cl[jj]];
             }
         }
-        // step 2: MLP candidates read the column of the cluster's own alias table
+        // step 2: MLP candidates read the fused record of the drawn slot of the cluster's own
+        // alias table: the column's and its alias's light, p_within and p_src in 32 bytes
+        float4 t2[8];
 #pragma unroll
         for (int jj = 0; jj < 8; ++jj) {
             if (cl[jj] >= 0 && cnt[jj] > 0) {
                 int32_t c = static_cast<int32_t>(static_cast<float>(cnt[jj]) * r1[jj]);
                 col[jj] = c < cnt[jj] - 1 ? c : cnt[jj] - 1;
-                t[jj] = a.clusterAlias[beg[jj] + col[jj]];
+                const float4 *rec = a.clusterFused + 2 * static_cast<size_t>(beg[jj] + col[jj]);
+                t[jj] = rec[0];
+                t2[jj] = rec[1];
             }
         }
         // step 3: the light; its probability under the table that drew it comes with the column
-        // (oriProb of the column or of its alias, as the alias table stores both)
+        // (oriProb of the column or of its alias, as the alias table stores both), and an MLP
+        // candidate's p_src with its record
 #pragma unroll
         for (int jj = 0; jj < 8; ++jj) {
             l[jj] = -1;
@@ -684,19 +690,17 @@ __global__ __launch_bounds__(2 * kProposeTile) void k_propose(const ProposeArgs 
                 ps[jj] = keep ? t[jj].z : t[jj].w;
             } else if (cl[jj] >= 0 && cnt[jj] > 0) {
                 const bool keep = t[jj].x > r2[jj];
-                l[jj] = a.clusterLights[beg[jj] + (keep ? col[jj] : __float_as_int(t[jj].y))];
-                pw[jj] = keep ? t[jj].z : t[jj].w;
+                l[jj] = __float_as_int(keep ? t[jj].y : t[jj].z);
+                pw[jj] = keep ? t[jj].w : t2[jj].x;
+                ps[jj] = keep ? t2[jj].y : t2[jj].z;
             }
         }
-        // step 4: the other table's probability of that light
+        // step 4: a source candidate's cluster and p_within
 #pragma unroll
         for (int jj = 0; jj < 8; ++jj) {
-            if (l[jj] < 0) continue;
-            if (cl[jj] == -1) {
+            if (l[jj] >= 0 && cl[jj] == -1) {
                 cl[jj] = a.lightCluster[l[jj]];
                 pw[jj] = a.lightPWithin[l[jj]];
-            } else {
-                ps[jj] = a.alias[l[jj]].z;
             }
         }
 #pragma unroll
@@ -1243,8 +1247,8 @@ hipError_t mlp_train(restir_mlp &m, const GView &g, f3 cam, const float lo[3], c
 hipError_t mlp_propose(const restir_mlp &m, const GView &g, f3 cam, const float lo[3], const float hi[3],
                        const SceneView &sc, const int32_t *lightCluster, const float *lightPWithin,
                        const int32_t *clusterBegin, const int32_t *clusterCount, const int32_t *clusterLights,
-                       const restir_alias_table_column *clusterAlias, uint32_t M, float mix, uint32_t seed,
-                       int32_t *candLight, float *candQ, hipStream_t s) {
+                       const restir_alias_table_column *clusterAlias, const float4 *clusterFused, uint32_t M,
+                       float mix, uint32_t seed, int32_t *candLight, float *candQ, hipStream_t s) {
     rm::ProposeArgs a{};
     a.f = rm::FeatArgs{g, cam, {lo[0], lo[1], lo[2]}, {hi[0], hi[1], hi[2]}};
     a.wbf = m.weights_bf16;
@@ -1262,6 +1266,7 @@ hipError_t mlp_propose(const restir_mlp &m, const GView &g, f3 cam, const float 
     a.clusterCount = clusterCount;
     a.clusterLights = clusterLights;
     a.clusterAlias = reinterpret_cast<const float4 *>(clusterAlias);
+    a.clusterFused = clusterFused;
     a.candLight = candLight;
     a.candQ = candQ;
     const uint32_t quads = ((g.W + 1) / 2) * ((g.H + 1) / 2);

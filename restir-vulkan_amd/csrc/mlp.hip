@@ -285,6 +285,8 @@ __global__ __launch_bounds__(256) void k_sum_partials(const SumSegments a) {
     const float *part = a.part[j];
     const int64_t count = a.count[j];
     float sum = 0.0f;
+    // unrolled so that the loads of 8 parts are in flight together; the additions stay in order
+#pragma unroll 8
     for (int s = 0; s < a.nParts[j]; ++s) sum += part[s * count + i];
     a.out[j][i] = sum;
 }

@@ -343,7 +343,10 @@ typedef struct restir_mlp {
 int restir_mlp_init(restir_ctx *ctx, restir_mlp *mlp, uint64_t seed, const float *out_prior,
                     void *stream);
 
-/* Binds the cluster tables of restir_build_light_clusters (device copies) for the proposal. */
+/* Binds the cluster tables of restir_build_light_clusters (device copies) for the proposal.
+ * Validates them (copying them to the host, synchronously) and builds from them a context-owned
+ * device table of 32 bytes per light (each cluster slot's light, p_within and p_src, for its
+ * column and its alias); the caller's buffers must stay alive while the context uses them. */
 int restir_set_light_clusters(restir_ctx *ctx, uint32_t k, const int32_t *light_cluster,
                               const float *light_pwithin, const int32_t *cluster_begin,
                               const int32_t *cluster_count, const int32_t *cluster_lights,

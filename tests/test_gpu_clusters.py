@@ -78,3 +78,27 @@ def test_device_clustering_1m_lights_4096_clusters(torch):
     # p_within sums to one in every non-empty cluster
     s = np.bincount(d.light_cluster, weights=d.light_pwithin.astype(np.float64), minlength=k)
     assert np.allclose(s[d.count > 0], 1.0, atol=1e-4)
+
+
+def test_device_clustering_rejects_bad_sizes(torch):
+    ctx = abi.Context(0)
+    pts = random_lights(100, 1)
+    with pytest.raises(abi.RestirError):
+        LightClusters(pts, 101, ctx=ctx)  # more clusters than lights
+    with pytest.raises(abi.RestirError):
+        LightClusters(pts[:0], 1, ctx=ctx)  # no lights
+
+
+@pytest.mark.parametrize("K", [300, 4224])
+def test_mlp_rejects_unsupported_cluster_counts(torch, K):
+    """K must be 64, 128, 256 or a multiple of 128 up to 4,096."""
+    from restir_amd.abi import MlpStruct, mlp_sizes
+    ctx = abi.Context(0)
+    nW, nB = mlp_sizes(K)
+    bufs = [torch.zeros(n, dtype=dt, device="cuda") for n, dt in
+            ((nW, torch.int16), (nW, torch.float32), (nB, torch.float32), (nW + nB, torch.float32),
+             (nW + nB, torch.float32))]
+    mlp = MlpStruct(K, 0, *[b.data_ptr() for b in bufs])
+    prior = torch.zeros(K, dtype=torch.float32, device="cuda")
+    with pytest.raises(abi.RestirError):
+        ctx.mlp_init(mlp, 1, prior)

@@ -173,7 +173,7 @@ void restir_destroy(restir_ctx *ctx);
 const char *restir_last_error(const restir_ctx *ctx);
 /* ABI revision; bumped whenever an entry point's signature changes. Bindings compare it with the
  * revision they were written against and refuse a mismatched library. */
-#define RESTIR_ABI_VERSION 9
+#define RESTIR_ABI_VERSION 10
 int restir_version(void);
 
 /* Light tables: device pointers to the reference buffer images (header + array). The counts are
@@ -183,7 +183,9 @@ int restir_version(void);
  * count must match). Extension: with both kinds present, an alias table of n_point + n_tri
  * columns (restir_build_alias_table_mixed) is one table over the point lights followed by the
  * triangle lights, and Pass A samples both kinds from it (a triangle light is drawn as the
- * reference draws it, restirOmni.glsl:122-131). */
+ * reference draws it, restirOmni.glsl:122-131). Binding lights unbinds the light clusters
+ * (restir_set_light_clusters): they describe the previous light set, so the learned-proposal
+ * entry points return RESTIR_E_NOT_READY until clusters are bound again. */
 int restir_set_lights(restir_ctx *ctx, const void *point_lights, size_t point_bytes,
                       const void *tri_lights, size_t tri_bytes, const void *alias_table,
                       size_t alias_bytes);
@@ -346,7 +348,9 @@ int restir_mlp_init(restir_ctx *ctx, restir_mlp *mlp, uint64_t seed, const float
 /* Binds the cluster tables of restir_build_light_clusters (device copies) for the proposal.
  * Validates them (copying them to the host, synchronously) and builds from them a context-owned
  * device table of 32 bytes per light (each cluster slot's light, p_within and p_src, for its
- * column and its alias); the caller's buffers must stay alive while the context uses them. */
+ * column and its alias); the caller's buffers must stay alive while the context uses them.
+ * Call it after restir_set_lights: the slot table snapshots the bound alias table, and binding
+ * new lights unbinds the clusters. */
 int restir_set_light_clusters(restir_ctx *ctx, uint32_t k, const int32_t *light_cluster,
                               const float *light_pwithin, const int32_t *cluster_begin,
                               const int32_t *cluster_count, const int32_t *cluster_lights,
@@ -391,6 +395,11 @@ int restir_mlp_forward(restir_ctx *ctx, const restir_mlp *mlp, const uint16_t *f
 int restir_mlp_loss_grad(restir_ctx *ctx, const restir_mlp *mlp, const uint16_t *features,
                          const int32_t *target, const uint32_t *mask, uint32_t n, float *grad_w,
                          float *grad_b, float *loss, void *stream);
+/* The Adam step of restir_mlp_train alone: from gradients grad_w, grad_b (fp32, parameter
+ * layouts) it advances adam_step (and *adam_step_device when set) and updates the fp32 master
+ * weights, biases, moments and the bf16 weight copy (beta 0.9 / 0.999, eps 1e-8, rate lr). */
+int restir_mlp_adam(restir_ctx *ctx, restir_mlp *mlp, const float *grad_w, const float *grad_b,
+                    float lr, void *stream);
 
 /* ----------------------------------------------------------------------------------------------
  * Scene-side helpers for synthetic inputs (no reference counterpart; the reference rasterises).

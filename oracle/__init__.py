@@ -46,6 +46,7 @@ SIGS = {
     "oracle_trace_brute_f64": (None, [_vp, _u32, _vp, _vp, _u32, _vp, _vp]),
     "oracle_count_reachable": (None, [C.POINTER(OracleScene), _vp, _vp, _u32, _vp]),
     "oracle_pass_a": (None, [C.POINTER(OracleScene), _vp, _vp, _vp, _vp, _vp]),
+    "oracle_pass_a_candidates": (None, [C.POINTER(OracleScene), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "oracle_pass_b": (None, [C.POINTER(OracleScene), _vp, _i32, _vp, _vp, _vp]),
     "oracle_pass_c": (None, [C.POINTER(OracleScene), _vp, _i32, _i32, _vp, _vp, _vp]),
     "oracle_cos_threshold": (_f32, [_f32]),
@@ -214,6 +215,19 @@ def pass_a(scene: Scene, u, gcur, gprev, prev_res, res_dtype):
     lib().oracle_pass_a(scene.ref(), C.addressof(u), C.addressof(gcur),
                         C.addressof(gprev) if gprev is not None else None, _p(prev_res),
                         out.ctypes.data)
+    return out
+
+
+def pass_a_candidates(scene: Scene, u, gcur, gprev, prev_res, cand_light, cand_q, res_dtype):
+    """Pass A over given candidates: cand_light/cand_q hold M x W*H entries (candidate-major)."""
+    W, H = u.screenSize[0], u.screenSize[1]
+    cl = np.ascontiguousarray(cand_light, dtype=np.int32).reshape(-1)
+    cq = np.ascontiguousarray(cand_q, dtype=np.float32).reshape(-1)
+    assert cl.size == cq.size == u.initialLightSampleCount * W * H
+    out = np.zeros(W * H, dtype=res_dtype)
+    lib().oracle_pass_a_candidates(scene.ref(), C.addressof(u), C.addressof(gcur),
+                                   C.addressof(gprev) if gprev is not None else None, _p(prev_res),
+                                   cl.ctypes.data, cq.ctypes.data, out.ctypes.data)
     return out
 
 

@@ -728,15 +728,26 @@ static inline float res_phat_at(const restir_reservoir *r, v3 P, v3 C, v3 N, flo
                   s->position_emissionLum[3], rough, metal);
 }
 
-void oracle_pass_a(const oracle_scene *s, const restir_uniforms *u, const restir_gbuffer *cur,
-                   const restir_gbuffer *prev, const restir_reservoir *prev_res,
-                   restir_reservoir *out) {
-    /* restirOmni.glsl:86-212 */
+/* Pass A (restirOmni.glsl:86-212). With cand_light == NULL the M candidates come from the alias
+ * table as in the reference; otherwise candidate i of pixel p is (cand_light, cand_q)[i*W*H + p]:
+ * a light of the alias table's index space (point lights first when the table covers both kinds)
+ * drawn with probability cand_q, weighted p_hat / q in the same reservoir update
+ * (addSampleToReservoir, reservoir.glsl:28-42). A candidate outside [0, n_lights) or with q <= 0
+ * is empty: p_hat 0, probability 1, still counted in M and still consuming the update's draw. A
+ * triangle-light candidate draws its point as the reference does (two draws, restirOmni.glsl:
+ * 122-131) and divides q by |cos| * area. */
+static void pass_a_impl(const oracle_scene *s, const restir_uniforms *u, const restir_gbuffer *cur,
+                        const restir_gbuffer *prev, const restir_reservoir *prev_res,
+                        const int32_t *cand_light, const float *cand_q, restir_reservoir *out) {
     uint32_t W = u->screenSize[0], H = u->screenSize[1];
     v3 cam = ld3(u->cameraPos);
     int32_t pointCount = (int32_t)s->n_points, aliasCount = (int32_t)s->n_alias;
     /* one table over both kinds (point lights first) when it has a column for every light */
     int mixed = pointCount > 0 && s->n_tri_lights > 0 && aliasCount == pointCount + (int32_t)s->n_tri_lights;
+    /* lights the table's index space names: the union, else point lights if any, else triangles */
+    int32_t nLights = mixed ? pointCount + (int32_t)s->n_tri_lights
+                            : (pointCount > 0 ? pointCount : (int32_t)s->n_tri_lights);
+    size_t npx = (size_t)W * H;
 #pragma omp parallel for schedule(dynamic, 4)
     for (int64_t yy = 0; yy < (int64_t)H; ++yy) {
         uint32_t y = (uint32_t)yy;
@@ -749,26 +760,40 @@ void oracle_pass_a(const oracle_scene *s, const restir_uniforms *u, const restir
             pcg_seed(rng, (uint64_t)u->frame, (uint64_t)(uint32_t)(y * 10007u + x));
             if (vdot(t.n, t.n) != 0.0f) {
                 for (uint32_t i = 0; i < u->initialLightSampleCount; ++i) {
-                    float r1 = pcg_float(rng), r2 = pcg_float(rng);
-                    /* aliasTableSample, :73-83 */
-                    int32_t col = (int32_t)((float)aliasCount * r1);
-                    if (col > aliasCount - 1) col = aliasCount - 1;
-                    const restir_alias_table_column *c = &s->alias[col];
                     int32_t idx;
                     float prob;
-                    if (c->prob > r2) { idx = col; prob = c->oriProb; }
-                    else { idx = c->alias; prob = c->aliasOriProb; }
+                    int point;
+                    if (cand_light) {
+                        size_t ci = (size_t)i * npx + (size_t)y * W + x;
+                        idx = cand_light[ci];
+                        prob = cand_q[ci];
+                        if (!(idx >= 0 && idx < nLights && prob > 0.0f)) {
+                            const float zero3[3] = {0, 0, 0}, zero4[4] = {0, 0, 0, 0};
+                            oracle_add_sample(&res, zero3, zero4, 0.0f, 0, 0.0f, 1.0f, rng);
+                            continue;
+                        }
+                        point = idx < pointCount;
+                    } else {
+                        float r1 = pcg_float(rng), r2 = pcg_float(rng);
+                        /* aliasTableSample, :73-83 */
+                        int32_t col = (int32_t)((float)aliasCount * r1);
+                        if (col > aliasCount - 1) col = aliasCount - 1;
+                        const restir_alias_table_column *c = &s->alias[col];
+                        if (c->prob > r2) { idx = col; prob = c->oriProb; }
+                        else { idx = c->alias; prob = c->aliasOriProb; }
+                        point = mixed ? idx < pointCount : pointCount != 0;
+                    }
                     v3 L;
                     float ln4[4] = {0, 0, 0, 0};
                     float lum;
                     int32_t lidx;
-                    if (mixed ? idx < pointCount : pointCount != 0) {
+                    if (point) {
                         const restir_point_light *pl = &s->points[idx];
                         L = ld3(pl->pos);
                         lum = pl->color_luminance[3];
                         lidx = idx;
                     } else {
-                        if (mixed) idx -= pointCount;
+                        if (cand_light || mixed) idx -= pointCount;
                         const restir_tri_light *tl = &s->tri_lights[idx];
                         float a = pcg_float(rng), b = pcg_float(rng);
                         float sa = sqrtf(a); /* pickPointOnTriangle, :68-71 */
@@ -825,6 +850,18 @@ void oracle_pass_a(const oracle_scene *s, const restir_uniforms *u, const restir
             out[(size_t)y * W + x] = res;
         }
     }
+}
+
+void oracle_pass_a(const oracle_scene *s, const restir_uniforms *u, const restir_gbuffer *cur,
+                   const restir_gbuffer *prev, const restir_reservoir *prev_res,
+                   restir_reservoir *out) {
+    pass_a_impl(s, u, cur, prev, prev_res, NULL, NULL, out);
+}
+
+void oracle_pass_a_candidates(const oracle_scene *s, const restir_uniforms *u, const restir_gbuffer *cur,
+                              const restir_gbuffer *prev, const restir_reservoir *prev_res,
+                              const int32_t *cand_light, const float *cand_q, restir_reservoir *out) {
+    pass_a_impl(s, u, cur, prev, prev_res, cand_light, cand_q, out);
 }
 
 float oracle_cos_threshold(float degrees) {

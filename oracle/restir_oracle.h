@@ -120,6 +120,11 @@ void oracle_count_reachable(const oracle_scene *s, const float *p1, const float 
 void oracle_pass_a(const oracle_scene *s, const restir_uniforms *u, const restir_gbuffer *cur,
                    const restir_gbuffer *prev, const restir_reservoir *prev_res,
                    restir_reservoir *out);
+/* Pass A with RIS over given candidates (the learned proposal's): candidate i of pixel p is
+ * (cand_light, cand_q)[i * W*H + p], weighted p_hat / q. */
+void oracle_pass_a_candidates(const oracle_scene *s, const restir_uniforms *u, const restir_gbuffer *cur,
+                              const restir_gbuffer *prev, const restir_reservoir *prev_res,
+                              const int32_t *cand_light, const float *cand_q, restir_reservoir *out);
 void oracle_pass_b(const oracle_scene *s, const restir_uniforms *u, int iter,
                    const restir_gbuffer *g, const restir_reservoir *in, restir_reservoir *out);
 void oracle_pass_c(const oracle_scene *s, const restir_uniforms *u, int k, int iter,

@@ -176,6 +176,9 @@ int restir_set_lights(restir_ctx *ctx, const void *pointBuf, size_t pointBytes, 
                       size_t triBytes, const void *aliasBuf, size_t aliasBytes) {
     if (!ctx) return RESTIR_E_INVALID_ARGUMENT;
     ctx->lightsSet = false;
+    // The clusters were built over the previous light set: their per-light tables are sized for its
+    // light count and k_propose reads a snapshot of its alias table, so they must be bound again.
+    ctx->clustersSet = false;
     if (int r = use_device(ctx)) return r;
     if (!pointBuf || !aliasBuf)
         return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "set_lights: point-light and alias buffers are required");

@@ -273,4 +273,15 @@ int restir_mlp_loss_grad(restir_ctx *ctx, const restir_mlp *m, const uint16_t *X
     return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "mlp_loss_grad launch");
 }
 
+int restir_mlp_adam(restir_ctx *ctx, restir_mlp *m, const float *gw, const float *gb, float lr, void *stream) {
+    if (!ctx) return RESTIR_E_INVALID_ARGUMENT;
+    if (int r = check_mlp(ctx, m, "mlp_adam")) return r;
+    if (!m->weights || !m->adam_m || !m->adam_v || !gw || !gb)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "mlp_adam: NULL buffer");
+    if (!(lr >= 0.0f)) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "mlp_adam: learning rate must be >= 0");
+    if (int r = use_device(ctx)) return r;
+    const hipError_t e = rk::mlp_adam(*m, gw, gb, nullptr, lr, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "mlp_adam launch");
+}
+
 }  // extern "C"

@@ -145,6 +145,8 @@ hipError_t mlp_train(restir_mlp &m, const GView &g, rt::f3 cam, const float lo[3
                      const restir_reservoir *res, const int32_t *lightCluster, int32_t nLights, int32_t triBase,
                      uint32_t n, float lr,
                      uint32_t seed, float *lossOut, void *scratch, hipStream_t s);
+hipError_t mlp_adam(restir_mlp &m, const float *gw, const float *gb, const uint32_t *nValid, float lr,
+                    hipStream_t s);
 hipError_t mlp_propose(const restir_mlp &m, const GView &g, rt::f3 cam, const float lo[3],
                        const float hi[3], const SceneView &sc, const int32_t *lightCluster,
                        const float *lightPWithin, const int32_t *clusterBegin,

@@ -19,6 +19,7 @@
 //   k_adam         Adam step on fp32 master weights, refreshes the bf16 copy
 //   k_mlp_init     seeded He-uniform initialisation
 #include "../../include/restir_abi.h"
+#include "carve.h"
 #include "kernels.h"
 #include "mlp.h"
 
@@ -322,10 +323,10 @@ __global__ void k_adam_count(uint32_t *step) {
     if (threadIdx.x == 0) atomicAdd(step, 1u);
 }
 
-// Adam step; skipped when the batch held no training sample (*nValid == 0).
+// Adam step; skipped when the batch held no training sample (*nValid == 0; nValid null: always).
 __global__ __launch_bounds__(256) void k_adam_guarded(const AdamArgs a, const uint32_t *nValid) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= a.nW + a.nB || *nValid == 0u) return;
+    if (i >= a.nW + a.nB || (nValid && *nValid == 0u)) return;
     const bool isW = i < a.nW;
     const float g = isW ? a.gw[i] : a.gb[i - a.nW];
     const uint32_t t = a.stepDev ? *a.stepDev : a.stepHost;
@@ -1023,37 +1024,32 @@ static int split_for(int M, int N, int n) {
     while (sp < kMaxSplit && tiles * sp < 256 && (n % (2 * sp * 32)) == 0) sp *= 2;
     return sp;
 }
-size_t mlp_scratch_bytes(uint32_t n, int K) {
-    const size_t nn = n, kk = static_cast<size_t>(K);
-    return 16 * 256 + 2 * (nn * kIn * 6 + nn * kk) + 4 * (nn * kk + 3 * nn) +
-           4 * (rm::n_weights(K) + rm::n_bias(K)) + 4 * part_floats(K, n) + 4 * (nn / 4) + 256;
-}
-static MlpScratch carve_mlp(void *base, uint32_t n, int K) {
-    char *p = static_cast<char *>(base);
-    size_t off = 0;
-    auto take = [&](size_t bytes) {
-        char *r = p + off;
-        off = (off + bytes + 255) & ~static_cast<size_t>(255);
-        return r;
-    };
+static MlpScratch carve_mlp(void *base, uint32_t n, int K, size_t *bytes = nullptr) {
+    Carver c(base);
     const size_t nn = n, kk = static_cast<size_t>(K);
     MlpScratch m;
-    m.X = reinterpret_cast<uint16_t *>(take(2 * nn * kIn));
-    for (int l = 0; l < 3; ++l) m.H[l] = reinterpret_cast<uint16_t *>(take(2 * nn * kHidden));
-    m.dA = reinterpret_cast<uint16_t *>(take(2 * nn * kHidden));
-    m.dB = reinterpret_cast<uint16_t *>(take(2 * nn * kHidden));
-    m.dZ = reinterpret_cast<uint16_t *>(take(2 * nn * kk));
-    m.Z = reinterpret_cast<float *>(take(4 * nn * kk));
-    m.target = reinterpret_cast<int32_t *>(take(4 * nn));
-    m.mask = reinterpret_cast<uint32_t *>(take(4 * nn));
-    m.pixels = reinterpret_cast<uint32_t *>(take(4 * nn));
-    m.nValid = reinterpret_cast<uint32_t *>(take(256));
-    m.loss = reinterpret_cast<float *>(take(256));
-    m.gw = reinterpret_cast<float *>(take(4 * rm::n_weights(K)));
-    m.gb = reinterpret_cast<float *>(take(4 * rm::n_bias(K)));
-    m.part = reinterpret_cast<float *>(take(4 * part_floats(K, n)));
-    m.lossPart = reinterpret_cast<float *>(take(4 * (nn / 4)));
+    m.X = c.take<uint16_t>(2 * nn * kIn);
+    for (int l = 0; l < 3; ++l) m.H[l] = c.take<uint16_t>(2 * nn * kHidden);
+    m.dA = c.take<uint16_t>(2 * nn * kHidden);
+    m.dB = c.take<uint16_t>(2 * nn * kHidden);
+    m.dZ = c.take<uint16_t>(2 * nn * kk);
+    m.Z = c.take<float>(4 * nn * kk);
+    m.target = c.take<int32_t>(4 * nn);
+    m.mask = c.take<uint32_t>(4 * nn);
+    m.pixels = c.take<uint32_t>(4 * nn);
+    m.nValid = c.take<uint32_t>(256);
+    m.loss = c.take<float>(256);
+    m.gw = c.take<float>(4 * rm::n_weights(K));
+    m.gb = c.take<float>(4 * rm::n_bias(K));
+    m.part = c.take<float>(4 * part_floats(K, n));
+    m.lossPart = c.take<float>(4 * (nn / 4));
+    if (bytes) *bytes = c.end();
     return m;
+}
+size_t mlp_scratch_bytes(uint32_t n, int K) {
+    size_t bytes = 0;
+    carve_mlp(nullptr, n, K, &bytes);
+    return bytes;
 }
 
 // forward through the GEMM path, keeping the hidden activations for the backward pass
@@ -1219,14 +1215,25 @@ hipError_t mlp_train(restir_mlp &m, const GView &g, f3 cam, const float lo[3], c
     RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, t.nValid, 1u);
     RK_LAUNCH("k_train_gather", rm::k_train_gather, dim3((n + 255) / 256), dim3(256), 0, s, ga);
     mlp_loss_grad_impl(m, t.X, t.target, t.mask, n, t, t.gw, t.gb, t.loss, false, s);
+    mlp_adam(m, t.gw, t.gb, t.nValid, lr, s);
+    if (lossOut) RK_LAUNCH("k_copy_f32", rm::k_copy_f32, dim3(1), dim3(1), 0, s, t.loss, lossOut);
+    return hipGetLastError();
+}
+
+// One Adam step (beta 0.9 / 0.999, eps 1e-8) from the gradients gw, gb: advances the step count
+// (host, and the device copy when set) and updates the fp32 master weights, the moments and the
+// bf16 copy. nValid (device, may be null): the step is skipped when it reads 0.
+hipError_t mlp_adam(restir_mlp &m, const float *gw, const float *gb, const uint32_t *nValid, float lr,
+                    hipStream_t s) {
+    const int K = static_cast<int>(m.n_clusters);
     m.adam_step += 1;
     const double b1 = 0.9, b2 = 0.999;
     rm::AdamArgs a{};
     a.w = m.weights;
     a.b = m.bias;
     a.wbf = m.weights_bf16;
-    a.gw = t.gw;
-    a.gb = t.gb;
+    a.gw = gw;
+    a.gb = gb;
     a.m = m.adam_m;
     a.v = m.adam_v;
     a.nW = static_cast<uint32_t>(rm::n_weights(K));
@@ -1241,8 +1248,7 @@ hipError_t mlp_train(restir_mlp &m, const GView &g, f3 cam, const float lo[3], c
     a.stepDev = m.adam_step_device;
     if (m.adam_step_device)
         RK_LAUNCH("k_adam_count", rm::k_adam_count, dim3(1), dim3(64), 0, s, m.adam_step_device);
-    RK_LAUNCH("k_adam_guarded", rm::k_adam_guarded, dim3((a.nW + a.nB + 255) / 256), dim3(256), 0, s, a, t.nValid);
-    if (lossOut) RK_LAUNCH("k_copy_f32", rm::k_copy_f32, dim3(1), dim3(1), 0, s, t.loss, lossOut);
+    RK_LAUNCH("k_adam_guarded", rm::k_adam_guarded, dim3((a.nW + a.nB + 255) / 256), dim3(256), 0, s, a, nValid);
     return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "carve.h"
 #include "kernels.h"
 #include "scan.h"
 
@@ -1086,28 +1087,22 @@ static hipError_t dispatch_a(int tile, const PassAArgs &args, hipStream_t s) {
         default: return run_a<32, 8, STACK>(args, s);
     }
 }
-size_t pass_a_queue_bytes(uint32_t npx) {
-    const size_t n = npx;
-    return 256 + 2 * sizeof(float4) * n + sizeof(uint64_t) * n + sizeof(uint32_t) * n + 4 * 256;
-}
-
-static AQueues carve_a_queues(void *scratch, uint32_t npx) {
-    auto align = [](size_t v) { return (v + 255) & ~static_cast<size_t>(255); };
-    char *p = static_cast<char *>(scratch);
-    size_t off = 0;
-    auto take = [&](size_t bytes) {
-        char *r = p + off;
-        off = align(off + bytes);
-        return r;
-    };
+static AQueues carve_a_queues(void *scratch, uint32_t npx, size_t *bytes = nullptr) {
+    Carver c(scratch);
     const size_t n = npx;
     AQueues q;
-    q.counter = reinterpret_cast<uint32_t *>(take(256));
-    q.p1 = reinterpret_cast<float4 *>(take(sizeof(float4) * n));
-    q.p2 = reinterpret_cast<float4 *>(take(sizeof(float4) * n));
-    q.rng = reinterpret_cast<uint64_t *>(take(sizeof(uint64_t) * n));
-    q.occ = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * n));
+    q.counter = c.take<uint32_t>(256);
+    q.p1 = c.take<float4>(sizeof(float4) * n);
+    q.p2 = c.take<float4>(sizeof(float4) * n);
+    q.rng = c.take<uint64_t>(sizeof(uint64_t) * n);
+    q.occ = c.take<uint32_t>(sizeof(uint32_t) * n);
+    if (bytes) *bytes = c.end();
     return q;
+}
+size_t pass_a_queue_bytes(uint32_t npx) {
+    size_t bytes = 0;
+    carve_a_queues(nullptr, npx, &bytes);
+    return bytes;
 }
 
 template <int TX, int TY, int STACK>
@@ -1213,36 +1208,30 @@ static hipError_t dispatch_c_queued(int tile, const PassCArgs &args, const CQueu
     }
 }
 
-size_t pass_c_queue_bytes(uint32_t npx, int k) {
-    const size_t n = npx, kk = static_cast<size_t>(k);
-    return 256 + 2 * sizeof(float4) * n * (2 + kk) + sizeof(uint32_t) * n * (3 + 2 * kk) + 7 * 256;
-}
-
 // Carves the queues out of one scratch allocation of pass_c_queue_bytes(npx, k) bytes.
-static CQueues carve_queues(void *scratch, uint32_t npx, int k) {
-    auto align = [](size_t v) { return (v + 255) & ~static_cast<size_t>(255); };
-    char *p = static_cast<char *>(scratch);
-    size_t off = 0;
-    auto take = [&](size_t bytes) {
-        char *r = p + off;
-        off = align(off + bytes);
-        return r;
-    };
+static CQueues carve_queues(void *scratch, uint32_t npx, int k, size_t *bytes = nullptr) {
+    Carver c(scratch);
     const size_t n = npx, kk = static_cast<size_t>(k);
     CQueues q;
-    q.counter = reinterpret_cast<uint32_t *>(take(256));
-    q.c1 = reinterpret_cast<float4 *>(take(sizeof(float4) * n));
-    q.c2 = reinterpret_cast<float4 *>(take(sizeof(float4) * n));
-    q.n1 = reinterpret_cast<float4 *>(take(sizeof(float4) * n * kk));
-    q.n2 = reinterpret_cast<float4 *>(take(sizeof(float4) * n * kk));
-    q.origM = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * n));
-    q.bits = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * n));
-    q.occ = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * n));
-    q.nM = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * n * kk));
-    q.nIdx = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t) * n * kk));
-    q.surf = reinterpret_cast<float4 *>(take(2 * sizeof(float4) * n));
+    q.counter = c.take<uint32_t>(256);
+    q.c1 = c.take<float4>(sizeof(float4) * n);
+    q.c2 = c.take<float4>(sizeof(float4) * n);
+    q.n1 = c.take<float4>(sizeof(float4) * n * kk);
+    q.n2 = c.take<float4>(sizeof(float4) * n * kk);
+    q.origM = c.take<uint32_t>(sizeof(uint32_t) * n);
+    q.bits = c.take<uint32_t>(sizeof(uint32_t) * n);
+    q.occ = c.take<uint32_t>(sizeof(uint32_t) * n);
+    q.nM = c.take<uint32_t>(sizeof(uint32_t) * n * kk);
+    q.nIdx = c.take<uint32_t>(sizeof(uint32_t) * n * kk);
+    q.surf = c.take<float4>(2 * sizeof(float4) * n);
     q.npx = npx;
+    if (bytes) *bytes = c.end();
     return q;
+}
+size_t pass_c_queue_bytes(uint32_t npx, int k) {
+    size_t bytes = 0;
+    carve_queues(nullptr, npx, k, &bytes);
+    return bytes;
 }
 
 __global__ void k_traced_flag(const CQueues q, uint32_t len, uint32_t *flag) {

@@ -21,7 +21,7 @@ TEMPORAL_REUSE_FLAG = 1 << 1
 TRACE_FIRST_HIT = 0
 TRACE_REFERENCE_LOOP = 1
 TRACE_ONE_RAY_PER_LANE = 2
-ABI_VERSION = 9  # RESTIR_ABI_VERSION of include/restir_abi.h
+ABI_VERSION = 10  # RESTIR_ABI_VERSION of include/restir_abi.h
 
 ERRORS = {
     -1: "RESTIR_E_INVALID_ARGUMENT",
@@ -196,6 +196,7 @@ SIGNATURES = {
     "restir_mlp_forward": (_i32, [_vp, C.POINTER(MlpStruct), _vp, _u32, _vp, _vp]),
     "restir_mlp_loss_grad": (_i32, [_vp, C.POINTER(MlpStruct), _vp, _vp, _vp, _u32, _vp, _vp, _vp,
                                     _vp]),
+    "restir_mlp_adam": (_i32, [_vp, C.POINTER(MlpStruct), _vp, _vp, _f32, _vp]),
 }
 
 
@@ -604,6 +605,10 @@ class Context:
     def mlp_forward(self, mlp, X, n, logits, stream=None):
         self._check(self.lib.restir_mlp_forward(self.h, C.byref(mlp), as_ptr(X), n, as_ptr(logits),
                                                 stream_ptr(stream)), "restir_mlp_forward")
+
+    def mlp_adam(self, mlp, gw, gb, lr, stream=None):
+        self._check(self.lib.restir_mlp_adam(self.h, C.byref(mlp), as_ptr(gw), as_ptr(gb), lr,
+                                             stream_ptr(stream)), "restir_mlp_adam")
 
     def mlp_loss_grad(self, mlp, X, target, mask, n, gw, gb, loss, stream=None):
         self._check(self.lib.restir_mlp_loss_grad(

@@ -48,7 +48,17 @@ def _exponential_chain():
     return t
 
 
+def _hand_worked_swap_partition():
+    """The 6-triangle tree of tests/golden/bvh_swap_partition_hand.json (worked by hand; its leaf
+    order shows the reference's unstable swap partition)."""
+    import json
+    from pathlib import Path
+    g = json.loads((Path(__file__).resolve().parent / "golden" / "bvh_swap_partition_hand.json").read_text())
+    return np.array([[cx - 0.5, 0, 0, cx + 0.5, 1, 0, cx, 0, 1] for cx in g["centroid_x"]], dtype=np.float32)
+
+
 CASES = {
+    "hand_worked_swap_partition": lambda rng: _hand_worked_swap_partition(),
     "two": lambda rng: rng.standard_normal((2, 9)).astype(np.float32),
     "three": lambda rng: rng.standard_normal((3, 9)).astype(np.float32),
     "soup_17": lambda rng: rng.standard_normal((17, 9)).astype(np.float32),

@@ -288,7 +288,7 @@ def _check_tree(nodes, tris_in, tris_out):
                 np.testing.assert_array_equal(mn, tmn)
                 np.testing.assert_array_equal(mx, tmx)
             else:
-                assert c > i  # breadth-first numbering: children after parents
+                assert c > i  # children after parents
                 child = nodes[c]
                 np.testing.assert_array_equal(mn, np.minimum(child["lmin"][:3], child["rmin"][:3]))
                 np.testing.assert_array_equal(mx, np.maximum(child["lmax"][:3], child["rmax"][:3]))
@@ -296,6 +296,72 @@ def _check_tree(nodes, tris_in, tris_out):
     assert np.all(seen == 1)
     assert sorted(order) == list(range(n - 1))
     assert np.all(nodes["lmin"][:, 3] == 1.0) and np.all(nodes["rmax"][:, 3] == 1.0)
+    assert bfs_order(nodes) == list(range(n - 1))
+
+
+def bfs_order(nodes):
+    """Node ids met by a level-order walk that queues each node's internal children left, then
+    right. The reference pops index ranges from a FIFO and numbers a node when its range is popped
+    (aabbTreeBuilder.cpp:84-210), so its ids are exactly 0, 1, 2, ... in this order: every id of
+    level d precedes every id of level d + 1, and within a level ids follow the parents' order."""
+    import collections
+    seen, q = [], collections.deque([0])
+    while q:
+        i = q.popleft()
+        seen.append(i)
+        for c in (int(nodes[i]["left"]), int(nodes[i]["right"])):
+            if c >= 0:
+                q.append(c)
+    return seen
+
+
+def _hand_tree_triangles():
+    g = json.loads((GOLDEN / "bvh_swap_partition_hand.json").read_text())
+    tri = np.array([[cx - 0.5, 0, 0, cx + 0.5, 1, 0, cx, 0, 1] for cx in g["centroid_x"]], dtype=np.float32)
+    return g, tri
+
+
+def _assert_hand_tree(nodes, g):
+    assert len(nodes) == len(g["nodes"])
+    for i, e in enumerate(g["nodes"]):
+        assert (int(nodes[i]["left"]), int(nodes[i]["right"])) == (e["left"], e["right"]), i
+        for f in ("lmin", "lmax", "rmin", "rmax"):
+            np.testing.assert_array_equal(nodes[i][f][:3], np.array(e[f], dtype=np.float32), err_msg=f"{i} {f}")
+
+
+def test_bvh_hand_worked_swap_partition_and_bfs_numbering():
+    """A 6-triangle tree worked by hand (tests/golden/bvh_swap_partition_hand.json) whose node
+    4 lists its leaves in the order the reference's unstable swap partition leaves them (a stable
+    partition swaps them) and whose ids are breadth-first (a depth-first numbering moves node 3)."""
+    g, tri = _hand_tree_triangles()
+    nodes, tris, depth = oracle.build_aabb_tree(tri, abi.NODE_DTYPE, abi.TRIANGLE_DTYPE)
+    _assert_hand_tree(nodes, g)
+    _check_tree(nodes, tri, tris)
+    assert depth == 3
+
+
+def test_bvh_hand_worked_tree_host_builder():
+    g, tri = _hand_tree_triangles()
+    nodes, tris, depth = abi.build_aabb_tree(tri)
+    _assert_hand_tree(nodes, g)
+
+
+def test_bfs_check_rejects_depth_first_numbering():
+    """The level-order check fails on the same tree renumbered depth-first."""
+    g, tri = _hand_tree_triangles()
+    nodes, _, _ = oracle.build_aabb_tree(tri, abi.NODE_DTYPE, abi.TRIANGLE_DTYPE)
+    dfs, st = [], [0]
+    while st:
+        i = st.pop()
+        dfs.append(i)
+        for c in (int(nodes[i]["right"]), int(nodes[i]["left"])):
+            if c >= 0:
+                st.append(c)
+    new_id = {old: new for new, old in enumerate(dfs)}
+    ren = nodes[dfs].copy()
+    for f in ("left", "right"):
+        ren[f] = [new_id[c] if c >= 0 else c for c in ren[f]]
+    assert bfs_order(ren) != list(range(len(ren)))
 
 
 def test_bvh_two_triangles():

@@ -1,0 +1,374 @@
+"""Pins of the oracle's per-pixel passes by invariants and closed forms the reference's shaders fix:
+
+- Pass B (spatialReuse.comp:30-85): k = 0 is the identity; neighbours across a depth step or a
+  normal change beyond the threshold are rejected (and accepted below it); neighbours outside the
+  image are clamped onto it (a 1x1 image combines the pixel with itself k times).
+- Pass A temporal reuse (restirOmni.glsl:163-209): a static camera reprojects every pixel onto
+  itself; the previous reservoir's M is capped at temporalSampleCountMultiplier * M; a zeroed
+  previous buffer changes nothing but W, which combineReservoirs recomputes as sumW / (M p_hat)
+  (reservoir.glsl:59-61).
+- Pass A over given candidates (the learned proposal's RIS): closed-form sums for repeated and empty
+  candidates, and the triangle-light point drawn from the first two numbers of the pixel's stream.
+
+Each check uses only pinned building blocks (PCG32 and p_hat are pinned in test_oracle_pins.py).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from restir_amd import abi, scenes
+from rt_helpers import HostScene, reservoir_mismatch
+
+PLANES = 13
+
+
+# ------------------------------------------------------------------------------ flat G-buffer
+def flat_gbuffer(W, H, depth=None, normal=None):
+    """A flat floor z = 0 seen from above: pixel (x, y) at (x, y, 0), normal +z, albedo 0.5,
+    roughness 0.5, metallic 0; depth and normal per pixel may be overridden ((H, W) / (H, W, 3))."""
+    P = W * H
+    pl = np.zeros((PLANES, P), dtype=np.float32)
+    ys, xs = np.mgrid[0:H, 0:W]
+    pl[0], pl[1] = xs.ravel(), ys.ravel()
+    nrm = np.zeros((H, W, 3), dtype=np.float32)
+    nrm[..., 2] = 1.0
+    if normal is not None:
+        nrm = normal.astype(np.float32)
+    pl[3:6] = nrm.reshape(-1, 3).T
+    pl[6:9] = 0.5
+    pl[10] = 0.5
+    pl[12] = 1.0 if depth is None else depth.astype(np.float32).ravel()
+    return pl.reshape(-1)
+
+
+def phat_at(pl, W, H, p, light_pos, cam):
+    P = W * H
+    pv = pl.reshape(PLANES, P)
+    pos = np.ascontiguousarray(pv[0:3, p], dtype=np.float32)
+    n = np.ascontiguousarray(pv[3:6, p], dtype=np.float32)
+    L = np.ascontiguousarray(light_pos, dtype=np.float32)
+    c = np.ascontiguousarray(cam, dtype=np.float32)
+    zero = np.zeros(3, dtype=np.float32)
+    alb_lum = oracle.lib().oracle_luminance(*[float(v) for v in pv[6:9, p]])
+    return oracle.lib().oracle_phat(pos.ctypes.data, L.ctypes.data, c.ctypes.data, n.ctypes.data,
+                                    zero.ctypes.data, 0, alb_lum, 5.0, float(pv[10, p]), float(pv[11, p]))
+
+
+def two_light_reservoirs(pl, W, H, cam, split_x):
+    """Pixels left of split_x hold light 0 at (W/4, H/2, 8); the others light 1 at (3W/4, H/2, 8).
+    Each reservoir is consistent: M = 1, p_hat at its own pixel, W = 1, sumW = p_hat."""
+    res = np.zeros(W * H, dtype=abi.RESERVOIR_DTYPE)
+    lights = [np.array([W / 4, H / 2, 8.0], np.float32), np.array([3 * W / 4, H / 2, 8.0], np.float32)]
+    for p in range(W * H):
+        k = 0 if (p % W) < split_x else 1
+        ph = phat_at(pl, W, H, p, lights[k], cam)
+        assert ph > 0
+        res[p]["pos_lum"][:3] = lights[k]
+        res[p]["pos_lum"][3] = 5.0
+        res[p]["lightIndex"] = k
+        res[p]["pHat"] = ph
+        res[p]["sumWeights"] = ph
+        res[p]["w"] = 1.0
+        res[p]["M"] = 1
+    return res
+
+
+def empty_scene():
+    tri = np.array([[0, 0, -50, 1, 0, -50, 0, 1, -50], [5, 0, -50, 6, 0, -50, 5, 1, -50]], np.float32)
+    nodes, tris, _ = abi.build_aabb_tree(tri)
+    pts = scenes.make_point_lights(np.zeros((1, 3), np.float32), np.ones((1, 3), np.float32))
+    return oracle.Scene(nodes, tris, pts, None, abi.build_alias_table(pts, None))
+
+
+def pass_b(W, H, k, pl, res, cam, it=0, frame=3, radius=30.0, normal_thr=25.0):
+    u = abi.make_uniforms(W, H, frame, cam, neighbors=k, radius=radius, normal_threshold=normal_thr)
+    return oracle.pass_b(empty_scene(), u, it, oracle.gbuffer(pl, W, H), res)
+
+
+CAM = np.array([16.0, 12.0, 30.0], np.float32)
+
+
+# ------------------------------------------------------------------------------------ Pass B
+def test_pass_b_zero_neighbours_is_identity():
+    W, H = 32, 24
+    pl = flat_gbuffer(W, H)
+    res = two_light_reservoirs(pl, W, H, CAM, W // 2)
+    out = pass_b(W, H, 0, pl, res, CAM)
+    assert not reservoir_mismatch(out, res).any()
+
+
+def _side_mixing(out, W, split_x):
+    x = np.arange(len(out)) % W
+    left_has_1 = (out["lightIndex"][x < split_x] == 1).sum()
+    right_has_0 = (out["lightIndex"][x >= split_x] == 0).sum()
+    return left_has_1, right_has_0
+
+
+def test_pass_b_rejects_across_a_depth_step():
+    W, H, k = 32, 24, 5
+    split = W // 2
+    pl0 = flat_gbuffer(W, H)
+    res = two_light_reservoirs(pl0, W, H, CAM, split)
+    # uniform depth: neighbours cross the middle, so some pixels pick up the other side's light
+    mixed = pass_b(W, H, k, pl0, res, CAM)
+    assert min(_side_mixing(mixed, W, split)) > 0
+    assert (mixed["M"] == 1 + k).all()  # every neighbour accepted, each adds M = 1
+    # depth 1 left, 1.5 right: |1.5 - 1| exceeds 0.1 * depth on both sides
+    depth = np.where(np.arange(W)[None, :] < split, 1.0, 1.5) * np.ones((H, 1))
+    pl = flat_gbuffer(W, H, depth=depth)
+    out = pass_b(W, H, k, pl, res, CAM)
+    assert _side_mixing(out, W, split) == (0, 0)
+    # a rejected neighbour adds nothing to M: M - 1 counts the same-side neighbours only
+    assert (out["M"] <= mixed["M"]).all() and (out["M"] < mixed["M"]).any()
+    # pixels whose whole 30 px disc stays on one side (none here: every disc crosses) would be
+    # unchanged; with a step of 0.09 (under 0.1 * depth) nothing is rejected
+    depth2 = np.where(np.arange(W)[None, :] < split, 1.0, 1.09) * np.ones((H, 1))
+    out2 = pass_b(W, H, k, flat_gbuffer(W, H, depth=depth2), res, CAM)
+    assert (out2["M"] == 1 + k).all()
+
+
+@pytest.mark.parametrize("tilt_deg,rejected", [(30.0, True), (20.0, False)])
+def test_pass_b_normal_threshold(tilt_deg, rejected):
+    """The right half's normal is tilted about y by tilt_deg: beyond the 25 degree threshold its
+    neighbours are rejected from the left half and vice versa; below it they are accepted."""
+    W, H, k = 32, 24, 5
+    split = W // 2
+    t = np.radians(tilt_deg)
+    nrm = np.zeros((H, W, 3), np.float32)
+    nrm[..., 2] = 1.0
+    nrm[:, split:, 0], nrm[:, split:, 2] = np.sin(t), np.cos(t)
+    pl = flat_gbuffer(W, H, normal=nrm)
+    res = two_light_reservoirs(pl, W, H, CAM, split)
+    out = pass_b(W, H, k, pl, res, CAM)
+    if rejected:
+        assert _side_mixing(out, W, split) == (0, 0)
+        assert (out["M"] < 1 + k).any()
+    else:
+        assert (out["M"] == 1 + k).all()
+        assert min(_side_mixing(out, W, split)) > 0
+
+
+def test_pass_b_single_pixel_combines_itself_k_times():
+    """1x1 image: every neighbour offset clamps onto the pixel, so it is combined with its own
+    input reservoir k times: M = (1 + k) M_in, sumW = sumW_in + k p_hat W_in M_in, W unchanged."""
+    k = 7
+    pl = flat_gbuffer(1, 1)
+    res = two_light_reservoirs(pl, 1, 1, CAM, 1)
+    res["M"] = 3
+    res["sumWeights"] = res["pHat"] * 3  # consistent: W = sumW / (M p_hat) = 1
+    out = pass_b(1, 1, k, pl, res, CAM)
+    ph = float(res["pHat"][0])
+    assert out["M"][0] == (1 + k) * 3
+    assert out["lightIndex"][0] == 0
+    assert out["sumWeights"][0] == pytest.approx(ph * 3 * (1 + k), rel=1e-6)
+    assert out["w"][0] == pytest.approx(1.0, rel=1e-6)
+
+
+def test_pass_b_clamps_neighbours_at_the_border():
+    """A 3x2 image with r = 30: nearly every offset leaves the image; clamped neighbours are
+    still combined (M grows by k for every pixel), and each output light names a pixel of the
+    image (distinct light per pixel)."""
+    W, H, k = 3, 2, 9
+    pl = flat_gbuffer(W, H)
+    res = two_light_reservoirs(pl, W, H, CAM, 2)
+    res["lightIndex"] = np.arange(W * H) + 100
+    out = pass_b(W, H, k, pl, res, CAM)
+    assert (out["M"] == 1 + k).all()
+    assert set(out["lightIndex"].tolist()) <= set(range(100, 100 + W * H))
+
+
+# ------------------------------------------------------------------------- Pass A temporal
+@pytest.fixture(scope="module")
+def cornell():
+    sc = scenes.cornell_box(1)
+    hs = HostScene(sc)
+    W, H = 96, 72
+    g = hs.oracle_gbuffer(W, H)
+    pv = abi.projection_view(sc.camera, W / H)
+    return sc, hs, W, H, g, pv
+
+
+def _prev_with_unique_lights(g, W, H, M, lum=1e6):
+    """Previous reservoirs naming light 100000 + p at a point near the ceiling, very bright, so
+    that a reused previous sample is nearly always the one selected."""
+    prev = np.zeros(W * H, dtype=abi.RESERVOIR_DTYPE)
+    prev["pos_lum"][:, :3] = [0.1, 1.9, 0.2]
+    prev["pos_lum"][:, 3] = lum
+    prev["lightIndex"] = 100000 + np.arange(W * H)
+    prev["pHat"] = 1.0
+    prev["w"] = 1.0
+    prev["sumWeights"] = float(M)
+    prev["M"] = M
+    return prev
+
+
+def _surface(g, W, H):
+    n = g[3 * W * H: 6 * W * H].reshape(3, -1)
+    return np.any(n != 0, axis=0)
+
+
+def test_pass_a_static_camera_reprojects_every_pixel_onto_itself(cornell):
+    sc, hs, W, H, g, pv = cornell
+    u = abi.make_uniforms(W, H, 5, sc.camera["position"], prev_pv=pv, M=32, flags=abi.TEMPORAL_REUSE_FLAG)
+    prev = _prev_with_unique_lights(g, W, H, 7)
+    og = oracle.gbuffer(g, W, H)
+    out = oracle.pass_a(hs.oracle, u, og, og, prev, abi.RESERVOIR_DTYPE)
+    surf = _surface(g, W, H)
+    assert (out["M"][surf] == 32 + 7).mean() > 0.999
+    reused = surf & (out["lightIndex"] >= 100000)
+    # (the rest face away from the previous samples' point: p_hat = 0 there)
+    assert reused.mean() > 0.8 * surf.mean()
+    # a reused sample is always the pixel's own previous one
+    np.testing.assert_array_equal(out["lightIndex"][reused], 100000 + np.flatnonzero(reused))
+    # background: M stays 0 (the cap 20 * 0 empties the previous reservoir)
+    assert (out["M"][~surf] == 0).all()
+
+
+@pytest.mark.parametrize("M,prevM", [(32, 100000), (8, 100000), (8, 150)])
+def test_pass_a_temporal_history_cap(cornell, M, prevM):
+    sc, hs, W, H, g, pv = cornell
+    u = abi.make_uniforms(W, H, 5, sc.camera["position"], prev_pv=pv, M=M, flags=abi.TEMPORAL_REUSE_FLAG)
+    prev = _prev_with_unique_lights(g, W, H, prevM)
+    og = oracle.gbuffer(g, W, H)
+    out = oracle.pass_a(hs.oracle, u, og, og, prev, abi.RESERVOIR_DTYPE)
+    surf = _surface(g, W, H)
+    assert (out["M"][surf] == M + min(prevM, 20 * M)).mean() > 0.999
+
+
+def test_pass_a_zeroed_history_only_recomputes_w(cornell):
+    """With an all-zero previous buffer, temporal reuse combines an empty reservoir: no draw, M
+    and sumW unchanged, and W = sumW / (M p_hat) replaces addSampleToReservoir's selection-time W
+    (reservoir.glsl:33 vs :59-61)."""
+    sc, hs, W, H, g, pv = cornell
+    og = oracle.gbuffer(g, W, H)
+    zero = np.zeros(W * H, dtype=abi.RESERVOIR_DTYPE)
+    u_t = abi.make_uniforms(W, H, 9, sc.camera["position"], prev_pv=pv, flags=abi.TEMPORAL_REUSE_FLAG)
+    u_n = abi.make_uniforms(W, H, 9, sc.camera["position"], prev_pv=pv, flags=0)
+    a_t = oracle.pass_a(hs.oracle, u_t, og, og, zero, abi.RESERVOIR_DTYPE)
+    a_n = oracle.pass_a(hs.oracle, u_n, og, og, zero, abi.RESERVOIR_DTYPE)
+    for f in ("lightIndex", "M", "sumWeights", "pHat", "pos_lum"):
+        np.testing.assert_array_equal(a_t[f], a_n[f], err_msg=f)
+    lit = a_n["w"] > 0
+    assert lit.mean() > 0.5
+    expect = (a_n["sumWeights"][lit] / (a_n["M"][lit].astype(np.float32) * a_n["pHat"][lit])).astype(np.float32)
+    np.testing.assert_array_equal(a_t["w"][lit], expect)
+    # selection-time W differs from the recomputed one wherever the last candidate was not selected
+    assert (a_t["w"][lit] != a_n["w"][lit]).mean() > 0.5
+
+
+# ----------------------------------------------------------------- Pass A over candidates
+def _cand_uniforms(sc, W, H, M, frame=4):
+    return abi.make_uniforms(W, H, frame, sc.camera["position"], M=M, flags=0)
+
+
+def test_candidates_repeated_light_closed_form(cornell):
+    """Every candidate of pixel p is light (p mod 64) with q = 0.25: the reservoir holds that
+    light with sumW = M p_hat / q (summed in order in fp32), M candidates, p_hat of that light."""
+    sc, hs, W, H, g, _ = cornell
+    M = 6
+    u = _cand_uniforms(sc, W, H, M)
+    npx = W * H
+    li = np.tile((np.arange(npx) % 64).astype(np.int32), (M, 1))
+    q = np.full((M, npx), 0.25, np.float32)
+    og = oracle.gbuffer(g, W, H)
+    out = oracle.pass_a_candidates(hs.oracle, u, og, None, None, li, q, abi.RESERVOIR_DTYPE)
+    surf = _surface(g, W, H)
+    s = out[surf]
+    assert (s["M"] == M).all()
+    lit = s["sumWeights"] > 0
+    assert lit.mean() > 0.5
+    # where the light faces the pixel it is selected; elsewhere (p_hat = 0) nothing is
+    np.testing.assert_array_equal(s["lightIndex"][lit], (np.flatnonzero(surf) % 64)[lit])
+    assert (s["lightIndex"][~lit] == 0).all() and (s["w"][~lit] == 0).all()
+    w1 = (s["pHat"][lit] / np.float32(0.25)).astype(np.float32)
+    acc = np.zeros_like(w1)
+    for _ in range(M):
+        acc = (acc + w1).astype(np.float32)
+    np.testing.assert_array_equal(s["sumWeights"][lit], acc)
+    # p_hat is the target function of that light at the pixel
+    pts = sc.point_lights
+    P = W * H
+    for p in np.flatnonzero(surf)[::97][:40]:
+        l = p % 64
+        pos = np.ascontiguousarray(g[[p, P + p, 2 * P + p]], np.float32)
+        n = np.ascontiguousarray(g[[3 * P + p, 4 * P + p, 5 * P + p]], np.float32)
+        alb = g[[6 * P + p, 7 * P + p, 8 * P + p]]
+        L = np.ascontiguousarray(pts["pos"][l, :3], np.float32)
+        cam = np.ascontiguousarray(sc.camera["position"], np.float32)
+        zero = np.zeros(3, np.float32)
+        ph = oracle.lib().oracle_phat(pos.ctypes.data, L.ctypes.data, cam.ctypes.data, n.ctypes.data,
+                                      zero.ctypes.data, 0, oracle.lib().oracle_luminance(*map(float, alb)),
+                                      float(pts["color_lum"][l, 3]), float(g[10 * P + p]), float(g[11 * P + p]))
+        assert out["pHat"][p] == np.float32(ph)
+
+
+def test_candidates_empty_entries_count_in_m_only(cornell):
+    """Entries with light -1, an index past the last light, or q = 0 count in M but add no
+    weight; the valid half gives sumW = (M/2) p_hat / q."""
+    sc, hs, W, H, g, _ = cornell
+    M = 8
+    u = _cand_uniforms(sc, W, H, M)
+    npx = W * H
+    li = np.tile((np.arange(npx) % 64).astype(np.int32), (M, 1))
+    q = np.full((M, npx), 0.5, np.float32)
+    li[1], li[3] = -1, 64  # before and past the light range
+    q[5] = 0.0
+    li[7] = -7
+    og = oracle.gbuffer(g, W, H)
+    out = oracle.pass_a_candidates(hs.oracle, u, og, None, None, li, q, abi.RESERVOIR_DTYPE)
+    surf = _surface(g, W, H)
+    s = out[surf]
+    assert (s["M"] == M).all()
+    lit = s["pHat"] > 0
+    w1 = (s["pHat"][lit] / np.float32(0.5)).astype(np.float32)
+    acc = np.zeros_like(w1)
+    for _ in range(4):
+        acc = (acc + w1).astype(np.float32)
+    np.testing.assert_array_equal(s["sumWeights"][lit], acc)
+
+
+def test_candidates_triangle_light_point_from_first_two_draws():
+    """One valid triangle-light candidate (then empty ones): its point is pickPointOnTriangle of
+    the pixel stream's first two numbers (restirOmni.glsl:68-71, 122-125) and its weight is
+    p_hat / (q / (|cos| area)); the reservoir names light -1 - t with the light normal set."""
+    sc = scenes.cornell_box(1)
+    tri = sc.tri_xyz.reshape(-1, 3, 3)
+    ceiling = np.flatnonzero(np.all(np.abs(tri[:, :, 1] - tri[:, :, 1].max()) < 1e-6, axis=1))[:2]
+    t = tri[ceiling].copy()
+    t[:, :, 1] -= 0.01
+    tls = scenes.make_tri_lights(t.reshape(-1, 9), np.full((2, 3), 3.0))
+    hs = HostScene(sc)
+    orc = oracle.Scene(hs.nodes, hs.tris, None, tls, abi.build_alias_table(None, tls))
+    W, H, M = 48, 36, 3
+    g = hs.oracle_gbuffer(W, H)
+    u = _cand_uniforms(sc, W, H, M, frame=11)
+    npx = W * H
+    li = np.full((M, npx), -1, np.int32)
+    li[0] = np.arange(npx) % 2
+    q = np.full((M, npx), 0.5, np.float32)
+    out = oracle.pass_a_candidates(orc, u, oracle.gbuffer(g, W, H), None, None, li, q, abi.RESERVOIR_DTYPE)
+    surf = _surface(g, W, H)
+    checked = 0
+    for p in np.flatnonzero(surf)[::7]:
+        x, y = p % W, p // W
+        r = oracle.pcg32_float(11, y * 10007 + x, 2)
+        k = p % 2
+        p1, p2, p3 = (tls[k][f][:3].astype(np.float32) for f in ("p1", "p2", "p3"))
+        sa = np.float32(np.sqrt(r[0]))
+        L = ((np.float32(1) - sa) * p1 + (sa * (np.float32(1) - r[1])) * p2) + (r[1] * sa) * p3
+        assert out["M"][p] == M
+        if out["sumWeights"][p] == 0:
+            continue
+        assert out["lightIndex"][p] == -1 - k and out["normal"][p][3] == 1.0
+        np.testing.assert_array_equal(out["pos_lum"][p][:3], L.astype(np.float32))
+        pos = g[[p, npx + p, 2 * npx + p]].astype(np.float64)
+        wi = pos - L
+        wi /= np.linalg.norm(wi)
+        nl = tls[k]["normal_area"][:3].astype(np.float64)
+        area = float(tls[k]["normal_area"][3])
+        expect = out["pHat"][p] / (0.5 / (abs(wi @ nl) * area))
+        assert out["sumWeights"][p] == pytest.approx(expect, rel=1e-5)
+        checked += 1
+    assert checked > 50

@@ -173,7 +173,7 @@ void restir_destroy(restir_ctx *ctx);
 const char *restir_last_error(const restir_ctx *ctx);
 /* ABI revision; bumped whenever an entry point's signature changes. Bindings compare it with the
  * revision they were written against and refuse a mismatched library. */
-#define RESTIR_ABI_VERSION 10
+#define RESTIR_ABI_VERSION 11
 int restir_version(void);
 
 /* Light tables: device pointers to the reference buffer images (header + array). The counts are
@@ -212,6 +212,23 @@ int restir_build_bvh(restir_ctx *ctx, const float *tri_xyz, uint32_t n_tris,
 int restir_initial_temporal(restir_ctx *ctx, const restir_uniforms *u, const restir_gbuffer *cur,
                             const restir_gbuffer *prev, const restir_reservoir *prev_res,
                             restir_reservoir *out, const restir_launch *launch, void *stream);
+
+/* No counterpart in the reference: restir_initial_temporal for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments and results (bit for bit); the queued schedule's scratch (ray queue,
+ * random-stream states, shadow bits: 44 bytes per pixel + 1 KB) comes from a static device array
+ * of RESTIR_NOALLOC_SCRATCH_BYTES that each device holds from library load until the process ends,
+ * instead of the context's allocation. Calls no hipMalloc/hipFree; frames above
+ * RESTIR_NOALLOC_MAX_PIXELS pixels are rejected (RESTIR_E_INVALID_ARGUMENT) on the queued schedule
+ * (the inline schedule needs no scratch). All contexts on a device share its array, which keeps
+ * the last call's contents: calls that overlap on one device (other streams, other threads) race.
+ * Per-kernel timing (restir_profile_enable) does not cover this entry point. */
+#define RESTIR_NOALLOC_SCRATCH_BYTES (96ull << 20)
+#define RESTIR_NOALLOC_MAX_PIXELS 2287792u /* e.g. 1920x1080 or 2048x1080 */
+int restir_initial_temporal_noalloc(restir_ctx *ctx, const restir_uniforms *u,
+                                    const restir_gbuffer *cur, const restir_gbuffer *prev,
+                                    const restir_reservoir *prev_res, restir_reservoir *out,
+                                    const restir_launch *launch, void *stream);
 
 /* Pass B: biased spatial reuse, push constant iter (spatialReuse.comp:30-85). */
 int restir_spatial(restir_ctx *ctx, const restir_uniforms *u, int iter, const restir_gbuffer *g,

@@ -6,11 +6,20 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "ctx.h"
 #include "kernels.h"
+
+// ---- static scratch of restir_initial_temporal_noalloc ------------------------------------
+// The queued Pass A schedule's scratch (ray queue, random-stream states, shadow bits:
+// pass_a_queue_bytes(W*H) bytes) for callers that cannot allocate device memory. Each device holds
+// its own copy from module load until the process ends. Host code never uses the name: its
+// device address is resolved once per device (rki::resolve_noalloc_scratch) and passed to the
+// kernels.
+static __device__ __align__(256) unsigned char scratch[RESTIR_NOALLOC_SCRATCH_BYTES];
 
 namespace rki {
 
@@ -140,6 +149,26 @@ int pick_tile(restir_ctx *ctx, const restir_launch *l, int *tile) {
     return RESTIR_OK;
 }
 
+constexpr int kMaxDevices = 64;
+std::mutex g_noallocMu;
+void *g_noallocPtr[kMaxDevices] = {};
+
+// The device address of `scratch` on ctx->device, resolved once per device; called by
+// restir_create with that device current.
+int resolve_noalloc_scratch(restir_ctx *ctx) {
+    if (ctx->device >= kMaxDevices)
+        return fail(ctx, RESTIR_E_NO_DEVICE, "device %d: at most %d devices", ctx->device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_noallocMu);
+    if (!g_noallocPtr[ctx->device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) return hip_fail(ctx, e, "hipGetSymbolAddress(scratch)");
+        g_noallocPtr[ctx->device] = ptr;
+    }
+    ctx->noallocScratch = g_noallocPtr[ctx->device];
+    return RESTIR_OK;
+}
+
 }  // namespace rki
 
 using namespace rki;
@@ -156,7 +185,8 @@ int restir_create(restir_ctx **out, int device) {
     if (device < 0 || device >= n) return RESTIR_E_NO_DEVICE;
     auto *ctx = new restir_ctx();
     ctx->device = device;
-    if (use_device(ctx) != RESTIR_OK || hipMalloc(&ctx->counters, 256) != hipSuccess) {
+    if (use_device(ctx) != RESTIR_OK || hipMalloc(&ctx->counters, 256) != hipSuccess ||
+        resolve_noalloc_scratch(ctx) != RESTIR_OK) {
         delete ctx;
         return RESTIR_E_HIP;
     }
@@ -275,24 +305,32 @@ int restir_set_bvh(restir_ctx *ctx, const restir_aabb_tree_node *nodes, uint32_t
     return bind_tree(ctx, nodes, nNodes, tris, depth, nullptr);
 }
 
+// Validation shared by the Pass A entry points (check_frame has passed).
+int check_pass_a(restir_ctx *ctx, const restir_uniforms *u, const restir_gbuffer *cur, const restir_gbuffer *prev,
+                 const restir_reservoir *prevRes, const restir_reservoir *out, const char *name) {
+    if (!ctx->lightsSet) return fail(ctx, RESTIR_E_NOT_READY, "%s: lights not set", name);
+    const bool vis = (u->flags & RESTIR_VISIBILITY_REUSE_FLAG) != 0;
+    const bool temporal = (u->flags & RESTIR_TEMPORAL_REUSE_FLAG) != 0;
+    if (vis && !ctx->bvhSet) return fail(ctx, RESTIR_E_NOT_READY, "%s: BVH not set", name);
+    const uint32_t W = u->screenSize[0], H = u->screenSize[1];
+    if (int r = check_gbuffer(ctx, cur, W, H, false, name)) return r;
+    if (!out) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "%s: output is NULL", name);
+    const size_t bytes = sizeof(restir_reservoir) * W * H;
+    if (temporal) {
+        if (int r = check_gbuffer(ctx, prev, W, H, false, name)) return r;
+        if (!prevRes) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "%s: previous reservoirs are NULL", name);
+        if (overlaps(prevRes, bytes, out, bytes))
+            return fail(ctx, RESTIR_E_ALIASING, "%s: output aliases the previous reservoirs", name);
+    }
+    return RESTIR_OK;
+}
+
 int restir_initial_temporal(restir_ctx *ctx, const restir_uniforms *u, const restir_gbuffer *cur,
                             const restir_gbuffer *prev, const restir_reservoir *prevRes,
                             restir_reservoir *out, const restir_launch *launch, void *stream) {
     if (int r = check_frame(ctx, u, "initial_temporal")) return r;
-    if (!ctx->lightsSet) return fail(ctx, RESTIR_E_NOT_READY, "initial_temporal: lights not set");
-    const bool vis = (u->flags & RESTIR_VISIBILITY_REUSE_FLAG) != 0;
-    const bool temporal = (u->flags & RESTIR_TEMPORAL_REUSE_FLAG) != 0;
-    if (vis && !ctx->bvhSet) return fail(ctx, RESTIR_E_NOT_READY, "initial_temporal: BVH not set");
+    if (int r = check_pass_a(ctx, u, cur, prev, prevRes, out, "initial_temporal")) return r;
     const uint32_t W = u->screenSize[0], H = u->screenSize[1];
-    if (int r = check_gbuffer(ctx, cur, W, H, false, "initial_temporal cur")) return r;
-    if (!out) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "initial_temporal: output is NULL");
-    const size_t bytes = sizeof(restir_reservoir) * W * H;
-    if (temporal) {
-        if (int r = check_gbuffer(ctx, prev, W, H, false, "initial_temporal prev")) return r;
-        if (!prevRes) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "initial_temporal: previous reservoirs are NULL");
-        if (overlaps(prevRes, bytes, out, bytes))
-            return fail(ctx, RESTIR_E_ALIASING, "initial_temporal: output aliases the previous reservoirs");
-    }
     int tile;
     if (int r = pick_tile(ctx, launch, &tile)) return r;
     void *queues = nullptr;
@@ -301,6 +339,34 @@ int restir_initial_temporal(restir_ctx *ctx, const restir_uniforms *u, const res
                                            prevRes, out, nullptr, nullptr, queues, ctx->persistentBlocks,
                                            static_cast<hipStream_t>(stream));
     return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "initial_temporal launch");
+}
+
+int restir_initial_temporal_noalloc(restir_ctx *ctx, const restir_uniforms *u, const restir_gbuffer *cur,
+                                    const restir_gbuffer *prev, const restir_reservoir *prevRes,
+                                    restir_reservoir *out, const restir_launch *launch, void *stream) {
+    if (int r = check_frame(ctx, u, "initial_temporal_noalloc")) return r;
+    const uint32_t W = u->screenSize[0], H = u->screenSize[1];
+    const uint32_t schedule = launch ? launch->schedule : RESTIR_SCHEDULE_QUEUED;
+    if (schedule != RESTIR_SCHEDULE_QUEUED && schedule != RESTIR_SCHEDULE_INLINE)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "initial_temporal_noalloc: unknown schedule %u", schedule);
+    const size_t need = rk::pass_a_queue_bytes(W * H);
+    if (schedule == RESTIR_SCHEDULE_QUEUED && need > RESTIR_NOALLOC_SCRATCH_BYTES)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT,
+                    "initial_temporal_noalloc: %ux%u pixels need %zu bytes of scratch; the static array "
+                    "holds %llu (at most %u pixels)", W, H, need,
+                    static_cast<unsigned long long>(RESTIR_NOALLOC_SCRATCH_BYTES), RESTIR_NOALLOC_MAX_PIXELS);
+    if (int r = check_pass_a(ctx, u, cur, prev, prevRes, out, "initial_temporal_noalloc")) return r;
+    int tile;
+    if (int r = pick_tile(ctx, launch, &tile)) return r;
+    if (int r = use_device(ctx)) return r;
+    if (!ctx->noallocScratch) return fail(ctx, RESTIR_E_HIP, "initial_temporal_noalloc: static scratch unresolved");
+    // no per-kernel timing here: the recorder's bookkeeping may allocate host memory
+    rk::tl_recorder = nullptr;
+    void *queues = schedule == RESTIR_SCHEDULE_QUEUED ? ctx->noallocScratch : nullptr;
+    const hipError_t e = rk::launch_pass_a(tile, ctx->stack, *u, ctx->scene, view_of(cur), view_of(prev),
+                                           prevRes, out, nullptr, nullptr, queues, ctx->persistentBlocks,
+                                           static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "initial_temporal_noalloc launch");
 }
 
 int restir_spatial(restir_ctx *ctx, const restir_uniforms *u, int iter, const restir_gbuffer *g,

@@ -34,6 +34,7 @@ struct restir_ctx {
     uint32_t *counters = nullptr;  // device work counters of the persistent kernels
     uint32_t cus = 256;                // compute units of the device
     uint32_t persistentBlocks = 1280;  // workgroups of the persistent tracers (see persistent_blocks)
+    void *noallocScratch = nullptr;  // this device's copy of abi.hip's static `scratch` array
     void *scratch = nullptr;       // ray queues of the queued Pass C schedule
     size_t scratchBytes = 0;
     uint32_t queuedNpx = 0;        // shape of the last queued Pass C (its rays stay in scratch)

@@ -21,7 +21,7 @@ TEMPORAL_REUSE_FLAG = 1 << 1
 TRACE_FIRST_HIT = 0
 TRACE_REFERENCE_LOOP = 1
 TRACE_ONE_RAY_PER_LANE = 2
-ABI_VERSION = 10  # RESTIR_ABI_VERSION of include/restir_abi.h
+ABI_VERSION = 11  # RESTIR_ABI_VERSION of include/restir_abi.h
 
 ERRORS = {
     -1: "RESTIR_E_INVALID_ARGUMENT",
@@ -154,6 +154,8 @@ SIGNATURES = {
     "restir_set_bvh": (_i32, [_vp, _vp, _u32, _vp, _u32]),
     "restir_build_bvh": (_i32, [_vp, _vp, _u32, _vp, _vp, C.POINTER(_u32), _vp]),
     "restir_initial_temporal": (_i32, [_vp, C.POINTER(Uniforms), C.POINTER(GBuffer),
+                                       C.POINTER(GBuffer), _vp, _vp, C.POINTER(Launch), _vp]),
+    "restir_initial_temporal_noalloc": (_i32, [_vp, C.POINTER(Uniforms), C.POINTER(GBuffer),
                                        C.POINTER(GBuffer), _vp, _vp, C.POINTER(Launch), _vp]),
     "restir_spatial": (_i32, [_vp, C.POINTER(Uniforms), _i32, C.POINTER(GBuffer), _vp, _vp,
                               C.POINTER(Launch), _vp]),
@@ -488,6 +490,12 @@ class Context:
             self.h, C.byref(u), C.byref(cur), C.byref(prev) if prev is not None else None,
             as_ptr(prev_res), as_ptr(out), C.byref(launch) if launch is not None else None,
             stream_ptr(stream)), "restir_initial_temporal")
+
+    def initial_temporal_noalloc(self, u, cur, prev, prev_res, out, launch=None, stream=None):
+        self._check(self.lib.restir_initial_temporal_noalloc(
+            self.h, C.byref(u), C.byref(cur), C.byref(prev) if prev is not None else None,
+            as_ptr(prev_res), as_ptr(out), C.byref(launch) if launch is not None else None,
+            stream_ptr(stream)), "restir_initial_temporal_noalloc")
 
     def spatial(self, u, it, g, res_in, res_out, launch=None, stream=None):
         self._check(self.lib.restir_spatial(

@@ -1215,7 +1215,8 @@ hipError_t mlp_train(restir_mlp &m, const GView &g, f3 cam, const float lo[3], c
     RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, t.nValid, 1u);
     RK_LAUNCH("k_train_gather", rm::k_train_gather, dim3((n + 255) / 256), dim3(256), 0, s, ga);
     mlp_loss_grad_impl(m, t.X, t.target, t.mask, n, t, t.gw, t.gb, t.loss, false, s);
-    mlp_adam(m, t.gw, t.gb, t.nValid, lr, s);
+    const hipError_t ea = mlp_adam(m, t.gw, t.gb, t.nValid, lr, s);
+    if (ea != hipSuccess) return ea;
     if (lossOut) RK_LAUNCH("k_copy_f32", rm::k_copy_f32, dim3(1), dim3(1), 0, s, t.loss, lossOut);
     return hipGetLastError();
 }

@@ -54,6 +54,11 @@ BYTES_PER_PX_PASS_A, BYTES_PER_PX_PASS_C = 208, 176
 PASS_A_KERNELS = ("k_pass_a", "k_pass_a_ris", "k_pass_a_finish")
 PASS_C_KERNELS = ("k_pack_surface", "k_pass_c_merge", "k_pass_c_finish", "k_pass_c")
 TRUTH_TRI_SAMPLES = 64  # stratified samples per triangle light in the FP64 ground truth
+# Error study: fixed views of the orbit (0.5 deg/frame), the same whatever --steps and --warmup
+# are; the proposal trains over the orbit's first ERROR_ORBIT frames.
+ERROR_ORBIT = 40
+ERROR_VIEWS = (6, 16, 26, 36)
+BOOTSTRAP = 2000
 
 
 def log(msg):
@@ -233,13 +238,16 @@ def main():
         f"({time.time() - t0:.1f}s)")
 
     nframes = WU + K
-    cams = [scenes.orbit_camera(scene.camera, f) for f in range(nframes)]
-    G = [torch.zeros(abi.GBUFFER_PLANES * npx, dtype=torch.float32, device=dev) for _ in range(nframes + 1)]
-    for f in range(nframes):
+    # the orbit's first ERROR_ORBIT frames also feed the proposal's training and the error study's
+    # fixed views, whatever --steps and --warmup are
+    n_orbit = max(nframes, ERROR_ORBIT)
+    cams = [scenes.orbit_camera(scene.camera, f) for f in range(n_orbit)]
+    G = [torch.zeros(abi.GBUFFER_PLANES * npx, dtype=torch.float32, device=dev) for _ in range(n_orbit + 1)]
+    for f in range(n_orbit):
         ctx.primary_gbuffer(cams[f], sd.d_tri_material, sd.d_materials, len(scene.materials), G[f + 1],
                             W, H, stream=stream)
     torch.cuda.synchronize()
-    log(f"g-buffers: {nframes} frames at {W}x{H} rendered")
+    log(f"g-buffers: {n_orbit} frames at {W}x{H} rendered")
     fr = Frames(torch, ctx, scene, cams, G, W, H, args, launch, stream)
 
     # ---- plain pipeline (the reference's configuration)
@@ -288,8 +296,14 @@ def main():
     Vn, Tn = ray_sets["neighbour"][-1]["V"], ray_sets["neighbour"][-1]["T"]
     cpu_rays = (q1[:nr].cpu().numpy(), q2[:nr].cpu().numpy()) if not args.no_cpu_baseline else None
     rays_per_s = nr / (trace_ms * 1e-3)
-    roof_rays = min(HBM_PEAK_GBS * 1e9 / (NODE_BYTES * Vn + TRI_BYTES * Tn),
-                    VALU_LANE_OPS / (VISIT_OPS * Vn + TEST_OPS * Tn))
+    # Rays/s roofline: the lane-op model is the binding roof. The node and triangle fetches
+    # (128V + 48T bytes per ray) come from L1/L2 and the Infinity Cache: the HBM bytes the counters
+    # measure per launch are a few percent of them, so the memory roof, priced at the measured
+    # bytes per ray when the profile has them, is far above the ALU one.
+    valu_rays = VALU_LANE_OPS / (VISIT_OPS * Vn + TEST_OPS * Tn)
+    meas = measured_traffic("k_trace_queue(neighbour)") if W * H == 1920 * 1080 else None
+    hbm_rays = HBM_PEAK_GBS * 1e9 / (meas / nr) if meas else None
+    roof_rays = min(valu_rays, hbm_rays) if hbm_rays else valu_rays
     log(f"traversal: {nr} neighbour rays, V={Vn:.1f} T={Tn:.2f}: {trace_ms:.3f} ms, "
         f"{rays_per_s / 1e6:.0f} Mrays/s, roofline {roof_rays / 1e6:.0f} Mrays/s")
 
@@ -310,7 +324,36 @@ def main():
                   "shade": pass_rate(("k_shade",), BYTES_PER_PX_SHADE, 1)}
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_frame"])
     dom_bytes = algo_bytes.get(dominant)
-    achieved = dom_bytes / (kernels[dominant]["avg_ms"] * 1e-3) / 1e9 if dom_bytes else None
+    dom_ms = kernels[dominant]["avg_ms"]
+    dom_traffic = measured_traffic(dominant) if W * H == 1920 * 1080 else None
+    if dominant.startswith("k_trace_queue"):
+        # VALU-bound: algorithmic lane-ops per launch (88 per 4-entry visit, 45 per triangle
+        # test, V and T of this kernel's own rays) over the average launch time
+        key = dominant[len("k_trace_queue("):-1]
+        ops = float(np.mean([r["rays"] * (VISIT_OPS * r["V"] + TEST_OPS * r["T"]) for r in ray_sets[key]]))
+        rays = float(np.mean([r["rays"] for r in ray_sets[key]]))
+        ach = ops / (dom_ms * 1e-3) / 1e12
+        roofline = {"kernel": dominant, "bound": "valu", "achieved": round(ach, 3),
+                    "peak": VALU_LANE_OPS / 1e12, "unit": "Tlane-op/s (fp32, non-packed)",
+                    "frac": round(ach * 1e12 / VALU_LANE_OPS, 4), "traffic": dom_traffic,
+                    "algorithmic_lane_ops_per_launch": round(ops), "rays_per_launch": round(rays),
+                    "avg_launch_ms": round(dom_ms, 4),
+                    "rays_per_s": round(rays / (dom_ms * 1e-3)),
+                    "roofline_rays_per_s": round(VALU_LANE_OPS * rays / ops),
+                    "hbm": ({"achieved_GBps": round(dom_traffic / (dom_ms * 1e-3) / 1e9, 1),
+                             "frac": round(dom_traffic / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                             "bytes": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, copy-calibrated"}
+                            if dom_traffic else None),
+                    "note": "model: 4 box tests of 16 lane-ops (6 FMA, 6 min/max, 4 compares) + 24 of "
+                            "walk control per 4-entry node visit, 45 per triangle test; node fetches "
+                            "are served by L1/L2, so the HBM roof does not bind"}
+    else:
+        ach = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_bytes else None
+        roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(ach, 1) if ach else None,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4) if ach else None, "traffic": dom_traffic,
+                    "algorithmic_bytes_per_launch": round(dom_bytes) if dom_bytes else None,
+                    "avg_launch_ms": round(dom_ms, 4)}
 
     def rounded(ks):
         for k, v in ks.items():
@@ -351,18 +394,10 @@ def main():
                       "node_bytes": NODE_BYTES, "ops_per_visit": VISIT_OPS, "ops_per_test": TEST_OPS,
                       "node_visits_per_ray": round(Vn, 2), "tri_tests_per_ray": round(Tn, 2),
                       "ms": round(trace_ms, 4), "roofline_mrays_per_s": round(roof_rays / 1e6, 1),
+                      "valu_roofline_mrays_per_s": round(valu_rays / 1e6, 1),
+                      "hbm_roofline_mrays_per_s": round(hbm_rays / 1e6, 1) if hbm_rays else None,
                       "frac": round(rays_per_s / roof_rays, 4), "pass_c_rays": ray_sets},
-        "roofline": {"kernel": dominant, "bound": "hbm",
-                     "achieved": round(achieved, 1) if achieved else None,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": measured_traffic(dominant) if W * H == 1920 * 1080 else None,
-                     "algorithmic_bytes_per_launch": round(dom_bytes) if dom_bytes else None,
-                     "avg_launch_ms": round(kernels[dominant]["avg_ms"], 4),
-                     "note": ("algorithmic bytes = node and triangle bytes per visit and test of each ray "
-                              "(128V + 48T); the tree is served by L1/L2 and the Infinity Cache, so the "
-                              "effective rate can pass the HBM peak while the HBM traffic stays at "
-                              "`traffic` bytes per launch" if dominant.startswith("k_trace") else None)},
+        "roofline": roofline,
     }
 
     if not args.no_learned:
@@ -388,15 +423,17 @@ def main():
 
 def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
     """The learned pipeline's timed frames, then both pipelines' per-frame error against the FP64
-    ground truth (GPU, restir_ground_truth) at the last camera of the orbit."""
+    ground truth (GPU, restir_ground_truth) at fixed views of the orbit (ERROR_VIEWS), with a
+    bootstrap confidence interval of the equal-error speedup."""
     from restir_amd.learned import LearnedProposal
 
     K, WU, W, H = args.steps, args.warmup, fr.W, fr.H
     nframes = WU + K
+    n_orbit = len(fr.cams)
     lp = LearnedProposal(ctx, scene, k=256, seed=7)
     t0 = time.time()
     for i in range(args.pretrain):  # online training over the orbit (untimed)
-        fr.run(i % nframes, args.learned_M, learned=lp)
+        fr.run(i % n_orbit, args.learned_M, learned=lp)
     torch.cuda.synchronize()
     log(f"learned: {args.pretrain} training frames ({time.time() - t0:.1f}s), loss {lp.loss.item():.3f}")
     sched_learned = fr.choose_pass_a_schedule(WU - 1, args.learned_M, learned=lp)
@@ -411,44 +448,63 @@ def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
         for key in ("launches_per_frame", "ms_per_frame", "avg_ms"):
             v[key] = round(v[key], 4)
 
-    # ---- per-frame error at a static camera (the orbit's last view), independent frames: the
+    # ---- per-frame error at static cameras (fixed views of the orbit), independent frames: the
     # history is cleared before each frame and the previous G-buffer is the current one
-    f_err = nframes - 1
-    gi = f_err + 1
-    cam = fr.cams[f_err]
-    nrm = fr.G[gi][3 * W * H: 6 * W * H].view(3, -1)
-    emissive = fr.G[gi][9 * W * H: 10 * W * H]  # emissive pixels show their emission, not lighting
-    surf = torch.nonzero((nrm.abs().sum(0) > 0) & (emissive < 0.5)).view(-1).cpu().numpy()
-    rng = np.random.default_rng(16)
-    pix = np.sort(rng.choice(surf, min(args.error_pixels, len(surf)), replace=False)).astype(np.uint32)
-    d_pix = torch.from_numpy(pix).to(fr.G[0].device)
-    n = len(pix)
-    gt = torch.zeros(3 * n, dtype=torch.float64, device=fr.G[0].device)
-    t0 = time.time()
-    ctx.ground_truth(cam["position"], fr.gview(gi), d_pix, n, gt, n, n_tri_samples=TRUTH_TRI_SAMPLES)
-    torch.cuda.synchronize()
-    gt = gt.view(3, n)
-    log(f"FP64 ground truth of {n} pixels: {time.time() - t0:.2f}s")
-    idx = d_pix.long()
+    views = [v for v in ERROR_VIEWS if v < n_orbit]
+    per_view = max(1, args.error_frames // len(views))
+    rp, rl, n_pix = [], [], 0
+    t_gt = 0.0
+    for vi, f_err in enumerate(views):
+        gi = f_err + 1
+        cam = fr.cams[f_err]
+        nrm = fr.G[gi][3 * W * H: 6 * W * H].view(3, -1)
+        emissive = fr.G[gi][9 * W * H: 10 * W * H]  # emissive pixels show their emission, not lighting
+        surf = torch.nonzero((nrm.abs().sum(0) > 0) & (emissive < 0.5)).view(-1).cpu().numpy()
+        rng = np.random.default_rng(16 + f_err)
+        pix = np.sort(rng.choice(surf, min(args.error_pixels, len(surf)), replace=False)).astype(np.uint32)
+        d_pix = torch.from_numpy(pix).to(fr.G[0].device)
+        n = len(pix)
+        n_pix += n
+        gt = torch.zeros(3 * n, dtype=torch.float64, device=fr.G[0].device)
+        t0 = time.time()
+        ctx.ground_truth(cam["position"], fr.gview(gi), d_pix, n, gt, n, n_tri_samples=TRUTH_TRI_SAMPLES)
+        torch.cuda.synchronize()
+        t_gt += time.time() - t0
+        gt = gt.view(3, n)
+        idx = d_pix.long()
 
-    def errors(M, learned):
-        rel = []
-        for e in range(args.error_frames):
-            for t in fr.res:
-                t.zero_()
-            fr.run(f_err, M, learned=learned, gcur=gi, gprev=gi, seed_frame=100000 + e)
-            x = torch.nan_to_num(fr.rgb.view(3, -1)[:, idx].double())
-            rel.append((((x - gt) ** 2) / (gt ** 2 + 1e-2)).mean().item())
-        return np.array(rel)
+        def errors(M, learned):
+            rel = []
+            for e in range(per_view):
+                for t in fr.res:
+                    t.zero_()
+                fr.run(f_err, M, learned=learned, gcur=gi, gprev=gi, seed_frame=100000 + 1000 * vi + e,
+                       train=False)
+                x = torch.nan_to_num(fr.rgb.view(3, -1)[:, idx].double())
+                rel.append((((x - gt) ** 2) / (gt ** 2 + 1e-2)).mean().item())
+            return np.array(rel)
 
-    rp = errors(args.M, None)
-    rl = errors(args.learned_M, lp)
+        rp.append(errors(args.M, None))
+        rl.append(errors(args.learned_M, lp))
     torch.cuda.synchronize()
-    eff_p, eff_l = plain_ms * rp.mean(), learned_ms * rl.mean()
-    speedup = eff_p / eff_l
-    log(f"relMSE plain {rp.mean():.3f} (median {np.median(rp):.3f}), learned {rl.mean():.3f} "
-        f"(median {np.median(rl):.3f}); equal-error speedup learned vs plain {speedup:.3f}")
-    value = learned_ms * rl.mean() / rp.mean()
+    log(f"FP64 ground truth of {n_pix} pixels over {len(views)} views: {t_gt:.2f}s")
+    rp, rl = np.stack(rp), np.stack(rl)  # [view, frame]
+    mp, ml = rp.mean(), rl.mean()  # every view weighs the same
+    speedup = (plain_ms * mp) / (learned_ms * ml)
+    # bootstrap over frames, resampled within each view
+    brng = np.random.default_rng(2024)
+    boots = np.empty(BOOTSTRAP)
+    for b in range(BOOTSTRAP):
+        ip = brng.integers(0, per_view, size=rp.shape)
+        il = brng.integers(0, per_view, size=rl.shape)
+        bp = np.take_along_axis(rp, ip, axis=1).mean()
+        bl = np.take_along_axis(rl, il, axis=1).mean()
+        boots[b] = (plain_ms * bp) / (learned_ms * bl)
+    lo, hi = np.quantile(boots, [0.025, 0.975])
+    med_ratio = (plain_ms * np.median(rp)) / (learned_ms * np.median(rl))
+    log(f"relMSE plain {mp:.3f} (median {np.median(rp):.3f}), learned {ml:.3f} (median {np.median(rl):.3f}); "
+        f"equal-error speedup learned vs plain {speedup:.3f} (95% CI {lo:.3f}-{hi:.3f}); of medians {med_ratio:.3f}")
+    value = learned_ms * ml / mp
     return {
         "value": round(value, 4),
         "ms_per_step": round(learned_ms, 4),
@@ -456,18 +512,25 @@ def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
                     "train_loss": round(lp.loss.item(), 4),
                     "pretrain_frames": args.pretrain},
         "equal_rmse": {
-            "protocol": "static camera, independent frames (history cleared), FP64 GPU ground truth on "
-                        f"{n} seeded non-emissive surface pixels (triangle lights by {TRUTH_TRI_SAMPLES} "
-                        "stratified samples each); relMSE = mean (x - gt)^2 / (gt^2 + 1e-2) per frame; "
-                        "time to a target error is proportional to ms/frame x mean relMSE",
-            "frames_per_method": args.error_frames,
-            "relmse_plain": round(float(rp.mean()), 4), "relmse_plain_median": round(float(np.median(rp)), 4),
-            "relmse_learned": round(float(rl.mean()), 4),
+            "protocol": f"static cameras at orbit frames {list(views)} (fixed; independent of --steps and "
+                        f"--warmup), {per_view} independent frames per view and method (history cleared), "
+                        f"FP64 GPU ground truth on up to {args.error_pixels} seeded non-emissive surface pixels "
+                        f"per view (triangle lights by {TRUTH_TRI_SAMPLES} stratified samples each); relMSE = "
+                        "mean (x - gt)^2 / (gt^2 + 1e-2) per frame; time to a target error is proportional to "
+                        "ms/frame x mean relMSE; 95% CI by bootstrap over frames within each view",
+            "views": list(views), "frames_per_view": per_view,
+            "relmse_plain": round(float(mp), 4), "relmse_plain_median": round(float(np.median(rp)), 4),
+            "relmse_learned": round(float(ml), 4),
             "relmse_learned_median": round(float(np.median(rl)), 4),
+            "relmse_plain_by_view": [round(float(v), 4) for v in rp.mean(axis=1)],
+            "relmse_learned_by_view": [round(float(v), 4) for v in rl.mean(axis=1)],
             "relmse_plain_p90_max": [round(float(np.quantile(rp, 0.9)), 3), round(float(rp.max()), 3)],
             "relmse_learned_p90_max": [round(float(np.quantile(rl, 0.9)), 3), round(float(rl.max()), 3)],
             "ms_plain": round(plain_ms, 4), "ms_learned": round(learned_ms, 4),
             "speedup_learned_vs_plain": round(float(speedup), 4),
+            "speedup_ci95": [round(float(lo), 4), round(float(hi), 4)],
+            "speedup_of_medians": round(float(med_ratio), 4),
+            "target": ">= 2x (not met)" if speedup < 2 else ">= 2x (met)",
         },
     }
 
@@ -630,7 +693,39 @@ def cpu_baseline(fr, scene, sd, args, rays):
             "sample": f"{args.cpu_frames} frames of the plain pipeline (M={args.M}, Pass A + "
                       f"{args.spatial_passes}x Pass C + shading) at {W}x{H}: {frame_s * args.cpu_frames:.1f} s; "
                       f"traversal of {len(p1)} Pass C neighbour rays x{reps}: {trace_s * reps:.1f} s",
-            "mrays_per_s": round(len(p1) / trace_s / 1e6, 2)}
+            "mrays_per_s": round(len(p1) / trace_s / 1e6, 2),
+            "c1_ms_per_frame": cpu_c1(args),
+            "reference_baseline": "lavapipe baseline not run: the reference's GLSL path needs glslc and a "
+                                  "Vulkan ICD (Mesa lavapipe), neither present on this machine; the "
+                                  "baseline is the project's C oracle on the same cores"}
+
+
+def cpu_c1(args):
+    """The CPU-runnable configuration on the oracle: the 256x256 Cornell box (34 triangles, 64
+    point lights), biased ReSTIR with M = 32, temporal reuse on a static camera and one Pass B with
+    k = 5 in a 30 px radius, plus shading; ms per frame over 2 frames after one."""
+    import oracle
+    from restir_amd import abi, scenes
+
+    sc = scenes.cornell_box(1)
+    W = H = 256
+    nodes, tris, _ = abi.build_aabb_tree(sc.tri_xyz)
+    alias = abi.scene_alias_table(sc.point_lights, sc.tri_lights)
+    orc = oracle.Scene(nodes, tris, sc.point_lights, None, alias)
+    planes = oracle.primary_gbuffer(tris, abi.camera_struct(sc.camera), sc.tri_material, sc.materials, W, H)
+    g = oracle.gbuffer(planes, W, H)
+    pv = abi.projection_view(sc.camera, W / H)
+    prev = np.zeros(W * H, dtype=abi.RESERVOIR_DTYPE)
+    times = []
+    for f in range(3):
+        u = abi.make_uniforms(W, H, f + 1, sc.camera["position"], pv, M=32, neighbors=5, radius=30.0)
+        t0 = time.perf_counter()
+        a = oracle.pass_a(orc, u, g, g, prev, abi.RESERVOIR_DTYPE)
+        b = oracle.pass_b(orc, u, 0, g, a)
+        oracle.shade(orc, sc.camera["position"], 1.0, g, b)
+        times.append(time.perf_counter() - t0)
+        prev = b
+    return round(float(np.mean(times[1:])) * 1e3, 2)
 
 
 def stress_4k(torch, args):

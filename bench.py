@@ -132,6 +132,7 @@ class Frames:
         self.flags = abi.VISIBILITY_REUSE_FLAG | abi.TEMPORAL_REUSE_FLAG
         self.learned = None
         self.launch_a = launch  # Pass A's shape and ray schedule (choose_pass_a_schedule)
+        self.normal_format = abi.NORMAL_F32
 
     def uniforms(self, f, M, seed_frame=None, static=False):
         fr = f + 1 if seed_frame is None else seed_frame
@@ -140,7 +141,7 @@ class Frames:
                                   temporal_mult=20, neighbors=self.args.k, radius=30.0, flags=self.flags)
 
     def gview(self, i):
-        return self.abi.gbuffer_struct(self.G[i].data_ptr(), self.W, self.H)
+        return self.abi.gbuffer_struct(self.G[i].data_ptr(), self.W, self.H, normal_format=self.normal_format)
 
     def frame(self, f, M, *, learned=None, gcur=None, gprev=None, seed_frame=None, train=True):
         """Enqueues frame f; yields after each Pass C (its queues then hold that pass's rays)."""
@@ -595,16 +596,7 @@ def batched_views(torch, args):
     from restir_amd import scenes
     from restir_amd.pipeline import BatchRenderer, FrameSequenceGraph, Renderer
 
-    rng = np.random.default_rng(3)
-    specs = [(110, 60, 1000), (160, 200, 5000), (220, 400, 20000), (300, 1000, 50000)]
-    # built as the 1080p scene: a third of the box triangles emissive, in one alias table with the
-    # point lights
-    scs = [scenes.terrain(20 + i, grid=g, n_boxes=b, n_point_lights=nl, extent=32.0, n_tri_lights=4 * b)
-           for i, (g, b, nl) in enumerate(specs)]
-    views = []
-    for v in range(16):
-        h = int(round(360 * 4.0 ** rng.random()))
-        views.append((scs[v % 4], int(round(h * 16 / 9)), h))
+    scs, views = scenes.batched_view_set(3)
     px = sum(w * h for _, w, h in views)
     frames = 8
     seq = [[scenes.orbit_camera(sc.camera, f) for sc, _, _ in views] for f in range(frames)]
@@ -749,7 +741,7 @@ def stress_4k(torch, args):
     tx, ty = (int(v) for v in args.tile.split("x"))
     launch = abi.Launch(tx, ty)
     t0 = time.time()
-    scene = scenes.terrain(5, grid=1024, n_boxes=20000, n_point_lights=1_000_000, light_intensity=(1e-3, 10.0))
+    scene = scenes.stress_scene()
     log(f"stress scene: {scene.n_tris} triangles, {len(scene.point_lights)} point lights ({time.time() - t0:.1f}s)")
     ctx = Context(0)
     torch.cuda.synchronize()
@@ -766,12 +758,13 @@ def stress_4k(torch, args):
     nframes = WU + K
     cams = [scenes.orbit_camera(scene.camera, f) for f in range(nframes)]
     G = [torch.zeros(abi.GBUFFER_PLANES * npx, dtype=torch.float32, device=dev) for _ in range(nframes + 1)]
-    for f in range(nframes):
+    for f in range(nframes):  # the workload's G-buffer keeps its normals as half planes
         ctx.primary_gbuffer(cams[f], sd.d_tri_material, sd.d_materials, len(scene.materials), G[f + 1],
-                            W, H, stream=stream)
+                            W, H, stream=stream, normal_format=abi.NORMAL_F16)
     torch.cuda.synchronize()
     fr = Frames(torch, ctx, scene, cams, G, W, H, args, launch, stream)
-    out = {"workload": "stress4k", "width": W, "height": H, "triangles": int(scene.n_tris),
+    fr.normal_format = abi.NORMAL_F16
+    out = {"workload": "stress4k", "normals": "fp16", "width": W, "height": H, "triangles": int(scene.n_tris),
            "point_lights": int(len(scene.point_lights)), "clusters": 4096, "steps": K, "warmup": WU,
            "tree_and_alias_build_s": round(setup_s, 3), "clustering_s": round(cluster_s, 3)}
     for name, M, learned in (("plain", args.M, None), ("learned", args.learned_M, lp)):
@@ -795,12 +788,13 @@ def stress_paths_graph(torch, args, scene, ctx, lp, launch, W, H):
     proposal, Pass A, two Pass C, shading and one Adam step per frame), one renderer per path on
     one context and one proposal, all captured into one HIP graph; timed against the same
     sequence run eagerly from the same starting state."""
-    from restir_amd import scenes
+    from restir_amd import abi, scenes
     from restir_amd.pipeline import Renderer
     paths, plen = args.stress_paths, 32
     cams = [[scenes.orbit_camera(scene.camera, 180 * p + f) for f in range(plen)] for p in range(paths)]
     rs = [Renderer(scene, W, H, mode="unbiased", k=args.k, M=args.learned_M, spatial_passes=args.spatial_passes,
-                   ctx=ctx, learned=lp, launch=launch, scene_build="device") for _ in range(paths)]
+                   ctx=ctx, learned=lp, launch=launch, scene_build="device", normal_format=abi.NORMAL_F16)
+          for _ in range(paths)]
     for r, c in zip(rs, cams):
         r.render_frame(c[0])  # sizes the library's scratch outside the capture
     torch.cuda.synchronize()

@@ -124,17 +124,23 @@ typedef struct restir_alias_table_column {
  * sampler2D images of the reference (gBufferPass.cpp:75-108, gBuffer.frag:27-80).
  * position/normal: 3 planes (x, y, z). albedo: 4 planes (r, g, b, emissive flag > 0.5).
  * material: 2 planes (roughness, metallic). depth: 1 plane of hardware depth (z/w, [0,1]).
- * A pixel whose normal is (0,0,0) is background. plane_stride is in floats (>= width*height). */
+ * A pixel whose normal is (0,0,0) is background. plane_stride is in elements (>= width*height).
+ * Normals may be stored as IEEE half planes (RESTIR_NORMAL_F16; the 4K stress workload's format):
+ * every pass reads them exactly as their float values. */
 typedef struct restir_gbuffer {
     const float *position;
-    const float *normal;
+    const void *normal;      /* 3 planes of float, or of IEEE half (uint16) with RESTIR_NORMAL_F16 */
     const float *albedo;
     const float *material;
     const float *depth;
-    uint64_t plane_stride;
+    uint64_t plane_stride;   /* in elements of each plane's type */
     uint32_t width;
     uint32_t height;
+    uint32_t normal_format;  /* RESTIR_NORMAL_F32 (0) or RESTIR_NORMAL_F16 */
+    uint32_t reserved;       /* 0 */
 } restir_gbuffer;
+#define RESTIR_NORMAL_F32 0u
+#define RESTIR_NORMAL_F16 1u
 
 /* Launch shape of the per-pixel passes: the workgroup tile (pixels per workgroup = tile_x *
  * tile_y, a multiple of 64; {0,0} = built-in default) and the schedule of the shadow rays of
@@ -173,7 +179,7 @@ void restir_destroy(restir_ctx *ctx);
 const char *restir_last_error(const restir_ctx *ctx);
 /* ABI revision; bumped whenever an entry point's signature changes. Bindings compare it with the
  * revision they were written against and refuse a mismatched library. */
-#define RESTIR_ABI_VERSION 11
+#define RESTIR_ABI_VERSION 12
 int restir_version(void);
 
 /* Light tables: device pointers to the reference buffer images (header + array). The counts are
@@ -438,8 +444,9 @@ typedef struct restir_camera {
 
 int restir_primary_gbuffer(restir_ctx *ctx, const restir_camera *cam, const int32_t *tri_material,
                            const float *materials, uint32_t n_materials, float *position,
-                           float *normal, float *albedo, float *material, float *depth,
-                           uint64_t plane_stride, uint32_t width, uint32_t height, void *stream);
+                           void *normal, uint32_t normal_format, float *albedo, float *material,
+                           float *depth, uint64_t plane_stride, uint32_t width, uint32_t height,
+                           void *stream);
 
 /* The 4x4 projection-view matrix of camera.h:25-50 for the given aspect, column-major, as the
  * reference writes it into RestirUniforms::prevFrameProjectionViewMatrix (app.cpp:779). */

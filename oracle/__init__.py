@@ -175,11 +175,12 @@ def gbuffer(planes: np.ndarray, width: int, height: int):
 
     class G(Structure):
         _fields_ = [("position", _vp), ("normal", _vp), ("albedo", _vp), ("material", _vp),
-                    ("depth", _vp), ("plane_stride", C.c_uint64), ("width", _u32), ("height", _u32)]
+                    ("depth", _vp), ("plane_stride", C.c_uint64), ("width", _u32), ("height", _u32),
+                    ("normal_format", _u32), ("reserved", _u32)]
 
     P = width * height
     base = planes.ctypes.data
-    g = G(base, base + 12 * P, base + 24 * P, base + 40 * P, base + 48 * P, P, width, height)
+    g = G(base, base + 12 * P, base + 24 * P, base + 40 * P, base + 48 * P, P, width, height, 0, 0)
     g._keep = planes
     return g
 

@@ -712,7 +712,7 @@ static inline texel fetch(const restir_gbuffer *g, uint32_t x, uint32_t y) {
     size_t i = (size_t)y * g->width + x, P = g->plane_stride;
     texel t;
     t.pos = mk(g->position[i], g->position[P + i], g->position[2 * P + i]);
-    t.n = mk(g->normal[i], g->normal[P + i], g->normal[2 * P + i]);
+    t.n = mk(((const float *)g->normal)[i], ((const float *)g->normal)[P + i], ((const float *)g->normal)[2 * P + i]);
     t.alb = mk(g->albedo[i], g->albedo[P + i], g->albedo[2 * P + i]);
     t.emissive = g->albedo[3 * P + i];
     t.rough = g->material[i];
@@ -899,7 +899,7 @@ void oracle_pass_b(const oracle_scene *sc, const restir_uniforms *u, int iter,
                 ny = ny < 0 ? 0 : (ny > (int32_t)H - 1 ? (int32_t)H - 1 : ny);
                 size_t ni = (size_t)ny * W + (size_t)nx;
                 float nd = g->depth[ni];
-                v3 nn = mk(g->normal[ni], g->normal[g->plane_stride + ni], g->normal[2 * g->plane_stride + ni]);
+                v3 nn = mk(((const float *)g->normal)[ni], ((const float *)g->normal)[g->plane_stride + ni], ((const float *)g->normal)[2 * g->plane_stride + ni]);
                 if (fabsf(nd - t.depth) > u->spatialPosThreshold * fabsf(t.depth) || vdot(nn, t.n) < cosThr)
                     continue;
                 restir_reservoir rr = in[ni];
@@ -958,7 +958,7 @@ void oracle_pass_c(const oracle_scene *s, const restir_uniforms *u, int k, int i
             for (int j = 0; j < k; ++j) {
                 size_t ni = nidx[j], P = g->plane_stride;
                 v3 nwp = mk(g->position[ni], g->position[P + ni], g->position[2 * P + ni]);
-                v3 nn = mk(g->normal[ni], g->normal[P + ni], g->normal[2 * P + ni]);
+                v3 nn = mk(((const float *)g->normal)[ni], ((const float *)g->normal)[P + ni], ((const float *)g->normal)[2 * P + ni]);
                 if (vdot(vsub(lp, nwp), nn) < 0.0f) continue;
                 if (vis && test_visibility(s, nwp, lp)) continue;
                 numSamples += nM[j];

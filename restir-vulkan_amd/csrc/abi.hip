@@ -62,6 +62,10 @@ int check_gbuffer(restir_ctx *ctx, const restir_gbuffer *g, uint32_t W, uint32_t
     if (g->plane_stride < static_cast<uint64_t>(W) * H)
         return fail(ctx, RESTIR_E_SIZE_MISMATCH, "%s: plane_stride %llu < width*height", name,
                     static_cast<unsigned long long>(g->plane_stride));
+    if (g->normal_format != RESTIR_NORMAL_F32 && g->normal_format != RESTIR_NORMAL_F16)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "%s: unknown normal format %u", name, g->normal_format);
+    if (reinterpret_cast<uintptr_t>(g->normal) % (g->normal_format == RESTIR_NORMAL_F16 ? 2 : 4) != 0)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "%s: normal planes are misaligned", name);
     return RESTIR_OK;
 }
 
@@ -70,6 +74,7 @@ rk::GView view_of(const restir_gbuffer *g) {
     if (!g) return v;
     v.pos = g->position;
     v.nrm = g->normal;
+    v.nrmHalf = g->normal_format == RESTIR_NORMAL_F16 ? 1u : 0u;
     v.alb = g->albedo;
     v.mat = g->material;
     v.depth = g->depth;
@@ -490,19 +495,22 @@ int restir_ground_truth(restir_ctx *ctx, const float cameraPos[3], const restir_
 
 int restir_primary_gbuffer(restir_ctx *ctx, const restir_camera *cam, const int32_t *triMaterial,
                            const float *materials, uint32_t nMaterials, float *position,
-                           float *normal, float *albedo, float *material, float *depth,
+                           void *normal, uint32_t normalFormat, float *albedo, float *material, float *depth,
                            uint64_t planeStride, uint32_t width, uint32_t height, void *stream) {
     if (!ctx) return RESTIR_E_INVALID_ARGUMENT;
     if (!ctx->bvhSet) return fail(ctx, RESTIR_E_NOT_READY, "primary_gbuffer: BVH not set");
     if (!cam || !triMaterial || !materials || !position || !normal || !albedo || !material || !depth)
         return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "primary_gbuffer: NULL argument");
+    if (normalFormat != RESTIR_NORMAL_F32 && normalFormat != RESTIR_NORMAL_F16)
+        return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "primary_gbuffer: unknown normal format %u", normalFormat);
     if (nMaterials == 0) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "primary_gbuffer: no materials");
     if (width == 0 || height == 0 || planeStride < static_cast<uint64_t>(width) * height)
         return fail(ctx, RESTIR_E_SIZE_MISMATCH, "primary_gbuffer: bad size");
     if (int r = use_device(ctx)) return r;
     const rk::CamBasis cb = rk::make_cam_basis(*cam, static_cast<float>(width) / static_cast<float>(height));
     const hipError_t e = rk::launch_primary(ctx->stack, ctx->scene, cb, triMaterial, materials, position,
-                                            normal, albedo, material, depth, planeStride, width, height,
+                                            normal, normalFormat == RESTIR_NORMAL_F16, albedo, material, depth,
+                                            planeStride, width, height,
                                             static_cast<hipStream_t>(stream));
     return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "primary_gbuffer launch");
 }

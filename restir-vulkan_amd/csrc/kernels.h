@@ -41,10 +41,23 @@ struct SceneView {
 };
 
 struct GView {
-    const float *pos, *nrm, *alb, *mat, *depth;
-    uint64_t P;  // plane stride in floats
+    const float *pos;
+    const void *nrm;  // 3 float planes, or 3 half planes when nrmHalf
+    const float *alb, *mat, *depth;
+    uint64_t P;  // plane stride in elements
     uint32_t W, H;
+    uint32_t nrmHalf;
 };
+
+// The normal of pixel i, from float or half planes (half -> float is exact).
+__device__ __forceinline__ rt::f3 load_normal(const GView &g, size_t i) {
+    if (g.nrmHalf) {
+        const _Float16 *h = static_cast<const _Float16 *>(g.nrm);
+        return rt::f3{static_cast<float>(h[i]), static_cast<float>(h[g.P + i]), static_cast<float>(h[2 * g.P + i])};
+    }
+    const float *f = static_cast<const float *>(g.nrm);
+    return rt::f3{f[i], f[g.P + i], f[2 * g.P + i]};
+}
 
 struct Tile {
     uint32_t tx, ty;
@@ -168,7 +181,7 @@ struct CamBasis {
 CamBasis make_cam_basis(const restir_camera &c, float aspect);
 hipError_t launch_primary(int stack, const SceneView &sc, const CamBasis &cb,
                           const int32_t *triMaterial, const float *materials, float *pos,
-                          float *nrm, float *alb, float *mat, float *depth, uint64_t P, uint32_t W,
+                          void *nrm, bool nrmHalf, float *alb, float *mat, float *depth, uint64_t P, uint32_t W,
                           uint32_t H, hipStream_t s);
 
 }  // namespace rk

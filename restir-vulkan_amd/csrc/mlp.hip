@@ -73,7 +73,7 @@ struct FeatArgs {
 
 __device__ __forceinline__ bool pixel_features(const FeatArgs &f, uint32_t p, uint16_t *dst) {
     const f3 pos = f3{f.g.pos[p], f.g.pos[f.g.P + p], f.g.pos[2 * f.g.P + p]};
-    const f3 n = f3{f.g.nrm[p], f.g.nrm[f.g.P + p], f.g.nrm[2 * f.g.P + p]};
+    const f3 n = rk::load_normal(f.g, p);
     make_features(pos, n, f.cam, f.lo, f.hi, dst);
     return rt::dot(n, n) != 0.0f;
 }
@@ -465,8 +465,8 @@ __device__ __forceinline__ uint32_t quad_pixel(const rk::GView &g, uint32_t qx, 
         const uint32_t x = x0 + (j & 1), y = y0 + (j >> 1);
         if (x >= g.W || y >= g.H) continue;
         const uint32_t p = y * g.W + x;
-        const float nx = g.nrm[p], ny = g.nrm[g.P + p], nz = g.nrm[2 * g.P + p];
-        if ((nx * nx + ny * ny) + nz * nz != 0.0f) return p;
+        const f3 nq = rk::load_normal(g, p);
+        if ((nq.x * nq.x + nq.y * nq.y) + nq.z * nq.z != 0.0f) return p;
     }
     return y0 * g.W + x0;
 }
@@ -635,7 +635,7 @@ __global__ __launch_bounds__(2 * kProposeTile) void k_propose(const ProposeArgs 
     for (uint32_t px = 2 * qx; px < 2 * qx + 2; ++px) {
     if (px >= W || py >= H) continue;
     const uint32_t p = py * W + px;
-    const f3 n = f3{f.g.nrm[p], f.g.nrm[f.g.P + p], f.g.nrm[2 * f.g.P + p]};
+    const f3 n = rk::load_normal(f.g, p);
     if (rt::dot(n, n) == 0.0f) continue;  // background: Pass A draws no candidates there
     rt::Pcg rng = rt::pcg_seed(a.seed, p);
     // Candidates in groups of 8, each step's loads issued together (no load waits on another
@@ -844,7 +844,7 @@ __global__ __launch_bounds__(512) void k_propose_wide(const ProposeArgs a) {
         pixOk[s] = false;
         if (quadOk && px < W && py < H) {
             const uint32_t p = pix[s];
-            const f3 n = f3{f.g.nrm[p], f.g.nrm[f.g.P + p], f.g.nrm[2 * f.g.P + p]};
+            const f3 n = rk::load_normal(f.g, p);
             pixOk[s] = rt::dot(n, n) != 0.0f;  // background: Pass A draws no candidates there
         }
     }

@@ -132,7 +132,7 @@ __device__ __forceinline__ PassASurface pass_a_surface(const PassAArgs &a, uint3
     p.idx = static_cast<size_t>(y) * a.u.screenSize[0] + x;
     const GView &g = a.cur;
     p.pos = plane3(g.pos, p.idx, g.P);
-    p.n = plane3(g.nrm, p.idx, g.P);
+    p.n = load_normal(g, p.idx);
     p.alb = plane3(g.alb, p.idx, g.P);
     const float rough = g.mat[p.idx], metal = g.mat[g.P + p.idx];
     const f3 cam{a.u.cameraPos[0], a.u.cameraPos[1], a.u.cameraPos[2]};
@@ -251,7 +251,7 @@ __device__ __forceinline__ void pass_a_temporal(const PassAArgs &a, const PassAS
             if (dot(pd, pd) < 0.01f) {
                 const f3 ad = alb - plane3(pg.alb, pi, pg.P);
                 if (dot(ad, ad) < 0.01f) {
-                    if (dot(n, plane3(pg.nrm, pi, pg.P)) > 0.5f) {
+                    if (dot(n, load_normal(pg, pi)) > 0.5f) {
                         Res pr = load_res(a.prevRes, pi);
                         const uint32_t cap = a.u.temporalSampleCountMultiplier * res.M;
                         pr.M = pr.M < cap ? pr.M : cap;
@@ -318,7 +318,7 @@ __device__ __forceinline__ void pass_b_px(const PassBArgs &a, uint32_t x, uint32
     const size_t idx = static_cast<size_t>(y) * W + x;
     const GView &g = a.g;
     const f3 pos = plane3(g.pos, idx, g.P);
-    const f3 n = plane3(g.nrm, idx, g.P);
+    const f3 n = load_normal(g, idx);
     const f3 alb = plane3(g.alb, idx, g.P);
     const float rough = g.mat[idx], metal = g.mat[g.P + idx];
     const float depth = g.depth[idx];
@@ -339,7 +339,7 @@ __device__ __forceinline__ void pass_b_px(const PassBArgs &a, uint32_t x, uint32
         ny = ny < 0 ? 0 : (ny > static_cast<int32_t>(H) - 1 ? static_cast<int32_t>(H) - 1 : ny);
         const size_t ni = static_cast<size_t>(ny) * W + static_cast<size_t>(nx);
         const float nd = g.depth[ni];
-        const f3 nn = plane3(g.nrm, ni, g.P);
+        const f3 nn = load_normal(g, ni);
         if (fabsf(nd - depth) > depthTol || dot(nn, n) < a.cosThr) continue;
         const Res rr = load_res(a.in, ni);
         const float ph = phat_of(s, rr);
@@ -420,7 +420,7 @@ __device__ __forceinline__ void pass_c_px(const PassCArgs &a, uint32_t x, uint32
     const size_t idx = static_cast<size_t>(y) * W + x;
     const GView &g = a.g;
     const f3 pos = plane3(g.pos, idx, g.P);
-    const f3 n = plane3(g.nrm, idx, g.P);
+    const f3 n = load_normal(g, idx);
     Res res = load_res(a.in, idx);
     if (dot(n, n) == 0.0f) {  // background: pass through (reference reads uninitialised fields)
         store_res(a.out, idx, res);
@@ -453,7 +453,7 @@ __device__ __forceinline__ void pass_c_px(const PassCArgs &a, uint32_t x, uint32
         for (int j = 0; j < k; ++j) {
             const uint32_t ni = nIdx[j];
             const f3 nwp = plane3(g.pos, ni, g.P);
-            const f3 nn = plane3(g.nrm, ni, g.P);
+            const f3 nn = load_normal(g, ni);
             if (dot(lp - nwp, nn) < 0.0f) continue;
             if (vis) {
                 f3 o, d;
@@ -522,7 +522,7 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, uint32_t coun
 __global__ __launch_bounds__(256) void k_pack_surface(const GView g, uint32_t npx, float4 *surf) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= npx) return;
-    const f3 p = plane3(g.pos, i, g.P), n = plane3(g.nrm, i, g.P);
+    const f3 p = plane3(g.pos, i, g.P), n = load_normal(g, i);
     surf[2 * static_cast<size_t>(i)] = make_float4(p.x, p.y, p.z, n.x);
     surf[2 * static_cast<size_t>(i) + 1] = make_float4(n.y, n.z, 0.0f, 0.0f);
 }
@@ -542,7 +542,7 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
     if (inside) {
         const GView &g = a.g;
         pos = plane3(g.pos, idx, g.P);
-        const f3 n = plane3(g.nrm, idx, g.P);
+        const f3 n = load_normal(g, idx);
         Res res = load_res(a.in, idx);
         if (dot(n, n) == 0.0f) {
             store_res(a.out, idx, res);
@@ -807,7 +807,7 @@ __device__ __forceinline__ void shade_px(const ShadeArgs &a, size_t i) {
     const GView &g = a.g;
     if (i >= static_cast<size_t>(g.W) * g.H) return;
     const f3 pos = plane3(g.pos, i, g.P);
-    const f3 n = plane3(g.nrm, i, g.P);
+    const f3 n = load_normal(g, i);
     const f3 alb = plane3(g.alb, i, g.P);
     const float emissive = g.alb[3 * g.P + i];
     f3 c{0.0f, 0.0f, 0.0f};
@@ -927,10 +927,29 @@ struct PrimaryArgs {
     CamBasis cb;
     const int32_t *triMaterial;
     const float *materials;
-    float *pos, *nrm, *alb, *mat, *depth;
+    float *pos;
+    void *nrm;  // float planes, or half planes when nrmHalf
+    float *alb, *mat, *depth;
     uint64_t P;
     uint32_t W, H;
+    bool nrmHalf;
 };
+
+// Writes normal n of pixel i as floats or as halves (round to nearest even).
+__device__ __forceinline__ void store_normal(const PrimaryArgs &a, size_t i, f3 n) {
+    const uint64_t P = a.P;
+    if (a.nrmHalf) {
+        _Float16 *h = static_cast<_Float16 *>(a.nrm);
+        h[i] = static_cast<_Float16>(n.x);
+        h[P + i] = static_cast<_Float16>(n.y);
+        h[2 * P + i] = static_cast<_Float16>(n.z);
+    } else {
+        float *f = static_cast<float *>(a.nrm);
+        f[i] = n.x;
+        f[P + i] = n.y;
+        f[2 * P + i] = n.z;
+    }
+}
 
 // Closest-hit slab test on a ray (not a segment): entry distance, or NaN on a miss (so every
 // comparison against it is false).
@@ -1036,7 +1055,7 @@ __global__ __launch_bounds__(256) void k_primary(const PrimaryArgs a) {
     const uint64_t P = a.P;
     if (bi < 0) {
         a.pos[i] = a.pos[P + i] = a.pos[2 * P + i] = 0.0f;
-        a.nrm[i] = a.nrm[P + i] = a.nrm[2 * P + i] = 0.0f;
+        store_normal(a, i, f3{0.0f, 0.0f, 0.0f});
         a.alb[i] = a.alb[P + i] = a.alb[2 * P + i] = a.alb[3 * P + i] = 0.0f;
         a.mat[i] = a.mat[P + i] = 0.0f;
         a.depth[i] = 1.0f;
@@ -1051,9 +1070,7 @@ __global__ __launch_bounds__(256) void k_primary(const PrimaryArgs a) {
     a.pos[i] = hp.x;
     a.pos[P + i] = hp.y;
     a.pos[2 * P + i] = hp.z;
-    a.nrm[i] = nn.x;
-    a.nrm[P + i] = nn.y;
-    a.nrm[2 * P + i] = nn.z;
+    store_normal(a, i, nn);
     a.alb[i] = m[0];
     a.alb[P + i] = m[1];
     a.alb[2 * P + i] = m[2];
@@ -1520,9 +1537,9 @@ CamBasis make_cam_basis(const restir_camera &c, float aspect) {
 
 hipError_t launch_primary(int stack, const SceneView &sc, const CamBasis &cb,
                           const int32_t *triMaterial, const float *materials, float *pos,
-                          float *nrm, float *alb, float *mat, float *depth, uint64_t P, uint32_t W,
+                          void *nrm, bool nrmHalf, float *alb, float *mat, float *depth, uint64_t P, uint32_t W,
                           uint32_t H, hipStream_t s) {
-    PrimaryArgs args{sc, cb, triMaterial, materials, pos, nrm, alb, mat, depth, P, W, H};
+    PrimaryArgs args{sc, cb, triMaterial, materials, pos, nrm, alb, mat, depth, P, W, H, nrmHalf};
     const size_t n = static_cast<size_t>(W) * H;
     const dim3 grid(static_cast<uint32_t>((n + 255) / 256));
     if (stack <= 32) RK_LAUNCH("k_primary", (k_primary<32>), grid, dim3(256), 0, s, args);

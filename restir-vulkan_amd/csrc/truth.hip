@@ -155,7 +155,8 @@ __global__ __launch_bounds__(256) void k_truth(const TruthArgs a) {
     const uint32_t p = a.pixels[i];
     const rk::GView &g = a.g;
     const d3 P = d3{g.pos[p], g.pos[g.P + p], g.pos[2 * g.P + p]};
-    const float nx = g.nrm[p], ny = g.nrm[g.P + p], nz = g.nrm[2 * g.P + p];
+    const rt::f3 nf = rk::load_normal(g, p);
+    const float nx = nf.x, ny = nf.y, nz = nf.z;
     const d3 alb = d3{g.alb[p], g.alb[g.P + p], g.alb[2 * g.P + p]};
     const float emissive = g.alb[3 * g.P + p];
     const double rough = g.mat[p], metal = g.mat[g.P + p];

@@ -21,7 +21,7 @@ TEMPORAL_REUSE_FLAG = 1 << 1
 TRACE_FIRST_HIT = 0
 TRACE_REFERENCE_LOOP = 1
 TRACE_ONE_RAY_PER_LANE = 2
-ABI_VERSION = 11  # RESTIR_ABI_VERSION of include/restir_abi.h
+ABI_VERSION = 12  # RESTIR_ABI_VERSION of include/restir_abi.h
 
 ERRORS = {
     -1: "RESTIR_E_INVALID_ARGUMENT",
@@ -80,9 +80,12 @@ class GBuffer(C.Structure):
         ("plane_stride", C.c_uint64),
         ("width", C.c_uint32),
         ("height", C.c_uint32),
+        ("normal_format", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
+NORMAL_F32, NORMAL_F16 = 0, 1  # RESTIR_NORMAL_*
 SCHEDULE_QUEUED = 0
 SCHEDULE_INLINE = 1
 LIGHT_BUFFER_HEADER_BYTES = 16  # RESTIR_LIGHT_BUFFER_HEADER_BYTES
@@ -123,7 +126,7 @@ def mlp_sizes(k: int):
     return 3 * 128 * 128 + k * 128, 3 * 128 + k
 
 
-assert C.sizeof(Uniforms) == 128 and C.sizeof(GBuffer) == 56 and C.sizeof(Camera) == 48
+assert C.sizeof(Uniforms) == 128 and C.sizeof(GBuffer) == 64 and C.sizeof(Camera) == 48
 assert C.sizeof(MlpStruct) == 56
 
 
@@ -171,8 +174,8 @@ SIGNATURES = {
                                            _vp, _vp, _vp]),
     "restir_build_light_clusters_device": (_i32, [_vp, _vp, _u32, _u32, C.c_uint64, _u32, _vp, _vp, _vp,
                                                   _vp, _vp, _vp, _vp]),
-    "restir_primary_gbuffer": (_i32, [_vp, C.POINTER(Camera), _vp, _vp, _u32, _vp, _vp, _vp, _vp,
-                                      _vp, _u64, _u32, _u32, _vp]),
+    "restir_primary_gbuffer": (_i32, [_vp, C.POINTER(Camera), _vp, _vp, _u32, _vp, _vp, _u32, _vp,
+                                      _vp, _vp, _u64, _u32, _u32, _vp]),
     "restir_camera_projection_view": (_i32, [C.POINTER(Camera), _f32, C.POINTER(_f32)]),
     "restir_mlp_init": (_i32, [_vp, C.POINTER(MlpStruct), C.c_uint64, _vp, _vp]),
     "restir_set_light_clusters": (_i32, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -413,10 +416,12 @@ def make_uniforms(width: int, height: int, frame: int, camera_pos, prev_pv=None,
     return u
 
 
-def gbuffer_struct(base_ptr: int, width: int, height: int, plane_stride: int | None = None) -> GBuffer:
+def gbuffer_struct(base_ptr: int, width: int, height: int, plane_stride: int | None = None,
+                   normal_format: int = NORMAL_F32) -> GBuffer:
     """G-buffer view of one float32 allocation holding 13 planes in the order
     position(3), normal(3), albedo(4: rgb + emissive flag), material(2: roughness, metallic),
-    depth(1)."""
+    depth(1). With NORMAL_F16 the three normal planes are halves (stride plane_stride halves)
+    at the start of the normal region."""
     P = plane_stride or width * height
     g = GBuffer()
     g.position = base_ptr
@@ -427,6 +432,7 @@ def gbuffer_struct(base_ptr: int, width: int, height: int, plane_stride: int | N
     g.plane_stride = P
     g.width = width
     g.height = height
+    g.normal_format = normal_format
     return g
 
 
@@ -624,11 +630,12 @@ class Context:
             as_ptr(gb), as_ptr(loss), stream_ptr(stream)), "restir_mlp_loss_grad")
 
     def primary_gbuffer(self, cam: dict, tri_material, materials, n_materials, gbuf_base, width,
-                        height, plane_stride=None, stream=None):
+                        height, plane_stride=None, stream=None, normal_format=NORMAL_F32):
+        """Fills a G-buffer laid out as gbuffer_struct describes (normals as float or half planes)."""
         P = plane_stride or width * height
         cs = camera_struct(cam)
         base = as_ptr(gbuf_base)
         self._check(self.lib.restir_primary_gbuffer(
             self.h, C.byref(cs), as_ptr(tri_material), as_ptr(materials), n_materials,
-            base, base + 4 * 3 * P, base + 4 * 6 * P, base + 4 * 10 * P, base + 4 * 12 * P, P,
+            base, base + 4 * 3 * P, normal_format, base + 4 * 6 * P, base + 4 * 10 * P, base + 4 * 12 * P, P,
             width, height, stream_ptr(stream)), "restir_primary_gbuffer")

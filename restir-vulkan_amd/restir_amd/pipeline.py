@@ -102,7 +102,8 @@ class Renderer:
     def __init__(self, scene, width, height, *, device_index=0, mode="unbiased", M=32, k=5,
                  spatial_passes=1, spatial_neighbors=4, radius=30.0, temporal=True,
                  visibility=True, temporal_mult=20, launch=None, ctx=None, learned=None,
-                 train_samples=4096, lr=1e-3, train=True, launches=None, scene_build="host"):
+                 train_samples=4096, lr=1e-3, train=True, launches=None, scene_build="host",
+                 normal_format=abi.NORMAL_F32):
         torch = _torch()
         self.torch = torch
         self.device = torch.device("cuda", device_index)
@@ -118,6 +119,7 @@ class Renderer:
         self.flags = (abi.VISIBILITY_REUSE_FLAG if visibility else 0) | \
                      (abi.TEMPORAL_REUSE_FLAG if temporal else 0)
         self.launch = launch
+        self.normal_format = normal_format  # G-buffer normals as float or half planes
         self.launches = {"a": launch, "b": launch, "c": launch}
         self.launches.update(launches or {})
         # learned proposal: a restir_amd.learned.LearnedProposal bound to this context, or None
@@ -136,12 +138,12 @@ class Renderer:
 
     # -------------------------------------------------------------------------------- pieces
     def gstruct(self, i):
-        return gbuffer_struct(self.gbuf[i].data_ptr(), self.W, self.H)
+        return gbuffer_struct(self.gbuf[i].data_ptr(), self.W, self.H, normal_format=self.normal_format)
 
     def render_gbuffer(self, cam: dict, i: int, stream=None):
         self.ctx.primary_gbuffer(cam, self.dev.d_tri_material, self.dev.d_materials,
                                  len(self.scene.materials), self.gbuf[i], self.W, self.H,
-                                 stream=stream)
+                                 stream=stream, normal_format=self.normal_format)
 
     def uniforms(self, cam: dict):
         pv_now = projection_view(cam, self.W / self.H)
@@ -222,7 +224,20 @@ class Renderer:
         return t.cpu().numpy().view(RESERVOIR_DTYPE)
 
     def gbuffer_host(self, i=None) -> np.ndarray:
-        return self.gbuf[self.cur if i is None else i].cpu().numpy()
+        """The 13 G-buffer planes as float32 on the host (half normals decoded to float, exactly)."""
+        return float_planes(self.gbuf[self.cur if i is None else i].cpu().numpy(), self.W * self.H,
+                            self.normal_format)
+
+
+def float_planes(planes: np.ndarray, npx: int, normal_format: int) -> np.ndarray:
+    """A host G-buffer (13 float32 planes of npx) with its normals as float planes: half normals
+    (stored in the first half of the normal region) are widened, which is exact."""
+    if normal_format == abi.NORMAL_F32:
+        return planes
+    out = planes.copy()
+    halves = planes[3 * npx: 6 * npx].view(np.float16)[: 3 * npx]
+    out[3 * npx: 6 * npx] = halves.astype(np.float32)
+    return out
 
 
 class BatchRenderer:
@@ -238,7 +253,7 @@ class BatchRenderer:
 
     def __init__(self, views, *, device_index=0, mode="unbiased", M=32, k=5, spatial_passes=1,
                  spatial_neighbors=4, radius=30.0, temporal=True, visibility=True, temporal_mult=20,
-                 launch=None, rays="persistent"):
+                 launch=None, rays="persistent", normal_format=abi.NORMAL_F32):
         torch = _torch()
         self.torch = torch
         self.device = torch.device("cuda", device_index)
@@ -252,6 +267,7 @@ class BatchRenderer:
         if rays not in ("queued", "persistent"):
             raise ValueError(f"rays must be 'queued' or 'persistent', not {rays!r}")
         self.rays = rays
+        self.normal_format = normal_format
         tx, ty = (launch.tile_x, launch.tile_y) if launch is not None else (0, 0)
         self.launch_q = abi.Launch(tx, ty, abi.SCHEDULE_QUEUED)
         self.scene_dev = {}
@@ -286,7 +302,7 @@ class BatchRenderer:
         self.frame, self.cur = 0, 1
 
     def _g(self, v, i):
-        return gbuffer_struct(v["gbuf"][i].data_ptr(), v["W"], v["H"])
+        return gbuffer_struct(v["gbuf"][i].data_ptr(), v["W"], v["H"], normal_format=self.normal_format)
 
     def render_frame(self, cams, stream=None):
         """One frame of every view; cams[v] is view v's camera."""
@@ -306,7 +322,8 @@ class BatchRenderer:
         for v, cam in zip(self.views, cams):
             v["cam"] = cam
             v["ctx"].primary_gbuffer(cam, v["dev"].d_tri_material, v["dev"].d_materials,
-                                     len(v["scene"].materials), v["gbuf"][i], v["W"], v["H"], stream=stream)
+                                     len(v["scene"].materials), v["gbuf"][i], v["W"], v["H"], stream=stream,
+                                     normal_format=self.normal_format)
             pv_now = projection_view(cam, v["W"] / v["H"])
             prev_pv = v["prev_pv"] if v["prev_pv"] is not None else pv_now
             u = make_uniforms(v["W"], v["H"], self.frame, cam["position"], prev_pv, M=self.M,

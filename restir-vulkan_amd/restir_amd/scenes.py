@@ -251,6 +251,30 @@ def orbit_camera(camera: dict, frame: int, degrees_per_frame: float = 0.5) -> di
     return cam
 
 
+def batched_view_set(seed: int = 3):
+    """The batched-views workload: 4 seeded terrains (grids 110..300, 60..1000 boxes: about
+    25k-200k triangles; 1k-50k point lights; a third of the box count times 4 emissive lamp
+    triangles, in one alias table with the point lights) and 16 views over them (view v shows
+    scene v mod 4) at 16:9 resolutions with heights log-uniform in [360, 1440]. Returns
+    (scenes, [(scene, width, height)] * 16)."""
+    rng = np.random.default_rng(seed)
+    specs = [(110, 60, 1000), (160, 200, 5000), (220, 400, 20000), (300, 1000, 50000)]
+    scs = [terrain(20 + i, grid=g, n_boxes=b, n_point_lights=nl, extent=32.0, n_tri_lights=4 * b)
+           for i, (g, b, nl) in enumerate(specs)]
+    views = []
+    for v in range(16):
+        h = int(round(360 * 4.0 ** rng.random()))
+        views.append((scs[v % 4], int(round(h * 16 / 9)), h))
+    return scs, views
+
+
+def stress_scene() -> Scene:
+    """The 4K stress workload's scene: a 1024 x 1024-vertex heightfield plus 20,000 boxes
+    (2 * 1023^2 + 240,000 = 2,333,058 triangles) and 1M point lights with log-uniform intensity
+    in [1e-3, 10]."""
+    return terrain(5, grid=1024, n_boxes=20000, n_point_lights=1_000_000, light_intensity=(1e-3, 10.0))
+
+
 def config_scene(name: str) -> Scene:
     if name in ("c1", "cornell"):
         return cornell_box(1)

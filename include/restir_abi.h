@@ -74,6 +74,13 @@ typedef struct restir_uniforms {
 
 #define RESTIR_VISIBILITY_REUSE_FLAG (1 << 0) /* restirStructs.glsl:37 */
 #define RESTIR_TEMPORAL_REUSE_FLAG (1 << 1)   /* restirStructs.glsl:38 */
+/* No counterpart in the reference (its default behaviour is this flag off): the estimator
+ * without the reference's two departures from RIS. Pass A ends its candidate loop with the
+ * standard contribution weight W = sumW / (M p_hat) instead of addSampleToReservoir's
+ * selection-time W (reservoir.glsl:28-42), and a triangle-light candidate's density is
+ * p / area (uniform by area) instead of p / (|cos| area) (restirOmni.glsl:129-131), whose
+ * estimator converges to the integral of f |cos_light| rather than of f. */
+#define RESTIR_STANDARD_WEIGHTS_FLAG (1 << 2)
 
 /* AabbTreeNode, src/shaders/include/structs/aabbTree.glsl:1-8 (80 B). Child < 0 is leaf ~triIndex. */
 typedef struct restir_aabb_tree_node {

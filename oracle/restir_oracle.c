@@ -748,6 +748,7 @@ static void pass_a_impl(const oracle_scene *s, const restir_uniforms *u, const r
     int32_t nLights = mixed ? pointCount + (int32_t)s->n_tri_lights
                             : (pointCount > 0 ? pointCount : (int32_t)s->n_tri_lights);
     size_t npx = (size_t)W * H;
+    int standard = (u->flags & RESTIR_STANDARD_WEIGHTS_FLAG) != 0;
 #pragma omp parallel for schedule(dynamic, 4)
     for (int64_t yy = 0; yy < (int64_t)H; ++yy) {
         uint32_t y = (uint32_t)yy;
@@ -803,7 +804,8 @@ static void pass_a_impl(const oracle_scene *s, const restir_uniforms *u, const r
                         lidx = -1 - idx;
                         v3 wi = vnorm(vsub(t.pos, L));
                         v3 nl = ld3(tl->normalArea);
-                        prob /= fabsf(vdot(wi, nl)) * tl->normalArea[3];
+                        if (standard) prob /= tl->normalArea[3]; /* uniform by area */
+                        else prob /= fabsf(vdot(wi, nl)) * tl->normalArea[3];
                         ln4[0] = nl.x; ln4[1] = nl.y; ln4[2] = nl.z; ln4[3] = 1.0f;
                     }
                     float pHat = phat_v(t.pos, L, cam, t.n, ld3(ln4), ln4[3] > 0.5f, albLum, lum,
@@ -811,6 +813,10 @@ static void pass_a_impl(const oracle_scene *s, const restir_uniforms *u, const r
                     float Lp[3] = {L.x, L.y, L.z};
                     oracle_add_sample(&res, Lp, ln4, lum, lidx, pHat, prob, rng);
                 }
+                /* RESTIR_STANDARD_WEIGHTS_FLAG: W = sumW / (M p_hat) after the candidates */
+                if (standard && res.samples[0].w > 0.0f)
+                    res.samples[0].w = res.samples[0].sumWeights /
+                                       ((float)res.numStreamSamples * res.samples[0].pHat);
             }
             /* visibility reuse, :148-160. A reservoir with sumWeights == 0 holds w == 0 and is
              * left unchanged by a shadow hit, so its ray is not traced. */
@@ -918,6 +924,7 @@ void oracle_pass_c(const oracle_scene *s, const restir_uniforms *u, int k, int i
     uint32_t W = u->screenSize[0], H = u->screenSize[1];
     v3 cam = ld3(u->cameraPos);
     int vis = (u->flags & RESTIR_VISIBILITY_REUSE_FLAG) != 0;
+    int standard = (u->flags & RESTIR_STANDARD_WEIGHTS_FLAG) != 0;
 #pragma omp parallel for schedule(dynamic, 4)
     for (int64_t yy = 0; yy < (int64_t)H; ++yy) {
         uint32_t y = (uint32_t)yy;
@@ -964,12 +971,16 @@ void oracle_pass_c(const oracle_scene *s, const restir_uniforms *u, int k, int i
                 numSamples += nM[j];
             }
             if (vis && test_visibility(s, t.pos, lp)) numSamples = 0;
-            if (numSamples > 0) {
+            if (numSamples > 0 && (!standard || res.samples[0].pHat > 0.0f)) {
                 res.samples[0].w = res.samples[0].sumWeights / ((float)numSamples * res.samples[0].pHat);
             } else {
                 res.samples[0].w = 0.0f;
                 res.samples[0].sumWeights = 0.0f;
             }
+            /* RESTIR_STANDARD_WEIGHTS_FLAG: keep sumW = p_hat W M, the weight a later pass gives
+             * every other input reservoir (the reference keeps p_hat W Z) */
+            if (standard)
+                res.samples[0].sumWeights = res.samples[0].w * (float)res.numStreamSamples * res.samples[0].pHat;
             out[idx] = res;
         }
     }

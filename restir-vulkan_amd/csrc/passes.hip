@@ -150,7 +150,7 @@ __device__ __forceinline__ Pcg pass_a_rng(const PassAArgs &a, uint32_t x, uint32
 // reservoir update run once for all lanes of a wave whatever kinds its lanes drew. `valid` false:
 // an empty candidate (p_hat 0, probability 1) that still counts in M and draws its number.
 __device__ __forceinline__ void add_candidate(Res &res, const Surface &s, f3 pos, const SceneView &sc, bool valid,
-                                              bool tri, int32_t li, float prob, Pcg &rng) {
+                                              bool tri, int32_t li, float prob, Pcg &rng, bool areaPdf) {
     f3 L{0.0f, 0.0f, 0.0f}, nl{0.0f, 0.0f, 0.0f};
     float lum = 0.0f;
     int32_t index = 0;
@@ -170,7 +170,8 @@ __device__ __forceinline__ void add_candidate(Res &res, const Surface &s, f3 pos
         L = ((1.0f - sa) * p1 + (sa * (1.0f - rb)) * p2) + (rb * sa) * p3;
         const f3 wi = normalize(pos - L);
         nl = xyz(na);
-        prob /= fabsf(dot(wi, nl)) * na.w;
+        if (areaPdf) prob /= na.w;  // uniform by area (RESTIR_STANDARD_WEIGHTS_FLAG)
+        else prob /= fabsf(dot(wi, nl)) * na.w;
         index = -1 - li;
     }
     const bool useLN = valid && tri;
@@ -187,6 +188,7 @@ __device__ __forceinline__ void pass_a_ris(const PassAArgs &a, const PassASurfac
     const Surface &s = ps.s;
     res = res_zero();
     const SceneView &sc = a.sc;
+    const bool standard = (a.u.flags & RESTIR_STANDARD_WEIGHTS_FLAG) != 0;
     if (dot(n, n) != 0.0f && a.candLight) {
         // RIS over the learned proposal's candidates: weight p_hat / q with the proposal's own
         // density q; a candidate from an empty cluster (l < 0) has weight 0 but still counts in M.
@@ -198,7 +200,7 @@ __device__ __forceinline__ void pass_a_ris(const PassAArgs &a, const PassASurfac
             const float q = a.candQ[i * npx + idx];
             const bool valid = li >= 0 && li < sc.nLights && q > 0.0f;
             const bool tri = li >= sc.nPoints;
-            add_candidate(res, s, pos, sc, valid, tri, tri ? li - sc.nPoints : li, q, rng);
+            add_candidate(res, s, pos, sc, valid, tri, tri ? li - sc.nPoints : li, q, rng, standard);
         }
     } else if (dot(n, n) != 0.0f) {
         const uint32_t M = a.u.initialLightSampleCount;
@@ -220,9 +222,11 @@ __device__ __forceinline__ void pass_a_ris(const PassAArgs &a, const PassASurfac
             // restirOmni.glsl:117: point lights if there are any, else triangle lights; a union
             // table names point lights first
             const bool tri = sc.mixed ? li >= sc.nPoints : sc.nPoints == 0;
-            add_candidate(res, s, pos, sc, true, tri, sc.mixed && tri ? li - sc.nPoints : li, prob, rng);
+            add_candidate(res, s, pos, sc, true, tri, sc.mixed && tri ? li - sc.nPoints : li, prob, rng, standard);
         }
     }
+    // the standard RIS contribution weight instead of the selection-time one
+    if (standard && res.w > 0.0f) res.w = res.sumW / (static_cast<float>(res.M) * res.pHat);
 }
 
 // Temporal reuse of the previous frame's reservoir at the reprojected pixel.
@@ -403,13 +407,20 @@ __device__ __forceinline__ void pass_c_merge(const PassCArgs &a, uint32_t x, uin
 }
 
 // W = sumWeights / (Z * pHat), or an empty reservoir when Z = 0 (unbiasedReuse.glsl:172-181).
-__device__ __forceinline__ void finish_w(float &w, float &sumW, float pHat, uint32_t numSamples) {
-    if (numSamples > 0) {
+// W of a merged reservoir from the count Z of the reservoirs that could have produced its sample
+// (unbiasedReuse.glsl:167-181). With RESTIR_STANDARD_WEIGHTS_FLAG an empty reservoir (p_hat 0)
+// gets W = 0 instead of 0/0, and sumW is reset to p_hat W M (M the merged count): a later pass
+// then weighs this reservoir like every other input, p_hat W M. The reference keeps sumW = p_hat W Z,
+// which a second pass weighs against M: each pass after the first is then biased.
+__device__ __forceinline__ void finish_w(float &w, float &sumW, float pHat, uint32_t numSamples, uint32_t M,
+                                         bool standard) {
+    if (numSamples > 0 && (!standard || pHat > 0.0f)) {
         w = sumW / (static_cast<float>(numSamples) * pHat);
     } else {
         w = 0.0f;
         sumW = 0.0f;
     }
+    if (standard) sumW = w * static_cast<float>(M) * pHat;
 }
 
 template <int TX, int TY, int STACK>
@@ -464,7 +475,7 @@ __device__ __forceinline__ void pass_c_px(const PassCArgs &a, uint32_t x, uint32
             numSamples += nM[j];
         }
     }
-    finish_w(res.w, res.sumW, res.pHat, numSamples);
+    finish_w(res.w, res.sumW, res.pHat, numSamples, res.M, (a.u.flags & RESTIR_STANDARD_WEIGHTS_FLAG) != 0);
     store_res(a.out, idx, res);
 }
 
@@ -563,7 +574,9 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
                 above |= 1u << j;
                 if (!vis) numSamples += q.nM[static_cast<size_t>(j) * q.npx + idx];
             }
-            if (!vis) finish_w(res.w, res.sumW, res.pHat, numSamples);
+            if (!vis)
+                finish_w(res.w, res.sumW, res.pHat, numSamples, res.M,
+                         (a.u.flags & RESTIR_STANDARD_WEIGHTS_FLAG) != 0);
             store_res(a.out, idx, res);
             q.origM[idx] = origM;
             q.occ[idx] = 0u;
@@ -652,7 +665,9 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_finish(const PassCArgs a, con
     }
     float4 *r = reinterpret_cast<float4 *>(a.out + idx) + 2;  // {lightIndex, pHat, sumW, w}
     float4 v = *r;
-    finish_w(v.w, v.z, v.y, numSamples);
+    const bool standard = (a.u.flags & RESTIR_STANDARD_WEIGHTS_FLAG) != 0;
+    const uint32_t M = standard ? __float_as_uint(r[1].x) : 0u;  // numStreamSamples
+    finish_w(v.w, v.z, v.y, numSamples, M, standard);
     *r = v;
 }
 

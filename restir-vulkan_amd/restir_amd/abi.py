@@ -18,6 +18,7 @@ LIB_PATH = PKG_DIR / "librestir_amd.so"
 
 VISIBILITY_REUSE_FLAG = 1 << 0
 TEMPORAL_REUSE_FLAG = 1 << 1
+STANDARD_WEIGHTS_FLAG = 1 << 2  # RIS with W = sumW / (M p_hat) and triangle lights by area
 TRACE_FIRST_HIT = 0
 TRACE_REFERENCE_LOOP = 1
 TRACE_ONE_RAY_PER_LANE = 2

@@ -103,7 +103,7 @@ class Renderer:
                  spatial_passes=1, spatial_neighbors=4, radius=30.0, temporal=True,
                  visibility=True, temporal_mult=20, launch=None, ctx=None, learned=None,
                  train_samples=4096, lr=1e-3, train=True, launches=None, scene_build="host",
-                 normal_format=abi.NORMAL_F32):
+                 normal_format=abi.NORMAL_F32, standard_weights=False):
         torch = _torch()
         self.torch = torch
         self.device = torch.device("cuda", device_index)
@@ -117,7 +117,8 @@ class Renderer:
         self.radius = radius
         self.temporal_mult = temporal_mult
         self.flags = (abi.VISIBILITY_REUSE_FLAG if visibility else 0) | \
-                     (abi.TEMPORAL_REUSE_FLAG if temporal else 0)
+                     (abi.TEMPORAL_REUSE_FLAG if temporal else 0) | \
+                     (abi.STANDARD_WEIGHTS_FLAG if standard_weights else 0)
         self.launch = launch
         self.normal_format = normal_format  # G-buffer normals as float or half planes
         self.launches = {"a": launch, "b": launch, "c": launch}

@@ -152,6 +152,23 @@ def test_unbiased_terrain_with_point_and_triangle_lights(torch):
 
 
 @pytest.mark.parametrize("schedule", [abi.SCHEDULE_QUEUED, abi.SCHEDULE_INLINE])
+def test_standard_weights_bit_exact(torch, schedule):
+    """RESTIR_STANDARD_WEIGHTS_FLAG (standard RIS W, triangle lights by area, sumW = p_hat W M
+    after Pass C) on the 1080p scene's point and triangle lights: every pass equals the oracle."""
+    sc = scenes.config_scene("terrain")
+    r = Renderer(sc, 320, 180, mode="unbiased", k=5, spatial_passes=2, standard_weights=True,
+                 launch=abi.Launch(16, 16, schedule))
+    assert r.flags & abi.STANDARD_WEIGHTS_FLAG
+    for f in range(3):
+        a, c = run_frame_and_compare(r, scenes.orbit_camera(sc.camera, f))
+    lit = c["w"] > 0
+    # the invariant a later pass relies on
+    np.testing.assert_array_equal(c["sumWeights"][lit],
+                                  (c["w"][lit] * c["M"][lit].astype(np.float32) * c["pHat"][lit]).astype(np.float32))
+    assert not np.isnan(c["w"]).any()
+
+
+@pytest.mark.parametrize("schedule", [abi.SCHEDULE_QUEUED, abi.SCHEDULE_INLINE])
 def test_biased_cornell_with_triangle_lights_bit_exact(torch, schedule):
     import dataclasses
     sc = scenes.cornell_box(1)

@@ -372,3 +372,90 @@ def test_candidates_triangle_light_point_from_first_two_draws():
         assert out["sumWeights"][p] == pytest.approx(expect, rel=1e-5)
         checked += 1
     assert checked > 50
+
+
+# ------------------------------------------------ standard weights (RESTIR_STANDARD_WEIGHTS_FLAG)
+def test_standard_weights_change_only_w(cornell):
+    """Point lights, no reuse: the flag leaves draws, selection, M and sumW unchanged and sets
+    W = sumW / (M p_hat) in place of the selection-time W."""
+    sc, hs, W, H, g, pv = cornell
+    og = oracle.gbuffer(g, W, H)
+    u0 = abi.make_uniforms(W, H, 3, sc.camera["position"], pv, M=8, flags=0)
+    u1 = abi.make_uniforms(W, H, 3, sc.camera["position"], pv, M=8, flags=abi.STANDARD_WEIGHTS_FLAG)
+    a0 = oracle.pass_a(hs.oracle, u0, og, None, None, abi.RESERVOIR_DTYPE)
+    a1 = oracle.pass_a(hs.oracle, u1, og, None, None, abi.RESERVOIR_DTYPE)
+    for f in ("lightIndex", "M", "sumWeights", "pHat", "pos_lum"):
+        np.testing.assert_array_equal(a0[f], a1[f], err_msg=f)
+    lit = a0["w"] > 0
+    assert lit.mean() > 0.5
+    expect = (a1["sumWeights"][lit] / (a1["M"][lit].astype(np.float32) * a1["pHat"][lit])).astype(np.float32)
+    np.testing.assert_array_equal(a1["w"][lit], expect)
+    assert (a1["w"][~lit] == 0).all()
+
+
+def test_standard_weights_make_ris_unbiased():
+    """RIS over M = 3 alias-table candidates at a 12x12 patch of the Cornell box, no visibility:
+    E[p_hat(Y) W] is the sum of p_hat over the 64 lights with the standard W (within 4 standard
+    errors over 10,000 frames; 30,000 give z = -0.6), and far off with the selection-time W
+    (z = 265 at 30,000 frames: addSampleToReservoir's W is biased on its own)."""
+    sc = scenes.cornell_box(1)
+    hs = HostScene(sc)
+    W = H = 12
+    g = hs.oracle_gbuffer(W, H)
+    og = oracle.gbuffer(g, W, H)
+    P = W * H
+    surf = np.flatnonzero(_surface(g, W, H))
+    pts = sc.point_lights
+    cam = np.ascontiguousarray(sc.camera["position"], np.float32)
+    zero = np.zeros(3, np.float32)
+    L = oracle.lib()
+    total = np.zeros(P)
+    for p in surf:
+        pos = np.ascontiguousarray(g[[p, P + p, 2 * P + p]], np.float32)
+        n = np.ascontiguousarray(g[[3 * P + p, 4 * P + p, 5 * P + p]], np.float32)
+        alb = L.oracle_luminance(*map(float, g[[6 * P + p, 7 * P + p, 8 * P + p]]))
+        for l in range(len(pts)):
+            lp = np.ascontiguousarray(pts["pos"][l, :3], np.float32)
+            total[p] += L.oracle_phat(pos.ctypes.data, lp.ctypes.data, cam.ctypes.data, n.ctypes.data,
+                                      zero.ctypes.data, 0, alb, float(pts["color_lum"][l, 3]),
+                                      float(g[10 * P + p]), float(g[11 * P + p]))
+    frames = 10000
+    z = {}
+    for flags in (0, abi.STANDARD_WEIGHTS_FLAG):
+        est = np.zeros((frames, len(surf)))
+        for f in range(frames):
+            u = abi.make_uniforms(W, H, 1000 + f, sc.camera["position"], M=3, flags=flags)
+            a = oracle.pass_a(hs.oracle, u, og, None, None, abi.RESERVOIR_DTYPE)
+            est[f] = (a["pHat"].astype(np.float64) * a["w"])[surf]
+        # one statistic over the patch: the mean relative estimate of the per-pixel totals
+        rel = est / total[surf]
+        per_frame = rel.mean(axis=1)
+        z[flags] = (per_frame.mean() - 1.0) / (per_frame.std(ddof=1) / np.sqrt(frames))
+    assert abs(z[abi.STANDARD_WEIGHTS_FLAG]) < 4.0, z
+    assert abs(z[0]) > 8.0, z
+
+
+def test_standard_weights_pass_c_keeps_sumw_consistent():
+    """With the flag, Pass C leaves sumW = p_hat W M (M the merged count), the weight a second
+    pass gives every other input, and W = 0 rather than 0/0 for an empty reservoir; without it
+    sumW = p_hat W Z (Z <= M the filtered count), as the reference leaves it."""
+    sc = scenes.cornell_box(1)
+    hs = HostScene(sc)
+    W, H = 48, 36
+    g = hs.oracle_gbuffer(W, H)
+    og = oracle.gbuffer(g, W, H)
+    pv = abi.projection_view(sc.camera, W / H)
+    for flags in (abi.VISIBILITY_REUSE_FLAG, abi.VISIBILITY_REUSE_FLAG | abi.STANDARD_WEIGHTS_FLAG):
+        u = abi.make_uniforms(W, H, 2, sc.camera["position"], pv, M=8, neighbors=5, flags=flags)
+        a = oracle.pass_a(hs.oracle, u, og, None, None, abi.RESERVOIR_DTYPE)
+        c = oracle.pass_c(hs.oracle, u, 5, 0, og, a)
+        lit = c["w"] > 0
+        assert lit.mean() > 0.2
+        pwm = (c["w"][lit] * c["M"][lit].astype(np.float32) * c["pHat"][lit]).astype(np.float32)
+        if flags & abi.STANDARD_WEIGHTS_FLAG:
+            np.testing.assert_array_equal(c["sumWeights"][lit], pwm)
+            assert not np.isnan(c["w"]).any()
+        else:
+            # the reference's sumW is p_hat W Z, below p_hat W M wherever a neighbour was not counted
+            assert (c["sumWeights"][lit] <= pwm * (1 + 1e-5)).all()
+            assert (c["sumWeights"][lit] < pwm * (1 - 1e-3)).mean() > 0.05

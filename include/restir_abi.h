@@ -186,7 +186,7 @@ void restir_destroy(restir_ctx *ctx);
 const char *restir_last_error(const restir_ctx *ctx);
 /* ABI revision; bumped whenever an entry point's signature changes. Bindings compare it with the
  * revision they were written against and refuse a mismatched library. */
-#define RESTIR_ABI_VERSION 12
+#define RESTIR_ABI_VERSION 13
 int restir_version(void);
 
 /* Light tables: device pointers to the reference buffer images (header + array). The counts are
@@ -454,6 +454,50 @@ int restir_primary_gbuffer(restir_ctx *ctx, const restir_camera *cam, const int3
                            void *normal, uint32_t normal_format, float *albedo, float *material,
                            float *depth, uint64_t plane_stride, uint32_t width, uint32_t height,
                            void *stream);
+
+/* ---- the reference's G-buffer attachments (gBufferPass.cpp:75-108, written by gBuffer.frag:27-80)
+ * as copied into device buffers (vkCmdCopyImageToBuffer, or images in exported memory with linear
+ * tiling): row y of an attachment starts at data + y * row_bytes, texels packed in a row. */
+typedef enum restir_texel_format {
+    RESTIR_TEXEL_RGBA8_SRGB = 1,   /* VK_FORMAT_R8G8B8A8_SRGB: albedo (rgb decoded sRGB -> linear, a unorm) */
+    RESTIR_TEXEL_RGBA8_UNORM = 2,  /* VK_FORMAT_R8G8B8A8_UNORM */
+    RESTIR_TEXEL_RGB16_SNORM = 3,  /* VK_FORMAT_R16G16B16_SNORM: normal, the reference's first choice */
+    RESTIR_TEXEL_RGBA16_SNORM = 4, /* VK_FORMAT_R16G16B16A16_SNORM */
+    RESTIR_TEXEL_RGB16_SFLOAT = 5, /* VK_FORMAT_R16G16B16_SFLOAT */
+    RESTIR_TEXEL_RGBA16_SFLOAT = 6,/* VK_FORMAT_R16G16B16A16_SFLOAT */
+    RESTIR_TEXEL_RGB32_SFLOAT = 7, /* VK_FORMAT_R32G32B32_SFLOAT: world position, normal fallback */
+    RESTIR_TEXEL_RGBA32_SFLOAT = 8,/* VK_FORMAT_R32G32B32A32_SFLOAT */
+    RESTIR_TEXEL_RG16_UNORM = 9,   /* VK_FORMAT_R16G16_UNORM: roughness, metallic */
+    RESTIR_TEXEL_D32_SFLOAT = 10,  /* depth aspect of VK_FORMAT_D32_SFLOAT or D32_SFLOAT_S8_UINT */
+    RESTIR_TEXEL_D24_UNORM = 11    /* depth aspect of VK_FORMAT_D24_UNORM_S8_UINT: 32-bit texels,
+                                      depth in the low 24 bits */
+} restir_texel_format;
+
+typedef struct restir_attachment {
+    const void *data;   /* device pointer */
+    uint32_t format;    /* restir_texel_format */
+    uint32_t row_bytes; /* >= width * texel size */
+} restir_attachment;
+
+typedef struct restir_gbuffer_attachments {
+    restir_attachment albedo;          /* RGBA8_SRGB, RGBA8_UNORM or RGBA32_SFLOAT; a > 0.5: emissive */
+    restir_attachment normal;          /* RGB16/RGBA16 SNORM or SFLOAT, RGB32/RGBA32 SFLOAT */
+    restir_attachment material;        /* RG16_UNORM or RGBA32_SFLOAT (roughness, metallic) */
+    restir_attachment world_position;  /* RGB32_SFLOAT or RGBA32_SFLOAT */
+    restir_attachment depth;           /* D32_SFLOAT or D24_UNORM */
+    uint32_t width, height;
+} restir_gbuffer_attachments;
+
+/* No counterpart in the reference, which samples its attachments: decodes them into the G-buffer
+ * planes restir_gbuffer describes (the arguments as restir_primary_gbuffer's outputs), the way a
+ * Vulkan sampler reads them: SNORM16 c -> max(c / 32767, -1), UNORM16 c / 65535, UNORM8 c / 255,
+ * sRGB8 rgb through the sRGB EOTF (c/255 <= 0.04045 ? (c/255) / 12.92 : ((c/255 + 0.055) / 1.055)^2.4,
+ * evaluated in double and rounded), half -> float exactly, D24 c / (2^24 - 1); normals are written
+ * as float or half planes (RESTIR_NORMAL_*). One pass over the pixels, stream-ordered. */
+int restir_gbuffer_from_attachments(restir_ctx *ctx, const restir_gbuffer_attachments *att,
+                                    float *position, void *normal, uint32_t normal_format,
+                                    float *albedo, float *material, float *depth,
+                                    uint64_t plane_stride, void *stream);
 
 /* The 4x4 projection-view matrix of camera.h:25-50 for the given aspect, column-major, as the
  * reference writes it into RestirUniforms::prevFrameProjectionViewMatrix (app.cpp:779). */

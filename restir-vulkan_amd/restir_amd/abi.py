@@ -22,7 +22,7 @@ STANDARD_WEIGHTS_FLAG = 1 << 2  # RIS with W = sumW / (M p_hat) and triangle lig
 TRACE_FIRST_HIT = 0
 TRACE_REFERENCE_LOOP = 1
 TRACE_ONE_RAY_PER_LANE = 2
-ABI_VERSION = 12  # RESTIR_ABI_VERSION of include/restir_abi.h
+ABI_VERSION = 13  # RESTIR_ABI_VERSION of include/restir_abi.h
 
 ERRORS = {
     -1: "RESTIR_E_INVALID_ARGUMENT",
@@ -87,6 +87,19 @@ class GBuffer(C.Structure):
 
 
 NORMAL_F32, NORMAL_F16 = 0, 1  # RESTIR_NORMAL_*
+# restir_texel_format: the reference's G-buffer attachment formats (gBufferPass.cpp:75-108)
+TEXEL = dict(RGBA8_SRGB=1, RGBA8_UNORM=2, RGB16_SNORM=3, RGBA16_SNORM=4, RGB16_SFLOAT=5, RGBA16_SFLOAT=6,
+             RGB32_SFLOAT=7, RGBA32_SFLOAT=8, RG16_UNORM=9, D32_SFLOAT=10, D24_UNORM=11)
+
+
+class Attachment(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("format", C.c_uint32), ("row_bytes", C.c_uint32)]
+
+
+class GBufferAttachments(C.Structure):
+    _fields_ = [("albedo", Attachment), ("normal", Attachment), ("material", Attachment),
+                ("world_position", Attachment), ("depth", Attachment), ("width", C.c_uint32),
+                ("height", C.c_uint32)]
 SCHEDULE_QUEUED = 0
 SCHEDULE_INLINE = 1
 LIGHT_BUFFER_HEADER_BYTES = 16  # RESTIR_LIGHT_BUFFER_HEADER_BYTES
@@ -175,6 +188,8 @@ SIGNATURES = {
                                            _vp, _vp, _vp]),
     "restir_build_light_clusters_device": (_i32, [_vp, _vp, _u32, _u32, C.c_uint64, _u32, _vp, _vp, _vp,
                                                   _vp, _vp, _vp, _vp]),
+    "restir_gbuffer_from_attachments": (_i32, [_vp, C.POINTER(GBufferAttachments), _vp, _vp, _u32, _vp, _vp,
+                                               _vp, _u64, _vp]),
     "restir_primary_gbuffer": (_i32, [_vp, C.POINTER(Camera), _vp, _vp, _u32, _vp, _vp, _u32, _vp,
                                       _vp, _vp, _u64, _u32, _u32, _vp]),
     "restir_camera_projection_view": (_i32, [C.POINTER(Camera), _f32, C.POINTER(_f32)]),
@@ -629,6 +644,21 @@ class Context:
         self._check(self.lib.restir_mlp_loss_grad(
             self.h, C.byref(mlp), as_ptr(X), as_ptr(target), as_ptr(mask), n, as_ptr(gw),
             as_ptr(gb), as_ptr(loss), stream_ptr(stream)), "restir_mlp_loss_grad")
+
+    def gbuffer_from_attachments(self, atts: dict, width, height, gbuf_base, plane_stride=None,
+                                 normal_format=NORMAL_F32, stream=None):
+        """atts: {"albedo" | "normal" | "material" | "world_position" | "depth": (device buffer,
+        TEXEL format name, row bytes)}; fills a G-buffer laid out as gbuffer_struct describes."""
+        P = plane_stride or width * height
+        ga = GBufferAttachments()
+        for k in ("albedo", "normal", "material", "world_position", "depth"):
+            buf, fmt, row = atts[k]
+            setattr(ga, k, Attachment(as_ptr(buf), TEXEL[fmt], row))
+        ga.width, ga.height = width, height
+        base = as_ptr(gbuf_base)
+        self._check(self.lib.restir_gbuffer_from_attachments(
+            self.h, C.byref(ga), base, base + 4 * 3 * P, normal_format, base + 4 * 6 * P, base + 4 * 10 * P,
+            base + 4 * 12 * P, P, stream_ptr(stream)), "restir_gbuffer_from_attachments")
 
     def primary_gbuffer(self, cam: dict, tri_material, materials, n_materials, gbuf_base, width,
                         height, plane_stride=None, stream=None, normal_format=NORMAL_F32):

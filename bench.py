@@ -274,6 +274,7 @@ def main():
     visits = torch.zeros_like(sh)
     tests = torch.zeros_like(sh)
     ray_sets = {"centre": [], "neighbour": []}
+    ctx.set_ray_export(True)  # the replay keeps its centre rays for the export
     for _j in fr.frame(f_last, args.M):
         for which, key in ((0, "centre"), (1, "neighbour")):
             n = ctx.unbiased_rays(which, q1, q2, cap, stream=stream)
@@ -283,6 +284,7 @@ def main():
                                   "T": float(tests[:n].double().mean()) if n else 0.0,
                                   "occluded": float(sh[:n].double().mean()) if n else 0.0})
     torch.cuda.synchronize()
+    ctx.set_ray_export(False)
     # standalone traversal throughput on the last Pass C's neighbour rays (left in q1/q2)
     nr = ray_sets["neighbour"][-1]["rays"]
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

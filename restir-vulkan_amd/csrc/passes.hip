@@ -406,7 +406,6 @@ __device__ __forceinline__ void pass_c_merge(const PassCArgs &a, uint32_t x, uin
     }
 }
 
-// W = sumWeights / (Z * pHat), or an empty reservoir when Z = 0 (unbiasedReuse.glsl:172-181).
 // W of a merged reservoir from the count Z of the reservoirs that could have produced its sample
 // (unbiasedReuse.glsl:167-181). With RESTIR_STANDARD_WEIGHTS_FLAG an empty reservoir (p_hat 0)
 // gets W = 0 instead of 0/0, and sumW is reset to p_hat W M (M the merged count): a later pass
@@ -496,15 +495,18 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c(const PassCArgs a) {
 // in pixel-tile order.
 struct CQueues {
     uint32_t *counter;     // [0] centre rays recorded, [1] neighbour rays queued, [3] their tracer's head
-    float4 *c1, *c2;       // centre ray end points (traced in the merge, kept for the export);
-                           // c1.w carries the pixel index
-    float4 *n1, *n2;       // neighbour ray end points; n1.w carries pixel | slot << 27
+    float4 *c1, *c2;       // centre ray end points, written only for the ray export (the merge
+                           // traces them itself); c1.w carries the pixel index
+    float4 *n1;            // neighbour rays: the neighbour's position, .w = pixel | slot << 27; the
+                           // other end is the light sample of `merged[pixel]`
+    const restir_reservoir *merged;  // the pass's output (merged reservoirs)
     uint32_t *origM;       // per pixel
     uint32_t *bits;        // per pixel: above-horizon neighbour mask (bits 0..15), flags below
     uint32_t *occ;         // per pixel: occluded neighbour mask
-    uint32_t *nM, *nIdx;   // [slot * npx + pixel]
+    uint32_t *nM, *nIdx;   // [slot * npx + pixel]: the neighbour's sample count and pixel
     float4 *surf;          // per pixel: {position.xyz, normal.x}, {normal.yz, 0, 0}
     uint32_t npx;
+    uint32_t exportCentre; // write c1/c2
 };
 constexpr uint32_t kBitBackground = 1u << 29;  // pass-through pixel, already final
 constexpr uint32_t kBitDone = 1u << 30;        // no visibility test: W already written
@@ -594,7 +596,7 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
         }
     }
     const uint32_t slot = wave_append(&q.counter[0], emit ? 1u : 0u);
-    if (emit) {
+    if (emit && q.exportCentre) {
         q.c1[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(static_cast<uint32_t>(idx)));
         q.c2[slot] = make_float4(lp.x, lp.y, lp.z, 0.0f);
     }
@@ -633,15 +635,12 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
         const uint32_t oWant = __shfl(want, static_cast<int>(o));
         const uint32_t oExcl = __shfl(excl, static_cast<int>(o));
         const uint32_t oIdx = __shfl(myIdx, static_cast<int>(o));
-        const float lx = __shfl(lp.x, static_cast<int>(o)), ly = __shfl(lp.y, static_cast<int>(o)),
-                    lz = __shfl(lp.z, static_cast<int>(o));
         if (r < total) {
             uint32_t m = oWant;
             for (uint32_t k = r - oExcl; k > 0; --k) m &= m - 1;  // the k-th set bit of the owner
             const uint32_t j = static_cast<uint32_t>(__ffs(static_cast<int>(m)) - 1);
             const float4 sp = q.surf[2 * static_cast<size_t>(q.nIdx[static_cast<size_t>(j) * q.npx + oIdx])];
             q.n1[base + r] = make_float4(sp.x, sp.y, sp.z, __uint_as_float(oIdx | (j << 27)));
-            q.n2[base + r] = make_float4(lx, ly, lz, 0.0f);
         }
     }
 }
@@ -725,7 +724,9 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
                         // a neighbour ray of a pixel whose centre ray was shadowed is not traced
                         if (PER_PIXEL || !(q.bits[tag & 0x07ffffffu] & kBitCentreShadowed)) {
                             f3 o, d;
-                            shadow_segment(xyz(a4), xyz(p2[i]), o, d);
+                            const float4 b4 = PER_PIXEL ? p2[i]
+                                                        : reinterpret_cast<const float4 *>(q.merged + (tag & 0x07ffffffu))[0];
+                            shadow_segment(xyz(a4), xyz(b4), o, d);
                             wwalk_begin(w, o, d, sc.treeNested != 0, sc.maxCoord);
                             ray = static_cast<int32_t>(i);
                         }
@@ -1223,7 +1224,7 @@ static hipError_t run_c_queued(PassCArgs args, const CQueues &q, uint32_t gridCa
     RK_LAUNCH("k_pass_c_merge", (k_pass_c_merge<TX, TY, STACK>), tiles, block, 0, s, args, q);
     if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
         RK_LAUNCH("k_trace_queue(neighbour)", (k_trace_queue<STACK, false>), dim3(gridCap), dim3(256), 0, s,
-                  args.sc, q.n1, q.n2, q.counter + 1, q.counter + 3, q);
+                  args.sc, q.n1, nullptr, q.counter + 1, q.counter + 3, q);
         RK_LAUNCH("k_pass_c_finish", (k_pass_c_finish<TX, TY>), tiles, block, 0, s, args, q);
     }
     return hipGetLastError();
@@ -1244,12 +1245,11 @@ static hipError_t dispatch_c_queued(int tile, const PassCArgs &args, const CQueu
 static CQueues carve_queues(void *scratch, uint32_t npx, int k, size_t *bytes = nullptr) {
     Carver c(scratch);
     const size_t n = npx, kk = static_cast<size_t>(k);
-    CQueues q;
+    CQueues q{};
     q.counter = c.take<uint32_t>(256);
     q.c1 = c.take<float4>(sizeof(float4) * n);
     q.c2 = c.take<float4>(sizeof(float4) * n);
     q.n1 = c.take<float4>(sizeof(float4) * n * kk);
-    q.n2 = c.take<float4>(sizeof(float4) * n * kk);
     q.origM = c.take<uint32_t>(sizeof(uint32_t) * n);
     q.bits = c.take<uint32_t>(sizeof(uint32_t) * n);
     q.occ = c.take<uint32_t>(sizeof(uint32_t) * n);
@@ -1276,13 +1276,16 @@ __global__ void k_traced_copy(const CQueues q, uint32_t len, const uint32_t *fla
                               float4 *p2, uint32_t cap) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= len || !flag[i] || off[i] >= cap) return;
-    p1[off[i]] = q.n1[i];
-    p2[off[i]] = q.n2[i];
+    const float4 a = q.n1[i];
+    const float4 b = reinterpret_cast<const float4 *>(q.merged + (__float_as_uint(a.w) & 0x07ffffffu))[0];
+    p1[off[i]] = a;
+    p2[off[i]] = make_float4(b.x, b.y, b.z, 0.0f);
 }
 
-hipError_t pass_c_rays(const void *queueScratch, uint32_t npx, int k, int which, float4 *p1,
-                       float4 *p2, uint32_t cap, uint32_t *n, hipStream_t s) {
-    const CQueues q = carve_queues(const_cast<void *>(queueScratch), npx, k);
+hipError_t pass_c_rays(const void *queueScratch, uint32_t npx, int k, const restir_reservoir *merged, int which,
+                       float4 *p1, float4 *p2, uint32_t cap, uint32_t *n, hipStream_t s) {
+    CQueues q = carve_queues(const_cast<void *>(queueScratch), npx, k);
+    q.merged = merged;
     uint32_t len = 0;
     hipError_t e = hipMemcpyAsync(&len, q.counter + which, sizeof len, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -1324,12 +1327,14 @@ hipError_t pass_c_rays(const void *queueScratch, uint32_t npx, int k, int which,
 
 hipError_t launch_pass_c(int tile, int stack, const restir_uniforms &u, int k, int iter,
                          const SceneView &sc, const GView &g, const restir_reservoir *in,
-                         restir_reservoir *out, void *queueScratch, uint32_t gridCap,
+                         restir_reservoir *out, void *queueScratch, bool exportRays, uint32_t gridCap,
                          hipStream_t s) {
     // unbiasedReuse.glsl:72 seeds with frame*17; iter > 0 offsets the seed of repeated passes
     PassCArgs args{u, sc, g, in, out, 0, 0, k, u.frame * 17u + static_cast<uint32_t>(iter)};
     if (queueScratch) {
-        const CQueues q = carve_queues(queueScratch, u.screenSize[0] * u.screenSize[1], k);
+        CQueues q = carve_queues(queueScratch, u.screenSize[0] * u.screenSize[1], k);
+        q.merged = out;
+        q.exportCentre = exportRays ? 1u : 0u;
         return by_stack(stack, [&](auto S) { return dispatch_c_queued<decltype(S)::value>(tile, args, q, gridCap, s); });
     }
     return by_stack(stack, [&](auto S) { return dispatch_c<decltype(S)::value>(tile, args, s); });

@@ -22,7 +22,7 @@ STANDARD_WEIGHTS_FLAG = 1 << 2  # RIS with W = sumW / (M p_hat) and triangle lig
 TRACE_FIRST_HIT = 0
 TRACE_REFERENCE_LOOP = 1
 TRACE_ONE_RAY_PER_LANE = 2
-ABI_VERSION = 13  # RESTIR_ABI_VERSION of include/restir_abi.h
+ABI_VERSION = 14  # RESTIR_ABI_VERSION of include/restir_abi.h
 
 ERRORS = {
     -1: "RESTIR_E_INVALID_ARGUMENT",
@@ -213,6 +213,7 @@ SIGNATURES = {
                                    _u64, _vp]),
     "restir_profile_enable": (_i32, [_vp, _i32]),
     "restir_profile_read": (_i32, [_vp, _vp, _u32, C.POINTER(_u32)]),
+    "restir_set_ray_export": (_i32, [_vp, _i32]),
     "restir_unbiased_rays": (_i32, [_vp, _i32, _vp, _vp, _u32, C.POINTER(_u32), _vp]),
     "restir_mlp_forward": (_i32, [_vp, C.POINTER(MlpStruct), _vp, _u32, _vp, _vp]),
     "restir_mlp_loss_grad": (_i32, [_vp, C.POINTER(MlpStruct), _vp, _vp, _vp, _u32, _vp, _vp, _vp,
@@ -591,6 +592,9 @@ class Context:
         self._check(self.lib.restir_profile_read(self.h, rows, cap, C.byref(n)), "restir_profile_read")
         return {rows[i].name.decode(): (rows[i].launches, rows[i].total_ms, rows[i].max_ms)
                 for i in range(min(n.value, cap))}
+
+    def set_ray_export(self, on=True):
+        self._check(self.lib.restir_set_ray_export(self.h, 1 if on else 0), "restir_set_ray_export")
 
     def unbiased_rays(self, which, p1, p2, capacity, stream=None) -> int:
         """Copies the last queued Pass C's centre (0) or neighbour (1) rays; returns the queue length."""

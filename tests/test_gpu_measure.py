@@ -47,8 +47,15 @@ def test_unbiased_rays_are_the_queued_rays(torch):
     u, _ = r.uniforms(sc.camera)
     g = r.gstruct(r.cur)
     out = torch.zeros_like(r.tmp)
+    # without the export switch the pass keeps no centre rays
     r.ctx.unbiased(u, k, 0, g, r.res[r.cur], out)
     npx = W * H
+    with pytest.raises(abi.RestirError):
+        r.ctx.unbiased_rays(0, torch.zeros(4, 4, device="cuda"), torch.zeros(4, 4, device="cuda"), 1)
+    ref_out = out.clone()
+    r.ctx.set_ray_export(True)
+    r.ctx.unbiased(u, k, 0, g, r.res[r.cur], out)
+    assert torch.equal(out, ref_out)  # the switch changes no result
     p1 = torch.zeros(npx * k, 4, dtype=torch.float32, device="cuda")
     p2 = torch.zeros_like(p1)
     nc = r.ctx.unbiased_rays(0, p1, p2, npx * k)

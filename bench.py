@@ -107,6 +107,10 @@ def parse():
                          "prints its own JSON line")
     ap.add_argument("--stress-paths", type=int, default=4,
                     help="stress4k: camera paths of 32 learned frames captured into one graph (0: skip)")
+    ap.add_argument("--weights", choices=("standard", "reference"), default="reference",
+                    help="reference (default): the reference's RIS weights, whose frames converge "
+                         "several percent from the FP64 truth; standard: RESTIR_STANDARD_WEIGHTS_FLAG, "
+                         "unbiased frames in both pipelines")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (x3), whose hardware byte counters "
                          "calibrate the traffic counters of the other kernels")
@@ -129,7 +133,8 @@ class Frames:
         self.res = [torch.zeros(16 * npx, dtype=torch.int32, device=dev) for _ in range(2)]
         self.tmp = torch.zeros(16 * npx, dtype=torch.int32, device=dev)
         self.rgb = torch.zeros(3 * npx, dtype=torch.float32, device=dev)
-        self.flags = abi.VISIBILITY_REUSE_FLAG | abi.TEMPORAL_REUSE_FLAG
+        self.flags = abi.VISIBILITY_REUSE_FLAG | abi.TEMPORAL_REUSE_FLAG | \
+            (abi.STANDARD_WEIGHTS_FLAG if getattr(args, "weights", "reference") == "standard" else 0)
         self.learned = None
         self.launch_a = launch  # Pass A's shape and ray schedule (choose_pass_a_schedule)
         self.normal_format = abi.NORMAL_F32
@@ -386,7 +391,7 @@ def main():
                         "(114-128-128-128-256, bf16 MFMA, Adam 1e-3, 4096 samples/frame)",
             "resolution": f"{W}x{H}", "triangles": scene.n_tris,
             "point_lights": int(len(scene.point_lights)), "triangle_lights": int(len(scene.tri_lights)),
-            "M_plain": args.M, "M_learned": args.learned_M,
+            "M_plain": args.M, "M_learned": args.learned_M, "weights": args.weights,
             "k": args.k, "spatial_passes": args.spatial_passes, "tile": args.tile,
         },
         "plain": {"ms_per_frame": round(plain_ms, 4), "pass_a_schedule": sched_plain,

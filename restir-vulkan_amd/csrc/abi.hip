@@ -119,9 +119,10 @@ uint32_t resident_blocks(uint32_t cus, uint32_t ldsBytes) {
     perCu = perCu > 8u ? 8u : (perCu < 1u ? 1u : perCu);
     return cus * perCu;
 }
-// the persistent tracers' 256-lane groups each hold a `stack`-entry LDS stack per lane
+// the queued tracers' 256-lane groups each hold a `stack`-entry LDS stack per lane and the
+// waves' triangle-test lists
 uint32_t persistent_blocks(uint32_t cus, int stack) {
-    return resident_blocks(cus, static_cast<uint32_t>(stack) * 256u * 4u);
+    return resident_blocks(cus, static_cast<uint32_t>(stack) * 256u * 4u + rk::kTraceExtraLds);
 }
 
 int ensure_scratch(restir_ctx *ctx, size_t bytes) { return ensure(ctx, &ctx->scratch, &ctx->scratchBytes, bytes); }

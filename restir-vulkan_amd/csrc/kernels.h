@@ -71,6 +71,11 @@ int tile_index(uint32_t tx, uint32_t ty);  // -1 if not compiled
 // candLight/candQ: the learned proposal's candidates ([M][npx]), or null for alias-table RIS.
 // queueScratch == nullptr: one dispatch, the visibility ray traced inline per pixel. Otherwise
 // the ray-queue schedule, using pass_a_queue_bytes(npx) bytes of device scratch.
+// LDS of a queued tracer's 256-lane workgroup besides its `stack`-entry stack per lane: each
+// wave's list of waiting triangle tests (fewer than 64 left after a round plus at most 4 per lane
+// from one visit; a 4-byte triangle index and an owner byte each) and a tally word per lane.
+constexpr uint32_t kTraceListEntries = 64 + 4 * 64;
+constexpr uint32_t kTraceExtraLds = 4u * kTraceListEntries * 5u + 256u * 4u;
 hipError_t launch_pass_a(int tile, int stack, const restir_uniforms &u, const SceneView &sc,
                          const GView &cur, const GView &prev, const restir_reservoir *prevRes,
                          restir_reservoir *out, const int32_t *candLight, const float *candQ,

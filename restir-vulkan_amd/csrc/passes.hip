@@ -678,23 +678,41 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_finish(const PassCArgs a, con
 // lanes busy when a wave's rays end at different depths. Chunks shrink as the queue drains
 // (guided self-scheduling: the rays left when the wave last took a chunk, over twice the grid's
 // waves, between kTraceMinChunk and kTraceMaxChunk), so that waves finish close together.
+//
+// Triangle tests are gathered per wave: a visit appends the triangles whose entries pass to the
+// wave's LDS ring (triangle index and owner lane), and the wave tests the oldest 64, one per
+// lane, whenever 64 are waiting, or all of them once fewer than kTriRoundWalkers lanes can still
+// walk. A lane's ray ends when no test of its is waiting and a test hit or its walk is over; the
+// lane takes no new ray before, and the tests of a ray already known to hit are skipped.
+//
 // Results go to the pixel named by the ray's tag: a per-pixel ray (PER_PIXEL: Pass A's
 // visibility ray) sets kBitCentreShadowed in bits[pixel]; a Pass C neighbour ray sets bit `slot`
 // of occ[pixel], and is not traced when its pixel's centre ray was shadowed.
 constexpr uint32_t kTraceMaxChunk = 128, kTraceMinChunk = 32, kTraceRefill = 32;
+constexpr uint32_t kTriList = kTraceListEntries, kTriRoundWalkers = 32;
 template <int STACK, bool PER_PIXEL>
 __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const float4 *__restrict__ p1,
                                                      const float4 *__restrict__ p2,
                                                      const uint32_t *__restrict__ count,
                                                      uint32_t *__restrict__ head, CQueues q) {
     __shared__ int32_t stack[STACK * 256];
+    __shared__ uint32_t triList[4 * kTriList];
+    __shared__ uint8_t triOwner[4 * kTriList];
+    __shared__ uint32_t tally[256];  // per lane: 256 x (its tests done in a round) + its hits
+    static_assert(sizeof(triList) + sizeof(triOwner) + sizeof(tally) == kTraceExtraLds, "LDS the grid is sized for");
     int32_t *stk = stack + threadIdx.x;
     const uint32_t n = *count;
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+    uint32_t *tri = triList + (threadIdx.x >> 6) * kTriList;
+    uint8_t *own = triOwner + (threadIdx.x >> 6) * kTriList;
+    uint32_t *tal = tally + (threadIdx.x & ~63u);
+    tally[threadIdx.x] = 0u;  // each wave uses its own 64 words only
     WWalk w;
     int32_t ray = -1;
     uint32_t tag = 0;
+    uint32_t waiting = 0;        // this lane's tests in the wave's list
+    uint32_t first = 0, listed = 0;  // wave-uniform: the ring's oldest entry and its length
     uint32_t next = 0, end = 0;  // wave-uniform: the rays of the wave's chunk not handed out yet
     uint32_t seen = 0;           // wave-uniform: the head just after this wave's last chunk
     bool drained = false;        // wave-uniform: the head has passed the last ray
@@ -739,9 +757,51 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
             if (drained && next >= end) break;
             continue;
         }
-        bool fin = false;
-        if (ray >= 0) fin = walk_step_w<STACK>(w, sc.wide, sc.nodes, sc.tris, stk, 256);
-        if (fin) {
+        // one visit per walking lane; its triangles join the wave's list
+        const bool walking = ray >= 0 && !w.hit && w.cur >= 0;
+        uint32_t leaves = 0;
+        int4 ids{0, 0, 0, 0};
+        if (walking) leaves = walk_step_defer<STACK>(w, sc.wide, sc.nodes, stk, 256, ids);
+        const uint32_t nLeaves = static_cast<uint32_t>(__popc(leaves));
+        const uint64_t c1 = __ballot(nLeaves & 1u), c2 = __ballot(nLeaves & 2u), c4 = __ballot(nLeaves & 4u);
+        if (c1 | c2 | c4) {
+            // listed < 64 before the visit, so the ring (kTriList entries) cannot overflow
+            uint32_t at = first + listed + static_cast<uint32_t>(__popcll(c1 & below) + 2 * __popcll(c2 & below) +
+                                                                 4 * __popcll(c4 & below));
+            for (uint32_t m = leaves; m; m &= m - 1u) {
+                const uint32_t slot = at < kTriList ? at : at - kTriList;
+                tri[slot] = static_cast<uint32_t>(~wide_entry(ids, static_cast<uint32_t>(__builtin_ctz(m))));
+                own[slot] = static_cast<uint8_t>(lane);
+                ++at;
+            }
+            listed += static_cast<uint32_t>(__popcll(c1) + 2 * __popcll(c2) + 4 * __popcll(c4));
+            waiting += nLeaves;
+        }
+        // triangle rounds: the oldest 64 tests, one per lane
+        const uint32_t walkers = static_cast<uint32_t>(__popcll(__ballot(ray >= 0 && !w.hit && w.cur >= 0)));
+        while (listed >= 64u || (listed > 0u && walkers < kTriRoundWalkers)) {
+            const uint64_t known = __ballot(ray >= 0 && w.hit);
+            const uint32_t take = listed < 64u ? listed : 64u;
+            const bool mine = lane < take;
+            uint32_t t = 0, o = 0;
+            if (mine) {
+                const uint32_t at = first + lane, slot = at < kTriList ? at : at - kTriList;
+                t = tri[slot];
+                o = own[slot];
+            }
+            const int src = static_cast<int>(o);
+            const f3 ro{__shfl(w.s.s.o.x, src), __shfl(w.s.s.o.y, src), __shfl(w.s.s.o.z, src)};
+            const f3 rd{__shfl(w.s.s.d.x, src), __shfl(w.s.s.d.y, src), __shfl(w.s.s.d.z, src)};
+            bool h = false;
+            if (mine && !((known >> o) & 1ull)) h = ray_tri(sc.tris, static_cast<int32_t>(t), ro, rd);
+            if (mine) atomicAdd(&tal[o], h ? 257u : 256u);
+            const uint32_t r = atomicExch(&tal[lane], 0u);  // after this wave's adds (LDS keeps a wave's order)
+            waiting -= r >> 8;
+            if (r & 0xffu) w.hit = true;
+            first = first + take < kTriList ? first + take : first + take - kTriList;
+            listed -= take;
+        }
+        if (ray >= 0 && waiting == 0 && (w.hit || w.cur < 0)) {
             if (w.hit) {
                 if (PER_PIXEL) q.bits[tag] |= kBitCentreShadowed;
                 else atomicOr(&q.occ[tag & 0x07ffffffu], 1u << (tag >> 27));

@@ -345,20 +345,16 @@ __device__ __forceinline__ float4 corner(float4 x, float4 y, float4 z, int k) {
                   : make_float4(x.w, y.w, z.w, 0.0f);
 }
 
-// One visit of node `cur`. Returns true when a triangle is hit; otherwise *next gets the node to
-// visit next (or -1 when the stack is empty), after pushing the left side's item if both sides
-// yield one. kQuick: every box test takes the quick form (all lanes' rays are quick).
-template <int kStack, bool kQuick>
-__device__ __forceinline__ bool wide_visit_k(const WSeg &ws, const WNode *__restrict__ wide,
-                                           const Node *__restrict__ nodes, const float4 *__restrict__ tris,
-                                           int32_t cur, int32_t *next, int32_t *stk, int stride, int &top,
-                                           uint32_t &nv, uint32_t &nt) {
-    const Seg &s = ws.s;
+// The entries of WNode `cur` that the segment passes: bit k for entry k when it exists, its box
+// passes and (under ws.full) so does the box of its side. `id` / `grp` get the node's entry and
+// group words. kQuick: every box test takes the quick form (all lanes' rays are quick).
+template <bool kQuick>
+__device__ __forceinline__ uint32_t wide_pass(const WSeg &ws, const WNode *__restrict__ wide,
+                                              const Node *__restrict__ nodes, int32_t cur, int4 &id, int4 &grp) {
     const float4 *p = reinterpret_cast<const float4 *>(wide + cur);
     const float4 lx = p[0], ly = p[1], lz = p[2], hx = p[3], hy = p[4], hz = p[5];
-    const int4 id = reinterpret_cast<const int4 *>(p)[6];
-    const int4 grp = reinterpret_cast<const int4 *>(p)[7];
-    ++nv;
+    id = reinterpret_cast<const int4 *>(p)[6];
+    grp = reinterpret_cast<const int4 *>(p)[7];
     bool sideL = true, sideR = true;
     if (ws.full) {
         const Node nd = load_node(nodes, cur);
@@ -370,20 +366,26 @@ __device__ __forceinline__ bool wide_visit_k(const WSeg &ws, const WNode *__rest
     const bool b1 = ray_box_w<kQuick>(ws, corner(lx, ly, lz, 1), corner(hx, hy, hz, 1));
     const bool b2 = ray_box_w<kQuick>(ws, corner(lx, ly, lz, 2), corner(hx, hy, hz, 2));
     const bool b3 = ray_box_w<kQuick>(ws, corner(lx, ly, lz, 3), corner(hx, hy, hz, 3));
-    const bool h0 = sideL & (id.x != kNoEntry) & b0;
-    const bool h1 = sideL & (id.y != kNoEntry) & b1;
-    const bool h2 = sideR & (id.z != kNoEntry) & b2;
-    const bool h3 = sideR & (id.w != kNoEntry) & b3;
-    uint32_t leaves = (h0 && id.x < 0 ? 1u : 0u) | (h1 && id.y < 0 ? 2u : 0u) |
-                      (h2 && id.z < 0 ? 4u : 0u) | (h3 && id.w < 0 ? 8u : 0u);
-    while (leaves) {
-        const uint32_t k = static_cast<uint32_t>(__builtin_ctz(leaves));
-        leaves &= leaves - 1u;
-        const int32_t e = k == 0 ? id.x : (k == 1 ? id.y : (k == 2 ? id.z : id.w));
-        ++nt;
-        if (ray_tri(tris, ~e, s.o, s.d)) return true;
-    }
-    const bool i0 = h0 && id.x >= 0, i1 = h1 && id.y >= 0, i2 = h2 && id.z >= 0, i3 = h3 && id.w >= 0;
+    return ((sideL & (id.x != kNoEntry) & b0) ? 1u : 0u) | ((sideL & (id.y != kNoEntry) & b1) ? 2u : 0u) |
+           ((sideR & (id.z != kNoEntry) & b2) ? 4u : 0u) | ((sideR & (id.w != kNoEntry) & b3) ? 8u : 0u);
+}
+
+// The triangle entries among the passed entries `h` (an existing entry < 0 is ~triangle).
+__device__ __forceinline__ uint32_t wide_leaves(uint32_t h, int4 id) {
+    return h & ((id.x < 0 ? 1u : 0u) | (id.y < 0 ? 2u : 0u) | (id.z < 0 ? 4u : 0u) | (id.w < 0 ? 8u : 0u));
+}
+
+__device__ __forceinline__ int32_t wide_entry(int4 id, uint32_t k) {
+    return k == 0 ? id.x : (k == 1 ? id.y : (k == 2 ? id.z : id.w));
+}
+
+// The node visited after the passed entries `h`: per side, its internal child itself when both of
+// that child's entries passed, else the one entry that passed; the left side's item is pushed
+// when both sides yield one; with neither, the stack's top (-1 when the stack is empty).
+template <int kStack>
+__device__ __forceinline__ int32_t wide_next(uint32_t h, int4 id, int4 grp, int32_t *stk, int stride, int &top) {
+    const bool i0 = (h & 1u) && id.x >= 0, i1 = (h & 2u) && id.y >= 0;
+    const bool i2 = (h & 4u) && id.z >= 0, i3 = (h & 8u) && id.w >= 0;
     const int32_t itemL = (i0 && i1) ? grp.x : (i0 ? id.x : (i1 ? id.y : -1));
     const int32_t itemR = (i2 && i3) ? grp.y : (i2 ? id.z : (i3 ? id.w : -1));
     int32_t nx = itemR;
@@ -399,7 +401,25 @@ __device__ __forceinline__ bool wide_visit_k(const WSeg &ws, const WNode *__rest
         --top;
         nx = stk[top * stride];
     }
-    *next = nx;
+    return nx;
+}
+
+// One visit of node `cur`. Returns true when a triangle is hit; otherwise *next gets the node to
+// visit next (or -1 when the stack is empty), after pushing the left side's item if both sides
+// yield one.
+template <int kStack, bool kQuick>
+__device__ __forceinline__ bool wide_visit_k(const WSeg &ws, const WNode *__restrict__ wide,
+                                           const Node *__restrict__ nodes, const float4 *__restrict__ tris,
+                                           int32_t cur, int32_t *next, int32_t *stk, int stride, int &top,
+                                           uint32_t &nv, uint32_t &nt) {
+    int4 id, grp;
+    const uint32_t h = wide_pass<kQuick>(ws, wide, nodes, cur, id, grp);
+    ++nv;
+    for (uint32_t leaves = wide_leaves(h, id); leaves; leaves &= leaves - 1u) {
+        ++nt;
+        if (ray_tri(tris, ~wide_entry(id, static_cast<uint32_t>(__builtin_ctz(leaves))), ws.s.o, ws.s.d)) return true;
+    }
+    *next = wide_next<kStack>(h, id, grp, stk, stride, top);
     return false;
 }
 
@@ -462,6 +482,24 @@ __device__ __forceinline__ bool walk_step_w(WWalk &w, const WNode *__restrict__ 
     }
     w.cur = next;
     return next < 0;
+}
+
+// A visit that leaves its triangle tests to the caller, for tracers that gather the tests of a
+// wave's rays and run them with the wave's lanes full: the visit of w.cur as walk_step_w makes
+// it (same boxes, same pushes, same next node), except that the triangles whose entries pass are
+// returned (bit k: entry k, triangle ~ids[k]) instead of tested. A segment is occluded exactly
+// when one of the triangles its visits return is hit, so the tests may come later and in any
+// order; the walk goes on past a leaf that would have ended it until the caller knows.
+template <int kStack>
+__device__ __forceinline__ uint32_t walk_step_defer(WWalk &w, const WNode *__restrict__ wide,
+                                                   const Node *__restrict__ nodes, int32_t *stk, int stride,
+                                                   int4 &ids) {
+    int4 grp;
+    const uint32_t h = __ballot(!w.s.quick) == 0ull ? wide_pass<true>(w.s, wide, nodes, w.cur, ids, grp)
+                                                    : wide_pass<false>(w.s, wide, nodes, w.cur, ids, grp);
+    ++w.nv;
+    w.cur = wide_next<kStack>(h, ids, grp, stk, stride, w.top);
+    return wide_leaves(h, ids);
 }
 
 // testVisibility, visibilityTest.glsl:1-30: shrink the segment p1 -> p2 by tMin = 0.001 at both

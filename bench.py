@@ -279,7 +279,6 @@ def main():
     visits = torch.zeros_like(sh)
     tests = torch.zeros_like(sh)
     ray_sets = {"centre": [], "neighbour": []}
-    ctx.set_ray_export(True)  # the replay keeps its centre rays for the export
     for _j in fr.frame(f_last, args.M):
         for which, key in ((0, "centre"), (1, "neighbour")):
             n = ctx.unbiased_rays(which, q1, q2, cap, stream=stream)
@@ -289,7 +288,6 @@ def main():
                                   "T": float(tests[:n].double().mean()) if n else 0.0,
                                   "occluded": float(sh[:n].double().mean()) if n else 0.0})
     torch.cuda.synchronize()
-    ctx.set_ray_export(False)
     # standalone traversal throughput on the last Pass C's neighbour rays (left in q1/q2)
     nr = ray_sets["neighbour"][-1]["rays"]
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -309,17 +307,24 @@ def main():
     # measure per launch are a few percent of them, so the memory roof, priced at the measured
     # bytes per ray when the profile has them, is far above the ALU one.
     valu_rays = VALU_LANE_OPS / (VISIT_OPS * Vn + TEST_OPS * Tn)
-    meas = measured_traffic("k_trace_queue(neighbour)") if W * H == 1920 * 1080 else None
+    meas = measured_traffic("k_trace_queue(pass_c)") if W * H == 1920 * 1080 else None
     hbm_rays = HBM_PEAK_GBS * 1e9 / (meas / nr) if meas else None
     roof_rays = min(valu_rays, hbm_rays) if hbm_rays else valu_rays
     log(f"traversal: {nr} neighbour rays, V={Vn:.1f} T={Tn:.2f}: {trace_ms:.3f} ms, "
         f"{rays_per_s / 1e6:.0f} Mrays/s, roofline {roof_rays / 1e6:.0f} Mrays/s")
 
+    # a Pass C's tracer launch traces its centre rays, then the neighbour rays of pixels whose
+    # centre ray found the light: per pass, both sets with their V and T weighted by ray count
+    ray_sets["pass_c"] = []
+    for c, nb in zip(ray_sets["centre"], ray_sets["neighbour"]):
+        n = c["rays"] + nb["rays"]
+        ray_sets["pass_c"].append({"rays": n, "V": (c["rays"] * c["V"] + nb["rays"] * nb["V"]) / max(n, 1),
+                                   "T": (c["rays"] * c["T"] + nb["rays"] * nb["T"]) / max(n, 1)})
+
     # ---- roofline of the dominant kernel: algorithmic bytes per launch / average launch time
     def ray_bytes(key):
         return float(np.mean([r["rays"] * (NODE_BYTES * r["V"] + TRI_BYTES * r["T"]) for r in ray_sets[key]]))
-    algo_bytes = {"k_trace_queue(neighbour)": ray_bytes("neighbour"),
-                  "k_trace_queue(centre)": ray_bytes("centre"),
+    algo_bytes = {"k_trace_queue(pass_c)": ray_bytes("pass_c"),
                   "k_shade": BYTES_PER_PX_SHADE * npx}
     def pass_rate(names, bytes_px, passes_per_frame):
         ms = sum(kernels[k]["ms_per_frame"] for k in names if k in kernels) / passes_per_frame

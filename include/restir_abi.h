@@ -184,9 +184,9 @@ typedef struct restir_ctx restir_ctx;
 int restir_create(restir_ctx **out, int device);
 void restir_destroy(restir_ctx *ctx);
 const char *restir_last_error(const restir_ctx *ctx);
-/* ABI revision; bumped whenever an entry point's signature changes. Bindings compare it with the
+/* ABI revision; bumped whenever an entry point's signature or documented behaviour changes. Bindings compare it with the
  * revision they were written against and refuse a mismatched library. */
-#define RESTIR_ABI_VERSION 14
+#define RESTIR_ABI_VERSION 15
 int restir_version(void);
 
 /* Light tables: device pointers to the reference buffer images (header + array). The counts are
@@ -564,17 +564,17 @@ int restir_profile_read(restir_ctx *ctx, restir_kernel_time *out, uint32_t capac
                         uint32_t *count);
 
 /* The shadow rays of the last queued-schedule restir_unbiased call on this context, unless a
- * queued Pass A ran since (the queues share scratch) (which = 0: the centre re-tests, kept only
- * while restir_set_ray_export is on; 1: the neighbour tests that were traced, i.e. those of pixels
- * whose centre ray found the light), as p1/p2 float4 pairs in queue order (p1.w holds a pixel tag,
- * ignored by restir_trace_shadow). A neighbour ray's light end is read from that call's output
- * reservoirs: export before they change. Copies min(number of rays, capacity) rays on `stream`
+ * queued Pass A ran since (the queues share scratch) (which = 0: the centre re-tests; 1: the
+ * neighbour tests of pixels whose centre ray found the light), as p1/p2 float4 pairs in queue
+ * order (p1.w holds a pixel tag, ignored by restir_trace_shadow). The light end of a ray is read
+ * from that call's output reservoirs: export before they change. Copies min(number of rays, capacity) rays on `stream`
  * and stores the number of rays in *count (a host pointer; the call synchronizes `stream`). Lets a
  * benchmark count node visits and triangle tests of exactly the rays the traversal kernel traced. */
 int restir_unbiased_rays(restir_ctx *ctx, int which, float *p1, float *p2, uint32_t capacity,
                          uint32_t *count, void *stream);
-/* Off by default: the queued Pass C traces its centre rays itself and stores no copy of them.
- * On: it also keeps them for restir_unbiased_rays (32 bytes per pixel more writes). */
+/* Kept for callers of ABI 14, where it switched on the copies of the centre rays that
+ * restir_unbiased_rays(which = 0) returns. The queued Pass C now keeps them in any case (its tracer
+ * reads them from the queue), so the switch has no effect. */
 int restir_set_ray_export(restir_ctx *ctx, int on);
 
 #ifdef __cplusplus

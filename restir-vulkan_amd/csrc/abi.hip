@@ -419,13 +419,12 @@ int restir_unbiased(restir_ctx *ctx, const restir_uniforms *u, int k, int iter,
         queues = ctx->scratch;
     }
     const hipError_t e = rk::launch_pass_c(tile, ctx->stack, *u, k, iter, ctx->scene, view_of(g), in, out,
-                                           queues, ctx->rayExport, ctx->persistentBlocks,
+                                           queues, ctx->persistentBlocks,
                                            static_cast<hipStream_t>(stream));
     // the queues hold this call's rays only when it traced any (otherwise they are stale)
     ctx->queuedNpx = (queues && vis && e == hipSuccess) ? W * H : 0;
     ctx->queuedK = k;
     ctx->queuedOut = out;
-    ctx->queuedExported = ctx->rayExport;
     return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "unbiased launch");
 }
 

@@ -39,8 +39,6 @@ struct restir_ctx {
     size_t scratchBytes = 0;
     uint32_t queuedNpx = 0;        // shape of the last queued Pass C (its rays stay in scratch)
     const restir_reservoir *queuedOut = nullptr;  // its output (the light end of its neighbour rays)
-    bool rayExport = false;        // restir_set_ray_export: queued Pass C keeps its centre rays
-    bool queuedExported = false;   // the last queued Pass C kept them
     int queuedK = 0;
     bool profiling = false;
     rk::Recorder recorder;

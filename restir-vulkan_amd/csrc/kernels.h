@@ -88,12 +88,11 @@ hipError_t launch_pass_b(int tile, const restir_uniforms &u, int iter, float cos
 // schedule, using pass_c_queue_bytes(npx, k) bytes of device scratch.
 hipError_t launch_pass_c(int tile, int stack, const restir_uniforms &u, int k, int iter,
                          const SceneView &sc, const GView &g, const restir_reservoir *in,
-                         restir_reservoir *out, void *queueScratch, bool exportRays, uint32_t gridCap,
-                         hipStream_t s);
+                         restir_reservoir *out, void *queueScratch, uint32_t gridCap, hipStream_t s);
 size_t pass_c_queue_bytes(uint32_t npx, int k);
 // Copies the shadow rays that the last queued Pass C over (npx, k) put in `queueScratch`
-// (which = 0: centre rays, written only with exportRays; 1: neighbour rays, whose light end is read
-// from that pass's output `merged`); *n receives the queue length.
+// (which = 0: centre rays; 1: the neighbour rays of pixels whose centre ray found the light; the
+// light end of both is read from that pass's output `merged`); *n receives the number of rays.
 hipError_t pass_c_rays(const void *queueScratch, uint32_t npx, int k, const restir_reservoir *merged, int which,
                        float4 *p1, float4 *p2, uint32_t cap, uint32_t *n, hipStream_t s);
 hipError_t launch_shade(const SceneView &sc, rt::f3 cam, float gamma, const GView &g,

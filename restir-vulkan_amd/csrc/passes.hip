@@ -488,15 +488,16 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c(const PassCArgs a) {
 
 // ------------------------------------------------------------------- Pass C, ray queues ---
 // The same pass as k_pass_c, with its shadow rays moved out of the per-pixel lanes into two
-// queues traced by the persistent tracer: (1) merge every pixel and queue its centre ray;
-// (2) trace; (3) queue the neighbour rays of pixels whose centre sees the light (the others have
-// Z = 0 whatever their neighbours give); (4) trace; (5) form Z and W. Rays are appended with one
-// atomic per wave, slots handed out by prefix sums over the wave, so a queue holds only live rays
-// in pixel-tile order.
+// queues traced by one launch of the persistent tracer: (1) merge every pixel, queue its centre
+// ray and the rays to its above-horizon neighbours; (2) trace the centre rays, then the
+// neighbour rays, skipping those of pixels whose centre ray was found shadowed (they have Z = 0
+// whatever their neighbours give); (3) form Z and W. Rays are appended with one atomic per wave,
+// slots handed out by prefix sums over the wave, so a queue holds only live rays in pixel-tile
+// order.
 struct CQueues {
-    uint32_t *counter;     // [0] centre rays recorded, [1] neighbour rays queued, [3] their tracer's head
-    float4 *c1, *c2;       // centre ray end points, written only for the ray export (the merge
-                           // traces them itself); c1.w carries the pixel index
+    uint32_t *counter;     // [0] centre rays queued, [1] neighbour rays queued, [3] their tracer's head
+    float4 *c1;            // centre rays: the pixel's position, .w = pixel | kCentreTag; the other
+                           // end is the light sample of `merged[pixel]`
     float4 *n1;            // neighbour rays: the neighbour's position, .w = pixel | slot << 27; the
                            // other end is the light sample of `merged[pixel]`
     const restir_reservoir *merged;  // the pass's output (merged reservoirs)
@@ -506,11 +507,11 @@ struct CQueues {
     uint32_t *nM, *nIdx;   // [slot * npx + pixel]: the neighbour's sample count and pixel
     float4 *surf;          // per pixel: {position.xyz, normal.x}, {normal.yz, 0, 0}
     uint32_t npx;
-    uint32_t exportCentre; // write c1/c2
 };
 constexpr uint32_t kBitBackground = 1u << 29;  // pass-through pixel, already final
 constexpr uint32_t kBitDone = 1u << 30;        // no visibility test: W already written
 constexpr uint32_t kBitCentreShadowed = 1u << 31;
+constexpr uint32_t kCentreTag = 1u << 31;  // a queued ray's tag: the pixel's centre ray
 
 // Appends `count` items for this lane to a queue: returns the lane's first slot. Every lane of
 // the wave must call it (count may be 0).
@@ -540,16 +541,15 @@ __global__ __launch_bounds__(256) void k_pack_surface(const GView g, uint32_t np
     surf[2 * static_cast<size_t>(i) + 1] = make_float4(n.y, n.z, 0.0f, 0.0f);
 }
 
-template <int TX, int TY, int STACK>
+template <int TX, int TY>
 __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, const CQueues q) {
-    __shared__ int32_t stack[STACK * TX * TY];
     uint32_t x, y;
     tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
     const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
     const bool inside = x < W && y < H;
     const bool vis = (a.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) != 0;
     const size_t idx = inside ? static_cast<size_t>(y) * W + x : 0;
-    bool emit = false, shadowed = false;
+    bool emit = false;
     uint32_t above = 0;  // above-horizon neighbour slots
     f3 pos{0.0f, 0.0f, 0.0f}, lp{0.0f, 0.0f, 0.0f};
     if (inside) {
@@ -582,31 +582,20 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
             store_res(a.out, idx, res);
             q.origM[idx] = origM;
             q.occ[idx] = 0u;
-            // the centre re-test (unbiasedReuse.glsl:157-166), traced here, one ray per lane in
-            // tile order
-            if (vis) {
-                f3 o, d;
-                shadow_segment(pos, lp, o, d);
-                TraceStats st{0, 0};
-                shadowed = occluded_w<STACK>(a.sc.wide, a.sc.nodes, a.sc.tris, a.sc.treeNested != 0, a.sc.maxCoord,
-                                             o, d, stack + threadIdx.x, TX * TY, st);
-            }
-            q.bits[idx] = above | (vis ? 0u : kBitDone) | (shadowed ? kBitCentreShadowed : 0u);
+            q.bits[idx] = above | (vis ? 0u : kBitDone);
             emit = vis;
         }
     }
+    // the centre re-test (unbiasedReuse.glsl:157-166)
     const uint32_t slot = wave_append(&q.counter[0], emit ? 1u : 0u);
-    if (emit && q.exportCentre) {
-        q.c1[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(static_cast<uint32_t>(idx)));
-        q.c2[slot] = make_float4(lp.x, lp.y, lp.z, 0.0f);
-    }
-    // One neighbour ray per above-horizon neighbour of a pixel whose centre ray found the light:
-    // a shadowed centre zeroes Z whatever the neighbours give (unbiasedReuse.glsl:157-166) and the
-    // neighbour loop draws no random numbers, so those rays are not needed. The wave's rays are written slot-major: lane i
-    // writes rays i, i + 64, ... of the wave's run, finding the pixel that owns a ray by binary
-    // search over the wave's prefix sums, so each store instruction covers consecutive queue
-    // entries.
-    const uint32_t want = emit && !shadowed ? above : 0u;
+    if (emit) q.c1[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(static_cast<uint32_t>(idx) | kCentreTag));
+    // One neighbour ray per above-horizon neighbour. The tracer skips those of a pixel whose
+    // centre ray it found shadowed: a shadowed centre zeroes Z whatever the neighbours give
+    // (unbiasedReuse.glsl:157-166) and the neighbour loop draws no random numbers. The wave's rays
+    // are written slot-major: lane i writes rays i, i + 64, ... of the wave's run, finding the
+    // pixel that owns a ray by binary search over the wave's prefix sums, so each store
+    // instruction covers consecutive queue entries.
+    const uint32_t want = emit ? above : 0u;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t cnt = static_cast<uint32_t>(__popc(want));
     uint32_t incl = cnt;
@@ -685,9 +674,13 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_finish(const PassCArgs a, con
 // walk. A lane's ray ends when no test of its is waiting and a test hit or its walk is over; the
 // lane takes no new ray before, and the tests of a ray already known to hit are skipped.
 //
-// Results go to the pixel named by the ray's tag: a per-pixel ray (PER_PIXEL: Pass A's
-// visibility ray) sets kBitCentreShadowed in bits[pixel]; a Pass C neighbour ray sets bit `slot`
-// of occ[pixel], and is not traced when its pixel's centre ray was shadowed.
+// PER_PIXEL: the rays are Pass A's visibility rays (p1, p2: end points; *count of them), and one
+// that is shadowed sets kBitCentreShadowed in bits[pixel]. Otherwise the rays are a Pass C's:
+// its count[0] centre rays (q.c1), then its count[1] neighbour rays (q.n1), both ending at the
+// light sample of merged[pixel]. A shadowed centre ray sets kBitCentreShadowed in bits[pixel];
+// a shadowed neighbour ray sets bit `slot` of occ[pixel], and is not traced when its pixel's
+// centre ray is already known to be shadowed (the centre rays come first in the index space, so
+// that holds for nearly all of them; tracing the others too changes nothing).
 constexpr uint32_t kTraceMaxChunk = 128, kTraceMinChunk = 32, kTraceRefill = 32;
 constexpr uint32_t kTriList = kTraceListEntries, kTriRoundWalkers = 32;
 template <int STACK, bool PER_PIXEL>
@@ -701,7 +694,8 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
     __shared__ uint32_t tally[256];  // per lane: 256 x (its tests done in a round) + its hits
     static_assert(sizeof(triList) + sizeof(triOwner) + sizeof(tally) == kTraceExtraLds, "LDS the grid is sized for");
     int32_t *stk = stack + threadIdx.x;
-    const uint32_t n = *count;
+    const uint32_t nCentre = PER_PIXEL ? 0u : count[0];
+    const uint32_t n = PER_PIXEL ? count[0] : nCentre + count[1];
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
     uint32_t *tri = triList + (threadIdx.x >> 6) * kTriList;
@@ -737,10 +731,11 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
                 if (ray < 0) {
                     const uint32_t i = next + static_cast<uint32_t>(__popcll(idle & below));
                     if (i < end) {
-                        const float4 a4 = p1[i];
+                        const bool centre = !PER_PIXEL && i < nCentre;
+                        const float4 a4 = PER_PIXEL ? p1[i] : (centre ? q.c1[i] : q.n1[i - nCentre]);
                         tag = __float_as_uint(a4.w);
                         // a neighbour ray of a pixel whose centre ray was shadowed is not traced
-                        if (PER_PIXEL || !(q.bits[tag & 0x07ffffffu] & kBitCentreShadowed)) {
+                        if (PER_PIXEL || centre || !(q.bits[tag & 0x07ffffffu] & kBitCentreShadowed)) {
                             f3 o, d;
                             const float4 b4 = PER_PIXEL ? p2[i]
                                                         : reinterpret_cast<const float4 *>(q.merged + (tag & 0x07ffffffu))[0];
@@ -803,7 +798,7 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
         }
         if (ray >= 0 && waiting == 0 && (w.hit || w.cur < 0)) {
             if (w.hit) {
-                if (PER_PIXEL) q.bits[tag] |= kBitCentreShadowed;
+                if (PER_PIXEL || (tag & kCentreTag)) q.bits[tag & 0x07ffffffu] |= kBitCentreShadowed;
                 else atomicOr(&q.occ[tag & 0x07ffffffu], 1u << (tag >> 27));
             }
             ray = -1;
@@ -1276,15 +1271,16 @@ static hipError_t dispatch_c(int tile, const PassCArgs &args, hipStream_t s) {
 }
 template <int TX, int TY, int STACK>
 static hipError_t run_c_queued(PassCArgs args, const CQueues &q, uint32_t gridCap, hipStream_t s) {
+    static_assert((kCentreTag & 0x07ffffffu) == 0 && (kCentreTag >> 27) > 15u, "the centre tag is no pixel or slot");
     grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
     const dim3 tiles(args.nTiles), block(TX * TY);
     RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 4u);
     const uint32_t npx = args.u.screenSize[0] * args.u.screenSize[1];
     RK_LAUNCH("k_pack_surface", k_pack_surface, dim3((npx + 255) / 256), dim3(256), 0, s, args.g, npx, q.surf);
-    RK_LAUNCH("k_pass_c_merge", (k_pass_c_merge<TX, TY, STACK>), tiles, block, 0, s, args, q);
+    RK_LAUNCH("k_pass_c_merge", (k_pass_c_merge<TX, TY>), tiles, block, 0, s, args, q);
     if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
-        RK_LAUNCH("k_trace_queue(neighbour)", (k_trace_queue<STACK, false>), dim3(gridCap), dim3(256), 0, s,
-                  args.sc, q.n1, nullptr, q.counter + 1, q.counter + 3, q);
+        RK_LAUNCH("k_trace_queue(pass_c)", (k_trace_queue<STACK, false>), dim3(gridCap), dim3(256), 0, s,
+                  args.sc, nullptr, nullptr, q.counter + 0, q.counter + 3, q);
         RK_LAUNCH("k_pass_c_finish", (k_pass_c_finish<TX, TY>), tiles, block, 0, s, args, q);
     }
     return hipGetLastError();
@@ -1308,7 +1304,6 @@ static CQueues carve_queues(void *scratch, uint32_t npx, int k, size_t *bytes = 
     CQueues q{};
     q.counter = c.take<uint32_t>(256);
     q.c1 = c.take<float4>(sizeof(float4) * n);
-    q.c2 = c.take<float4>(sizeof(float4) * n);
     q.n1 = c.take<float4>(sizeof(float4) * n * kk);
     q.origM = c.take<uint32_t>(sizeof(uint32_t) * n);
     q.bits = c.take<uint32_t>(sizeof(uint32_t) * n);
@@ -1331,6 +1326,15 @@ __global__ void k_traced_flag(const CQueues q, uint32_t len, uint32_t *flag) {
     if (i >= len) return;
     const uint32_t tag = __float_as_uint(q.n1[i].w);
     flag[i] = (q.bits[tag & 0x07ffffffu] & kBitCentreShadowed) ? 0u : 1u;
+}
+__global__ void k_centre_copy(const CQueues q, uint32_t len, float4 *p1, float4 *p2) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= len) return;
+    const float4 a = q.c1[i];
+    const uint32_t px = __float_as_uint(a.w) & 0x07ffffffu;
+    const float4 b = reinterpret_cast<const float4 *>(q.merged + px)[0];
+    p1[i] = make_float4(a.x, a.y, a.z, __uint_as_float(px));
+    p2[i] = make_float4(b.x, b.y, b.z, 0.0f);
 }
 __global__ void k_traced_copy(const CQueues q, uint32_t len, const uint32_t *flag, const uint32_t *off, float4 *p1,
                               float4 *p2, uint32_t cap) {
@@ -1357,9 +1361,8 @@ hipError_t pass_c_rays(const void *queueScratch, uint32_t npx, int k, const rest
         *n = len;
         const uint32_t m = len < cap ? len : cap;
         if (m == 0) return hipSuccess;
-        e = hipMemcpyAsync(p1, q.c1, sizeof(float4) * m, hipMemcpyDeviceToDevice, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(p2, q.c2, sizeof(float4) * m, hipMemcpyDeviceToDevice, s);
-        return e;
+        RK_LAUNCH("k_centre_copy", k_centre_copy, dim3((m + 255) / 256), dim3(256), 0, s, q, m, p1, p2);
+        return hipGetLastError();
     }
     // neighbour rays: the ones the tracer traced (centre ray not shadowed), in queue order
     *n = 0;
@@ -1387,14 +1390,12 @@ hipError_t pass_c_rays(const void *queueScratch, uint32_t npx, int k, const rest
 
 hipError_t launch_pass_c(int tile, int stack, const restir_uniforms &u, int k, int iter,
                          const SceneView &sc, const GView &g, const restir_reservoir *in,
-                         restir_reservoir *out, void *queueScratch, bool exportRays, uint32_t gridCap,
-                         hipStream_t s) {
+                         restir_reservoir *out, void *queueScratch, uint32_t gridCap, hipStream_t s) {
     // unbiasedReuse.glsl:72 seeds with frame*17; iter > 0 offsets the seed of repeated passes
     PassCArgs args{u, sc, g, in, out, 0, 0, k, u.frame * 17u + static_cast<uint32_t>(iter)};
     if (queueScratch) {
         CQueues q = carve_queues(queueScratch, u.screenSize[0] * u.screenSize[1], k);
         q.merged = out;
-        q.exportCentre = exportRays ? 1u : 0u;
         return by_stack(stack, [&](auto S) { return dispatch_c_queued<decltype(S)::value>(tile, args, q, gridCap, s); });
     }
     return by_stack(stack, [&](auto S) { return dispatch_c<decltype(S)::value>(tile, args, s); });

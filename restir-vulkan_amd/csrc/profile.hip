@@ -109,9 +109,6 @@ int restir_unbiased_rays(restir_ctx *ctx, int which, float *p1, float *p2, uint3
         return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "unbiased_rays: NULL output");
     if (ctx->queuedNpx == 0)
         return fail(ctx, RESTIR_E_NOT_READY, "unbiased_rays: no queued Pass C has run on this context");
-    if (which == 0 && !ctx->queuedExported)
-        return fail(ctx, RESTIR_E_NOT_READY,
-                    "unbiased_rays: the last queued Pass C kept no centre rays (restir_set_ray_export was off)");
     if (int r = use_device(ctx)) return r;
     const hipError_t e = rk::pass_c_rays(ctx->scratch, ctx->queuedNpx, ctx->queuedK, ctx->queuedOut, which,
                                          reinterpret_cast<float4 *>(p1), reinterpret_cast<float4 *>(p2),
@@ -120,9 +117,8 @@ int restir_unbiased_rays(restir_ctx *ctx, int which, float *p1, float *p2, uint3
 }
 
 int restir_set_ray_export(restir_ctx *ctx, int on) {
-    if (!ctx) return RESTIR_E_INVALID_ARGUMENT;
-    ctx->rayExport = on != 0;
-    return RESTIR_OK;
+    (void)on;
+    return ctx ? RESTIR_OK : RESTIR_E_INVALID_ARGUMENT;
 }
 
 }  // extern "C"

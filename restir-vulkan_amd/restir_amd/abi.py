@@ -22,7 +22,7 @@ STANDARD_WEIGHTS_FLAG = 1 << 2  # RIS with W = sumW / (M p_hat) and triangle lig
 TRACE_FIRST_HIT = 0
 TRACE_REFERENCE_LOOP = 1
 TRACE_ONE_RAY_PER_LANE = 2
-ABI_VERSION = 14  # RESTIR_ABI_VERSION of include/restir_abi.h
+ABI_VERSION = 15  # RESTIR_ABI_VERSION of include/restir_abi.h
 
 ERRORS = {
     -1: "RESTIR_E_INVALID_ARGUMENT",

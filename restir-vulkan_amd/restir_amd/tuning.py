@@ -17,7 +17,7 @@ _PASS_KERNELS = {
     "a": ("k_pass_a", "k_pass_a_ris", "k_trace_queue(pass_a)", "k_pass_a_finish"),
     "b": ("k_pass_b",),
     "c": ("k_pack_surface", "k_pass_c_merge",
-          "k_trace_queue(neighbour)", "k_pass_c_finish", "k_pass_c"),
+          "k_trace_queue(pass_c)", "k_pass_c_finish", "k_pass_c"),
 }
 
 

@@ -291,11 +291,12 @@ def main():
     # standalone traversal throughput on the last Pass C's neighbour rays (left in q1/q2)
     nr = ray_sets["neighbour"][-1]["rays"]
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    # one ray per lane in queue order: the tracer shape of the passes' ray queues
-    ctx.trace_shadow(q1, q2, nr, sh, one_ray_per_lane=True, stream=stream)
+    # restir_trace_shadow's default schedule: the passes' persistent queue tracer (rays handed to
+    # idle lanes in queue order, triangle tests gathered per wave)
+    ctx.trace_shadow(q1, q2, nr, sh, stream=stream)
     e0.record(stream)
     for _ in range(args.trace_reps):
-        ctx.trace_shadow(q1, q2, nr, sh, one_ray_per_lane=True, stream=stream)
+        ctx.trace_shadow(q1, q2, nr, sh, stream=stream)
     e1.record(stream)
     torch.cuda.synchronize()
     trace_ms = e0.elapsed_time(e1) / args.trace_reps
@@ -403,7 +404,7 @@ def main():
                   "kernels": rounded(kernels), "resampling_passes": resampling},
         "mrays_per_s": round(rays_per_s / 1e6, 1),
         "traversal": {"rays": nr, "ray_set": "last frame's second Pass C neighbour rays",
-                      "walk": "4-entry nodes (two tree levels per visit), one ray per lane in queue order",
+                      "walk": "4-entry nodes (two tree levels per visit), persistent queue tracer, triangle tests gathered per wave",
                       "node_bytes": NODE_BYTES, "ops_per_visit": VISIT_OPS, "ops_per_test": TEST_OPS,
                       "node_visits_per_ray": round(Vn, 2), "tri_tests_per_ray": round(Tn, 2),
                       "ms": round(trace_ms, 4), "roofline_mrays_per_s": round(roof_rays / 1e6, 1),

@@ -674,28 +674,31 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_finish(const PassCArgs a, con
 // walk. A lane's ray ends when no test of its is waiting and a test hit or its walk is over; the
 // lane takes no new ray before, and the tests of a ray already known to hit are skipped.
 //
-// PER_PIXEL: the rays are Pass A's visibility rays (p1, p2: end points; *count of them), and one
-// that is shadowed sets kBitCentreShadowed in bits[pixel]. Otherwise the rays are a Pass C's:
-// its count[0] centre rays (q.c1), then its count[1] neighbour rays (q.n1), both ending at the
-// light sample of merged[pixel]. A shadowed centre ray sets kBitCentreShadowed in bits[pixel];
-// a shadowed neighbour ray sets bit `slot` of occ[pixel], and is not traced when its pixel's
-// centre ray is already known to be shadowed (the centre rays come first in the index space, so
-// that holds for nearly all of them; tracing the others too changes nothing).
+// MODE kTracePassA: the rays are Pass A's visibility rays (p1, p2: end points; *count of them),
+// and one that is shadowed sets kBitCentreShadowed in bits[pixel]. kTracePassC: the rays are a
+// Pass C's: its count[0] centre rays (q.c1), then its count[1] neighbour rays (q.n1), both
+// ending at the light sample of merged[pixel]. A shadowed centre ray sets kBitCentreShadowed in
+// bits[pixel]; a shadowed neighbour ray sets bit `slot` of occ[pixel], and is not traced when its
+// pixel's centre ray is already known to be shadowed (the centre rays come first in the index
+// space, so that holds for nearly all of them; tracing the others too changes nothing).
+// kTraceBatch: nBatch rays p1[i] -> p2[i] (restir_trace_shadow); out[i] gets 1 if shadowed, else 0.
 constexpr uint32_t kTraceMaxChunk = 128, kTraceMinChunk = 32, kTraceRefill = 32;
 constexpr uint32_t kTriList = kTraceListEntries, kTriRoundWalkers = 32;
-template <int STACK, bool PER_PIXEL>
+constexpr int kTracePassA = 0, kTracePassC = 1, kTraceBatch = 2;
+template <int STACK, int MODE>
 __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const float4 *__restrict__ p1,
                                                      const float4 *__restrict__ p2,
-                                                     const uint32_t *__restrict__ count,
-                                                     uint32_t *__restrict__ head, CQueues q) {
+                                                     const uint32_t *__restrict__ count, uint32_t nBatch,
+                                                     uint32_t *__restrict__ head, CQueues q, uint32_t *__restrict__ out) {
+    constexpr bool PER_PIXEL = MODE == kTracePassA, BATCH = MODE == kTraceBatch;
     __shared__ int32_t stack[STACK * 256];
     __shared__ uint32_t triList[4 * kTriList];
     __shared__ uint8_t triOwner[4 * kTriList];
     __shared__ uint32_t tally[256];  // per lane: 256 x (its tests done in a round) + its hits
     static_assert(sizeof(triList) + sizeof(triOwner) + sizeof(tally) == kTraceExtraLds, "LDS the grid is sized for");
     int32_t *stk = stack + threadIdx.x;
-    const uint32_t nCentre = PER_PIXEL ? 0u : count[0];
-    const uint32_t n = PER_PIXEL ? count[0] : nCentre + count[1];
+    const uint32_t nCentre = MODE == kTracePassC ? count[0] : 0u;
+    const uint32_t n = BATCH ? nBatch : (PER_PIXEL ? count[0] : nCentre + count[1]);
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
     uint32_t *tri = triList + (threadIdx.x >> 6) * kTriList;
@@ -731,14 +734,15 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
                 if (ray < 0) {
                     const uint32_t i = next + static_cast<uint32_t>(__popcll(idle & below));
                     if (i < end) {
-                        const bool centre = !PER_PIXEL && i < nCentre;
-                        const float4 a4 = PER_PIXEL ? p1[i] : (centre ? q.c1[i] : q.n1[i - nCentre]);
-                        tag = __float_as_uint(a4.w);
+                        const bool centre = MODE == kTracePassC && i < nCentre;
+                        const float4 a4 = MODE != kTracePassC ? p1[i] : (centre ? q.c1[i] : q.n1[i - nCentre]);
+                        tag = BATCH ? i : __float_as_uint(a4.w);
                         // a neighbour ray of a pixel whose centre ray was shadowed is not traced
-                        if (PER_PIXEL || centre || !(q.bits[tag & 0x07ffffffu] & kBitCentreShadowed)) {
+                        if (MODE != kTracePassC || centre || !(q.bits[tag & 0x07ffffffu] & kBitCentreShadowed)) {
                             f3 o, d;
-                            const float4 b4 = PER_PIXEL ? p2[i]
-                                                        : reinterpret_cast<const float4 *>(q.merged + (tag & 0x07ffffffu))[0];
+                            const float4 b4 = MODE != kTracePassC
+                                                  ? p2[i]
+                                                  : reinterpret_cast<const float4 *>(q.merged + (tag & 0x07ffffffu))[0];
                             shadow_segment(xyz(a4), xyz(b4), o, d);
                             wwalk_begin(w, o, d, sc.treeNested != 0, sc.maxCoord);
                             ray = static_cast<int32_t>(i);
@@ -797,7 +801,9 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
             listed -= take;
         }
         if (ray >= 0 && waiting == 0 && (w.hit || w.cur < 0)) {
-            if (w.hit) {
+            if (BATCH) {
+                out[tag] = w.hit ? 1u : 0u;
+            } else if (w.hit) {
                 if (PER_PIXEL || (tag & kCentreTag)) q.bits[tag & 0x07ffffffu] |= kBitCentreShadowed;
                 else atomicOr(&q.occ[tag & 0x07ffffffu], 1u << (tag >> 27));
             }
@@ -1202,8 +1208,8 @@ static hipError_t run_a_queued(PassAArgs args, const AQueues &q, uint32_t gridCa
     if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
         CQueues cq{};
         cq.bits = q.occ;  // a shadowed ray sets kBitCentreShadowed in occ[pixel]
-        RK_LAUNCH("k_trace_queue(pass_a)", (k_trace_queue<STACK, true>), dim3(gridCap), dim3(256), 0, s,
-                  args.sc, q.p1, q.p2, q.counter + 0, q.counter + 1, cq);
+        RK_LAUNCH("k_trace_queue(pass_a)", (k_trace_queue<STACK, kTracePassA>), dim3(gridCap), dim3(256), 0, s,
+                  args.sc, q.p1, q.p2, q.counter + 0, 0u, q.counter + 1, cq, nullptr);
     }
     RK_LAUNCH("k_pass_a_finish", (k_pass_a_finish<TX, TY>), tiles, block, 0, s, args, q);
     return hipGetLastError();
@@ -1279,8 +1285,8 @@ static hipError_t run_c_queued(PassCArgs args, const CQueues &q, uint32_t gridCa
     RK_LAUNCH("k_pack_surface", k_pack_surface, dim3((npx + 255) / 256), dim3(256), 0, s, args.g, npx, q.surf);
     RK_LAUNCH("k_pass_c_merge", (k_pass_c_merge<TX, TY>), tiles, block, 0, s, args, q);
     if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
-        RK_LAUNCH("k_trace_queue(pass_c)", (k_trace_queue<STACK, false>), dim3(gridCap), dim3(256), 0, s,
-                  args.sc, nullptr, nullptr, q.counter + 0, q.counter + 3, q);
+        RK_LAUNCH("k_trace_queue(pass_c)", (k_trace_queue<STACK, kTracePassC>), dim3(gridCap), dim3(256), 0, s,
+                  args.sc, nullptr, nullptr, q.counter + 0, 0u, q.counter + 3, q, nullptr);
         RK_LAUNCH("k_pass_c_finish", (k_pass_c_finish<TX, TY>), tiles, block, 0, s, args, q);
     }
     return hipGetLastError();
@@ -1582,6 +1588,12 @@ static void run_trace(bool persistent, uint32_t gridCap, uint32_t *counter, cons
     }
     RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, counter, 1u);
     const uint32_t grid = blocks < gridCap ? blocks : gridCap;
+    if (!REF && !args.visits && !args.tests) {
+        // no counts asked for: the passes' queue tracer (gathered triangle tests)
+        RK_LAUNCH("k_trace_queue(batch)", (k_trace_queue<STACK, kTraceBatch>), dim3(grid), dim3(256), 0, s, args.sc,
+                  args.p1, args.p2, nullptr, args.n, counter, CQueues{}, args.shadowed);
+        return;
+    }
     RK_LAUNCH("k_trace_pool", (k_trace_pool<STACK, REF>), dim3(grid), dim3(256), 0, s, args, counter);
 }
 

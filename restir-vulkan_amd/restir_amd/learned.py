@@ -31,8 +31,7 @@ class LearnedProposal:
         self.d_count = t(cl.count)
         self.d_lights = t(cl.lights)
         self.d_alias = t(cl.alias.view(np.uint8))
-        ctx.set_light_clusters(k, self.d_light_cluster, self.d_light_pwithin, self.d_begin,
-                               self.d_count, self.d_lights, self.d_alias)
+        self.bind()
         nW, nB = mlp_sizes(k)
         self.w_bf16 = torch.zeros(nW, dtype=torch.int16, device=device)
         self.w = torch.zeros(nW, dtype=torch.float32, device=device)
@@ -54,6 +53,12 @@ class LearnedProposal:
         self.cand = {}
         self.loss = torch.zeros(1, dtype=torch.float32, device=device)
         self.train_seed = seed * 7919
+
+    def bind(self):
+        """(Re)binds the clusters to the context: restir_set_lights unbinds them, so a renderer
+        that sets the scene's lights on a shared context calls this afterwards."""
+        self.ctx.set_light_clusters(self.k, self.d_light_cluster, self.d_light_pwithin, self.d_begin,
+                                    self.d_count, self.d_lights, self.d_alias)
 
     def buffers(self, npx: int, M: int):
         key = (npx, M)

@@ -124,7 +124,10 @@ class Renderer:
         self.launches = {"a": launch, "b": launch, "c": launch}
         self.launches.update(launches or {})
         # learned proposal: a restir_amd.learned.LearnedProposal bound to this context, or None
+        # (setting the scene's lights above unbound its clusters)
         self.learned = learned
+        if learned is not None:
+            learned.bind()
         self.train_samples, self.lr, self.train_enabled = train_samples, lr, train
         n = width * height
         self.gbuf = [torch.zeros(GBUFFER_PLANES * n, dtype=torch.float32, device=self.device)

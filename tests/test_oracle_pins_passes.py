@@ -9,6 +9,10 @@
   (reservoir.glsl:59-61).
 - Pass A over given candidates (the learned proposal's RIS): closed-form sums for repeated and empty
   candidates, and the triangle-light point drawn from the first two numbers of the pixel's stream.
+- Pass C (unbiasedReuse.glsl:50-185): a 1x1 image combines the pixel with itself k times and
+  keeps W; an occluded centre ray zeroes W and sumW; neighbours whose ray to the light is blocked,
+  or whose surface faces away from it, drop out of Z only, so W scales by (1 + k) / (1 + n) for
+  the n neighbours that still count, with everything else unchanged.
 
 Each check uses only pinned building blocks (PCG32 and p_hat are pinned in test_oracle_pins.py).
 """
@@ -459,3 +463,113 @@ def test_standard_weights_pass_c_keeps_sumw_consistent():
             # the reference's sumW is p_hat W Z, below p_hat W M wherever a neighbour was not counted
             assert (c["sumWeights"][lit] <= pwm * (1 + 1e-5)).all()
             assert (c["sumWeights"][lit] < pwm * (1 - 1e-3)).mean() > 0.05
+
+
+# ------------------------------------------------------------------------------------ Pass C
+LIGHT = np.array([0.0, 0.0, 8.0], np.float32)
+
+
+def occluder_scene(blocker):
+    """Two far-away triangles, plus (blocker) a small square at z = 0.5 around x = 0.94, which
+    cuts the segment from (1, 0, 0) to LIGHT but not the one from (0, 0, 0)."""
+    tri = [[0, 0, -50, 1, 0, -50, 0, 1, -50], [5, 0, -50, 6, 0, -50, 5, 1, -50]]
+    if blocker:
+        tri += [[0.8, -0.2, 0.5, 1.1, -0.2, 0.5, 1.1, 0.2, 0.5], [0.8, -0.2, 0.5, 1.1, 0.2, 0.5, 0.8, 0.2, 0.5]]
+    nodes, tris, _ = abi.build_aabb_tree(np.array(tri, np.float32))
+    pts = scenes.make_point_lights(LIGHT[None, :], np.ones((1, 3), np.float32))
+    return oracle.Scene(nodes, tris, pts, None, abi.build_alias_table(pts, None))
+
+
+def light_reservoirs(pl, W, H, cam, M):
+    """Every pixel holds light 0 at LIGHT, consistent: p_hat at its own pixel, W = 1, sumW = p_hat M."""
+    res = np.zeros(W * H, dtype=abi.RESERVOIR_DTYPE)
+    for p in range(W * H):
+        ph = phat_at(pl, W, H, p, LIGHT, cam)
+        res[p]["pos_lum"][:3] = LIGHT
+        res[p]["pos_lum"][3] = 5.0
+        res[p]["lightIndex"] = 0
+        res[p]["pHat"] = ph
+        res[p]["sumWeights"] = ph * M
+        res[p]["w"] = 1.0
+        res[p]["M"] = M
+    return res
+
+
+def pass_c(W, H, k, pl, res, scene, frame=3):
+    u = abi.make_uniforms(W, H, frame, CAM_C, neighbors=k, radius=30.0, flags=abi.VISIBILITY_REUSE_FLAG)
+    return oracle.pass_c(scene, u, k, 0, oracle.gbuffer(pl, W, H), res)
+
+
+CAM_C = np.array([0.5, 0.0, 20.0], np.float32)
+
+
+def test_pass_c_single_pixel_keeps_w():
+    k, M = 6, 3
+    pl = flat_gbuffer(1, 1)
+    res = light_reservoirs(pl, 1, 1, CAM_C, M)
+    out = pass_c(1, 1, k, pl, res, occluder_scene(False))
+    ph = float(res["pHat"][0])
+    assert out["M"][0] == (1 + k) * M and out["lightIndex"][0] == 0
+    assert out["sumWeights"][0] == pytest.approx(ph * M * (1 + k), rel=1e-6)
+    assert out["w"][0] == pytest.approx(1.0, rel=1e-6)  # Z = (1 + k) M: every copy sees the light
+
+
+def test_pass_c_occluded_centre_zeroes_w():
+    """Pixel 1's own ray to the light is blocked: Z = 0 whatever its neighbours see."""
+    k, M = 6, 2
+    pl = flat_gbuffer(2, 1)
+    res = light_reservoirs(pl, 2, 1, CAM_C, M)
+    free = pass_c(2, 1, k, pl, res, occluder_scene(False))
+    occ = pass_c(2, 1, k, pl, res, occluder_scene(True))
+    assert free["w"][1] > 0 and occ["w"][1] == 0.0 and occ["sumWeights"][1] == 0.0
+    assert occ["M"][1] == free["M"][1] == (1 + k) * M  # M still counts every neighbour
+
+
+def _z_ratio(free, occ, p, k):
+    """(1 + n) for the W ratio free / occluded = (1 + n) / (1 + k): returns n (a float)."""
+    return free["w"][p] / occ["w"][p] * (1 + k) - 1
+
+
+def _blocked_picks(frame, k=8, M=2):
+    """Pixel 0 sees the light; pixel 1's ray to it is blocked. For pixel 0, each neighbour pick
+    that clamps onto pixel 1 drops its M out of Z (and nothing else changes): with n picks on
+    pixel 0 itself, W_free / W_blocked = (1 + n) / (1 + k) for an integer n in [0, k]. Returns n."""
+    pl = flat_gbuffer(2, 1)
+    res = light_reservoirs(pl, 2, 1, CAM_C, M)
+    free = pass_c(2, 1, k, pl, res, occluder_scene(False), frame)
+    occ = pass_c(2, 1, k, pl, res, occluder_scene(True), frame)
+    for f in ("M", "lightIndex", "sumWeights", "pHat"):
+        assert free[f][0] == occ[f][0], f
+    n = _z_ratio(free, occ, 0, k)
+    assert abs(n - round(n)) < 1e-4 and 0 <= round(n) <= k, n
+    return round(n)
+
+
+@pytest.mark.parametrize("frame", [3, 4, 5, 6])
+def test_pass_c_blocked_neighbours_drop_out_of_z(frame):
+    _blocked_picks(frame)
+
+
+def test_pass_c_blocked_neighbour_counts_vary():
+    ns = {_blocked_picks(f) for f in (3, 4, 5, 6, 7, 8)}
+    assert len(ns) > 1 and max(ns) < 8  # some picks land on pixel 1, and not always as many
+
+
+def test_pass_c_neighbour_facing_away_drops_out_of_z():
+    """Pixel 1's normal points down (-z): the light is below its horizon, so its picks drop out of
+    pixel 0's Z as a blocked ray's do (no occluder in the scene)."""
+    k, M = 8, 2
+    nrm = np.zeros((1, 2, 3), np.float32)
+    nrm[..., 2] = 1.0
+    pl_up = flat_gbuffer(2, 1, normal=nrm)
+    nrm2 = nrm.copy()
+    nrm2[0, 1, 2] = -1.0
+    pl_down = flat_gbuffer(2, 1, normal=nrm2)
+    res = light_reservoirs(pl_up, 2, 1, CAM_C, M)
+    free = pass_c(2, 1, k, pl_up, res, occluder_scene(False))
+    away = pass_c(2, 1, k, pl_down, res, occluder_scene(False))
+    blocked = pass_c(2, 1, k, pl_up, res, occluder_scene(True))
+    assert away["sumWeights"][0] == free["sumWeights"][0] and away["M"][0] == free["M"][0]
+    n = _z_ratio(free, away, 0, k)
+    assert abs(n - round(n)) < 1e-4 and 0 <= round(n) < k
+    assert away["w"][0] == blocked["w"][0]  # the same picks drop out either way

@@ -642,7 +642,13 @@ __global__ __launch_bounds__(256) void k_wide_build(const rt::Node *__restrict__
     o[4] = make_float4(mx[0].y, mx[1].y, mx[2].y, mx[3].y);
     o[5] = make_float4(mx[0].z, mx[1].z, mx[2].z, mx[3].z);
     reinterpret_cast<int4 *>(o)[6] = make_int4(id[0], id[1], id[2], id[3]);
-    reinterpret_cast<int4 *>(o)[7] = make_int4(grp[0], grp[1], 0, 0);
+    // entry masks: bit k set when entry k exists, bit 4 + k when it is a triangle leaf
+    uint32_t masks = 0;
+    for (int k = 0; k < 4; ++k) {
+        if (id[k] != rt::kNoEntry) masks |= 1u << k;
+        if (id[k] != rt::kNoEntry && id[k] < 0) masks |= 16u << k;
+    }
+    reinterpret_cast<int4 *>(o)[7] = make_int4(grp[0], grp[1], static_cast<int32_t>(masks), 0);
     if (!ok) atomicOr(&info[0], 1u);
     const float m = fmaxf(fmaxf(fmaxf(box_max(b.lmin), box_max(b.lmax)), box_max(b.rmin)), box_max(b.rmax));
     const bool nan = b.lmin.x != b.lmin.x || b.lmin.y != b.lmin.y || b.lmin.z != b.lmin.z || b.lmax.x != b.lmax.x ||

@@ -724,7 +724,9 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
                 size = size < kTraceMinChunk ? kTraceMinChunk : (size > kTraceMaxChunk ? kTraceMaxChunk : size);
                 uint32_t base = 0;
                 if (lane == 0) base = atomicAdd(head, size);
-                base = __shfl(base, 0);
+                // every lane is active here: the first active lane is lane 0, and the value stays
+                // in a scalar register, so the loop's exit test is seen as wave-uniform
+                base = __builtin_amdgcn_readfirstlane(base);
                 seen = base + size;
                 next = base < n ? base : n;
                 end = n - next < size ? n : next + size;

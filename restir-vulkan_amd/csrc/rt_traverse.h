@@ -278,7 +278,8 @@ struct alignas(16) WNode {
     float4 lox, loy, loz;  // low corner of entry k in component k
     float4 hix, hiy, hiz;  // high corner
     int4 id;               // entry: >= 0 internal node, < 0 ~triangle, kNoEntry: none
-    int4 grp;              // .x / .y: the left / right child when it is internal, else -1
+    int4 grp;              // .x / .y: the left / right child when it is internal, else -1;
+                           // .z: bit k when entry k exists, bit 4 + k when it is a triangle
 };
 static_assert(sizeof(WNode) == 128, "one 128-byte line per node");
 constexpr int32_t kNoEntry = INT32_MIN;
@@ -366,13 +367,14 @@ __device__ __forceinline__ uint32_t wide_pass(const WSeg &ws, const WNode *__res
     const bool b1 = ray_box_w<kQuick>(ws, corner(lx, ly, lz, 1), corner(hx, hy, hz, 1));
     const bool b2 = ray_box_w<kQuick>(ws, corner(lx, ly, lz, 2), corner(hx, hy, hz, 2));
     const bool b3 = ray_box_w<kQuick>(ws, corner(lx, ly, lz, 3), corner(hx, hy, hz, 3));
-    return ((sideL & (id.x != kNoEntry) & b0) ? 1u : 0u) | ((sideL & (id.y != kNoEntry) & b1) ? 2u : 0u) |
-           ((sideR & (id.z != kNoEntry) & b2) ? 4u : 0u) | ((sideR & (id.w != kNoEntry) & b3) ? 8u : 0u);
+    const uint32_t sides = (sideL ? 3u : 0u) | (sideR ? 12u : 0u);
+    return ((b0 ? 1u : 0u) | (b1 ? 2u : 0u) | (b2 ? 4u : 0u) | (b3 ? 8u : 0u)) & sides &
+           static_cast<uint32_t>(grp.z);
 }
 
-// The triangle entries among the passed entries `h` (an existing entry < 0 is ~triangle).
-__device__ __forceinline__ uint32_t wide_leaves(uint32_t h, int4 id) {
-    return h & ((id.x < 0 ? 1u : 0u) | (id.y < 0 ? 2u : 0u) | (id.z < 0 ? 4u : 0u) | (id.w < 0 ? 8u : 0u));
+// The triangle entries among the passed entries `h`.
+__device__ __forceinline__ uint32_t wide_leaves(uint32_t h, int4 grp) {
+    return h & (static_cast<uint32_t>(grp.z) >> 4);
 }
 
 __device__ __forceinline__ int32_t wide_entry(int4 id, uint32_t k) {
@@ -384,8 +386,8 @@ __device__ __forceinline__ int32_t wide_entry(int4 id, uint32_t k) {
 // when both sides yield one; with neither, the stack's top (-1 when the stack is empty).
 template <int kStack>
 __device__ __forceinline__ int32_t wide_next(uint32_t h, int4 id, int4 grp, int32_t *stk, int stride, int &top) {
-    const bool i0 = (h & 1u) && id.x >= 0, i1 = (h & 2u) && id.y >= 0;
-    const bool i2 = (h & 4u) && id.z >= 0, i3 = (h & 8u) && id.w >= 0;
+    const uint32_t in = h & ~(static_cast<uint32_t>(grp.z) >> 4);  // internal entries that passed
+    const bool i0 = (in & 1u) != 0, i1 = (in & 2u) != 0, i2 = (in & 4u) != 0, i3 = (in & 8u) != 0;
     const int32_t itemL = (i0 && i1) ? grp.x : (i0 ? id.x : (i1 ? id.y : -1));
     const int32_t itemR = (i2 && i3) ? grp.y : (i2 ? id.z : (i3 ? id.w : -1));
     int32_t nx = itemR;
@@ -415,7 +417,7 @@ __device__ __forceinline__ bool wide_visit_k(const WSeg &ws, const WNode *__rest
     int4 id, grp;
     const uint32_t h = wide_pass<kQuick>(ws, wide, nodes, cur, id, grp);
     ++nv;
-    for (uint32_t leaves = wide_leaves(h, id); leaves; leaves &= leaves - 1u) {
+    for (uint32_t leaves = wide_leaves(h, grp); leaves; leaves &= leaves - 1u) {
         ++nt;
         if (ray_tri(tris, ~wide_entry(id, static_cast<uint32_t>(__builtin_ctz(leaves))), ws.s.o, ws.s.d)) return true;
     }
@@ -499,7 +501,7 @@ __device__ __forceinline__ uint32_t walk_step_defer(WWalk &w, const WNode *__res
                                                     : wide_pass<false>(w.s, wide, nodes, w.cur, ids, grp);
     ++w.nv;
     w.cur = wide_next<kStack>(h, ids, grp, stk, stride, w.top);
-    return wide_leaves(h, ids);
+    return wide_leaves(h, grp);
 }
 
 // testVisibility, visibilityTest.glsl:1-30: shrink the segment p1 -> p2 by tMin = 0.001 at both

@@ -186,7 +186,7 @@ void restir_destroy(restir_ctx *ctx);
 const char *restir_last_error(const restir_ctx *ctx);
 /* ABI revision; bumped whenever an entry point's signature or documented behaviour changes. Bindings compare it with the
  * revision they were written against and refuse a mismatched library. */
-#define RESTIR_ABI_VERSION 15
+#define RESTIR_ABI_VERSION 16
 int restir_version(void);
 
 /* Light tables: device pointers to the reference buffer images (header + array). The counts are
@@ -260,6 +260,16 @@ int restir_unbiased(restir_ctx *ctx, const restir_uniforms *u, int k, int iter,
 int restir_shade(restir_ctx *ctx, const float camera_pos[3], float gamma, const restir_gbuffer *g,
                  const restir_reservoir *res, float *rgb_planes, uint64_t rgb_plane_stride,
                  void *stream);
+/* restir_shade with options (no counterpart in the reference's lighting pass). With
+ * RESTIR_SHADE_VISIBILITY_FLAG, the final shading re-tests visibility: a pixel whose sample has
+ * nonzero W traces the segment from its surface to the sample (testVisibility semantics,
+ * visibilityTest.glsl:1-30) and shades to 0 when it is shadowed (the BVH must be set). After an
+ * unbiased Pass C this changes nothing (a shadowed centre already has W = 0); after Pass B it
+ * removes the light the biased reuse carries into shadow. */
+#define RESTIR_SHADE_VISIBILITY_FLAG 1u
+int restir_shade_ex(restir_ctx *ctx, const float camera_pos[3], float gamma, uint32_t flags,
+                    const restir_gbuffer *g, const restir_reservoir *res, float *rgb_planes,
+                    uint64_t rgb_plane_stride, void *stream);
 
 /* Shadow rays p1 -> p2 (float4 each, w ignored) through testVisibility semantics: the segment is
  * shrunk by 0.001 at both ends. shadowed[i] = 1 if occluded. node_visits / tri_tests may be NULL.

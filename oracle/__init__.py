@@ -51,6 +51,7 @@ SIGS = {
     "oracle_pass_c": (None, [C.POINTER(OracleScene), _vp, _i32, _i32, _vp, _vp, _vp]),
     "oracle_cos_threshold": (_f32, [_f32]),
     "oracle_shade": (None, [C.POINTER(OracleScene), _vp, _f32, _vp, _vp, _vp, _u64]),
+    "oracle_shade_visibility": (None, [C.POINTER(OracleScene), _vp, _f32, _vp, _vp, _vp, _u64]),
     "oracle_primary_gbuffer": (None, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64,
                                       _u32, _u32]),
     "oracle_projection_view": (None, [_vp, _f32, _vp]),
@@ -246,12 +247,12 @@ def pass_c(scene: Scene, u, k, it, g, res_in):
     return out
 
 
-def shade(scene: Scene, cam_pos, gamma, g, res):
+def shade(scene: Scene, cam_pos, gamma, g, res, visibility=False):
     W, H = g.width, g.height
     out = np.zeros(3 * W * H, dtype=np.float32)
     cp = np.asarray(cam_pos[:3], dtype=np.float32)
-    lib().oracle_shade(scene.ref(), cp.ctypes.data, gamma, C.addressof(g), res.ctypes.data,
-                       out.ctypes.data, W * H)
+    fn = lib().oracle_shade_visibility if visibility else lib().oracle_shade
+    fn(scene.ref(), cp.ctypes.data, gamma, C.addressof(g), res.ctypes.data, out.ctypes.data, W * H)
     return out.reshape(3, H, W)
 
 

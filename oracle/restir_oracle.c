@@ -986,9 +986,10 @@ void oracle_pass_c(const oracle_scene *s, const restir_uniforms *u, int k, int i
     }
 }
 
-void oracle_shade(const oracle_scene *s, const float camf[3], float gamma, const restir_gbuffer *g,
-                  const restir_reservoir *res, float *rgb, uint64_t rs) {
-    /* lighting.frag:49-71, 103 */
+static void shade_impl(const oracle_scene *s, const float camf[3], float gamma, int visibility,
+                       const restir_gbuffer *g, const restir_reservoir *res, float *rgb, uint64_t rs) {
+    /* lighting.frag:49-71, 103; with `visibility`, the sample of a pixel whose weight is nonzero
+     * is shadow-tested from the pixel first (testVisibility) and a shadowed one shades to 0 */
     uint32_t W = g->width, H = g->height;
     v3 cam = ld3(camf);
 #pragma omp parallel for schedule(static)
@@ -1009,12 +1010,22 @@ void oracle_shade(const oracle_scene *s, const float camf[3], float gamma, const
             phat_full_v(t.pos, ld3(sm->position_emissionLum), cam, t.n, ld3(sm->normal),
                         sm->normal[3] > 0.5f, alb, em, t.rough, t.metal, c);
             for (int j = 0; j < 3; ++j) c[j] = c[j] * sm->w;
+            if (visibility && sm->w != 0.0f && test_visibility(s, t.pos, ld3(sm->position_emissionLum)))
+                c[0] = c[1] = c[2] = 0.0f;
         }
         if (t.emissive > 0.5f) { c[0] = t.alb.x; c[1] = t.alb.y; c[2] = t.alb.z; }
         if (gamma != 1.0f)
             for (int j = 0; j < 3; ++j) c[j] = powf(c[j], 1.0f / gamma);
         rgb[i] = c[0]; rgb[rs + i] = c[1]; rgb[2 * rs + i] = c[2];
     }
+}
+void oracle_shade(const oracle_scene *s, const float camf[3], float gamma, const restir_gbuffer *g,
+                  const restir_reservoir *res, float *rgb, uint64_t rs) {
+    shade_impl(s, camf, gamma, 0, g, res, rgb, rs);
+}
+void oracle_shade_visibility(const oracle_scene *s, const float camf[3], float gamma, const restir_gbuffer *g,
+                             const restir_reservoir *res, float *rgb, uint64_t rs) {
+    shade_impl(s, camf, gamma, 1, g, res, rgb, rs);
 }
 
 /* ------------------------------------------------------------------ primary G-buffer ------ */

@@ -133,6 +133,10 @@ float oracle_cos_threshold(float degrees);
 /* lighting.frag:49-71, 103 */
 void oracle_shade(const oracle_scene *s, const float cam[3], float gamma, const restir_gbuffer *g,
                   const restir_reservoir *res, float *rgb_planes, uint64_t rgb_plane_stride);
+/* The same with the visibility re-test of the final shading (the 1080p workload's setting): a
+ * pixel whose sample has nonzero W and is shadowed from it shades to 0. */
+void oracle_shade_visibility(const oracle_scene *s, const float cam[3], float gamma, const restir_gbuffer *g,
+                             const restir_reservoir *res, float *rgb_planes, uint64_t rgb_plane_stride);
 
 /* Primary-ray G-buffer by brute force over all triangles (closest t, ties to lower index). */
 void oracle_primary_gbuffer(const restir_triangle *tris, uint32_t n_tris,

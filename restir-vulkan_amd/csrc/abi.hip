@@ -430,8 +430,16 @@ int restir_unbiased(restir_ctx *ctx, const restir_uniforms *u, int k, int iter,
 
 int restir_shade(restir_ctx *ctx, const float cameraPos[3], float gamma, const restir_gbuffer *g,
                  const restir_reservoir *res, float *rgb, uint64_t rgbStride, void *stream) {
+    return restir_shade_ex(ctx, cameraPos, gamma, 0u, g, res, rgb, rgbStride, stream);
+}
+
+int restir_shade_ex(restir_ctx *ctx, const float cameraPos[3], float gamma, uint32_t flags, const restir_gbuffer *g,
+                    const restir_reservoir *res, float *rgb, uint64_t rgbStride, void *stream) {
     if (!ctx) return RESTIR_E_INVALID_ARGUMENT;
+    if (flags & ~RESTIR_SHADE_VISIBILITY_FLAG) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "shade: unknown flags %u", flags);
+    const bool vis = (flags & RESTIR_SHADE_VISIBILITY_FLAG) != 0;
     if (!ctx->lightsSet) return fail(ctx, RESTIR_E_NOT_READY, "shade: lights not set");
+    if (vis && !ctx->bvhSet) return fail(ctx, RESTIR_E_NOT_READY, "shade: BVH not set (visibility re-test)");
     if (!cameraPos || !res || !rgb) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "shade: NULL argument");
     if (!g) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "shade: G-buffer is NULL");
     if (int r = check_gbuffer(ctx, g, g->width, g->height, false, "shade")) return r;
@@ -440,7 +448,7 @@ int restir_shade(restir_ctx *ctx, const float cameraPos[3], float gamma, const r
     if (!(gamma > 0.0f)) return fail(ctx, RESTIR_E_INVALID_ARGUMENT, "shade: gamma must be > 0");
     if (int r = use_device(ctx)) return r;
     const hipError_t e = rk::launch_shade(ctx->scene, rt::f3{cameraPos[0], cameraPos[1], cameraPos[2]},
-                                          gamma, view_of(g), res, rgb, rgbStride,
+                                          gamma, view_of(g), res, rgb, rgbStride, vis, ctx->stack,
                                           static_cast<hipStream_t>(stream));
     return e == hipSuccess ? RESTIR_OK : hip_fail(ctx, e, "shade launch");
 }

@@ -95,8 +95,10 @@ size_t pass_c_queue_bytes(uint32_t npx, int k);
 // light end of both is read from that pass's output `merged`); *n receives the number of rays.
 hipError_t pass_c_rays(const void *queueScratch, uint32_t npx, int k, const restir_reservoir *merged, int which,
                        float4 *p1, float4 *p2, uint32_t cap, uint32_t *n, hipStream_t s);
+// visibility: the final visibility re-test (tree traversal with a `stack`-entry stack)
 hipError_t launch_shade(const SceneView &sc, rt::f3 cam, float gamma, const GView &g,
-                        const restir_reservoir *res, float *rgb, uint64_t rgbP, hipStream_t s);
+                        const restir_reservoir *res, float *rgb, uint64_t rgbP, bool visibility, int stack,
+                        hipStream_t s);
 // persistent: fixed grid of at most gridCap workgroups pulling rays through *counter (which the
 // launch clears with a kernel, so the sequence can be captured in a graph).
 hipError_t launch_trace(int stack, bool refLoop, bool persistent, uint32_t gridCap,

@@ -882,7 +882,8 @@ struct ShadeArgs {
     uint64_t rgbP;
 };
 
-__device__ __forceinline__ void shade_px(const ShadeArgs &a, size_t i) {
+// occluded: the pixel's sample failed the final visibility re-test (it then shades to 0)
+__device__ __forceinline__ void shade_px(const ShadeArgs &a, size_t i, bool occluded = false) {
     const GView &g = a.g;
     if (i >= static_cast<size_t>(g.W) * g.H) return;
     const f3 pos = plane3(g.pos, i, g.P);
@@ -904,6 +905,7 @@ __device__ __forceinline__ void shade_px(const ShadeArgs &a, size_t i) {
         const f3 f = eval_phat_full(s, alb, r.pos, f3{r.normal.x, r.normal.y, r.normal.z},
                                     r.normal.w > 0.5f, f3{em[0], em[1], em[2]});
         c = f3{f.x * r.w, f.y * r.w, f.z * r.w};
+        if (occluded) c = f3{0.0f, 0.0f, 0.0f};
     }
     if (emissive > 0.5f) c = alb;
     if (a.gamma != 1.0f) {
@@ -915,8 +917,28 @@ __device__ __forceinline__ void shade_px(const ShadeArgs &a, size_t i) {
     a.rgb[2 * a.rgbP + i] = c.z;
 }
 
+// VIS: the visibility re-test of the final shading (the 1080p workload's setting): a pixel whose
+// sample has nonzero W traces the segment from its surface to the sample (testVisibility) first,
+// one ray per lane.
+template <int STACK, bool VIS>
 __global__ __launch_bounds__(256) void k_shade(const ShadeArgs a) {
-    shade_px(a, static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x);
+    __shared__ int32_t stack[VIS ? STACK * 256 : 1];
+    const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    bool occ = false;
+    if (VIS && i < static_cast<size_t>(a.g.W) * a.g.H) {
+        const f3 n = load_normal(a.g, i);
+        if (dot(n, n) != 0.0f) {
+            const Res r = load_res(a.res, i);
+            if (r.w != 0.0f) {
+                f3 o, d;
+                shadow_segment(plane3(a.g.pos, i, a.g.P), r.pos, o, d);
+                TraceStats st{0, 0};
+                occ = occluded_w<STACK>(a.sc.wide, a.sc.nodes, a.sc.tris, a.sc.treeNested != 0, a.sc.maxCoord, o, d,
+                                        stack + threadIdx.x, 256, st);
+            }
+        }
+    }
+    shade_px(a, i, occ);
 }
 
 // ------------------------------------------------------------------------- batch trace ----
@@ -1573,11 +1595,19 @@ hipError_t launch_shade_views(const ViewShade *views, uint32_t n, float gamma, v
 }
 
 hipError_t launch_shade(const SceneView &sc, f3 cam, float gamma, const GView &g,
-                        const restir_reservoir *res, float *rgb, uint64_t rgbP, hipStream_t s) {
+                        const restir_reservoir *res, float *rgb, uint64_t rgbP, bool visibility, int stack,
+                        hipStream_t s) {
     ShadeArgs args{sc, cam, gamma, g, res, rgb, rgbP};
     const size_t n = static_cast<size_t>(g.W) * g.H;
-    RK_LAUNCH("k_shade", k_shade, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, s, args);
-    return hipGetLastError();
+    const dim3 grid(static_cast<uint32_t>((n + 255) / 256));
+    if (!visibility) {
+        RK_LAUNCH("k_shade", (k_shade<24, false>), grid, dim3(256), 0, s, args);
+        return hipGetLastError();
+    }
+    return by_stack(stack, [&](auto S) {
+        RK_LAUNCH("k_shade", (k_shade<decltype(S)::value, true>), grid, dim3(256), 0, s, args);
+        return hipGetLastError();
+    });
 }
 
 template <int STACK, bool REF>

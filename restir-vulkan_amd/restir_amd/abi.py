@@ -19,10 +19,11 @@ LIB_PATH = PKG_DIR / "librestir_amd.so"
 VISIBILITY_REUSE_FLAG = 1 << 0
 TEMPORAL_REUSE_FLAG = 1 << 1
 STANDARD_WEIGHTS_FLAG = 1 << 2  # RIS with W = sumW / (M p_hat) and triangle lights by area
+SHADE_VISIBILITY_FLAG = 1  # RESTIR_SHADE_VISIBILITY_FLAG: final shading re-tests visibility
 TRACE_FIRST_HIT = 0
 TRACE_REFERENCE_LOOP = 1
 TRACE_ONE_RAY_PER_LANE = 2
-ABI_VERSION = 15  # RESTIR_ABI_VERSION of include/restir_abi.h
+ABI_VERSION = 16  # RESTIR_ABI_VERSION of include/restir_abi.h
 
 ERRORS = {
     -1: "RESTIR_E_INVALID_ARGUMENT",
@@ -179,6 +180,7 @@ SIGNATURES = {
     "restir_unbiased": (_i32, [_vp, C.POINTER(Uniforms), _i32, _i32, C.POINTER(GBuffer), _vp,
                                _vp, C.POINTER(Launch), _vp]),
     "restir_shade": (_i32, [_vp, C.POINTER(_f32), _f32, C.POINTER(GBuffer), _vp, _vp, _u64, _vp]),
+    "restir_shade_ex": (_i32, [_vp, C.POINTER(_f32), _f32, _u32, C.POINTER(GBuffer), _vp, _vp, _u64, _vp]),
     "restir_trace_shadow": (_i32, [_vp, _vp, _vp, _u32, _vp, _vp, _vp, _i32, _vp]),
     "restir_build_aabb_tree": (_i32, [_vp, _u32, _vp, _vp, C.POINTER(_u32)]),
     "restir_build_alias_table": (_i32, [_vp, _u32, _vp, _u32, _vp]),
@@ -530,8 +532,14 @@ class Context:
             self.h, C.byref(u), k, it, C.byref(g), as_ptr(res_in), as_ptr(res_out),
             C.byref(launch) if launch is not None else None, stream_ptr(stream)), "restir_unbiased")
 
-    def shade(self, camera_pos, gamma, g, res, rgb, rgb_stride, stream=None):
+    def shade(self, camera_pos, gamma, g, res, rgb, rgb_stride, stream=None, visibility=False):
+        """restir_shade, or with visibility=True restir_shade_ex with the final visibility re-test."""
         cp = (C.c_float * 3)(*[float(v) for v in camera_pos[:3]])
+        if visibility:
+            self._check(self.lib.restir_shade_ex(self.h, cp, float(gamma), SHADE_VISIBILITY_FLAG, C.byref(g),
+                                                 as_ptr(res), as_ptr(rgb), rgb_stride, stream_ptr(stream)),
+                        "restir_shade_ex")
+            return
         self._check(self.lib.restir_shade(self.h, cp, float(gamma), C.byref(g), as_ptr(res),
                                           as_ptr(rgb), rgb_stride, stream_ptr(stream)),
                     "restir_shade")

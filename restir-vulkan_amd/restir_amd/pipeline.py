@@ -103,7 +103,7 @@ class Renderer:
                  spatial_passes=1, spatial_neighbors=4, radius=30.0, temporal=True,
                  visibility=True, temporal_mult=20, launch=None, ctx=None, learned=None,
                  train_samples=4096, lr=1e-3, train=True, launches=None, scene_build="host",
-                 normal_format=abi.NORMAL_F32, standard_weights=False):
+                 normal_format=abi.NORMAL_F32, standard_weights=False, shade_visibility=False):
         torch = _torch()
         self.torch = torch
         self.device = torch.device("cuda", device_index)
@@ -121,6 +121,7 @@ class Renderer:
                      (abi.STANDARD_WEIGHTS_FLAG if standard_weights else 0)
         self.launch = launch
         self.normal_format = normal_format  # G-buffer normals as float or half planes
+        self.shade_visibility = shade_visibility  # final shading re-tests visibility
         self.launches = {"a": launch, "b": launch, "c": launch}
         self.launches.update(launches or {})
         # learned proposal: a restir_amd.learned.LearnedProposal bound to this context, or None
@@ -195,7 +196,7 @@ class Renderer:
 
     def shade(self, stream=None, gamma=1.0):
         self.ctx.shade(self.cam["position"], gamma, self.gstruct(self.cur), self.res[self.cur],
-                       self.rgb, self.W * self.H, stream)
+                       self.rgb, self.W * self.H, stream, visibility=self.shade_visibility)
 
     def train_step(self, u, stream=None):
         """One online step of the learned proposal on this frame's final reservoirs."""

@@ -184,6 +184,34 @@ def test_biased_cornell_with_triangle_lights_bit_exact(torch, schedule):
     run_frame_and_compare(r, sc.camera)
 
 
+def test_shade_visibility_retest_bit_exact(torch):
+    """The final shading's visibility re-test (restir_shade_ex with RESTIR_SHADE_VISIBILITY_FLAG):
+    after biased Pass B frames on the terrain it equals the oracle's re-tested shading bit for bit
+    and darkens the pixels whose reused sample is shadowed; after an unbiased Pass C it changes
+    no value (a shadowed centre already has W = 0)."""
+    sc = scenes.config_scene("terrain")
+    for mode, kw in (("biased", dict(spatial_passes=1, spatial_neighbors=5)), ("unbiased", dict(k=5, spatial_passes=2))):
+        r = Renderer(sc, 320, 180, mode=mode, **kw)
+        for f in range(2):
+            cam = scenes.orbit_camera(sc.camera, f)
+            _, final_ref = run_frame_and_compare(r, cam)
+            plain = r.image().copy()
+            r.shade_visibility = True
+            r.shade()
+            vis = r.image()
+            r.shade_visibility = False
+            og = oracle.gbuffer(host_planes(r, r.cur), r.W, r.H)
+            ref = oracle.shade(oracle_for(r), cam["position"], 1.0, og, final_ref, visibility=True)
+            eq = floats_equal_nan(vis, ref)
+            assert eq.all(), f"{mode} frame {f}: {int((~eq).sum())} values differ"
+            changed = int((~floats_equal_nan(vis, plain)).sum())
+            print(f"[shade visibility] {mode} frame {f}: {changed} values darkened", flush=True)
+            if mode == "unbiased":
+                assert changed == 0
+            else:
+                assert changed > 0 and np.all((vis <= plain) | np.isnan(plain))
+
+
 def test_trace_shadow_matches_oracle(torch):
     for sc in (scenes.cornell_box(1), scenes.terrain(2)):
         hs = HostScene(sc)

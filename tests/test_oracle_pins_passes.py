@@ -573,3 +573,17 @@ def test_pass_c_neighbour_facing_away_drops_out_of_z():
     n = _z_ratio(free, away, 0, k)
     assert abs(n - round(n)) < 1e-4 and 0 <= round(n) < k
     assert away["w"][0] == blocked["w"][0]  # the same picks drop out either way
+
+
+def test_shade_visibility_zeroes_only_blocked_pixels():
+    M = 2
+    pl = flat_gbuffer(2, 1)
+    res = light_reservoirs(pl, 2, 1, CAM_C, M)
+    g = oracle.gbuffer(pl, 2, 1)
+    plain = oracle.shade(occluder_scene(True), CAM_C, 1.0, g, res)
+    free = oracle.shade(occluder_scene(False), CAM_C, 1.0, g, res, visibility=True)
+    blocked = oracle.shade(occluder_scene(True), CAM_C, 1.0, g, res, visibility=True)
+    assert (plain[:, 0, :] > 0).all()
+    assert np.array_equal(free, plain)
+    assert np.array_equal(blocked[:, 0, 0], plain[:, 0, 0])  # pixel 0 sees the light
+    assert (blocked[:, 0, 1] == 0).all()                      # pixel 1's segment is cut

@@ -393,7 +393,8 @@ def main():
         "config": {
             "workload": "unbiased ReSTIR 1920x1080, terrain 136,050 tris + 2k emissive lamp triangles, "
                         "10k point lights + the 2k emissive triangles (one alias table over both), temporal cap 20M, 2x unbiased spatial k=5 "
-                        "r=30, orbit 0.5 deg/frame; plain RIS M=32 vs online-trained MLP proposal M=8 "
+                        "r=30 (the second Pass C's centre ray is the final sample's visibility re-test), "
+                        "orbit 0.5 deg/frame; plain RIS M=32 vs online-trained MLP proposal M=8 "
                         "(114-128-128-128-256, bf16 MFMA, Adam 1e-3, 4096 samples/frame)",
             "resolution": f"{W}x{H}", "triangles": scene.n_tris,
             "point_lights": int(len(scene.point_lights)), "triangle_lights": int(len(scene.tri_lights)),

@@ -305,9 +305,7 @@ int restir_build_alias_table(const restir_point_light *points, uint32_t n_point,
  * restir_build_alias_table makes, bit for bit, from device light records (point lights if
  * n_point > 0, else triangle lights; plain arrays, no SSBO header) into device columns[n]. One
  * workgroup, one launch, no host round trip (it may be captured into a graph); the work is
- * ordered on `stream`. Context scratch: 20 bytes per light.
- * Currently switched off: returns RESTIR_E_NOT_READY without launching (an unexplained GPU fault
- * occurred while its kernel ran; build on the host with restir_build_alias_table). */
+ * ordered on `stream`. Context scratch: 20 bytes per light. */
 int restir_build_alias_table_device(restir_ctx *ctx, const restir_point_light *points,
                                     uint32_t n_point, const restir_tri_light *tris, uint32_t n_tri,
                                     restir_alias_table_column *columns, void *stream);

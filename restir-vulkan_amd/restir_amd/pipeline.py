@@ -61,9 +61,10 @@ class SceneOnDevice:
         tl = pack_buffer(scene.tri_lights, abi.TRI_LIGHT_DTYPE)
         self.d_points = to_device(pl, torch, device)
         self.d_trilights = to_device(tl, torch, device)
-        # the alias table is built on the host in both modes (the device alias build is
-        # switched off in the library); with both kinds of lights, one table over both
-        # (abi.uses_union) unless union_lights=False asks for the reference's point-light table
+        # the alias table is built on the host in both modes (restir_build_alias_table_device
+        # builds the reference's one-kind table; the union table is host-built); with both kinds
+        # of lights, one table over both (abi.uses_union) unless union_lights=False asks for the
+        # reference's point-light table
         self.union = abi.uses_union(scene.point_lights, scene.tri_lights, union_lights)
         self._alias = abi.scene_alias_table(scene.point_lights, scene.tri_lights, union_lights)
         al = pack_buffer(self._alias, abi.ALIAS_DTYPE)

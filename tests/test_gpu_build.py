@@ -142,10 +142,10 @@ def test_frame_on_device_built_scene_equals_host_built(torch):
 
 
 # ---- alias table: restir_build_alias_table_device against createAliasTable's host restatement.
-# Skipped: the library switches that entry point off (it refuses with RESTIR_E_NOT_READY) after a
-# test run faulted the GPU while its 5000-round "dominant" case ran, with no cause identified.
-ALIAS_SKIP = pytest.mark.skip(reason="device alias build switched off: unexplained GPU fault in its "
-                                     "'dominant' case, cause not identified")
+# The "dominant" case (one light paired with every small in turn) once faulted the GPU: the kernel
+# lacked a barrier between building its queues and the first pairing step, so lane 0 could read
+# a queue entry before its writer stored it (found by the kernel body's CPU build under
+# ThreadSanitizer and AddressSanitizer, tests/test_alias_kernel_cpu.py).
 def _points(lum):
     pts = np.zeros(len(lum), dtype=abi.POINT_LIGHT_DTYPE)
     pts["color_lum"][:, 3] = np.asarray(lum, dtype=np.float32)
@@ -175,7 +175,6 @@ def _device_alias(torch, ctx, points=None, tri_lights=None):
     return cols.cpu().numpy()
 
 
-@ALIAS_SKIP
 @pytest.mark.parametrize("case", list(ALIAS_CASES))
 def test_device_alias_table_matches_host_build(torch, ctx, case):
     pts = ALIAS_CASES[case](np.random.default_rng(5))
@@ -190,7 +189,6 @@ def test_device_alias_table_matches_host_build(torch, ctx, case):
                              f"device {dev.view(abi.ALIAS_DTYPE)[i]} host {host[i]}")
 
 
-@ALIAS_SKIP
 def test_device_alias_table_triangle_lights(torch, ctx):
     rng = np.random.default_rng(8)
     tri = rng.standard_normal((700, 9)).astype(np.float32)
@@ -199,9 +197,11 @@ def test_device_alias_table_triangle_lights(torch, ctx):
     assert np.array_equal(_device_alias(torch, ctx, tri_lights=tls), host.view(np.uint8))
 
 
-def test_device_alias_build_refuses_while_switched_off(torch, ctx):
-    """The switched-off entry point returns an error and launches nothing."""
+def test_device_alias_build_rejects_bad_arguments(torch, ctx):
+    """NULL buffers and columns that overlap the light records are refused before any launch."""
     pts = torch.zeros(32 * 4, dtype=torch.uint8, device="cuda")
     cols = torch.zeros(16 * 4, dtype=torch.uint8, device="cuda")
-    with pytest.raises(abi.RestirError, match="switched off"):
-        ctx.build_alias_table(pts, 4, None, 0, cols)
+    with pytest.raises(abi.RestirError):
+        ctx.build_alias_table(None, 4, None, 0, cols)
+    with pytest.raises(abi.RestirError):
+        ctx.build_alias_table(pts, 4, None, 0, pts)

@@ -555,7 +555,9 @@ def scene_builds(torch, scene):
     (restir_build_aabb_tree: the reference's sequential code, one core), on the 1080p scene and
     on a seeded 1M-triangle terrain. Device time: host wall clock around a build (it waits for
     the stream once per tree level), best of 5 after one warm build; the device tree is compared
-    byte for byte with the host one."""
+    byte for byte with the host one. Then the device alias table (restir_build_alias_table_device)
+    against the host build over the 1080p scene's 10k point lights and over 1M lognormal lights:
+    device time by events on the stream, best of 5; tables compared byte for byte."""
     from restir_amd import abi, scenes
     from restir_amd.abi import Context
 
@@ -596,6 +598,33 @@ def scene_builds(torch, scene):
         log(f"scene build {name}: tree {best:.2f} ms on the GPU vs {host_ms:.0f} ms host "
             f"(identical {same_tree})")
         del x, nodes, tris
+    rng = np.random.default_rng(17)
+    big_lights = np.zeros(1_000_000, dtype=abi.POINT_LIGHT_DTYPE)
+    big_lights["color_lum"][:, :3] = 1.0
+    big_lights["color_lum"][:, 3] = rng.lognormal(0.0, 2.0, len(big_lights)).astype(np.float32)
+    for name, pts in (("1080p point lights", scene.point_lights), ("1M lognormal lights", big_lights)):
+        pts = np.ascontiguousarray(pts, dtype=abi.POINT_LIGHT_DTYPE)
+        n = len(pts)
+        t = time.perf_counter()
+        host = abi.build_alias_table(pts, None)
+        host_ms = (time.perf_counter() - t) * 1e3
+        d_pts = torch.from_numpy(pts.view(np.uint8).reshape(-1).copy()).cuda()
+        cols = torch.empty(16 * n, dtype=torch.uint8, device="cuda")
+        ctx.build_alias_table(d_pts, n, None, 0, cols)
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(5):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            ctx.build_alias_table(d_pts, n, None, 0, cols)
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        same = bool(np.array_equal(cols.cpu().numpy(), host.view(np.uint8)))
+        rows.append({"alias_table": name, "lights": n, "alias_device_ms": round(best, 3),
+                     "alias_host_ms": round(host_ms, 1), "alias_identical": same})
+        log(f"alias table {name}: {best:.2f} ms on the GPU vs {host_ms:.0f} ms host (identical {same})")
+        del d_pts, cols
     ctx.close()
     return rows
 
@@ -763,7 +792,7 @@ def stress_4k(torch, args):
     sd = SceneOnDevice(ctx, scene, dev, build="device")
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
-    log(f"device tree (depth {sd.depth}) + host alias table over 1M lights: {setup_s:.2f} s")
+    log(f"device tree (depth {sd.depth}) + device alias table over 1M lights: {setup_s:.2f} s")
     t0 = time.perf_counter()
     lp = LearnedProposal(ctx, scene, k=4096, seed=7, iterations=16, device_clustering=True)
     torch.cuda.synchronize()

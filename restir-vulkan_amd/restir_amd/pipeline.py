@@ -35,10 +35,12 @@ def to_device(arr: np.ndarray, torch, device):
 class SceneOnDevice:
     """BVH, triangles, light tables and materials resident in HBM and bound to a context.
 
-    ``build="host"`` builds the tree on the host (restir_build_aabb_tree) and uploads it;
-    ``build="device"`` uploads the vertices and builds the same tree in HBM (restir_build_bvh);
-    ``nodes`` / ``tris`` are then read back on first access, for checking. The alias table is
-    built on the host (restir_build_alias_table) in both modes."""
+    ``build="host"`` builds the tree on the host (restir_build_aabb_tree) and the alias table
+    (restir_build_alias_table) and uploads them; ``build="device"`` uploads the vertices and light
+    records and builds the same tree (restir_build_bvh) and, for one kind of lights, the same
+    alias table (restir_build_alias_table_device) in HBM; ``nodes`` / ``tris`` / ``alias`` are
+    then read back on first access, for checking. A table over both kinds of lights (the union)
+    is built on the host in both modes."""
 
     def __init__(self, ctx: Context, scene, device="cuda", build="host", union_lights=None):
         torch = _torch()
@@ -61,14 +63,25 @@ class SceneOnDevice:
         tl = pack_buffer(scene.tri_lights, abi.TRI_LIGHT_DTYPE)
         self.d_points = to_device(pl, torch, device)
         self.d_trilights = to_device(tl, torch, device)
-        # the alias table is built on the host in both modes (restir_build_alias_table_device
-        # builds the reference's one-kind table; the union table is host-built); with both kinds
-        # of lights, one table over both (abi.uses_union) unless union_lights=False asks for the
-        # reference's point-light table
+        # with both kinds of lights, one table over both (abi.uses_union) unless
+        # union_lights=False asks for the reference's point-light table
         self.union = abi.uses_union(scene.point_lights, scene.tri_lights, union_lights)
-        self._alias = abi.scene_alias_table(scene.point_lights, scene.tri_lights, union_lights)
-        al = pack_buffer(self._alias, abi.ALIAS_DTYPE)
-        self.d_alias = to_device(al, torch, device)
+        hdr = abi.LIGHT_BUFFER_HEADER_BYTES
+        if build == "device" and not self.union:
+            # the reference's table (point lights if any, else triangle lights) built in HBM
+            n_pt, n_tl = len(scene.point_lights), len(scene.tri_lights)
+            n_al = n_pt if n_pt else n_tl
+            al = pack_buffer(np.zeros(n_al, dtype=abi.ALIAS_DTYPE), abi.ALIAS_DTYPE)  # header + space
+            self.d_alias = to_device(al, torch, device)
+            if n_al:
+                ctx.build_alias_table(self.d_points.data_ptr() + hdr if n_pt else None, n_pt,
+                                      None if n_pt else self.d_trilights.data_ptr() + hdr, 0 if n_pt else n_tl,
+                                      self.d_alias.data_ptr() + hdr)
+            self._alias = None
+        else:
+            self._alias = abi.scene_alias_table(scene.point_lights, scene.tri_lights, union_lights)
+            al = pack_buffer(self._alias, abi.ALIAS_DTYPE)
+            self.d_alias = to_device(al, torch, device)
         self.d_tri_material = to_device(scene.tri_material.astype(np.int32), torch, device)
         self.d_materials = to_device(scene.materials.astype(np.float32), torch, device)
         ctx.set_lights(self.d_points, pl.nbytes, self.d_trilights, tl.nbytes, self.d_alias, al.nbytes)
@@ -87,6 +100,9 @@ class SceneOnDevice:
 
     @property
     def alias(self) -> np.ndarray:
+        if self._alias is None:
+            hdr = abi.LIGHT_BUFFER_HEADER_BYTES
+            self._alias = self.d_alias[hdr:].cpu().numpy().view(abi.ALIAS_DTYPE)
         return self._alias
 
 

@@ -153,6 +153,12 @@ int main(int argc, char **argv) {
         }
         ok &= run_case("triangle_lights", {}, t, 8);
     }
+    if (g_only && std::strcmp(g_only, "lognormal_1m") == 0) {  // only when asked for by name (slow)
+        std::normal_distribution<float> nd(0.0f, 2.0f);
+        std::vector<float> l(1000000);
+        for (auto &v : l) v = std::exp(nd(g));
+        ok &= run_case("lognormal_1m", points_of(l), none, 10);
+    }
     std::printf(ok ? "all cases equal to the host build\n" : "some case differs\n");
     return ok ? 0 : 1;
 }

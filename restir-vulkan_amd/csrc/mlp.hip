@@ -421,7 +421,9 @@ struct ProposeArgs {
     float *candQ;
 };
 
-constexpr int kTilePx = 256;                 // pixel quads per workgroup of k_propose_wide
+// pixel quads per workgroup of k_propose_wide: 128 quads, 256 lanes and 69.6 KB of LDS, so that
+// two workgroups share a CU and one's barriers and weight loads overlap the other's work
+constexpr int kTilePx = 128;
 constexpr int kRow = kIn + 8;                // padded LDS row (bf16) against bank conflicts
 
 __device__ __forceinline__ void stage_weights(uint16_t *wbuf, const uint16_t *src, int rows) {
@@ -756,10 +758,10 @@ __device__ __forceinline__ void super_block_sums(const uint2 *row, float (&bs)[k
     }
 }
 
-__global__ __launch_bounds__(512) void k_propose_wide(const ProposeArgs a) {
+__global__ __launch_bounds__(2 * kTilePx) void k_propose_wide(const ProposeArgs a) {
     __shared__ __align__(16) uint16_t lds[2 * kTilePx * kRow];
     static_assert(kTilePx * kERow <= kTilePx * kRow, "a column block's numerators fit the weight buffer");
-    static_assert(kTilePx == 256, "512 lanes, two per quad");
+    static_assert(kTilePx % 32 == 0, "32 quads per wave in the MFMA layers, two lanes per quad");
     uint16_t *act = lds;
     uint16_t *wbuf = lds + kTilePx * kRow;
     __half *etile = reinterpret_cast<__half *>(wbuf);

@@ -284,14 +284,17 @@ struct alignas(16) WNode {
 static_assert(sizeof(WNode) == 128, "one 128-byte line per node");
 constexpr int32_t kNoEntry = INT32_MIN;
 
-// A segment for the 4-entry walk. `full`: test the skipped boxes too (see above). `quick`: o and
-// d are regular and every slab distance of every box of the tree stays below 1e30 in magnitude,
-// because (max |box coordinate| + max |o|) * max |1/d| < 1e30; ray_box_w then forms each slab
-// distance with one FMA and needs no per-box finiteness test.
+// A segment for the 4-entry walk. `full`: test the skipped boxes too (see above). `quick`: the
+// tree's boxes are finite, ordered and nested, o and d are regular, and every slab distance of
+// every box of the tree stays below 1e30 in magnitude, because (max |box coordinate| + max |o|) *
+// max |1/d| < 1e30; box_quick then forms each slab distance with one FMA and needs no finiteness
+// test. ox, oy, oz: byte offsets of the entries' near planes in a WNode row pair (0 when that
+// component of d is positive, so the low corner is met first, else 48).
 struct WSeg {
     Seg s;
-    f3 noi;      // -RN(o * inv), per component
-    float eAbs;  // absolute slack of the quick test
+    f3 noi;     // -RN(o * inv), per component
+    float eq;   // slack of the quick decision: 32u + 4u max |o * inv|, u = 2^-24
+    uint32_t ox, oy, oz;
     bool full, quick;
 };
 __device__ __forceinline__ WSeg make_wseg(f3 o, f3 d, bool treeNested, float maxCoord) {
@@ -307,36 +310,41 @@ __device__ __forceinline__ WSeg make_wseg(f3 o, f3 d, bool treeNested, float max
     const float invMax = fmaxf(fmaxf(fabsf(inv.x), fabsf(inv.y)), fabsf(inv.z));
     const float invMin = fminf(fminf(fabsf(inv.x), fabsf(inv.y)), fabsf(inv.z));
     const float oMax = fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z));
-    // comparisons are false for NaN: a NaN maxCoord or an infinite 1/d leaves quick off
-    w.quick = regular && invMin > 1e-30f && (maxCoord + oMax) * invMax < 1e30f;
-    const float oiMax = fmaxf(fmaxf(fabsf(w.noi.x), fabsf(w.noi.y)), fabsf(w.noi.z));
-    w.eAbs = oiMax * 2.3841858e-7f + 1e-30f;  // 4u max |o * inv| + 1e-30, u = 2^-24
     w.full = !(treeNested && regular);
+    // comparisons are false for NaN: a NaN maxCoord or an infinite 1/d leaves quick off
+    w.quick = !w.full && invMin > 1e-30f && (maxCoord + oMax) * invMax < 1e30f;
+    const float oiMax = fmaxf(fmaxf(fabsf(w.noi.x), fabsf(w.noi.y)), fabsf(w.noi.z));
+    w.eq = fmaf(oiMax, 2.3841858e-7f, 1.9073486e-6f);
+    w.ox = d.x > 0.0f ? 0u : 48u;
+    w.oy = d.y > 0.0f ? 0u : 48u;
+    w.oz = d.z > 0.0f ? 0u : 48u;
     return w;
 }
 
-// ray_box for a WSeg. Quick rays form each slab distance as t = RN(b * inv - oi) with one FMA
-// (oi = RN(o * inv)). With inv = (1/d)(1 + d1) and oi = o inv (1 + d2), t differs from the exact
-// (b - o)/d by at most 3.02u |t| + 1.02u |o inv|, and from the reference's RN(RN(b - o)/d) by one
-// more rounding: the widened intervals below carry 8u |v| + 4u max |o inv| + 1e-30, more than
-// twice that, and min/max keep such bounds. Undecided cases take the division form, as in
-// ray_box. All distances are finite by the `quick` condition, so fminf/fmaxf see no NaN.
-template <bool kQuick>
-__device__ __forceinline__ bool ray_box_w(const WSeg &w, float4 mn, float4 mx) {
-    if (!kQuick) return ray_box(w.s, mn, mx);
+// ray_box for a quick WSeg, given each axis's near and far plane (the low and high corner in the
+// order the segment meets them). Each slab distance is t = RN(b * inv - oi) with one FMA
+// (oi = RN(o * inv)); with inv = (1/d)(1 + d1) and oi = o inv (1 + d2), t differs from the exact
+// (b - o)/d by at most 2.01u |t| + 1.01u |o inv|, so from the reference's RN(RN(b - o)/d) by at
+// most E(t) = 5.03u |t| + 1.02u A (A = max |o inv|). t is monotone in b for a fixed ray, in both
+// forms, so near <= far per axis, rmin = max3 of the near distances and rmax = min3 of the far
+// ones are the reference's max-of-mins and min-of-maxes, and the max and min of values each within
+// E of its counterpart stay within E of the counterpart's max and min (E grows with |t|, and t +-
+// E(t) is monotone). The decision rmin < 1, rmax >= rmin, rmax > 0 is then the reference's as soon
+// as |rmin - 1|, |rmax| and |rmax - rmin| are all at least eq = 32u + 4uA: each of the three
+// differences is then larger than the error of its two sides (for |t| <= 2 the errors are below
+// 10.1u + 1.02uA per side; for larger |t| the distance to the threshold grows with |t| while
+// 5.03u |t| does not catch up), except rmax - rmin when |rmin| or |rmax| exceeds 2, and there
+// either rmin > 1 or rmax < 0 already decides both forms, or rmax - rmin > 1 in both. Closer
+// cases take the division form, which gives the same decision with near and far as with the low
+// and high corners (its per-axis min and max only swap equal operands). All distances are finite
+// by the `quick` condition, so fminf/fmaxf see no NaN.
+__device__ __forceinline__ bool box_quick(const WSeg &w, float nx, float ny, float nz, float fx, float fy, float fz) {
     const f3 inv = w.s.inv, noi = w.noi;
-    const float ax = fmaf(mn.x, inv.x, noi.x), ay = fmaf(mn.y, inv.y, noi.y), az = fmaf(mn.z, inv.z, noi.z);
-    const float bx = fmaf(mx.x, inv.x, noi.x), by = fmaf(mx.y, inv.y, noi.y), bz = fmaf(mx.z, inv.z, noi.z);
-    const float rmin = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
-    const float rmax = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
-    constexpr float kRel = 8.0f * 5.9604645e-8f;
-    const float emin = fmaf(fabsf(rmin), kRel, w.eAbs), emax = fmaf(fabsf(rmax), kRel, w.eAbs);
-    const float minLo = rmin - emin, minHi = rmin + emin;
-    const float maxLo = rmax - emax, maxHi = rmax + emax;
-    const bool surelyIn = minHi < 1.0f && maxLo >= minHi && maxLo > 0.0f;
-    const bool surelyOut = minLo >= 1.0f || maxHi < minLo || maxHi <= 0.0f;
-    if (surelyIn || surelyOut) return surelyIn;
-    return ray_box_div(w.s.o, w.s.d, mn, mx);
+    const float rmin = fmaxf(fmaxf(fmaf(nx, inv.x, noi.x), fmaf(ny, inv.y, noi.y)), fmaf(nz, inv.z, noi.z));
+    const float rmax = fminf(fminf(fmaf(fx, inv.x, noi.x), fmaf(fy, inv.y, noi.y)), fmaf(fz, inv.z, noi.z));
+    const float t1 = rmin - 1.0f, gap = rmax - rmin;
+    if (fminf(fminf(fabsf(t1), fabsf(rmax)), fabsf(gap)) >= w.eq) return t1 < 0.0f && gap >= 0.0f && rmax > 0.0f;
+    return ray_box_div(w.s.o, w.s.d, make_float4(nx, ny, nz, 0.0f), make_float4(fx, fy, fz, 0.0f));
 }
 
 __device__ __forceinline__ float4 corner(float4 x, float4 y, float4 z, int k) {
@@ -348,28 +356,50 @@ __device__ __forceinline__ float4 corner(float4 x, float4 y, float4 z, int k) {
 
 // The entries of WNode `cur` that the segment passes: bit k for entry k when it exists, its box
 // passes and (under ws.full) so does the box of its side. `id` / `grp` get the node's entry and
-// group words. kQuick: every box test takes the quick form (all lanes' rays are quick).
+// group words. kQuick: every active lane's ray is quick; each axis's near and far rows are then
+// loaded by the ray's direction, so the four box tests need no per-axis min and max.
 template <bool kQuick>
 __device__ __forceinline__ uint32_t wide_pass(const WSeg &ws, const WNode *__restrict__ wide,
                                               const Node *__restrict__ nodes, int32_t cur, int4 &id, int4 &grp) {
-    const float4 *p = reinterpret_cast<const float4 *>(wide + cur);
-    const float4 lx = p[0], ly = p[1], lz = p[2], hx = p[3], hy = p[4], hz = p[5];
-    id = reinterpret_cast<const int4 *>(p)[6];
-    grp = reinterpret_cast<const int4 *>(p)[7];
-    bool sideL = true, sideR = true;
-    if (ws.full) {
-        const Node nd = load_node(nodes, cur);
-        sideL = ray_box_w<kQuick>(ws, nd.lmin, nd.lmax);
-        sideR = ray_box_w<kQuick>(ws, nd.rmin, nd.rmax);
+    uint32_t pass;
+    if (kQuick) {
+        // row pairs (low, high) of x, y, z at byte offsets (0, 48), (16, 64), (32, 80) of the node:
+        // 32-bit offsets from the uniform base, so each load takes one offset register
+        const char *base = reinterpret_cast<const char *>(wide);
+        const uint32_t row = static_cast<uint32_t>(cur) << 7;
+        const float4 nx = *reinterpret_cast<const float4 *>(base + (row + ws.ox));
+        const float4 fx = *reinterpret_cast<const float4 *>(base + (row + (ws.ox ^ 48u)));
+        const float4 ny = *reinterpret_cast<const float4 *>(base + (row + ws.oy) + 16);
+        const float4 fy = *reinterpret_cast<const float4 *>(base + (row + (ws.oy ^ 48u)) + 16);
+        const float4 nz = *reinterpret_cast<const float4 *>(base + (row + ws.oz) + 32);
+        const float4 fz = *reinterpret_cast<const float4 *>(base + (row + (ws.oz ^ 48u)) + 32);
+        id = *reinterpret_cast<const int4 *>(base + row + 96);
+        grp = *reinterpret_cast<const int4 *>(base + row + 112);
+        // all four boxes are tested (an empty entry holds a zero box), without a branch per entry
+        const bool b0 = box_quick(ws, nx.x, ny.x, nz.x, fx.x, fy.x, fz.x);
+        const bool b1 = box_quick(ws, nx.y, ny.y, nz.y, fx.y, fy.y, fz.y);
+        const bool b2 = box_quick(ws, nx.z, ny.z, nz.z, fx.z, fy.z, fz.z);
+        const bool b3 = box_quick(ws, nx.w, ny.w, nz.w, fx.w, fy.w, fz.w);
+        pass = (b0 ? 1u : 0u) | (b1 ? 2u : 0u) | (b2 ? 4u : 0u) | (b3 ? 8u : 0u);
+    } else {
+        const float4 *p = reinterpret_cast<const float4 *>(wide + cur);
+        id = reinterpret_cast<const int4 *>(p)[6];
+        grp = reinterpret_cast<const int4 *>(p)[7];
+        const float4 lx = p[0], ly = p[1], lz = p[2], hx = p[3], hy = p[4], hz = p[5];
+        bool sideL = true, sideR = true;
+        if (ws.full) {
+            const Node nd = load_node(nodes, cur);
+            sideL = ray_box(ws.s, nd.lmin, nd.lmax);
+            sideR = ray_box(ws.s, nd.rmin, nd.rmax);
+        }
+        const bool b0 = ray_box(ws.s, corner(lx, ly, lz, 0), corner(hx, hy, hz, 0));
+        const bool b1 = ray_box(ws.s, corner(lx, ly, lz, 1), corner(hx, hy, hz, 1));
+        const bool b2 = ray_box(ws.s, corner(lx, ly, lz, 2), corner(hx, hy, hz, 2));
+        const bool b3 = ray_box(ws.s, corner(lx, ly, lz, 3), corner(hx, hy, hz, 3));
+        const uint32_t sides = (sideL ? 3u : 0u) | (sideR ? 12u : 0u);
+        pass = ((b0 ? 1u : 0u) | (b1 ? 2u : 0u) | (b2 ? 4u : 0u) | (b3 ? 8u : 0u)) & sides;
     }
-    // all four boxes are tested (an empty entry holds a zero box), without a branch per entry
-    const bool b0 = ray_box_w<kQuick>(ws, corner(lx, ly, lz, 0), corner(hx, hy, hz, 0));
-    const bool b1 = ray_box_w<kQuick>(ws, corner(lx, ly, lz, 1), corner(hx, hy, hz, 1));
-    const bool b2 = ray_box_w<kQuick>(ws, corner(lx, ly, lz, 2), corner(hx, hy, hz, 2));
-    const bool b3 = ray_box_w<kQuick>(ws, corner(lx, ly, lz, 3), corner(hx, hy, hz, 3));
-    const uint32_t sides = (sideL ? 3u : 0u) | (sideR ? 12u : 0u);
-    return ((b0 ? 1u : 0u) | (b1 ? 2u : 0u) | (b2 ? 4u : 0u) | (b3 ? 8u : 0u)) & sides &
-           static_cast<uint32_t>(grp.z);
+    return pass & static_cast<uint32_t>(grp.z);
 }
 
 // The triangle entries among the passed entries `h`.

@@ -682,6 +682,12 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_finish(const PassCArgs a, con
 // pixel's centre ray is already known to be shadowed (the centre rays come first in the index
 // space, so that holds for nearly all of them; tracing the others too changes nothing).
 // kTraceBatch: nBatch rays p1[i] -> p2[i] (restir_trace_shadow); out[i] gets 1 if shadowed, else 0.
+// The set bits of a wave mask below the calling lane.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
 constexpr uint32_t kTraceMaxChunk = 128, kTraceMinChunk = 32, kTraceRefill = 32;
 constexpr uint32_t kTriList = kTraceListEntries, kTriRoundWalkers = 32;
 constexpr int kTracePassA = 0, kTracePassC = 1, kTraceBatch = 2;
@@ -700,7 +706,6 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
     const uint32_t nCentre = MODE == kTracePassC ? count[0] : 0u;
     const uint32_t n = BATCH ? nBatch : (PER_PIXEL ? count[0] : nCentre + count[1]);
     const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
     uint32_t *tri = triList + (threadIdx.x >> 6) * kTriList;
     uint8_t *own = triOwner + (threadIdx.x >> 6) * kTriList;
     uint32_t *tal = tally + (threadIdx.x & ~63u);
@@ -734,7 +739,7 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
             }
             if (next < end) {
                 if (ray < 0) {
-                    const uint32_t i = next + static_cast<uint32_t>(__popcll(idle & below));
+                    const uint32_t i = next + lanes_below(idle);
                     if (i < end) {
                         const bool centre = MODE == kTracePassC && i < nCentre;
                         const float4 a4 = MODE != kTracePassC ? p1[i] : (centre ? q.c1[i] : q.n1[i - nCentre]);
@@ -761,20 +766,24 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
         // one visit per walking lane; its triangles join the wave's list
         const bool walking = ray >= 0 && !w.hit && w.cur >= 0;
         uint32_t leaves = 0;
-        int4 ids{0, 0, 0, 0};
+        int4 ids;  // read only where leaves has bits, that is after a visit wrote it
         if (walking) leaves = walk_step_defer<STACK>(w, sc.wide, sc.nodes, stk, 256, ids);
         const uint32_t nLeaves = static_cast<uint32_t>(__popc(leaves));
         const uint64_t c1 = __ballot(nLeaves & 1u), c2 = __ballot(nLeaves & 2u), c4 = __ballot(nLeaves & 4u);
         if (c1 | c2 | c4) {
             // listed < 64 before the visit, so the ring (kTriList entries) cannot overflow
-            uint32_t at = first + listed + static_cast<uint32_t>(__popcll(c1 & below) + 2 * __popcll(c2 & below) +
-                                                                 4 * __popcll(c4 & below));
-            for (uint32_t m = leaves; m; m &= m - 1u) {
+            uint32_t at = first + listed + lanes_below(c1) + 2u * lanes_below(c2) + 4u * lanes_below(c4);
+            // one step per entry, so that each entry's id is a register rather than a select
+            const auto put = [&](int32_t id) {
                 const uint32_t slot = at < kTriList ? at : at - kTriList;
-                tri[slot] = static_cast<uint32_t>(~wide_entry(ids, static_cast<uint32_t>(__builtin_ctz(m))));
+                tri[slot] = static_cast<uint32_t>(~id);
                 own[slot] = static_cast<uint8_t>(lane);
                 ++at;
-            }
+            };
+            if (leaves & 1u) put(ids.x);
+            if (leaves & 2u) put(ids.y);
+            if (leaves & 4u) put(ids.z);
+            if (leaves & 8u) put(ids.w);
             listed += static_cast<uint32_t>(__popcll(c1) + 2 * __popcll(c2) + 4 * __popcll(c4));
             waiting += nLeaves;
         }
@@ -975,7 +984,6 @@ __global__ __launch_bounds__(256) void k_trace_pool(const TraceArgs a, uint32_t 
     __shared__ int32_t stack[STACK * 256];
     int32_t *stk = stack + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
     std::conditional_t<REF, Walk, WWalk> w;  // the reference loop walks the binary nodes
     int32_t ray = -1;
     bool drained = false;  // wave-uniform: the counter has passed the last ray
@@ -989,7 +997,7 @@ __global__ __launch_bounds__(256) void k_trace_pool(const TraceArgs a, uint32_t 
             base = __shfl(base, static_cast<int>(leader));
             drained = base + nIdle >= a.n;
             if (ray < 0) {
-                const uint32_t i = base + static_cast<uint32_t>(__popcll(idle & below));
+                const uint32_t i = base + lanes_below(idle);
                 if (i < a.n) {
                     f3 o, d;
                     shadow_segment(xyz(a.p1[i]), xyz(a.p2[i]), o, d);

@@ -144,13 +144,15 @@ __device__ __forceinline__ Pcg pass_a_rng(const PassAArgs &a, uint32_t x, uint32
 }
 
 // One light candidate of Pass A (restirOmni.glsl:113-145) with source probability prob: point
-// light li, or (tri) a point on triangle light li picked with two draws (pickPointOnTriangle,
-// restirOmni.glsl:68-71), whose probability per unit area seen from pos is prob / (|cos| area).
-// Only the choice of the sample point depends on the kind; the target function and the
-// reservoir update run once for all lanes of a wave whatever kinds its lanes drew. `valid` false:
-// an empty candidate (p_hat 0, probability 1) that still counts in M and draws its number.
+// light li, or (tri) a point on triangle light li picked with the draws ra, rb
+// (pickPointOnTriangle, restirOmni.glsl:68-71), whose probability per unit area seen from pos is
+// prob / (|cos| area); u is the reservoir update's draw. Only the choice of the sample point
+// depends on the kind; the target function and the reservoir update run once for all lanes of a
+// wave whatever kinds its lanes drew. `valid` false: an empty candidate (p_hat 0, probability 1)
+// that still counts in M and draws its update number.
 __device__ __forceinline__ void add_candidate(Res &res, const Surface &s, f3 pos, const SceneView &sc, bool valid,
-                                              bool tri, int32_t li, float prob, Pcg &rng, bool areaPdf) {
+                                              bool tri, int32_t li, float prob, float ra, float rb, float u,
+                                              bool areaPdf) {
     f3 L{0.0f, 0.0f, 0.0f}, nl{0.0f, 0.0f, 0.0f};
     float lum = 0.0f;
     int32_t index = 0;
@@ -165,7 +167,6 @@ __device__ __forceinline__ void add_candidate(Res &res, const Surface &s, f3 pos
         const f3 p3 = xyz(*reinterpret_cast<const float4 *>(tl.p3));
         const float4 na = *reinterpret_cast<const float4 *>(tl.normalArea);
         lum = tl.emission_luminance[3];
-        const float ra = pcg_float(rng), rb = pcg_float(rng);
         const float sa = sqrtf(ra);
         L = ((1.0f - sa) * p1 + (sa * (1.0f - rb)) * p2) + (rb * sa) * p3;
         const f3 wi = normalize(pos - L);
@@ -176,8 +177,14 @@ __device__ __forceinline__ void add_candidate(Res &res, const Surface &s, f3 pos
     }
     const bool useLN = valid && tri;
     const float ph = valid ? eval_phat(s, L, nl, useLN, lum) : 0.0f;
-    res_add_sample(res, L, make_float4(nl.x, nl.y, nl.z, useLN ? 1.0f : 0.0f), lum, index, ph,
-                   valid ? prob : 1.0f, rng);
+    res_add_sample_u(res, L, make_float4(nl.x, nl.y, nl.z, useLN ? 1.0f : 0.0f), lum, index, ph,
+                     valid ? prob : 1.0f, u);
+}
+
+// aliasTableSample's column, restirOmni.glsl:73-83
+__device__ __forceinline__ int32_t alias_column(const SceneView &sc, float r1) {
+    const int32_t col = static_cast<int32_t>(static_cast<float>(sc.nAlias) * r1);
+    return col < sc.nAlias - 1 ? col : sc.nAlias - 1;
 }
 
 // Initial RIS over M candidates (alias table or the learned proposal's list).
@@ -189,27 +196,45 @@ __device__ __forceinline__ void pass_a_ris(const PassAArgs &a, const PassASurfac
     res = res_zero();
     const SceneView &sc = a.sc;
     const bool standard = (a.u.flags & RESTIR_STANDARD_WEIGHTS_FLAG) != 0;
+    const uint32_t M = a.u.initialLightSampleCount;
     if (dot(n, n) != 0.0f && a.candLight) {
         // RIS over the learned proposal's candidates: weight p_hat / q with the proposal's own
         // density q; a candidate from an empty cluster (l < 0) has weight 0 but still counts in M.
         // Candidates index the lights the alias table indexes (the union: point lights first).
-        const uint32_t M = a.u.initialLightSampleCount;
         const size_t npx = static_cast<size_t>(W) * H;
         for (uint32_t i = 0; i < M; ++i) {
             const int32_t li = a.candLight[i * npx + idx];
             const float q = a.candQ[i * npx + idx];
             const bool valid = li >= 0 && li < sc.nLights && q > 0.0f;
             const bool tri = li >= sc.nPoints;
-            add_candidate(res, s, pos, sc, valid, tri, tri ? li - sc.nPoints : li, q, rng, standard);
+            float ra = 0.0f, rb = 0.0f;
+            if (valid && tri) {
+                ra = pcg_float(rng);
+                rb = pcg_float(rng);
+            }
+            const float u = pcg_float(rng);
+            add_candidate(res, s, pos, sc, valid, tri, tri ? li - sc.nPoints : li, q, ra, rb, u, standard);
         }
-    } else if (dot(n, n) != 0.0f) {
-        const uint32_t M = a.u.initialLightSampleCount;
+    } else if (dot(n, n) != 0.0f && M > 0) {
+        // A candidate draws r1, r2 (the alias column and its coin), then a triangle light's two
+        // point draws, then the reservoir update's: 3 draws, or 5 after a triangle light. Each
+        // state is reached by jumping ahead from the candidate's first one, so the next candidate's
+        // column (where it starts if this one is a point light) is loaded before this candidate
+        // is evaluated, taking the column load off the chain from one candidate to the next; after
+        // a triangle light the column is loaded again from the right state. Same draws in the
+        // same order as drawing them one by one.
+        const uint64_t inc = rng.inc;
+        const uint64_t c3 = inc * pcg_mul_sum(3);
+        uint64_t st = rng.state;
+        int32_t col = alias_column(sc, pcg_float_at(st));
+        float4 c = sc.alias[col];
         for (uint32_t i = 0; i < M; ++i) {
-            const float r1 = pcg_float(rng), r2 = pcg_float(rng);
+            const uint64_t s3 = st * pcg_mul_pow(3) + c3;
+            int32_t colN = alias_column(sc, pcg_float_at(s3));
+            float4 cN = sc.alias[colN];
+            const uint64_t s1 = st * kPcgMul + inc, s2 = s1 * kPcgMul + inc;
+            const float r2 = pcg_float_at(s1);
             // aliasTableSample, restirOmni.glsl:73-83
-            int32_t col = static_cast<int32_t>(static_cast<float>(sc.nAlias) * r1);
-            col = col < sc.nAlias - 1 ? col : sc.nAlias - 1;
-            const float4 c = sc.alias[col];
             int32_t li;
             float prob;
             if (c.x > r2) {
@@ -222,8 +247,27 @@ __device__ __forceinline__ void pass_a_ris(const PassAArgs &a, const PassASurfac
             // restirOmni.glsl:117: point lights if there are any, else triangle lights; a union
             // table names point lights first
             const bool tri = sc.mixed ? li >= sc.nPoints : sc.nPoints == 0;
-            add_candidate(res, s, pos, sc, true, tri, sc.mixed && tri ? li - sc.nPoints : li, prob, rng, standard);
+            float ra = 0.0f, rb = 0.0f, u;
+            uint64_t next;
+            if (tri) {
+                const uint64_t s4 = s3 * kPcgMul + inc;
+                ra = pcg_float_at(s2);
+                rb = pcg_float_at(s3);
+                u = pcg_float_at(s4);
+                next = s4 * kPcgMul + inc;
+                colN = alias_column(sc, pcg_float_at(next));
+                cN = sc.alias[colN];
+            } else {
+                u = pcg_float_at(s2);
+                next = s3;
+            }
+            add_candidate(res, s, pos, sc, true, tri, sc.mixed && tri ? li - sc.nPoints : li, prob, ra, rb, u,
+                          standard);
+            st = next;
+            col = colN;
+            c = cN;
         }
+        rng.state = st;
     }
     // the standard RIS contribution weight instead of the selection-time one
     if (standard && res.w > 0.0f) res.w = res.sumW / (static_cast<float>(res.M) * res.pHat);

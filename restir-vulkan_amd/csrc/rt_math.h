@@ -44,12 +44,31 @@ RT_HD float gmix(float x, float y, float a) { return x * (1.0f - a) + y * a; }
 struct Pcg {
     uint64_t state, inc;
 };
-RT_HD uint32_t pcg_next(Pcg &r) {
-    const uint64_t old = r.state;
-    r.state = old * 6364136223846793005ull + r.inc;
+constexpr uint64_t kPcgMul = 6364136223846793005ull;
+// The number drawn at state `old` (the output permutation; the state then steps to old * a + inc).
+RT_HD uint32_t pcg_output(uint64_t old) {
     const uint32_t xs = static_cast<uint32_t>(((old >> 18u) ^ old) >> 27u);
     const uint32_t rot = static_cast<uint32_t>(old >> 59u);
     return (xs >> rot) | (xs << ((0u - rot) & 31u));
+}
+RT_HD uint32_t pcg_next(Pcg &r) {
+    const uint64_t old = r.state;
+    r.state = old * kPcgMul + r.inc;
+    return pcg_output(old);
+}
+// Jump-ahead: k steps after state s the state is a^k s + inc (1 + a + ... + a^(k-1)) (mod 2^64).
+constexpr uint64_t pcg_mul_pow(int k) {
+    uint64_t m = 1;
+    for (int i = 0; i < k; ++i) m *= kPcgMul;
+    return m;
+}
+constexpr uint64_t pcg_mul_sum(int k) {
+    uint64_t sum = 0, m = 1;
+    for (int i = 0; i < k; ++i) {
+        sum += m;
+        m *= kPcgMul;
+    }
+    return sum;
 }
 RT_HD Pcg pcg_seed(uint64_t seed, uint64_t seq) {
     Pcg r{0ull, (seq << 1u) | 1u};
@@ -60,6 +79,8 @@ RT_HD Pcg pcg_seed(uint64_t seed, uint64_t seq) {
 }
 // u32 -> float rounds to nearest even; the scale by 2^-32 is exact.
 RT_HD float pcg_float(Pcg &r) { return static_cast<float>(pcg_next(r)) * 2.3283064365386963e-10f; }
+// The float pcg_float draws at state s.
+RT_HD float pcg_float_at(uint64_t s) { return static_cast<float>(pcg_output(s)) * 2.3283064365386963e-10f; }
 
 // ---------------------------------------------------------------------------------- sin/cos
 RT_HD void sincos_f(float xf, float &s_out, float &c_out) {
@@ -241,11 +262,12 @@ RT_HD Res res_zero() {
     return r;
 }
 // updateReservoirAt, reservoir.glsl:6-26
-RT_HD void res_update(Res &r, float weight, f3 pos, float4 n, float emLum, int32_t li, float pHat,
-                      float w, Pcg &rng) {
+// u: the update's uniform draw
+RT_HD void res_update_u(Res &r, float weight, f3 pos, float4 n, float emLum, int32_t li, float pHat,
+                        float w, float u) {
     r.sumW += weight;
     const float rp = weight / r.sumW;
-    if (pcg_float(rng) < rp) {
+    if (u < rp) {
         r.pos = pos;
         r.emLum = emLum;
         r.normal = n;
@@ -254,13 +276,21 @@ RT_HD void res_update(Res &r, float weight, f3 pos, float4 n, float emLum, int32
         r.w = w;
     }
 }
-// addSampleToReservoir, reservoir.glsl:28-42
-RT_HD void res_add_sample(Res &r, f3 pos, float4 n, float emLum, int32_t li, float pHat,
-                          float sampleP, Pcg &rng) {
+RT_HD void res_update(Res &r, float weight, f3 pos, float4 n, float emLum, int32_t li, float pHat,
+                      float w, Pcg &rng) {
+    res_update_u(r, weight, pos, n, emLum, li, pHat, w, pcg_float(rng));
+}
+// addSampleToReservoir, reservoir.glsl:28-42 (u: the update's uniform draw)
+RT_HD void res_add_sample_u(Res &r, f3 pos, float4 n, float emLum, int32_t li, float pHat,
+                            float sampleP, float u) {
     const float weight = pHat / sampleP;
     r.M += 1u;
     const float w = (r.sumW + weight) / (static_cast<float>(r.M) * pHat);
-    res_update(r, weight, pos, n, emLum, li, pHat, w, rng);
+    res_update_u(r, weight, pos, n, emLum, li, pHat, w, u);
+}
+RT_HD void res_add_sample(Res &r, f3 pos, float4 n, float emLum, int32_t li, float pHat,
+                          float sampleP, Pcg &rng) {
+    res_add_sample_u(r, pos, n, emLum, li, pHat, sampleP, pcg_float(rng));
 }
 // combineReservoirs, reservoir.glsl:44-64
 RT_HD void res_combine(Res &self, const Res &o, float pHat, Pcg &rng) {

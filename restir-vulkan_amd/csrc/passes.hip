@@ -548,7 +548,7 @@ struct CQueues {
     uint32_t *origM;       // per pixel
     uint32_t *bits;        // per pixel: above-horizon neighbour mask (bits 0..15), flags below
     uint32_t *occ;         // per pixel: occluded neighbour mask
-    uint32_t *nM, *nIdx;   // [slot * npx + pixel]: the neighbour's sample count and pixel
+    uint32_t *nM;          // [slot * npx + pixel]: the neighbour's sample count
     float4 *surf;          // per pixel: {position.xyz, normal.x}, {normal.yz, 0, 0}
     uint32_t npx;
 };
@@ -587,6 +587,10 @@ __global__ __launch_bounds__(256) void k_pack_surface(const GView g, uint32_t np
 
 template <int TX, int TY>
 __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, const CQueues q) {
+    constexpr uint32_t kT = TX * TY;
+    // neighbour pixel of slot j (j < k <= 16) of thread t at [j * kT + t]: read back by the same
+    // wave only (its horizon tests and ray stores), which sees its own LDS writes in order
+    __shared__ uint32_t nbr[16 * kT];
     uint32_t x, y;
     tile_pixel<TX, TY>(a.tilesX, a.nTiles, x, y);
     const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
@@ -607,19 +611,28 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
         } else {
             const uint32_t origM = res.M;
             pass_c_merge(a, x, y, idx, pos, n, res, [&](int i, uint32_t ni, uint32_t m) {
-                q.nIdx[static_cast<size_t>(i) * q.npx + idx] = ni;
+                nbr[static_cast<uint32_t>(i) * kT + threadIdx.x] = ni;
                 q.nM[static_cast<size_t>(i) * q.npx + idx] = m;
             });
             lp = res.pos;
             uint32_t numSamples = origM;
-            for (int j = 0; j < a.k; ++j) {
-                const uint32_t ni = q.nIdx[static_cast<size_t>(j) * q.npx + idx];
+            // which neighbours see the merged sample above their horizon; unrolled for the usual
+            // neighbour count, so that the surface loads of all slots are in flight together
+            const auto horizon = [&](int j) {
+                const uint32_t ni = nbr[static_cast<uint32_t>(j) * kT + threadIdx.x];
                 const float4 s0 = q.surf[2 * static_cast<size_t>(ni)], s1 = q.surf[2 * static_cast<size_t>(ni) + 1];
                 const f3 nwp{s0.x, s0.y, s0.z}, nn{s0.w, s1.x, s1.y};
-                if (dot(lp - nwp, nn) < 0.0f) continue;
-                above |= 1u << j;
-                if (!vis) numSamples += q.nM[static_cast<size_t>(j) * q.npx + idx];
+                above |= dot(lp - nwp, nn) < 0.0f ? 0u : 1u << j;
+            };
+            if (a.k == 5) {
+#pragma unroll
+                for (int j = 0; j < 5; ++j) horizon(j);
+            } else {
+                for (int j = 0; j < a.k; ++j) horizon(j);
             }
+            if (!vis)
+                for (uint32_t m = above; m; m &= m - 1)
+                    numSamples += q.nM[static_cast<size_t>(__builtin_ctz(m)) * q.npx + idx];
             if (!vis)
                 finish_w(res.w, res.sumW, res.pHat, numSamples, res.M,
                          (a.u.flags & RESTIR_STANDARD_WEIGHTS_FLAG) != 0);
@@ -672,7 +685,7 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
             uint32_t m = oWant;
             for (uint32_t k = r - oExcl; k > 0; --k) m &= m - 1;  // the k-th set bit of the owner
             const uint32_t j = static_cast<uint32_t>(__ffs(static_cast<int>(m)) - 1);
-            const float4 sp = q.surf[2 * static_cast<size_t>(q.nIdx[static_cast<size_t>(j) * q.npx + oIdx])];
+            const float4 sp = q.surf[2 * static_cast<size_t>(nbr[j * kT + (threadIdx.x & ~63u) + o])];
             q.n1[base + r] = make_float4(sp.x, sp.y, sp.z, __uint_as_float(oIdx | (j << 27)));
         }
     }
@@ -1391,7 +1404,6 @@ static CQueues carve_queues(void *scratch, uint32_t npx, int k, size_t *bytes = 
     q.bits = c.take<uint32_t>(sizeof(uint32_t) * n);
     q.occ = c.take<uint32_t>(sizeof(uint32_t) * n);
     q.nM = c.take<uint32_t>(sizeof(uint32_t) * n * kk);
-    q.nIdx = c.take<uint32_t>(sizeof(uint32_t) * n * kk);
     q.surf = c.take<float4>(2 * sizeof(float4) * n);
     q.npx = npx;
     if (bytes) *bytes = c.end();

@@ -191,11 +191,12 @@ int restir_create(restir_ctx **out, int device) {
     if (device < 0 || device >= n) return RESTIR_E_NO_DEVICE;
     auto *ctx = new restir_ctx();
     ctx->device = device;
-    if (use_device(ctx) != RESTIR_OK || hipMalloc(&ctx->counters, 256) != hipSuccess ||
+    if (use_device(ctx) != RESTIR_OK || hipMalloc(&ctx->counters, rk::kCounterBytes) != hipSuccess ||
         resolve_noalloc_scratch(ctx) != RESTIR_OK) {
         delete ctx;
         return RESTIR_E_HIP;
     }
+    ctx->scene.traceHeads = ctx->counters + 64;  // past the 256 bytes of single counters
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
         ctx->cus = static_cast<uint32_t>(cus);

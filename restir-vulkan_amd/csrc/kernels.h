@@ -38,7 +38,14 @@ struct SceneView {
     const rt::WNode *wide;  // the 4-entry nodes over `nodes` (same indices), built by the library
     int32_t treeNested;     // 1: every box finite and inside the box stored for its parent
     float maxCoord;         // largest |box coordinate| of the tree
+    uint32_t *traceHeads;   // the queue tracer's work counters (the context's): kTraceShards words,
+                            // kTraceShardStride apart, cleared by each tracer launch
 };
+// The queue tracer's rays are taken in chunks of kTraceChunk through kTraceShards counters, each in
+// a 128-byte line of its own: device-wide atomics on one address are served one at a time (about
+// 11 ns each on the MI355X), on different lines in parallel.
+constexpr uint32_t kTraceShards = 64, kTraceShardStride = 32, kTraceChunk = 64;
+constexpr size_t kCounterBytes = 256 + sizeof(uint32_t) * kTraceShards * kTraceShardStride;
 
 struct GView {
     const float *pos;
@@ -108,6 +115,8 @@ hipError_t launch_trace(int stack, bool refLoop, bool persistent, uint32_t gridC
 
 // clears n words (a kernel rather than hipMemsetAsync, so the sequence can be graph-captured)
 __global__ void k_zero_u32(uint32_t *p, uint32_t n);
+// clears n < 64 words of p and the queue tracer's counters (one workgroup of 64 lanes)
+__global__ void k_zero_counters(uint32_t *p, uint32_t n, uint32_t *heads);
 
 // ---- batched views: one persistent launch over the tiles of n views (each view its own
 // uniforms, scene tables, G-buffers and reservoirs); `table` holds views_table_bytes(n) of

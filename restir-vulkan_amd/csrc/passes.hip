@@ -539,7 +539,7 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c(const PassCArgs a) {
 // slots handed out by prefix sums over the wave, so a queue holds only live rays in pixel-tile
 // order.
 struct CQueues {
-    uint32_t *counter;     // [0] centre rays queued, [1] neighbour rays queued, [3] their tracer's head
+    uint32_t *counter;     // [0] centre rays queued, [1] neighbour rays queued
     float4 *c1;            // centre rays: the pixel's position, .w = pixel | kCentreTag; the other
                            // end is the light sample of `merged[pixel]`
     float4 *n1;            // neighbour rays: the neighbour's position, .w = pixel | slot << 27; the
@@ -716,14 +716,22 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_finish(const PassCArgs a, con
     *r = v;
 }
 
+// The set bits of a wave mask below the calling lane.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
 // Tracer over a queue whose length a previous kernel left in *count. A fixed grid of workgroups
-// (as many as fit the device at once); each wave takes chunks of consecutive rays from a global
-// head, one atomic per chunk, and hands a chunk's rays to its idle lanes in queue order whenever
-// kTraceRefill or more lanes are idle. Queue order keeps the rays of neighbouring pixels
-// together, which walk much of the same tree (the L1 serves most node fetches); the refills keep
-// lanes busy when a wave's rays end at different depths. Chunks shrink as the queue drains
-// (guided self-scheduling: the rays left when the wave last took a chunk, over twice the grid's
-// waves, between kTraceMinChunk and kTraceMaxChunk), so that waves finish close together.
+// (as many as fit the device at once); each wave takes chunks of kTraceChunk consecutive rays and
+// hands a chunk's rays to its idle lanes in queue order whenever kTraceRefill or more lanes are
+// idle. Queue order keeps the rays of neighbouring pixels together, which walk much of the same
+// tree (the L1 serves most node fetches); the refills keep lanes busy when a wave's rays end at
+// different depths. Chunk c belongs to shard c mod kTraceShards, whose counter (sc.traceHeads, one
+// 128-byte line per shard) hands out its chunks in order, one atomic per chunk: a wave starts at
+// its own shard and moves on to the next one when that is exhausted, so the atomics of the grid
+// spread over all the lines (atomics on one line are served one at a time) while the chunks in
+// flight stay near the front of the queue.
 //
 // Triangle tests are gathered per wave: a visit appends the triangles whose entries pass to the
 // wave's LDS ring (triangle index and owner lane), and the wave tests the oldest 64, one per
@@ -739,20 +747,14 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_finish(const PassCArgs a, con
 // pixel's centre ray is already known to be shadowed (the centre rays come first in the index
 // space, so that holds for nearly all of them; tracing the others too changes nothing).
 // kTraceBatch: nBatch rays p1[i] -> p2[i] (restir_trace_shadow); out[i] gets 1 if shadowed, else 0.
-// The set bits of a wave mask below the calling lane.
-__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
-                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
-}
-
-constexpr uint32_t kTraceMaxChunk = 128, kTraceMinChunk = 32, kTraceRefill = 32;
+constexpr uint32_t kTraceRefill = 32;
 constexpr uint32_t kTriList = kTraceListEntries, kTriRoundWalkers = 32;
 constexpr int kTracePassA = 0, kTracePassC = 1, kTraceBatch = 2;
 template <int STACK, int MODE>
 __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const float4 *__restrict__ p1,
                                                      const float4 *__restrict__ p2,
-                                                     const uint32_t *__restrict__ count, uint32_t nBatch,
-                                                     uint32_t *__restrict__ head, CQueues q, uint32_t *__restrict__ out) {
+                                                     const uint32_t *__restrict__ count, uint32_t nBatch, CQueues q,
+                                                     uint32_t *__restrict__ out) {
     constexpr bool PER_PIXEL = MODE == kTracePassA, BATCH = MODE == kTraceBatch;
     __shared__ int32_t stack[STACK * 256];
     __shared__ uint32_t triList[4 * kTriList];
@@ -773,26 +775,39 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
     uint32_t waiting = 0;        // this lane's tests in the wave's list
     uint32_t first = 0, listed = 0;  // wave-uniform: the ring's oldest entry and its length
     uint32_t next = 0, end = 0;  // wave-uniform: the rays of the wave's chunk not handed out yet
-    uint32_t seen = 0;           // wave-uniform: the head just after this wave's last chunk
-    bool drained = false;        // wave-uniform: the head has passed the last ray
-    const uint32_t nWaves = gridDim.x * (256u / 64u);
+    const uint32_t nChunks = (n + kTraceChunk - 1) / kTraceChunk;
+    uint32_t shard = (blockIdx.x * 4u + (threadIdx.x >> 6)) % kTraceShards;  // wave-uniform
+    uint32_t spent = 0;          // wave-uniform: shards this wave found exhausted
+    bool drained = false;        // wave-uniform: every shard is exhausted
     for (;;) {
         const uint64_t idle = __ballot(ray < 0);
         const uint32_t nIdle = static_cast<uint32_t>(__popcll(idle));
         if (nIdle >= kTraceRefill) {
             if (next >= end && !drained) {
-                const uint32_t left = n > seen ? n - seen : 0u;
-                uint32_t size = left / (2u * nWaves);
-                size = size < kTraceMinChunk ? kTraceMinChunk : (size > kTraceMaxChunk ? kTraceMaxChunk : size);
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(head, size);
-                // every lane is active here: the first active lane is lane 0, and the value stays
-                // in a scalar register, so the loop's exit test is seen as wave-uniform
-                base = __builtin_amdgcn_readfirstlane(base);
-                seen = base + size;
-                next = base < n ? base : n;
-                end = n - next < size ? n : next + size;
-                drained = base + size >= n;
+                // every lane is active here: the first active lane is lane 0, and the values stay
+                // in scalar registers, so the loop's tests are seen as wave-uniform
+                for (;;) {
+                    uint32_t *h = sc.traceHeads + shard * kTraceShardStride;
+                    // a shard already past its last chunk is passed over without an atomic
+                    uint32_t t = __builtin_amdgcn_readfirstlane(
+                        __hip_atomic_load(h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                    if (t * kTraceShards + shard < nChunks) {
+                        uint32_t v = 0;
+                        if (lane == 0) v = atomicAdd(h, 1u);
+                        t = __builtin_amdgcn_readfirstlane(v);
+                    }
+                    const uint32_t c = t * kTraceShards + shard;
+                    if (c < nChunks) {
+                        next = c * kTraceChunk;
+                        end = n - next < kTraceChunk ? n : next + kTraceChunk;
+                        break;
+                    }
+                    if (++spent == kTraceShards) {
+                        drained = true;
+                        break;
+                    }
+                    shard = shard + 1 == kTraceShards ? 0u : shard + 1;
+                }
             }
             if (next < end) {
                 if (ray < 0) {
@@ -886,7 +901,7 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
 // scratch, and the ray queued; (2) the persistent tracer marks shadowed pixels; (3) per pixel:
 // the shadow verdict, temporal reuse, store. Same draws, same order, same result as k_pass_a.
 struct AQueues {
-    uint32_t *counter;  // [0] rays queued, [1] tracer head
+    uint32_t *counter;  // [0] rays queued
     float4 *p1, *p2;    // p1.w carries the pixel index
     uint64_t *rng;      // per pixel: the random stream's state after RIS
     uint32_t *occ;      // per pixel: kBitCentreShadowed once its ray hit
@@ -1085,6 +1100,10 @@ __global__ __launch_bounds__(256) void k_trace_pool(const TraceArgs a, uint32_t 
 __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
+}
+__global__ void k_zero_counters(uint32_t *p, uint32_t n, uint32_t *heads) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0u;
+    if (threadIdx.x < kTraceShards) heads[threadIdx.x * kTraceShardStride] = 0u;
 }
 
 // ---------------------------------------------------------------------- primary G-buffer --
@@ -1292,13 +1311,13 @@ template <int TX, int TY, int STACK>
 static hipError_t run_a_queued(PassAArgs args, const AQueues &q, uint32_t gridCap, hipStream_t s) {
     grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
     const dim3 tiles(args.nTiles), block(TX * TY);
-    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 2u);
+    RK_LAUNCH("k_zero_counters", k_zero_counters, dim3(1), dim3(64), 0, s, q.counter, 1u, args.sc.traceHeads);
     RK_LAUNCH("k_pass_a_ris", (k_pass_a_ris<TX, TY>), tiles, block, 0, s, args, q);
     if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
         CQueues cq{};
         cq.bits = q.occ;  // a shadowed ray sets kBitCentreShadowed in occ[pixel]
         RK_LAUNCH("k_trace_queue(pass_a)", (k_trace_queue<STACK, kTracePassA>), dim3(gridCap), dim3(256), 0, s,
-                  args.sc, q.p1, q.p2, q.counter + 0, 0u, q.counter + 1, cq, nullptr);
+                  args.sc, q.p1, q.p2, q.counter + 0, 0u, cq, nullptr);
     }
     RK_LAUNCH("k_pass_a_finish", (k_pass_a_finish<TX, TY>), tiles, block, 0, s, args, q);
     return hipGetLastError();
@@ -1369,13 +1388,13 @@ static hipError_t run_c_queued(PassCArgs args, const CQueues &q, uint32_t gridCa
     static_assert((kCentreTag & 0x07ffffffu) == 0 && (kCentreTag >> 27) > 15u, "the centre tag is no pixel or slot");
     grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
     const dim3 tiles(args.nTiles), block(TX * TY);
-    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, q.counter, 4u);
+    RK_LAUNCH("k_zero_counters", k_zero_counters, dim3(1), dim3(64), 0, s, q.counter, 2u, args.sc.traceHeads);
     const uint32_t npx = args.u.screenSize[0] * args.u.screenSize[1];
     RK_LAUNCH("k_pack_surface", k_pack_surface, dim3((npx + 255) / 256), dim3(256), 0, s, args.g, npx, q.surf);
     RK_LAUNCH("k_pass_c_merge", (k_pass_c_merge<TX, TY>), tiles, block, 0, s, args, q);
     if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
         RK_LAUNCH("k_trace_queue(pass_c)", (k_trace_queue<STACK, kTracePassC>), dim3(gridCap), dim3(256), 0, s,
-                  args.sc, nullptr, nullptr, q.counter + 0, 0u, q.counter + 3, q, nullptr);
+                  args.sc, nullptr, nullptr, q.counter + 0, 0u, q, nullptr);
         RK_LAUNCH("k_pass_c_finish", (k_pass_c_finish<TX, TY>), tiles, block, 0, s, args, q);
     }
     return hipGetLastError();
@@ -1682,12 +1701,12 @@ static void run_trace(bool persistent, uint32_t gridCap, uint32_t *counter, cons
         RK_LAUNCH("k_trace", (k_trace<STACK, REF>), dim3(blocks), dim3(256), 0, s, args);
         return;
     }
-    RK_LAUNCH("k_zero_u32", k_zero_u32, dim3(1), dim3(64), 0, s, counter, 1u);
+    RK_LAUNCH("k_zero_counters", k_zero_counters, dim3(1), dim3(64), 0, s, counter, 1u, args.sc.traceHeads);
     const uint32_t grid = blocks < gridCap ? blocks : gridCap;
     if (!REF && !args.visits && !args.tests) {
         // no counts asked for: the passes' queue tracer (gathered triangle tests)
         RK_LAUNCH("k_trace_queue(batch)", (k_trace_queue<STACK, kTraceBatch>), dim3(grid), dim3(256), 0, s, args.sc,
-                  args.p1, args.p2, nullptr, args.n, counter, CQueues{}, args.shadowed);
+                  args.p1, args.p2, nullptr, args.n, CQueues{}, args.shadowed);
         return;
     }
     RK_LAUNCH("k_trace_pool", (k_trace_pool<STACK, REF>), dim3(grid), dim3(256), 0, s, args, counter);

@@ -557,21 +557,44 @@ constexpr uint32_t kBitDone = 1u << 30;        // no visibility test: W already 
 constexpr uint32_t kBitCentreShadowed = 1u << 31;
 constexpr uint32_t kCentreTag = 1u << 31;  // a queued ray's tag: the pixel's centre ray
 
-// Appends `count` items for this lane to a queue: returns the lane's first slot. Every lane of
-// the wave must call it (count may be 0).
-__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, uint32_t count) {
-    const uint32_t lane = threadIdx.x & 63u;
-    uint32_t incl = count;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t t = __shfl_up(incl, off);
-        if (lane >= static_cast<uint32_t>(off)) incl += t;
+// The set bits of a wave mask below the calling lane.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
+// Reserves room in two queues for every wave of a workgroup of kWaves waves: w0 and w1 are this
+// wave's item counts (the same on all its lanes); the queues' counters are the low and high halves
+// of *counter. One 64-bit atomic per workgroup (device-wide atomics on one address are served one
+// at a time, about 11 ns each on the MI355X, so one per wave would bound the producing kernel);
+// returns this wave's first slot in each queue. Every thread of the workgroup must call it (it
+// holds two barriers); lds: kWaves + 1 words of the workgroup's shared memory.
+template <uint32_t kWaves>
+__device__ __forceinline__ uint2 group_reserve(unsigned long long *counter, uint32_t w0, uint32_t w1, uint2 *lds) {
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0) lds[wave] = make_uint2(w0, w1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s0 = 0, s1 = 0;
+        for (uint32_t v = 0; v < kWaves; ++v) {
+            const uint2 t = lds[v];
+            lds[v] = make_uint2(s0, s1);
+            s0 += t.x;
+            s1 += t.y;
+        }
+        unsigned long long old = 0;
+        if (s0 | s1) old = atomicAdd(counter, (static_cast<unsigned long long>(s1) << 32) | s0);
+        lds[kWaves] = make_uint2(static_cast<uint32_t>(old), static_cast<uint32_t>(old >> 32));
     }
-    const uint32_t total = __shfl(incl, 63);
-    uint32_t base = 0;
-    if (lane == 0 && total > 0) base = atomicAdd(counter, total);
-    base = __shfl(base, 0);
-    return base + incl - count;
+    __syncthreads();
+    const uint2 off = lds[wave], base = lds[kWaves];
+    return make_uint2(base.x + off.x, base.y + off.y);
+}
+
+// The lane's rank among the wave's lanes with `emit` set, and their number.
+__device__ __forceinline__ uint2 wave_rank(bool emit) {
+    const uint64_t m = __ballot(emit);
+    return make_uint2(lanes_below(m), static_cast<uint32_t>(__popcll(m)));
 }
 
 // Position and normal of every pixel in one 32-byte record: the merge's horizon test and the
@@ -643,15 +666,13 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
             emit = vis;
         }
     }
-    // the centre re-test (unbiasedReuse.glsl:157-166)
-    const uint32_t slot = wave_append(&q.counter[0], emit ? 1u : 0u);
-    if (emit) q.c1[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(static_cast<uint32_t>(idx) | kCentreTag));
-    // One neighbour ray per above-horizon neighbour. The tracer skips those of a pixel whose
-    // centre ray it found shadowed: a shadowed centre zeroes Z whatever the neighbours give
-    // (unbiasedReuse.glsl:157-166) and the neighbour loop draws no random numbers. The wave's rays
-    // are written slot-major: lane i writes rays i, i + 64, ... of the wave's run, finding the
-    // pixel that owns a ray by binary search over the wave's prefix sums, so each store
-    // instruction covers consecutive queue entries.
+    // The centre re-test (unbiasedReuse.glsl:157-166), and one neighbour ray per above-horizon
+    // neighbour. The tracer skips the neighbour rays of a pixel whose centre ray it found shadowed:
+    // a shadowed centre zeroes Z whatever the neighbours give (unbiasedReuse.glsl:157-166) and the
+    // neighbour loop draws no random numbers. Both queues' room is reserved per workgroup. The
+    // wave's neighbour rays are written slot-major: lane i writes rays i, i + 64, ... of the wave's
+    // run, finding the pixel that owns a ray by binary search over the wave's prefix sums, so each
+    // store instruction covers consecutive queue entries.
     const uint32_t want = emit ? above : 0u;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t cnt = static_cast<uint32_t>(__popc(want));
@@ -663,10 +684,14 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
     }
     const uint32_t excl = incl - cnt;
     const uint32_t total = __shfl(incl, 63);
+    const uint2 rank = wave_rank(emit);
+    __shared__ uint2 reserve[kT / 64 + 1];
+    const uint2 first = group_reserve<kT / 64>(reinterpret_cast<unsigned long long *>(q.counter), rank.y, total, reserve);
+    if (emit)
+        q.c1[first.x + rank.x] =
+            make_float4(pos.x, pos.y, pos.z, __uint_as_float(static_cast<uint32_t>(idx) | kCentreTag));
     if (total == 0) return;  // uniform: the whole wave leaves
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&q.counter[1], total);
-    base = __shfl(base, 0);
+    const uint32_t base = first.y;
     const uint32_t myIdx = static_cast<uint32_t>(idx);
     for (uint32_t r0 = 0; r0 < total; r0 += 64) {  // every lane runs every round
         const uint32_t r = r0 + lane;
@@ -714,12 +739,6 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_finish(const PassCArgs a, con
     const uint32_t M = standard ? __float_as_uint(r[1].x) : 0u;  // numStreamSamples
     finish_w(v.w, v.z, v.y, numSamples, M, standard);
     *r = v;
-}
-
-// The set bits of a wave mask below the calling lane.
-__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
-                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
 }
 
 // Tracer over a queue whose length a previous kernel left in *count. A fixed grid of workgroups
@@ -928,7 +947,10 @@ __global__ __launch_bounds__(TX *TY) void k_pass_a_ris(const PassAArgs a, const 
         emit = (a.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) && res.sumW != 0.0f;
         lp = res.pos;
     }
-    const uint32_t slot = wave_append(&q.counter[0], emit ? 1u : 0u);
+    __shared__ uint2 reserve[TX * TY / 64 + 1];
+    const uint2 rank = wave_rank(emit);
+    const uint32_t slot = group_reserve<TX * TY / 64>(reinterpret_cast<unsigned long long *>(q.counter), rank.y, 0u,
+                                                      reserve).x + rank.x;
     if (emit) {
         q.p1[slot] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(static_cast<uint32_t>(idx)));
         q.p2[slot] = make_float4(lp.x, lp.y, lp.z, 0.0f);
@@ -1311,7 +1333,7 @@ template <int TX, int TY, int STACK>
 static hipError_t run_a_queued(PassAArgs args, const AQueues &q, uint32_t gridCap, hipStream_t s) {
     grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
     const dim3 tiles(args.nTiles), block(TX * TY);
-    RK_LAUNCH("k_zero_counters", k_zero_counters, dim3(1), dim3(64), 0, s, q.counter, 1u, args.sc.traceHeads);
+    RK_LAUNCH("k_zero_counters", k_zero_counters, dim3(1), dim3(64), 0, s, q.counter, 2u, args.sc.traceHeads);
     RK_LAUNCH("k_pass_a_ris", (k_pass_a_ris<TX, TY>), tiles, block, 0, s, args, q);
     if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
         CQueues cq{};

@@ -162,9 +162,13 @@ class Frames:
         else:
             ctx.initial_temporal(u, g, gp, self.res[p], first, self.launch_a, s)
         src = first
+        # the second Pass C reads the first's visibility-tested output: it does not retrace the
+        # segments found unoccluded there (RESTIR_INPUT_VISIBLE_FLAG; the same results)
+        later = self.abi.Uniforms.from_buffer_copy(u)
+        later.flags |= self.abi.INPUT_VISIBLE_FLAG
         for j in range(n_sp):
             dst = self.res[i] if (n_sp - j) % 2 == 1 else self.tmp
-            ctx.unbiased(u, self.args.k, j, g, src, dst, self.launch, s)
+            ctx.unbiased(later if j > 0 else u, self.args.k, j, g, src, dst, self.launch, s)
             yield j
             src = dst
         ctx.shade(self.cams[f]["position"], 1.0, g, self.res[i], self.rgb, self.W * self.H, s)

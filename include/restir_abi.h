@@ -81,6 +81,15 @@ typedef struct restir_uniforms {
  * p / area (uniform by area) instead of p / (|cos| area) (restirOmni.glsl:129-131), whose
  * estimator converges to the integral of f |cos_light| rather than of f. */
 #define RESTIR_STANDARD_WEIGHTS_FLAG (1 << 2)
+/* No counterpart in the reference: a promise about a Pass C's input, read by restir_unbiased only.
+ * Every input reservoir with W > 0 holds a sample whose segment from its own pixel was found
+ * unoccluded, as the output of a restir_unbiased with RESTIR_VISIBILITY_REUSE_FLAG over the same
+ * G-buffer guarantees (its centre re-test leaves W = 0 otherwise): the second and later Pass C of a
+ * frame may set it. The pass then does not trace the rays that retrace such a segment: the
+ * centre ray when the merged sample is the pixel's own, and the neighbour rays from the pixel
+ * whose reservoir supplied the merged sample. The results are the same as without the flag when
+ * the promise holds; when it does not, those rays count as unoccluded. */
+#define RESTIR_INPUT_VISIBLE_FLAG (1 << 3)
 
 /* AabbTreeNode, src/shaders/include/structs/aabbTree.glsl:1-8 (80 B). Child < 0 is leaf ~triIndex. */
 typedef struct restir_aabb_tree_node {
@@ -186,7 +195,7 @@ void restir_destroy(restir_ctx *ctx);
 const char *restir_last_error(const restir_ctx *ctx);
 /* ABI revision; bumped whenever an entry point's signature or documented behaviour changes. Bindings compare it with the
  * revision they were written against and refuse a mismatched library. */
-#define RESTIR_ABI_VERSION 16
+#define RESTIR_ABI_VERSION 17
 int restir_version(void);
 
 /* Light tables: device pointers to the reference buffer images (header + array). The counts are

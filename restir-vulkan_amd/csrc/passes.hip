@@ -418,10 +418,13 @@ struct PassCArgs {
 // The neighbour merge of unbiasedReuse.glsl:84-124 for pixel (x, y): k random neighbours in a
 // disk, each merged into `res` with the target evaluated at this pixel. Records each
 // neighbour's pixel index and sample count for the later Z computation.
-// `keep(i, ni, M)` records neighbour i's pixel and sample count for the visibility step.
+// `keep(i, ni, M)` records neighbour i's pixel and sample count for the visibility step; *src
+// gets the pixel whose input reservoir supplied the merged sample (this pixel when no neighbour's
+// was taken), or kNoPixel when that is this pixel's own input and its W is 0.
+constexpr uint32_t kNoPixel = 0xffffffffu;
 template <typename Keep>
 __device__ __forceinline__ void pass_c_merge(const PassCArgs &a, uint32_t x, uint32_t y, size_t idx,
-                                             f3 pos, f3 n, Res &res, Keep keep) {
+                                             f3 pos, f3 n, Res &res, Keep keep, uint32_t *src = nullptr) {
     const uint32_t W = a.u.screenSize[0], H = a.u.screenSize[1];
     const GView &g = a.g;
     const f3 alb = plane3(g.alb, idx, g.P);
@@ -431,6 +434,7 @@ __device__ __forceinline__ void pass_c_merge(const PassCArgs &a, uint32_t x, uin
     Pcg rng = pcg_seed(static_cast<uint64_t>(a.seed), static_cast<uint64_t>(y * 10007u + x));
     const float R = a.u.spatialRadius;
     const int k = a.k;
+    uint32_t from = res.w > 0.0f ? static_cast<uint32_t>(idx) : kNoPixel;
     for (int i = 0; i < k; ++i) {
         const float angle = pcg_float(rng) * 2.0f * kPi;
         const float radius = sqrtf(pcg_float(rng)) * R;
@@ -446,8 +450,11 @@ __device__ __forceinline__ void pass_c_merge(const PassCArgs &a, uint32_t x, uin
         res.M += rr.M;
         const float ph = phat_of(s, rr);
         const float weight = ph * rr.w * static_cast<float>(rr.M);
-        if (weight > 0.0f) res_update(res, weight, rr.pos, rr.normal, rr.emLum, rr.lightIndex, ph, rr.w, rng);
+        if (weight > 0.0f && res_update_u(res, weight, rr.pos, rr.normal, rr.emLum, rr.lightIndex, ph, rr.w,
+                                          pcg_float(rng)))
+            from = ni;
     }
+    if (src) *src = from;
 }
 
 // W of a merged reservoir from the count Z of the reservoirs that could have produced its sample
@@ -620,8 +627,12 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
     const bool inside = x < W && y < H;
     const bool vis = (a.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) != 0;
     const size_t idx = inside ? static_cast<size_t>(y) * W + x : 0;
-    bool emit = false;
-    uint32_t above = 0;  // above-horizon neighbour slots
+    // RESTIR_INPUT_VISIBLE_FLAG: a ray from the pixel whose input reservoir supplied the merged
+    // sample retraces the segment that pixel's own centre re-test found unoccluded: not queued
+    const bool known = (a.u.flags & RESTIR_INPUT_VISIBLE_FLAG) != 0;
+    bool emit = false;    // the centre ray is queued
+    uint32_t above = 0;   // above-horizon neighbour slots
+    uint32_t queued = 0;  // the slots whose rays are queued
     f3 pos{0.0f, 0.0f, 0.0f}, lp{0.0f, 0.0f, 0.0f};
     if (inside) {
         const GView &g = a.g;
@@ -633,10 +644,15 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
             q.bits[idx] = kBitBackground;
         } else {
             const uint32_t origM = res.M;
-            pass_c_merge(a, x, y, idx, pos, n, res, [&](int i, uint32_t ni, uint32_t m) {
-                nbr[static_cast<uint32_t>(i) * kT + threadIdx.x] = ni;
-                q.nM[static_cast<size_t>(i) * q.npx + idx] = m;
-            });
+            uint32_t src = kNoPixel;
+            pass_c_merge(
+                a, x, y, idx, pos, n, res,
+                [&](int i, uint32_t ni, uint32_t m) {
+                    nbr[static_cast<uint32_t>(i) * kT + threadIdx.x] = ni;
+                    q.nM[static_cast<size_t>(i) * q.npx + idx] = m;
+                },
+                &src);
+            if (!known) src = kNoPixel;
             lp = res.pos;
             uint32_t numSamples = origM;
             // which neighbours see the merged sample above their horizon; unrolled for the usual
@@ -645,7 +661,9 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
                 const uint32_t ni = nbr[static_cast<uint32_t>(j) * kT + threadIdx.x];
                 const float4 s0 = q.surf[2 * static_cast<size_t>(ni)], s1 = q.surf[2 * static_cast<size_t>(ni) + 1];
                 const f3 nwp{s0.x, s0.y, s0.z}, nn{s0.w, s1.x, s1.y};
-                above |= dot(lp - nwp, nn) < 0.0f ? 0u : 1u << j;
+                const uint32_t bit = dot(lp - nwp, nn) < 0.0f ? 0u : 1u << j;
+                above |= bit;
+                queued |= ni == src ? 0u : bit;
             };
             if (a.k == 5) {
 #pragma unroll
@@ -663,7 +681,8 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
             q.origM[idx] = origM;
             q.occ[idx] = 0u;
             q.bits[idx] = above | (vis ? 0u : kBitDone);
-            emit = vis;
+            emit = vis && src != static_cast<uint32_t>(idx);
+            if (!vis) queued = 0u;
         }
     }
     // The centre re-test (unbiasedReuse.glsl:157-166), and one neighbour ray per above-horizon
@@ -673,7 +692,7 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c_merge(const PassCArgs a, cons
     // wave's neighbour rays are written slot-major: lane i writes rays i, i + 64, ... of the wave's
     // run, finding the pixel that owns a ray by binary search over the wave's prefix sums, so each
     // store instruction covers consecutive queue entries.
-    const uint32_t want = emit ? above : 0u;
+    const uint32_t want = queued;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t cnt = static_cast<uint32_t>(__popc(want));
     uint32_t incl = cnt;

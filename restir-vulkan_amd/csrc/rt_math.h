@@ -262,8 +262,8 @@ RT_HD Res res_zero() {
     return r;
 }
 // updateReservoirAt, reservoir.glsl:6-26
-// u: the update's uniform draw
-RT_HD void res_update_u(Res &r, float weight, f3 pos, float4 n, float emLum, int32_t li, float pHat,
+// u: the update's uniform draw; returns true when the sample is taken
+RT_HD bool res_update_u(Res &r, float weight, f3 pos, float4 n, float emLum, int32_t li, float pHat,
                         float w, float u) {
     r.sumW += weight;
     const float rp = weight / r.sumW;
@@ -274,7 +274,9 @@ RT_HD void res_update_u(Res &r, float weight, f3 pos, float4 n, float emLum, int
         r.lightIndex = li;
         r.pHat = pHat;
         r.w = w;
+        return true;
     }
+    return false;
 }
 RT_HD void res_update(Res &r, float weight, f3 pos, float4 n, float emLum, int32_t li, float pHat,
                       float w, Pcg &rng) {

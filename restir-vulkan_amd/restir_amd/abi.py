@@ -19,11 +19,12 @@ LIB_PATH = PKG_DIR / "librestir_amd.so"
 VISIBILITY_REUSE_FLAG = 1 << 0
 TEMPORAL_REUSE_FLAG = 1 << 1
 STANDARD_WEIGHTS_FLAG = 1 << 2  # RIS with W = sumW / (M p_hat) and triangle lights by area
+INPUT_VISIBLE_FLAG = 1 << 3  # Pass C input from a visibility-tested Pass C: known segments are not retraced
 SHADE_VISIBILITY_FLAG = 1  # RESTIR_SHADE_VISIBILITY_FLAG: final shading re-tests visibility
 TRACE_FIRST_HIT = 0
 TRACE_REFERENCE_LOOP = 1
 TRACE_ONE_RAY_PER_LANE = 2
-ABI_VERSION = 16  # RESTIR_ABI_VERSION of include/restir_abi.h
+ABI_VERSION = 17  # RESTIR_ABI_VERSION of include/restir_abi.h
 
 ERRORS = {
     -1: "RESTIR_E_INVALID_ARGUMENT",

@@ -120,7 +120,8 @@ class Renderer:
                  spatial_passes=1, spatial_neighbors=4, radius=30.0, temporal=True,
                  visibility=True, temporal_mult=20, launch=None, ctx=None, learned=None,
                  train_samples=4096, lr=1e-3, train=True, launches=None, scene_build="host",
-                 normal_format=abi.NORMAL_F32, standard_weights=False, shade_visibility=False):
+                 normal_format=abi.NORMAL_F32, standard_weights=False, shade_visibility=False,
+                 reuse_verdicts=True):
         torch = _torch()
         self.torch = torch
         self.device = torch.device("cuda", device_index)
@@ -139,6 +140,9 @@ class Renderer:
         self.launch = launch
         self.normal_format = normal_format  # G-buffer normals as float or half planes
         self.shade_visibility = shade_visibility  # final shading re-tests visibility
+        # the second and later Pass C do not retrace the segments the previous one found
+        # unoccluded (RESTIR_INPUT_VISIBLE_FLAG; the same results)
+        self.reuse_verdicts = reuse_verdicts
         self.launches = {"a": launch, "b": launch, "c": launch}
         self.launches.update(launches or {})
         # learned proposal: a restir_amd.learned.LearnedProposal bound to this context, or None
@@ -200,9 +204,13 @@ class Renderer:
             first = self.tmp if n % 2 == 1 else self.res[i]
             pass_a(first)
             src = first
+            later = u
+            if self.reuse_verdicts and (u.flags & abi.VISIBILITY_REUSE_FLAG):
+                later = abi.Uniforms.from_buffer_copy(u)
+                later.flags |= abi.INPUT_VISIBLE_FLAG
             for j in range(n):
                 dst = self.res[i] if (n - j) % 2 == 1 else self.tmp
-                self.ctx.unbiased(u, self.k, j, g, src, dst, self.launches["c"], stream)
+                self.ctx.unbiased(later if j > 0 else u, self.k, j, g, src, dst, self.launches["c"], stream)
                 src = dst
         else:
             pass_a(self.res[i])

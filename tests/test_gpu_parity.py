@@ -57,9 +57,15 @@ def run_frame_and_compare(r, cam, *, check_shade=True):
         a_ref = oracle.pass_a(orc, u, og, ogp, prev_res, abi.RESERVOIR_DTYPE)
         assert_res_equal(host_res(first), a_ref, f"pass A frame {r.frame}")
         src, src_ref = first, a_ref
+        # as the renderer runs them: a later Pass C with RESTIR_INPUT_VISIBLE_FLAG (it does not
+        # retrace the segments the previous pass found unoccluded); the oracle traces every ray
+        later = u
+        if r.reuse_verdicts and (u.flags & abi.VISIBILITY_REUSE_FLAG):
+            later = abi.Uniforms.from_buffer_copy(u)
+            later.flags |= abi.INPUT_VISIBLE_FLAG
         for j in range(n):
             dst = r.res[i] if (n - j) % 2 == 1 else r.tmp
-            r.ctx.unbiased(u, r.k, j, r.gstruct(i), src, dst)
+            r.ctx.unbiased(later if j > 0 else u, r.k, j, r.gstruct(i), src, dst)
             c_ref = oracle.pass_c(orc, u, r.k, j, og, src_ref)
             assert_res_equal(host_res(dst), c_ref, f"pass C{j} frame {r.frame}")
             src, src_ref = dst, c_ref
@@ -149,6 +155,40 @@ def test_unbiased_terrain_with_point_and_triangle_lights(torch):
         a, _ = run_frame_and_compare(r, scenes.orbit_camera(sc.camera, f))
     li = a["lightIndex"][a["sumWeights"] > 0]
     assert (li < 0).any() and (li >= 0).any()  # both kinds were drawn
+
+
+def test_input_visible_flag_same_reservoirs_fewer_rays(torch):
+    """RESTIR_INPUT_VISIBLE_FLAG on the second Pass C of a frame (its input is the first's
+    visibility-tested output): the same reservoirs as without it, bit for bit, and fewer rays in
+    the queues: the centre rays of pixels that kept their own sample and the neighbour rays from
+    the pixel that supplied the merged sample retrace segments the first pass found unoccluded."""
+    sc = scenes.config_scene("terrain")
+    W, H, k = 480, 270, 5
+    r = Renderer(sc, W, H, mode="unbiased", k=k, spatial_passes=2)
+    for f in range(3):  # temporal history
+        r.render_frame(scenes.orbit_camera(sc.camera, f))
+    cam = scenes.orbit_camera(sc.camera, 3)
+    r.begin_frame(cam)
+    u, _ = r.uniforms(cam)
+    i, p = r.cur, 1 - r.cur
+    g = r.gstruct(i)
+    r.ctx.initial_temporal(u, g, r.gstruct(p), r.res[p], r.res[i])
+    r.ctx.unbiased(u, k, 0, g, r.res[i], r.tmp)
+    later = abi.Uniforms.from_buffer_copy(u)
+    later.flags |= abi.INPUT_VISIBLE_FLAG
+    npx = W * H
+    p1 = torch.zeros(npx * k, 4, dtype=torch.float32, device="cuda")
+    p2 = torch.zeros_like(p1)
+    outs, rays = [], []
+    for uu in (u, later):
+        out = torch.zeros_like(r.tmp)
+        r.ctx.unbiased(uu, k, 1, g, r.tmp, out)
+        rays.append((r.ctx.unbiased_rays(0, p1, p2, npx * k), r.ctx.unbiased_rays(1, p1, p2, npx * k)))
+        outs.append(out.cpu().numpy())
+    np.testing.assert_array_equal(outs[0], outs[1])
+    (c0, n0), (c1, n1) = rays
+    print(f"[input visible] centre rays {c0} -> {c1}, neighbour rays {n0} -> {n1}", flush=True)
+    assert c1 < 0.9 * c0 and n1 < 0.95 * n0, rays
 
 
 @pytest.mark.parametrize("schedule", [abi.SCHEDULE_QUEUED, abi.SCHEDULE_INLINE])

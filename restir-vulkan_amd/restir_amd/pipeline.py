@@ -283,7 +283,7 @@ class BatchRenderer:
 
     def __init__(self, views, *, device_index=0, mode="unbiased", M=32, k=5, spatial_passes=1,
                  spatial_neighbors=4, radius=30.0, temporal=True, visibility=True, temporal_mult=20,
-                 launch=None, rays="persistent", normal_format=abi.NORMAL_F32):
+                 launch=None, rays="persistent", normal_format=abi.NORMAL_F32, reuse_verdicts=True):
         torch = _torch()
         self.torch = torch
         self.device = torch.device("cuda", device_index)
@@ -298,6 +298,7 @@ class BatchRenderer:
             raise ValueError(f"rays must be 'queued' or 'persistent', not {rays!r}")
         self.rays = rays
         self.normal_format = normal_format
+        self.reuse_verdicts = reuse_verdicts  # later queued Pass C: RESTIR_INPUT_VISIBLE_FLAG
         tx, ty = (launch.tile_x, launch.tile_y) if launch is not None else (0, 0)
         self.launch_q = abi.Launch(tx, ty, abi.SCHEDULE_QUEUED)
         self.scene_dev = {}
@@ -383,6 +384,9 @@ class BatchRenderer:
                     self.ctx.unbiased_views(vc, self.k, j, self.launch, stream)
                 else:
                     for v, u, a, b in zip(self.views, us, src, dst):
+                        if j > 0 and self.reuse_verdicts and (u.flags & abi.VISIBILITY_REUSE_FLAG):
+                            u = abi.Uniforms.from_buffer_copy(u)
+                            u.flags |= abi.INPUT_VISIBLE_FLAG
                         v["ctx"].unbiased(u, self.k, j, self._g(v, i), a, b, self.launch_q, stream)
                 src = dst
         else:

@@ -26,6 +26,25 @@ def test_header_and_binding_declare_the_same_entry_points():
     assert declared_functions() == set(abi.SIGNATURES)
 
 
+def test_header_and_binding_flags_agree():
+    text = HEADER.read_text()
+
+    def value(name):
+        m = re.search(rf"#define {name} \(?(?:1 << (\d+)|(\d+)u?)\)?", text)
+        assert m, name
+        return 1 << int(m.group(1)) if m.group(1) else int(m.group(2))
+
+    for name, v in (("RESTIR_VISIBILITY_REUSE_FLAG", abi.VISIBILITY_REUSE_FLAG),
+                    ("RESTIR_TEMPORAL_REUSE_FLAG", abi.TEMPORAL_REUSE_FLAG),
+                    ("RESTIR_STANDARD_WEIGHTS_FLAG", abi.STANDARD_WEIGHTS_FLAG),
+                    ("RESTIR_INPUT_VISIBLE_FLAG", abi.INPUT_VISIBLE_FLAG),
+                    ("RESTIR_SHADE_VISIBILITY_FLAG", abi.SHADE_VISIBILITY_FLAG)):
+        assert value(name) == v, name
+    uniform_flags = [abi.VISIBILITY_REUSE_FLAG, abi.TEMPORAL_REUSE_FLAG, abi.STANDARD_WEIGHTS_FLAG,
+                     abi.INPUT_VISIBLE_FLAG]
+    assert len(set(uniform_flags)) == len(uniform_flags)
+
+
 def test_library_exports_every_declared_symbol(product_lib):
     out = subprocess.run(["nm", "-D", "--defined-only", str(abi.LIB_PATH)], capture_output=True,
                          text=True, check=True).stdout

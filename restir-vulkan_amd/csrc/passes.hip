@@ -144,21 +144,22 @@ __device__ __forceinline__ Pcg pass_a_rng(const PassAArgs &a, uint32_t x, uint32
 }
 
 // One light candidate of Pass A (restirOmni.glsl:113-145) with source probability prob: point
-// light li, or (tri) a point on triangle light li picked with the draws ra, rb
+// light li (`point`: its position and luminance, point_light()), or (tri) a point on triangle
+// light li picked with the draws ra, rb
 // (pickPointOnTriangle, restirOmni.glsl:68-71), whose probability per unit area seen from pos is
 // prob / (|cos| area); u is the reservoir update's draw. Only the choice of the sample point
 // depends on the kind; the target function and the reservoir update run once for all lanes of a
 // wave whatever kinds its lanes drew. `valid` false: an empty candidate (p_hat 0, probability 1)
 // that still counts in M and draws its update number.
 __device__ __forceinline__ void add_candidate(Res &res, const Surface &s, f3 pos, const SceneView &sc, bool valid,
-                                              bool tri, int32_t li, float prob, float ra, float rb, float u,
-                                              bool areaPdf) {
+                                              bool tri, int32_t li, float prob, float4 point, float ra, float rb,
+                                              float u, bool areaPdf) {
     f3 L{0.0f, 0.0f, 0.0f}, nl{0.0f, 0.0f, 0.0f};
     float lum = 0.0f;
     int32_t index = 0;
     if (valid && !tri) {
-        L = xyz(*reinterpret_cast<const float4 *>(sc.points[li].pos));
-        lum = sc.points[li].color_luminance[3];
+        L = xyz(point);
+        lum = point.w;
         index = li;
     } else if (valid) {
         const restir_tri_light &tl = sc.triLights[li];
@@ -179,6 +180,12 @@ __device__ __forceinline__ void add_candidate(Res &res, const Surface &s, f3 pos
     const float ph = valid ? eval_phat(s, L, nl, useLN, lum) : 0.0f;
     res_add_sample_u(res, L, make_float4(nl.x, nl.y, nl.z, useLN ? 1.0f : 0.0f), lum, index, ph,
                      valid ? prob : 1.0f, u);
+}
+
+// Point light li's position and luminance (pointLight, light.glsl).
+__device__ __forceinline__ float4 point_light(const SceneView &sc, int32_t li) {
+    const float4 p = *reinterpret_cast<const float4 *>(sc.points[li].pos);
+    return make_float4(p.x, p.y, p.z, sc.points[li].color_luminance[3]);
 }
 
 // aliasTableSample's column, restirOmni.glsl:73-83
@@ -213,59 +220,79 @@ __device__ __forceinline__ void pass_a_ris(const PassAArgs &a, const PassASurfac
                 rb = pcg_float(rng);
             }
             const float u = pcg_float(rng);
-            add_candidate(res, s, pos, sc, valid, tri, tri ? li - sc.nPoints : li, q, ra, rb, u, standard);
+            const float4 point = valid && !tri ? point_light(sc, li) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            add_candidate(res, s, pos, sc, valid, tri, tri ? li - sc.nPoints : li, q, point, ra, rb, u, standard);
         }
     } else if (dot(n, n) != 0.0f && M > 0) {
         // A candidate draws r1, r2 (the alias column and its coin), then a triangle light's two
-        // point draws, then the reservoir update's: 3 draws, or 5 after a triangle light. Each
-        // state is reached by jumping ahead from the candidate's first one, so the next candidate's
-        // column (where it starts if this one is a point light) is loaded before this candidate
-        // is evaluated, taking the column load off the chain from one candidate to the next; after
-        // a triangle light the column is loaded again from the right state. Same draws in the
-        // same order as drawing them one by one.
+        // point draws, then the reservoir update's: 3 draws, or 5 after a triangle light. The
+        // candidates run as a pipeline: while candidate i is evaluated, candidate i + 1 is already
+        // resolved (its column was loaded during candidate i - 1) with its point light's record in
+        // flight, and candidate i + 2's column is in flight, each from its exact first state
+        // (jumping 3 or 5 steps ahead once the kind before it is known). Same draws in the same
+        // order as drawing them one by one.
         const uint64_t inc = rng.inc;
-        const uint64_t c3 = inc * pcg_mul_sum(3);
-        uint64_t st = rng.state;
-        int32_t col = alias_column(sc, pcg_float_at(st));
-        float4 c = sc.alias[col];
-        for (uint32_t i = 0; i < M; ++i) {
-            const uint64_t s3 = st * pcg_mul_pow(3) + c3;
-            int32_t colN = alias_column(sc, pcg_float_at(s3));
-            float4 cN = sc.alias[colN];
-            const uint64_t s1 = st * kPcgMul + inc, s2 = s1 * kPcgMul + inc;
-            const float r2 = pcg_float_at(s1);
-            // aliasTableSample, restirOmni.glsl:73-83
-            int32_t li;
+        const uint64_t c3 = inc * pcg_mul_sum(3), c5 = inc * pcg_mul_sum(5);
+        // the state after candidate i's draws, from its first state and kind
+        const auto after = [&](uint64_t first, bool tri) {
+            return tri ? first * pcg_mul_pow(5) + c5 : first * pcg_mul_pow(3) + c3;
+        };
+        // aliasTableSample (restirOmni.glsl:73-83) from column col (record cv) and the coin r2;
+        // restirOmni.glsl:117: point lights if there are any, else triangle lights; a union table
+        // names point lights first
+        struct Cand {
+            int32_t li;  // index among its kind's lights
             float prob;
-            if (c.x > r2) {
+            bool tri;
+            float4 point;  // a point light's record
+        };
+        const auto resolve = [&](int32_t col, float4 cv, float r2) {
+            Cand k;
+            int32_t li;
+            if (cv.x > r2) {
                 li = col;
-                prob = c.z;
+                k.prob = cv.z;
             } else {
-                li = __float_as_int(c.y);
-                prob = c.w;
+                li = __float_as_int(cv.y);
+                k.prob = cv.w;
             }
-            // restirOmni.glsl:117: point lights if there are any, else triangle lights; a union
-            // table names point lights first
-            const bool tri = sc.mixed ? li >= sc.nPoints : sc.nPoints == 0;
+            k.tri = sc.mixed ? li >= sc.nPoints : sc.nPoints == 0;
+            k.li = sc.mixed && k.tri ? li - sc.nPoints : li;
+            k.point = k.tri ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : point_light(sc, k.li);
+            return k;
+        };
+        uint64_t st = rng.state;  // candidate i's first state
+        int32_t col = alias_column(sc, pcg_float_at(st));
+        Cand cur = resolve(col, sc.alias[col], pcg_float_at(st * kPcgMul + inc));
+        uint64_t stN = after(st, cur.tri);  // candidate i + 1's first state
+        int32_t colN = alias_column(sc, pcg_float_at(stN));
+        float4 cN = sc.alias[colN];  // candidate i + 1's column
+        for (uint32_t i = 0; i < M; ++i) {
+            Cand nxt = cur;
+            uint64_t stNN = stN;
+            int32_t colNN = colN;
+            float4 cNN = cN;
+            if (i + 1 < M) {
+                nxt = resolve(colN, cN, pcg_float_at(stN * kPcgMul + inc));
+                stNN = after(stN, nxt.tri);
+                colNN = alias_column(sc, pcg_float_at(stNN));
+                cNN = sc.alias[colNN];
+            }
+            const uint64_t s2 = st * pcg_mul_pow(2) + inc * pcg_mul_sum(2), s3 = s2 * kPcgMul + inc;
             float ra = 0.0f, rb = 0.0f, u;
-            uint64_t next;
-            if (tri) {
-                const uint64_t s4 = s3 * kPcgMul + inc;
+            if (cur.tri) {
                 ra = pcg_float_at(s2);
                 rb = pcg_float_at(s3);
-                u = pcg_float_at(s4);
-                next = s4 * kPcgMul + inc;
-                colN = alias_column(sc, pcg_float_at(next));
-                cN = sc.alias[colN];
+                u = pcg_float_at(s3 * kPcgMul + inc);
             } else {
                 u = pcg_float_at(s2);
-                next = s3;
             }
-            add_candidate(res, s, pos, sc, true, tri, sc.mixed && tri ? li - sc.nPoints : li, prob, ra, rb, u,
-                          standard);
-            st = next;
-            col = colN;
-            c = cN;
+            add_candidate(res, s, pos, sc, true, cur.tri, cur.li, cur.prob, cur.point, ra, rb, u, standard);
+            st = stN;
+            cur = nxt;
+            stN = stNN;
+            colN = colNN;
+            cN = cNN;
         }
         rng.state = st;
     }

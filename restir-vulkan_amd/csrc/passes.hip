@@ -569,9 +569,9 @@ __global__ __launch_bounds__(TX *TY) void k_pass_c(const PassCArgs a) {
 // queues traced by one launch of the persistent tracer: (1) merge every pixel, queue its centre
 // ray and the rays to its above-horizon neighbours; (2) trace the centre rays, then the
 // neighbour rays, skipping those of pixels whose centre ray was found shadowed (they have Z = 0
-// whatever their neighbours give); (3) form Z and W. Rays are appended with one atomic per wave,
-// slots handed out by prefix sums over the wave, so a queue holds only live rays in pixel-tile
-// order.
+// whatever their neighbours give); (3) form Z and W. Rays are appended with one atomic per
+// workgroup for both queues, slots handed out by prefix sums over the wave and the workgroup, so
+// a queue holds only live rays in pixel-tile order.
 struct CQueues {
     uint32_t *counter;     // [0] centre rays queued, [1] neighbour rays queued
     float4 *c1;            // centre rays: the pixel's position, .w = pixel | kCentreTag; the other

@@ -94,8 +94,14 @@ def parse():
     ap.add_argument("--tile", type=str, default="16x16")
     ap.add_argument("--trace-reps", type=int, default=20)
     ap.add_argument("--pretrain", type=int, default=1024, help="untimed training frames of the MLP")
+    ap.add_argument("--mix", type=float, default=0.1,
+                    help="share of the learned candidates drawn from the alias table (bounds q below)")
+    ap.add_argument("--train-every", type=int, default=1,
+                    help="learned pipeline: one Adam step every this many frames (pretraining too)")
     ap.add_argument("--error-frames", type=int, default=256, help="independent frames per method")
     ap.add_argument("--error-pixels", type=int, default=16384)
+    ap.add_argument("--error-seed", type=int, default=0,
+                    help="selects another set of independent frames for the error study")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-views", action="store_true", help="skip the batched 16-view measurement")
     ap.add_argument("--no-builds", action="store_true", help="skip the scene-builder measurement")
@@ -172,7 +178,7 @@ class Frames:
             yield j
             src = dst
         ctx.shade(self.cams[f]["position"], 1.0, g, self.res[i], self.rgb, self.W * self.H, s)
-        if learned is not None and train:
+        if learned is not None and train and f % self.args.train_every == 0:
             learned.train(u, g, self.res[i], 4096, 1e-3, s)
 
     def run(self, f, M, **kw):
@@ -449,7 +455,7 @@ def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
     K, WU, W, H = args.steps, args.warmup, fr.W, fr.H
     nframes = WU + K
     n_orbit = len(fr.cams)
-    lp = LearnedProposal(ctx, scene, k=256, seed=7)
+    lp = LearnedProposal(ctx, scene, k=256, seed=7, mix=args.mix)
     t0 = time.time()
     for i in range(args.pretrain):  # online training over the orbit (untimed)
         fr.run(i % n_orbit, args.learned_M, learned=lp)
@@ -497,7 +503,8 @@ def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
             for e in range(per_view):
                 for t in fr.res:
                     t.zero_()
-                fr.run(f_err, M, learned=learned, gcur=gi, gprev=gi, seed_frame=100000 + 1000 * vi + e,
+                fr.run(f_err, M, learned=learned, gcur=gi, gprev=gi,
+                       seed_frame=100000 + 10000 * args.error_seed + 1000 * vi + e,
                        train=False)
                 x = torch.nan_to_num(fr.rgb.view(3, -1)[:, idx].double())
                 rel.append((((x - gt) ** 2) / (gt ** 2 + 1e-2)).mean().item())
@@ -529,7 +536,7 @@ def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
         "ms_per_step": round(learned_ms, 4),
         "learned": {"ms_per_frame": round(learned_ms, 4), "pass_a_schedule": sched_learned, "kernels": lk,
                     "train_loss": round(lp.loss.item(), 4),
-                    "pretrain_frames": args.pretrain},
+                    "pretrain_frames": args.pretrain, "mix": args.mix, "train_every": args.train_every},
         "equal_rmse": {
             "protocol": f"static cameras at orbit frames {list(views)} (fixed; independent of --steps and "
                         f"--warmup), {per_view} independent frames per view and method (history cleared), "
@@ -537,7 +544,7 @@ def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
                         f"per view (triangle lights by {TRUTH_TRI_SAMPLES} stratified samples each); relMSE = "
                         "mean (x - gt)^2 / (gt^2 + 1e-2) per frame; time to a target error is proportional to "
                         "ms/frame x mean relMSE; 95% CI by bootstrap over frames within each view",
-            "views": list(views), "frames_per_view": per_view,
+            "views": list(views), "frames_per_view": per_view, "error_seed": args.error_seed,
             "relmse_plain": round(float(mp), 4), "relmse_plain_median": round(float(np.median(rp)), 4),
             "relmse_learned": round(float(ml), 4),
             "relmse_learned_median": round(float(np.median(rl)), 4),

@@ -644,9 +644,10 @@ def batched_views(torch, args):
     """16 views (resolutions log-uniform in [640x360, 2560x1440], seeded) over 4 seeded terrains
     (about 25k-200k triangles, 1k-50k point lights), unbiased ReSTIR M = 32 with temporal reuse
     and 2 Pass C: 8 orbiting frames of all views captured in one HIP graph (shading as one
-    persistent multi-view launch; Passes A and C either per view on the queued ray schedule or as
-    persistent multi-view launches, both timed), against the same frames rendered view by view
-    (eager, one launch sequence per view, queued Pass C)."""
+    persistent multi-view launch; Passes A and C either per view on the queued ray schedule, each
+    scene's views on a stream of their own, or as persistent multi-view launches, both timed),
+    against the same frames rendered view by view (eager, one launch sequence per view, queued
+    Pass C, one stream)."""
     from restir_amd import scenes
     from restir_amd.pipeline import BatchRenderer, FrameSequenceGraph, Renderer
 

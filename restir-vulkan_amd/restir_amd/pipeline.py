@@ -279,11 +279,18 @@ class BatchRenderer:
     either the same way (``rays="persistent"``, the default: one launch per pass over all views,
     rays traced per pixel) or view by view on the queued ray schedule of the view's scene context
     (``rays="queued"``: each view's rays compacted and traced by the persistent tracer). Both give
-    each view's output bit-identical to a Renderer of that view alone."""
+    each view's output bit-identical to a Renderer of that view alone.
+
+    On the queued schedule the views of different scenes share nothing (each scene context has its
+    own scratch and queues), so with ``concurrent_scenes`` (default) each scene's views run in order
+    on a stream of their own, forked from the caller's stream after the frame's start and joined
+    back before the multi-view shading: small views and the tails of one scene's launches overlap
+    another scene's work. A graph capture records the fork and the join as dependencies."""
 
     def __init__(self, views, *, device_index=0, mode="unbiased", M=32, k=5, spatial_passes=1,
                  spatial_neighbors=4, radius=30.0, temporal=True, visibility=True, temporal_mult=20,
-                 launch=None, rays="persistent", normal_format=abi.NORMAL_F32, reuse_verdicts=True):
+                 launch=None, rays="persistent", normal_format=abi.NORMAL_F32, reuse_verdicts=True,
+                 concurrent_scenes=True):
         torch = _torch()
         self.torch = torch
         self.device = torch.device("cuda", device_index)
@@ -321,6 +328,13 @@ class BatchRenderer:
             })
             off += 3 * n
         self.images = [v["rgb"] for v in self.views]
+        # view indices per scene context, in order of first appearance; a stream per scene
+        groups = {}
+        for vi, v in enumerate(self.views):
+            groups.setdefault(id(v["ctx"]), []).append(vi)
+        self.groups = list(groups.values())
+        self.concurrent = concurrent_scenes and rays == "queued" and len(self.groups) > 1
+        self.side = [torch.cuda.Stream(device=self.device) for _ in self.groups] if self.concurrent else []
         self.frame = 0
         self.cur = 1
 
@@ -352,9 +366,6 @@ class BatchRenderer:
         us = []
         for v, cam in zip(self.views, cams):
             v["cam"] = cam
-            v["ctx"].primary_gbuffer(cam, v["dev"].d_tri_material, v["dev"].d_materials,
-                                     len(v["scene"].materials), v["gbuf"][i], v["W"], v["H"], stream=stream,
-                                     normal_format=self.normal_format)
             pv_now = projection_view(cam, v["W"] / v["H"])
             prev_pv = v["prev_pv"] if v["prev_pv"] is not None else pv_now
             u = make_uniforms(v["W"], v["H"], self.frame, cam["position"], prev_pv, M=self.M,
@@ -367,29 +378,56 @@ class BatchRenderer:
             first = [v["tmp"] if n_sp % 2 == 1 else v["res"][i] for v in self.views]
         else:
             first = [v["res"][i] for v in self.views]
-        if self.rays == "persistent":
-            va = [view(v, u, self._g(v, i), self._g(v, p), v["res"][p], f)
-                  for v, u, f in zip(self.views, us, first)]
-            self.ctx.initial_temporal_views(va, self.launch, stream)
-        else:
-            for v, u, f in zip(self.views, us, first):
-                v["ctx"].initial_temporal(u, self._g(v, i), self._g(v, p), v["res"][p], f, self.launch_q, stream)
-        if self.mode == "unbiased":
-            src = first
+
+        def gbuffer(v, s):
+            v["ctx"].primary_gbuffer(v["cam"], v["dev"].d_tri_material, v["dev"].d_materials,
+                                     len(v["scene"].materials), v["gbuf"][i], v["W"], v["H"], stream=s,
+                                     normal_format=self.normal_format)
+
+        def queued_passes(v, u, f, s):
+            """Pass A and (unbiased) every Pass C of one view on the queued schedule, on stream s."""
+            v["ctx"].initial_temporal(u, self._g(v, i), self._g(v, p), v["res"][p], f, self.launch_q, s)
+            if self.mode != "unbiased":
+                return
+            src = f
             for j in range(n_sp):
-                dst = [v["res"][i] if (n_sp - j) % 2 == 1 else v["tmp"] for v in self.views]
-                if self.rays == "persistent":
-                    vc = [view(v, u, self._g(v, i), self._g(v, i), a, b)
-                          for v, u, a, b in zip(self.views, us, src, dst)]
-                    self.ctx.unbiased_views(vc, self.k, j, self.launch, stream)
-                else:
-                    for v, u, a, b in zip(self.views, us, src, dst):
-                        if j > 0 and self.reuse_verdicts and (u.flags & abi.VISIBILITY_REUSE_FLAG):
-                            u = abi.Uniforms.from_buffer_copy(u)
-                            u.flags |= abi.INPUT_VISIBLE_FLAG
-                        v["ctx"].unbiased(u, self.k, j, self._g(v, i), a, b, self.launch_q, stream)
+                dst = v["res"][i] if (n_sp - j) % 2 == 1 else v["tmp"]
+                uj = u
+                if j > 0 and self.reuse_verdicts and (u.flags & abi.VISIBILITY_REUSE_FLAG):
+                    uj = abi.Uniforms.from_buffer_copy(u)
+                    uj.flags |= abi.INPUT_VISIBLE_FLAG
+                v["ctx"].unbiased(uj, self.k, j, self._g(v, i), src, dst, self.launch_q, s)
                 src = dst
+
+        if self.concurrent:
+            # fork: every scene's stream starts after the work already on the caller's stream
+            main = stream if stream is not None else self.torch.cuda.current_stream(self.device)
+            for g, side in zip(self.groups, self.side):
+                side.wait_stream(main)
+                for vi in g:
+                    gbuffer(self.views[vi], side)
+                    queued_passes(self.views[vi], us[vi], first[vi], side)
+            for side in self.side:  # join
+                main.wait_stream(side)
         else:
+            for v in self.views:
+                gbuffer(v, stream)
+            if self.rays == "persistent":
+                va = [view(v, u, self._g(v, i), self._g(v, p), v["res"][p], f)
+                      for v, u, f in zip(self.views, us, first)]
+                self.ctx.initial_temporal_views(va, self.launch, stream)
+                if self.mode == "unbiased":
+                    src = first
+                    for j in range(n_sp):
+                        dst = [v["res"][i] if (n_sp - j) % 2 == 1 else v["tmp"] for v in self.views]
+                        vc = [view(v, u, self._g(v, i), self._g(v, i), a, b)
+                              for v, u, a, b in zip(self.views, us, src, dst)]
+                        self.ctx.unbiased_views(vc, self.k, j, self.launch, stream)
+                        src = dst
+            else:
+                for v, u, f in zip(self.views, us, first):
+                    queued_passes(v, u, f, stream)
+        if self.mode != "unbiased":
             for j in range(self.spatial_passes):
                 vb = [view(v, u, self._g(v, i), self._g(v, i), v["res"][i], v["res"][p])
                       for v, u in zip(self.views, us)]

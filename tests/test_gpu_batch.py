@@ -148,12 +148,16 @@ def batch_views():
     return [(a, 64, 48), (b, 80, 45), (a, 33, 70)]
 
 
-@pytest.mark.parametrize("mode,rays", [("unbiased", "queued"), ("unbiased", "persistent"),
-                                       ("biased", "queued"), ("biased", "persistent")])
-def test_batched_views_match_single_view_renderers(torch, mode, rays):
+@pytest.mark.parametrize("mode,rays,concurrent", [("unbiased", "queued", True), ("unbiased", "queued", False),
+                                                  ("unbiased", "persistent", True), ("biased", "queued", True),
+                                                  ("biased", "persistent", True)])
+def test_batched_views_match_single_view_renderers(torch, mode, rays, concurrent):
+    """Each view of a batch equals a Renderer of that view alone; on the queued schedule the two
+    scenes' views run on streams of their own (concurrent) or one after another on one stream."""
     views = batch_views()
     kw = dict(mode=mode, k=3, M=8, spatial_passes=2)
-    batch = BatchRenderer(views, rays=rays, **kw)
+    batch = BatchRenderer(views, rays=rays, concurrent_scenes=concurrent, **kw)
+    assert batch.concurrent == (concurrent and rays == "queued")
     singles = [Renderer(sc, w, h, launch=abi.Launch(16, 16, abi.SCHEDULE_INLINE), **kw) for sc, w, h in views]
     for f in range(3):
         cams = [scenes.orbit_camera(sc.camera, f) for sc, _, _ in views]

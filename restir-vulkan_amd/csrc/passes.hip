@@ -821,6 +821,10 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
                                                      const uint32_t *__restrict__ count, uint32_t nBatch, CQueues q,
                                                      uint32_t *__restrict__ out) {
     constexpr bool PER_PIXEL = MODE == kTracePassA, BATCH = MODE == kTraceBatch;
+    // visits per loop step: two for Pass A's rays, where spreading the step's refill, round and
+    // finish checks over two visits pays for the lower occupancy it costs (106 VGPRs, 4 waves
+    // per SIMD); one for Pass C's, where it does not
+    constexpr int kVisits = MODE == kTracePassA ? 2 : 1;
     __shared__ int32_t stack[STACK * 256];
     __shared__ uint32_t triList[4 * kTriList];
     __shared__ uint8_t triOwner[4 * kTriList];
@@ -900,29 +904,34 @@ __global__ __launch_bounds__(256) void k_trace_queue(const SceneView sc, const f
             if (drained && next >= end) break;
             continue;
         }
-        // one visit per walking lane; its triangles join the wave's list
-        const bool walking = ray >= 0 && !w.hit && w.cur >= 0;
-        uint32_t leaves = 0;
-        int4 ids;  // read only where leaves has bits, that is after a visit wrote it
-        if (walking) leaves = walk_step_defer<STACK>(w, sc.wide, sc.nodes, stk, 256, ids);
-        const uint32_t nLeaves = static_cast<uint32_t>(__popc(leaves));
-        const uint64_t c1 = __ballot(nLeaves & 1u), c2 = __ballot(nLeaves & 2u), c4 = __ballot(nLeaves & 4u);
-        if (c1 | c2 | c4) {
-            // listed < 64 before the visit, so the ring (kTriList entries) cannot overflow
-            uint32_t at = first + listed + lanes_below(c1) + 2u * lanes_below(c2) + 4u * lanes_below(c4);
-            // one step per entry, so that each entry's id is a register rather than a select
-            const auto put = [&](int32_t id) {
-                const uint32_t slot = at < kTriList ? at : at - kTriList;
-                tri[slot] = static_cast<uint32_t>(~id);
-                own[slot] = static_cast<uint8_t>(lane);
-                ++at;
-            };
-            if (leaves & 1u) put(ids.x);
-            if (leaves & 2u) put(ids.y);
-            if (leaves & 4u) put(ids.z);
-            if (leaves & 8u) put(ids.w);
-            listed += static_cast<uint32_t>(__popcll(c1) + 2 * __popcll(c2) + 4 * __popcll(c4));
-            waiting += nLeaves;
+        // kVisits visits per walking lane (the second only while the list holds fewer than 64
+        // tests); their triangles join the wave's list
+#pragma unroll
+        for (int rep = 0; rep < kVisits; ++rep) {
+            if (rep > 0 && listed >= 64u) break;  // wave-uniform
+            const bool walking = ray >= 0 && !w.hit && w.cur >= 0;
+            uint32_t leaves = 0;
+            int4 ids;  // read only where leaves has bits, that is after a visit wrote it
+            if (walking) leaves = walk_step_defer<STACK>(w, sc.wide, sc.nodes, stk, 256, ids);
+            const uint32_t nLeaves = static_cast<uint32_t>(__popc(leaves));
+            const uint64_t c1 = __ballot(nLeaves & 1u), c2 = __ballot(nLeaves & 2u), c4 = __ballot(nLeaves & 4u);
+            if (c1 | c2 | c4) {
+                // listed < 64 before the visit, so the ring (kTriList entries) cannot overflow
+                uint32_t at = first + listed + lanes_below(c1) + 2u * lanes_below(c2) + 4u * lanes_below(c4);
+                // one step per entry, so that each entry's id is a register rather than a select
+                const auto put = [&](int32_t id) {
+                    const uint32_t slot = at < kTriList ? at : at - kTriList;
+                    tri[slot] = static_cast<uint32_t>(~id);
+                    own[slot] = static_cast<uint8_t>(lane);
+                    ++at;
+                };
+                if (leaves & 1u) put(ids.x);
+                if (leaves & 2u) put(ids.y);
+                if (leaves & 4u) put(ids.z);
+                if (leaves & 8u) put(ids.w);
+                listed += static_cast<uint32_t>(__popcll(c1) + 2 * __popcll(c2) + 4 * __popcll(c4));
+                waiting += nLeaves;
+            }
         }
         // triangle rounds: the oldest 64 tests, one per lane
         const uint32_t walkers = static_cast<uint32_t>(__popcll(__ballot(ray >= 0 && !w.hit && w.cur >= 0)));

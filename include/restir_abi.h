@@ -87,8 +87,11 @@ typedef struct restir_uniforms {
  * G-buffer guarantees (its centre re-test leaves W = 0 otherwise): the second and later Pass C of a
  * frame may set it. The pass then does not trace the rays that retrace such a segment: the
  * centre ray when the merged sample is the pixel's own, and the neighbour rays from the pixel
- * whose reservoir supplied the merged sample. The results are the same as without the flag when
- * the promise holds; when it does not, those rays count as unoccluded. */
+ * whose reservoir supplied the merged sample. On the queued schedule, when the previous call on
+ * this context was a queued restir_unbiased over the same G-buffer planes, size and k (no queued
+ * Pass A since), the pass also reuses the position and normal records that call packed from the
+ * planes, which the promise implies are unchanged. The results are the same as without the flag
+ * when the promise holds; when it does not, those rays count as unoccluded. */
 #define RESTIR_INPUT_VISIBLE_FLAG (1 << 3)
 
 /* AabbTreeNode, src/shaders/include/structs/aabbTree.glsl:1-8 (80 B). Child < 0 is leaf ~triIndex. */

@@ -125,7 +125,10 @@ uint32_t persistent_blocks(uint32_t cus, int stack) {
     return resident_blocks(cus, static_cast<uint32_t>(stack) * 256u * 4u + rk::kTraceExtraLds);
 }
 
-int ensure_scratch(restir_ctx *ctx, size_t bytes) { return ensure(ctx, &ctx->scratch, &ctx->scratchBytes, bytes); }
+int ensure_scratch(restir_ctx *ctx, size_t bytes) {
+    if (bytes > ctx->scratchBytes) ctx->surfValid = false;  // a new allocation holds no records
+    return ensure(ctx, &ctx->scratch, &ctx->scratchBytes, bytes);
+}
 
 // Binds the device and, for the queued schedule (the default), sizes the context scratch for
 // Pass A's ray queue; *queues stays null for the inline schedule. The Pass C ray export is
@@ -140,6 +143,7 @@ int pass_a_queues(restir_ctx *ctx, const restir_launch *launch, uint32_t npx, vo
         if (int r = ensure_scratch(ctx, rk::pass_a_queue_bytes(npx))) return r;
         *queues = ctx->scratch;
         ctx->queuedNpx = 0;
+        ctx->surfValid = false;
     }
     return RESTIR_OK;
 }
@@ -415,13 +419,22 @@ int restir_unbiased(restir_ctx *ctx, const restir_uniforms *u, int k, int iter,
     if (static_cast<uint64_t>(W) * H >= (1ull << 27)) schedule = RESTIR_SCHEDULE_INLINE;
     if (int r = use_device(ctx)) return r;
     void *queues = nullptr;
+    bool pack = true;
+    const rk::GView gv = view_of(g);
     if (schedule == RESTIR_SCHEDULE_QUEUED) {
         if (int r = ensure_scratch(ctx, rk::pass_c_queue_bytes(W * H, k))) return r;
         queues = ctx->scratch;
+        // RESTIR_INPUT_VISIBLE_FLAG promises the previous Pass C's output over the same G-buffer:
+        // when that Pass C ran here on these planes, size and k, its surface records are reused
+        const auto &key = ctx->surfKey;
+        pack = !((u->flags & RESTIR_INPUT_VISIBLE_FLAG) && ctx->surfValid && key.pos == gv.pos &&
+                 key.nrm == gv.nrm && key.P == gv.P && key.W == W && key.H == H && key.nrmHalf == gv.nrmHalf &&
+                 key.k == k);
     }
-    const hipError_t e = rk::launch_pass_c(tile, ctx->stack, *u, k, iter, ctx->scene, view_of(g), in, out,
-                                           queues, ctx->persistentBlocks,
-                                           static_cast<hipStream_t>(stream));
+    const hipError_t e = rk::launch_pass_c(tile, ctx->stack, *u, k, iter, ctx->scene, gv, in, out, queues,
+                                           ctx->persistentBlocks, static_cast<hipStream_t>(stream), pack);
+    ctx->surfValid = queues && e == hipSuccess;
+    if (ctx->surfValid) ctx->surfKey = {gv.pos, gv.nrm, gv.P, W, H, gv.nrmHalf, k};
     // the queues hold this call's rays only when it traced any (otherwise they are stale)
     ctx->queuedNpx = (queues && vis && e == hipSuccess) ? W * H : 0;
     ctx->queuedK = k;

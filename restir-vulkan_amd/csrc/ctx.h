@@ -40,6 +40,15 @@ struct restir_ctx {
     uint32_t queuedNpx = 0;        // shape of the last queued Pass C (its rays stay in scratch)
     const restir_reservoir *queuedOut = nullptr;  // its output (the light end of its neighbour rays)
     int queuedK = 0;
+    // the G-buffer planes, size and neighbour count whose surface records (k_pack_surface) the
+    // last queued Pass C left in scratch; surfValid is cleared when anything else writes scratch
+    struct {
+        const void *pos, *nrm;
+        uint64_t P;
+        uint32_t W, H, nrmHalf;
+        int k;
+    } surfKey{};
+    bool surfValid = false;
     bool profiling = false;
     rk::Recorder recorder;
     void *mlpScratch = nullptr;    // training tensors of the learned proposal

@@ -95,7 +95,8 @@ hipError_t launch_pass_b(int tile, const restir_uniforms &u, int iter, float cos
 // schedule, using pass_c_queue_bytes(npx, k) bytes of device scratch.
 hipError_t launch_pass_c(int tile, int stack, const restir_uniforms &u, int k, int iter,
                          const SceneView &sc, const GView &g, const restir_reservoir *in,
-                         restir_reservoir *out, void *queueScratch, uint32_t gridCap, hipStream_t s);
+                         restir_reservoir *out, void *queueScratch, uint32_t gridCap, hipStream_t s,
+                         bool packSurface = true);
 size_t pass_c_queue_bytes(uint32_t npx, int k);
 // Copies the shadow rays that the last queued Pass C over (npx, k) put in `queueScratch`
 // (which = 0: centre rays; 1: the neighbour rays of pixels whose centre ray found the light; the

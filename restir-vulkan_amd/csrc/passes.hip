@@ -1461,13 +1461,16 @@ static hipError_t dispatch_c(int tile, const PassCArgs &args, hipStream_t s) {
     }
 }
 template <int TX, int TY, int STACK>
-static hipError_t run_c_queued(PassCArgs args, const CQueues &q, uint32_t gridCap, hipStream_t s) {
+static hipError_t run_c_queued(PassCArgs args, const CQueues &q, uint32_t gridCap, bool packSurface,
+                               hipStream_t s) {
     static_assert((kCentreTag & 0x07ffffffu) == 0 && (kCentreTag >> 27) > 15u, "the centre tag is no pixel or slot");
     grid_for<TX, TY>(args.u.screenSize[0], args.u.screenSize[1], args.tilesX, args.nTiles);
     const dim3 tiles(args.nTiles), block(TX * TY);
     RK_LAUNCH("k_zero_counters", k_zero_counters, dim3(1), dim3(64), 0, s, q.counter, 2u, args.sc.traceHeads);
     const uint32_t npx = args.u.screenSize[0] * args.u.screenSize[1];
-    RK_LAUNCH("k_pack_surface", k_pack_surface, dim3((npx + 255) / 256), dim3(256), 0, s, args.g, npx, q.surf);
+    // packSurface false: q.surf already holds this G-buffer's records (the previous Pass C's)
+    if (packSurface)
+        RK_LAUNCH("k_pack_surface", k_pack_surface, dim3((npx + 255) / 256), dim3(256), 0, s, args.g, npx, q.surf);
     RK_LAUNCH("k_pass_c_merge", (k_pass_c_merge<TX, TY>), tiles, block, 0, s, args, q);
     if (args.u.flags & RESTIR_VISIBILITY_REUSE_FLAG) {
         RK_LAUNCH("k_trace_queue(pass_c)", (k_trace_queue<STACK, kTracePassC>), dim3(gridCap), dim3(256), 0, s,
@@ -1478,13 +1481,13 @@ static hipError_t run_c_queued(PassCArgs args, const CQueues &q, uint32_t gridCa
 }
 template <int STACK>
 static hipError_t dispatch_c_queued(int tile, const PassCArgs &args, const CQueues &q,
-                                    uint32_t gridCap, hipStream_t s) {
+                                    uint32_t gridCap, bool packSurface, hipStream_t s) {
     switch (tile) {
-        case 0: return run_c_queued<16, 16, STACK>(args, q, gridCap, s);
-        case 1: return run_c_queued<64, 1, STACK>(args, q, gridCap, s);
-        case 2: return run_c_queued<8, 8, STACK>(args, q, gridCap, s);
-        case 3: return run_c_queued<16, 8, STACK>(args, q, gridCap, s);
-        default: return run_c_queued<32, 8, STACK>(args, q, gridCap, s);
+        case 0: return run_c_queued<16, 16, STACK>(args, q, gridCap, packSurface, s);
+        case 1: return run_c_queued<64, 1, STACK>(args, q, gridCap, packSurface, s);
+        case 2: return run_c_queued<8, 8, STACK>(args, q, gridCap, packSurface, s);
+        case 3: return run_c_queued<16, 8, STACK>(args, q, gridCap, packSurface, s);
+        default: return run_c_queued<32, 8, STACK>(args, q, gridCap, packSurface, s);
     }
 }
 
@@ -1580,13 +1583,16 @@ hipError_t pass_c_rays(const void *queueScratch, uint32_t npx, int k, const rest
 
 hipError_t launch_pass_c(int tile, int stack, const restir_uniforms &u, int k, int iter,
                          const SceneView &sc, const GView &g, const restir_reservoir *in,
-                         restir_reservoir *out, void *queueScratch, uint32_t gridCap, hipStream_t s) {
+                         restir_reservoir *out, void *queueScratch, uint32_t gridCap, hipStream_t s,
+                         bool packSurface) {
     // unbiasedReuse.glsl:72 seeds with frame*17; iter > 0 offsets the seed of repeated passes
     PassCArgs args{u, sc, g, in, out, 0, 0, k, u.frame * 17u + static_cast<uint32_t>(iter)};
     if (queueScratch) {
         CQueues q = carve_queues(queueScratch, u.screenSize[0] * u.screenSize[1], k);
         q.merged = out;
-        return by_stack(stack, [&](auto S) { return dispatch_c_queued<decltype(S)::value>(tile, args, q, gridCap, s); });
+        return by_stack(stack, [&](auto S) {
+            return dispatch_c_queued<decltype(S)::value>(tile, args, q, gridCap, packSurface, s);
+        });
     }
     return by_stack(stack, [&](auto S) { return dispatch_c<decltype(S)::value>(tile, args, s); });
 }

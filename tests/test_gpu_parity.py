@@ -191,6 +191,57 @@ def test_input_visible_flag_same_reservoirs_fewer_rays(torch):
     assert c1 < 0.9 * c0 and n1 < 0.95 * n0, rays
 
 
+def test_input_visible_flag_reuses_surface_records(torch):
+    """A flagged Pass C right after a queued Pass C on the same G-buffer planes, size and k reuses
+    that pass's packed surface records (no k_pack_surface launch). On a copy of the planes, after
+    a queued Pass A on the context, or after a Pass C with another k it packs its own. The
+    reservoirs equal those of an unflagged pass in every case."""
+    sc = scenes.config_scene("terrain")
+    W, H, k = 160, 90, 5
+    r = Renderer(sc, W, H, mode="unbiased", k=k, spatial_passes=2)
+    for f in range(2):
+        r.render_frame(scenes.orbit_camera(sc.camera, f))
+    cam = scenes.orbit_camera(sc.camera, 2)
+    r.begin_frame(cam)
+    u, _ = r.uniforms(cam)
+    i, p = r.cur, 1 - r.cur
+    g = r.gstruct(i)
+    r.ctx.initial_temporal(u, g, r.gstruct(p), r.res[p], r.res[i])
+    later = abi.Uniforms.from_buffer_copy(u)
+    later.flags |= abi.INPUT_VISIBLE_FLAG
+    first, junk = torch.zeros_like(r.tmp), torch.zeros_like(r.tmp)
+    r.ctx.unbiased(u, k, 0, g, r.res[i], first)
+    want = torch.zeros_like(r.tmp)
+    r.ctx.unbiased(u, k, 1, g, first, want)  # unflagged: the reference result
+    planes = r.gbuf[i].clone()
+    g_copy = abi.gbuffer_struct(planes.data_ptr(), W, H)
+
+    def packs_and_out(before, g2):
+        r.ctx.unbiased(u, k, 0, g, r.res[i], junk)  # a queued Pass C on g packs its records
+        before()
+        torch.cuda.synchronize()
+        r.ctx.profile_read()
+        r.ctx.profile_enable(True)
+        out = torch.zeros_like(r.tmp)
+        r.ctx.unbiased(later, k, 1, g2, first, out)
+        torch.cuda.synchronize()
+        rows = r.ctx.profile_read()
+        r.ctx.profile_enable(False)
+        return rows.get("k_pack_surface", (0,))[0], out
+
+    cases = {
+        "same planes": (lambda: None, g, 0),
+        "copied planes": (lambda: None, g_copy, 1),
+        "queued Pass A between": (lambda: r.ctx.initial_temporal(u, g, r.gstruct(p), r.res[p], torch.zeros_like(r.tmp)),
+                                  g, 1),
+        "other k between": (lambda: r.ctx.unbiased(u, 3, 0, g, r.res[i], torch.zeros_like(r.tmp)), g, 1),
+    }
+    for name, (before, g2, packs) in cases.items():
+        n, out = packs_and_out(before, g2)
+        assert n == packs, (name, n)
+        assert torch.equal(out, want), name
+
+
 @pytest.mark.parametrize("schedule", [abi.SCHEDULE_QUEUED, abi.SCHEDULE_INLINE])
 def test_standard_weights_bit_exact(torch, schedule):
     """RESTIR_STANDARD_WEIGHTS_FLAG (standard RIS W, triangle lights by area, sumW = p_hat W M

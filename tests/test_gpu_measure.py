@@ -30,7 +30,8 @@ def test_profile_records_each_kernel(torch):
     assert rows["k_pass_a_ris"][0] == 1 and rows["k_pass_a_finish"][0] == 1
     assert rows["k_trace_queue(pass_a)"][0] == 1
     assert rows["k_pass_c_merge"][0] == 2 and rows["k_trace_queue(pass_c)"][0] == 2
-    assert rows["k_pack_surface"][0] == 2
+    # the second, flagged Pass C reuses the surface records the first packed from the same planes
+    assert rows["k_pack_surface"][0] == 1
     assert rows["k_shade"][0] == 1
     for name, (n, total, mx) in rows.items():
         assert n >= 1 and total > 0.0 and 0.0 < mx <= total + 1e-6, (name, n, total, mx)

@@ -208,21 +208,69 @@ class Frames:
         return {"choice": best, "ms": {k: round(v, 4) for k, v in table.items()}}
 
     def timed(self, first, count, M, learned=None):
-        """count frames from `first`, every kernel timed with events; (wall ms/frame, kernels)."""
+        """count frames from `first`, twice from the same state: eagerly, every kernel timed with
+        the library's events (the per-kernel table), then captured into one HIP graph and replayed
+        once, as a renderer replays its recorded command buffers (the frame time). The two runs
+        compute the same frames bit for bit and leave the reservoirs, and a learned proposal's
+        parameters and Adam state, as the eager run left them. Returns (ms/frame, kernels);
+        self.last_timing records both times (the eager one alone if the capture fails)."""
         torch, ctx = self.torch, self.ctx
         torch.cuda.synchronize()
+        bufs = (*self.res, self.tmp)
+        saved = [t.clone() for t in bufs]
+        start = learned.state() if learned is not None else None
         ctx.profile_read()
         ctx.profile_enable(True)
         t0 = time.perf_counter()
         for f in range(first, first + count):
             self.run(f, M, learned=learned)
         torch.cuda.synchronize()
-        wall = (time.perf_counter() - t0) * 1e3 / count
+        eager = (time.perf_counter() - t0) * 1e3 / count
         prof = ctx.profile_read()
         ctx.profile_enable(False)
         kernels = {name: {"launches_per_frame": n / count, "ms_per_frame": tot / count, "avg_ms": tot / n}
                    for name, (n, tot, _mx) in prof.items()}
-        return wall, kernels
+        after = [t.clone() for t in bufs]
+        end = learned.state() if learned is not None else None
+
+        def set_state(values, st):
+            for t, v in zip(bufs, values):
+                t.copy_(v)
+            if learned is not None:
+                learned.load_state(st)
+            torch.cuda.synchronize()
+
+        set_state(saved, start)
+        own = self.stream
+        cs = torch.cuda.Stream()
+        cs.wait_stream(torch.cuda.current_stream())
+        graph = torch.cuda.CUDAGraph()
+        try:
+            self.stream = cs
+            with torch.cuda.graph(graph, stream=cs):
+                for f in range(first, first + count):
+                    self.run(f, M, learned=learned)
+            self.stream = own
+            set_state(saved, start)  # capturing ran nothing, but advanced the host-side seeds
+            t0 = time.perf_counter()
+            graph.replay()
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / count
+            same = all(torch.equal(t, v) for t, v in zip(bufs, after))
+            self.last_timing = {"frame_time": "one HIP graph of the timed frames, replayed once",
+                                "ms_per_frame_graph": round(ms, 4), "ms_per_frame_eager": round(eager, 4),
+                                "graph_equals_eager": same}
+            if not same:
+                raise RuntimeError("the graph replay's reservoirs differ from the eager run's")
+        except RuntimeError as e:
+            self.stream = own
+            torch.cuda.synchronize()
+            ms = eager
+            self.last_timing = {"frame_time": f"eager (graph capture failed: {str(e)[:200]})",
+                                "ms_per_frame_eager": round(eager, 4)}
+        del graph
+        set_state(after, end)
+        return ms, kernels
 
 
 def main():
@@ -275,7 +323,8 @@ def main():
     for f in range(WU):
         fr.run(f, args.M)
     plain_ms, kernels = fr.timed(WU, K, args.M)
-    log(f"plain M={args.M}: {plain_ms:.3f} ms/frame; " + ", ".join(
+    plain_timing = fr.last_timing
+    log(f"plain M={args.M}: {plain_ms:.3f} ms/frame (eager {plain_timing['ms_per_frame_eager']:.3f}); " + ", ".join(
         f"{k} {v['ms_per_frame']:.3f}" for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_frame"])))
 
     # ---- the last timed frame again, untimed and bit-identical (same inputs and seeds; a frame
@@ -411,7 +460,7 @@ def main():
             "M_plain": args.M, "M_learned": args.learned_M, "weights": args.weights,
             "k": args.k, "spatial_passes": args.spatial_passes, "tile": args.tile,
         },
-        "plain": {"ms_per_frame": round(plain_ms, 4), "pass_a_schedule": sched_plain,
+        "plain": {"ms_per_frame": round(plain_ms, 4), "frame_timing": plain_timing, "pass_a_schedule": sched_plain,
                   "kernels": rounded(kernels), "resampling_passes": resampling},
         "mrays_per_s": round(rays_per_s / 1e6, 1),
         "traversal": {"rays": nr, "ray_set": "last frame's second Pass C neighbour rays",
@@ -467,7 +516,9 @@ def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
     for f in range(WU):
         fr.run(f, args.learned_M, learned=lp)
     learned_ms, lk = fr.timed(WU, K, args.learned_M, learned=lp)
-    log(f"learned M={args.learned_M}: {learned_ms:.3f} ms/frame; " + ", ".join(
+    learned_timing = fr.last_timing
+    log(f"learned M={args.learned_M}: {learned_ms:.3f} ms/frame (eager {learned_timing['ms_per_frame_eager']:.3f}); "
+        + ", ".join(
         f"{k} {v['ms_per_frame']:.3f}" for k, v in sorted(lk.items(), key=lambda kv: -kv[1]["ms_per_frame"])))
     for v in lk.values():
         for key in ("launches_per_frame", "ms_per_frame", "avg_ms"):
@@ -534,7 +585,8 @@ def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
     return {
         "value": round(value, 4),
         "ms_per_step": round(learned_ms, 4),
-        "learned": {"ms_per_frame": round(learned_ms, 4), "pass_a_schedule": sched_learned, "kernels": lk,
+        "learned": {"ms_per_frame": round(learned_ms, 4), "frame_timing": learned_timing,
+                    "pass_a_schedule": sched_learned, "kernels": lk,
                     "train_loss": round(lp.loss.item(), 4),
                     "pretrain_frames": args.pretrain, "mix": args.mix, "train_every": args.train_every},
         "equal_rmse": {
@@ -828,10 +880,11 @@ def stress_4k(torch, args):
         for f in range(WU):
             fr.run(f, M, learned=learned)
         ms, kernels = fr.timed(WU, K, M, learned=learned)
-        log(f"{name} M={M}: {ms:.2f} ms/frame; " + ", ".join(
+        log(f"{name} M={M}: {ms:.2f} ms/frame (eager {fr.last_timing['ms_per_frame_eager']:.2f}); " + ", ".join(
             f"{k} {v['ms_per_frame']:.3f}" for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms_per_frame"])[:8]))
         out[f"{name}_M"] = M
         out[f"{name}_ms_per_frame"] = round(ms, 3)
+        out[f"{name}_frame_timing"] = fr.last_timing
         out[f"{name}_kernels_ms_per_frame"] = {k: round(v["ms_per_frame"], 4) for k, v in kernels.items()}
     if args.stress_paths > 0:
         out.update(stress_paths_graph(torch, args, scene, ctx, lp, launch, W, H))

@@ -98,6 +98,10 @@ def parse():
                     help="share of the learned candidates drawn from the alias table (bounds q below)")
     ap.add_argument("--train-every", type=int, default=1,
                     help="learned pipeline: one Adam step every this many frames (pretraining too)")
+    ap.add_argument("--train-on", choices=("final", "first"), default="final",
+                    help="learned pipeline: the reservoirs its Adam step trains on, the frame's final ones "
+                         "(after the last Pass C) or the first Pass C's output, which lets the step run on a "
+                         "second stream during the later Pass C")
     ap.add_argument("--error-frames", type=int, default=256, help="independent frames per method")
     ap.add_argument("--error-pixels", type=int, default=16384)
     ap.add_argument("--error-seed", type=int, default=0,
@@ -132,6 +136,7 @@ class Frames:
         from restir_amd.abi import make_uniforms, projection_view
         self.torch, self.ctx, self.cams, self.G, self.W, self.H = torch, ctx, cams, G, W, H
         self.args, self.launch, self.stream = args, launch, stream
+        self.side = torch.cuda.Stream()  # the learned pipeline's Adam step, when it trains early
         self.abi, self.make_uniforms = abi, make_uniforms
         self.pvs = [projection_view(c, W / H) for c in cams]
         npx = W * H
@@ -172,13 +177,22 @@ class Frames:
         # segments found unoccluded there (RESTIR_INPUT_VISIBLE_FLAG; the same results)
         later = self.abi.Uniforms.from_buffer_copy(u)
         later.flags |= self.abi.INPUT_VISIBLE_FLAG
+        train_now = learned is not None and train and f % self.args.train_every == 0
+        early = train_now and self.args.train_on == "first" and n_sp > 1
         for j in range(n_sp):
             dst = self.res[i] if (n_sp - j) % 2 == 1 else self.tmp
             ctx.unbiased(later if j > 0 else u, self.args.k, j, g, src, dst, self.launch, s)
+            if early and j == 0:
+                # the Adam step on the first Pass C's output, on a second stream while the later
+                # passes run (it reads that output, which they only read too, and no buffer of theirs)
+                self.side.wait_stream(s)
+                learned.train(u, g, dst, 4096, 1e-3, self.side)
             yield j
             src = dst
         ctx.shade(self.cams[f]["position"], 1.0, g, self.res[i], self.rgb, self.W * self.H, s)
-        if learned is not None and train and f % self.args.train_every == 0:
+        if early:
+            s.wait_stream(self.side)
+        elif train_now:
             learned.train(u, g, self.res[i], 4096, 1e-3, s)
 
     def run(self, f, M, **kw):
@@ -588,7 +602,8 @@ def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
         "learned": {"ms_per_frame": round(learned_ms, 4), "frame_timing": learned_timing,
                     "pass_a_schedule": sched_learned, "kernels": lk,
                     "train_loss": round(lp.loss.item(), 4),
-                    "pretrain_frames": args.pretrain, "mix": args.mix, "train_every": args.train_every},
+                    "pretrain_frames": args.pretrain, "mix": args.mix, "train_every": args.train_every,
+                    "train_on": args.train_on},
         "equal_rmse": {
             "protocol": f"static cameras at orbit frames {list(views)} (fixed; independent of --steps and "
                         f"--warmup), {per_view} independent frames per view and method (history cleared), "

@@ -102,6 +102,11 @@ def parse():
                     help="learned pipeline: the reservoirs its Adam step trains on, the frame's final ones "
                          "(after the last Pass C) or the first Pass C's output, which lets the step run on a "
                          "second stream during the later Pass C")
+    ap.add_argument("--tuned-M", type=int, default=12,
+                    help="the second learned setting's candidates (equal_rmse_tuned)")
+    ap.add_argument("--tuned-mix", type=float, default=0.5,
+                    help="the second learned setting's share of alias-table candidates")
+    ap.add_argument("--no-tuned", action="store_true", help="skip the second learned setting")
     ap.add_argument("--error-frames", type=int, default=256, help="independent frames per method")
     ap.add_argument("--error-pixels", type=int, default=16384)
     ap.add_argument("--error-seed", type=int, default=0,
@@ -509,40 +514,67 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def speedup_stats(rp, rl, plain_ms, learned_ms):
+    """Equal-error speedup of learned over plain from per-frame relMSE arrays [view, frame]: the
+    ratio of ms/frame x mean relMSE, its bootstrap 95% CI (frames resampled within each view) and
+    the same ratio of medians."""
+    mp, ml = rp.mean(), rl.mean()  # every view weighs the same
+    speedup = (plain_ms * mp) / (learned_ms * ml)
+    brng = np.random.default_rng(2024)
+    boots = np.empty(BOOTSTRAP)
+    n = rp.shape[1]
+    for b in range(BOOTSTRAP):
+        bp = np.take_along_axis(rp, brng.integers(0, n, size=rp.shape), axis=1).mean()
+        bl = np.take_along_axis(rl, brng.integers(0, n, size=rl.shape), axis=1).mean()
+        boots[b] = (plain_ms * bp) / (learned_ms * bl)
+    lo, hi = np.quantile(boots, [0.025, 0.975])
+    med_ratio = (plain_ms * np.median(rp)) / (learned_ms * np.median(rl))
+    return speedup, lo, hi, med_ratio
+
+
 def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
     """The learned pipeline's timed frames, then both pipelines' per-frame error against the FP64
     ground truth (GPU, restir_ground_truth) at fixed views of the orbit (ERROR_VIEWS), with a
-    bootstrap confidence interval of the equal-error speedup."""
+    bootstrap confidence interval of the equal-error speedup. The workload's setting (M, mix)
+    gives `value`; unless --no-tuned, a second proposal trained the same way with --tuned-M
+    candidates and a --tuned-mix share of source draws is timed and measured against the same
+    ground truth and plain frames (equal_rmse_tuned)."""
     from restir_amd.learned import LearnedProposal
 
     K, WU, W, H = args.steps, args.warmup, fr.W, fr.H
-    nframes = WU + K
     n_orbit = len(fr.cams)
-    lp = LearnedProposal(ctx, scene, k=256, seed=7, mix=args.mix)
-    t0 = time.time()
-    for i in range(args.pretrain):  # online training over the orbit (untimed)
-        fr.run(i % n_orbit, args.learned_M, learned=lp)
-    torch.cuda.synchronize()
-    log(f"learned: {args.pretrain} training frames ({time.time() - t0:.1f}s), loss {lp.loss.item():.3f}")
-    sched_learned = fr.choose_pass_a_schedule(WU - 1, args.learned_M, learned=lp)
-    for t in fr.res:
-        t.zero_()
-    for f in range(WU):
-        fr.run(f, args.learned_M, learned=lp)
-    learned_ms, lk = fr.timed(WU, K, args.learned_M, learned=lp)
-    learned_timing = fr.last_timing
-    log(f"learned M={args.learned_M}: {learned_ms:.3f} ms/frame (eager {learned_timing['ms_per_frame_eager']:.3f}); "
-        + ", ".join(
-        f"{k} {v['ms_per_frame']:.3f}" for k, v in sorted(lk.items(), key=lambda kv: -kv[1]["ms_per_frame"])))
-    for v in lk.values():
-        for key in ("launches_per_frame", "ms_per_frame", "avg_ms"):
-            v[key] = round(v[key], 4)
+
+    def train_and_time(M, mix):
+        lp = LearnedProposal(ctx, scene, k=256, seed=7, mix=mix)
+        t0 = time.time()
+        for i in range(args.pretrain):  # online training over the orbit (untimed)
+            fr.run(i % n_orbit, M, learned=lp)
+        torch.cuda.synchronize()
+        log(f"learned (M={M}, mix {mix}): {args.pretrain} training frames ({time.time() - t0:.1f}s), "
+            f"loss {lp.loss.item():.3f}")
+        sched = fr.choose_pass_a_schedule(WU - 1, M, learned=lp)
+        for t in fr.res:
+            t.zero_()
+        for f in range(WU):
+            fr.run(f, M, learned=lp)
+        ms, kern = fr.timed(WU, K, M, learned=lp)
+        timing = fr.last_timing
+        log(f"learned M={M}: {ms:.3f} ms/frame (eager {timing['ms_per_frame_eager']:.3f}); " + ", ".join(
+            f"{k} {v['ms_per_frame']:.3f}" for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms_per_frame"])))
+        for v in kern.values():
+            for key in ("launches_per_frame", "ms_per_frame", "avg_ms"):
+                v[key] = round(v[key], 4)
+        return {"lp": lp, "M": M, "mix": mix, "ms": ms, "kernels": kern, "timing": timing, "sched": sched}
+
+    runs = [train_and_time(args.learned_M, args.mix)]
+    if not args.no_tuned:
+        runs.append(train_and_time(args.tuned_M, args.tuned_mix))
 
     # ---- per-frame error at static cameras (fixed views of the orbit), independent frames: the
     # history is cleared before each frame and the previous G-buffer is the current one
     views = [v for v in ERROR_VIEWS if v < n_orbit]
     per_view = max(1, args.error_frames // len(views))
-    rp, rl, n_pix = [], [], 0
+    rp, rls, n_pix = [], [[] for _ in runs], 0
     t_gt = 0.0
     for vi, f_err in enumerate(views):
         gi = f_err + 1
@@ -564,6 +596,8 @@ def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
         idx = d_pix.long()
 
         def errors(M, learned):
+            if learned is not None:
+                learned.bind()  # the clusters of the proposal in use (every proposal here has the same)
             rel = []
             for e in range(per_view):
                 for t in fr.res:
@@ -576,56 +610,72 @@ def learned_and_error(torch, ctx, scene, sd, fr, args, plain_ms):
             return np.array(rel)
 
         rp.append(errors(args.M, None))
-        rl.append(errors(args.learned_M, lp))
+        for r, rl in zip(runs, rls):
+            rl.append(errors(r["M"], r["lp"]))
     torch.cuda.synchronize()
     log(f"FP64 ground truth of {n_pix} pixels over {len(views)} views: {t_gt:.2f}s")
-    rp, rl = np.stack(rp), np.stack(rl)  # [view, frame]
-    mp, ml = rp.mean(), rl.mean()  # every view weighs the same
-    speedup = (plain_ms * mp) / (learned_ms * ml)
-    # bootstrap over frames, resampled within each view
-    brng = np.random.default_rng(2024)
-    boots = np.empty(BOOTSTRAP)
-    for b in range(BOOTSTRAP):
-        ip = brng.integers(0, per_view, size=rp.shape)
-        il = brng.integers(0, per_view, size=rl.shape)
-        bp = np.take_along_axis(rp, ip, axis=1).mean()
-        bl = np.take_along_axis(rl, il, axis=1).mean()
-        boots[b] = (plain_ms * bp) / (learned_ms * bl)
-    lo, hi = np.quantile(boots, [0.025, 0.975])
-    med_ratio = (plain_ms * np.median(rp)) / (learned_ms * np.median(rl))
-    log(f"relMSE plain {mp:.3f} (median {np.median(rp):.3f}), learned {ml:.3f} (median {np.median(rl):.3f}); "
-        f"equal-error speedup learned vs plain {speedup:.3f} (95% CI {lo:.3f}-{hi:.3f}); of medians {med_ratio:.3f}")
-    value = learned_ms * ml / mp
-    return {
-        "value": round(value, 4),
-        "ms_per_step": round(learned_ms, 4),
-        "learned": {"ms_per_frame": round(learned_ms, 4), "frame_timing": learned_timing,
-                    "pass_a_schedule": sched_learned, "kernels": lk,
-                    "train_loss": round(lp.loss.item(), 4),
-                    "pretrain_frames": args.pretrain, "mix": args.mix, "train_every": args.train_every,
-                    "train_on": args.train_on},
-        "equal_rmse": {
-            "protocol": f"static cameras at orbit frames {list(views)} (fixed; independent of --steps and "
-                        f"--warmup), {per_view} independent frames per view and method (history cleared), "
-                        f"FP64 GPU ground truth on up to {args.error_pixels} seeded non-emissive surface pixels "
-                        f"per view (triangle lights by {TRUTH_TRI_SAMPLES} stratified samples each); relMSE = "
-                        "mean (x - gt)^2 / (gt^2 + 1e-2) per frame; time to a target error is proportional to "
-                        "ms/frame x mean relMSE; 95% CI by bootstrap over frames within each view",
-            "views": list(views), "frames_per_view": per_view, "error_seed": args.error_seed,
-            "relmse_plain": round(float(mp), 4), "relmse_plain_median": round(float(np.median(rp)), 4),
+    rp = np.stack(rp)  # [view, frame]
+    mp = rp.mean()
+    protocol = (f"static cameras at orbit frames {list(views)} (fixed; independent of --steps and "
+                f"--warmup), {per_view} independent frames per view and method (history cleared), "
+                f"FP64 GPU ground truth on up to {args.error_pixels} seeded non-emissive surface pixels "
+                f"per view (triangle lights by {TRUTH_TRI_SAMPLES} stratified samples each); relMSE = "
+                "mean (x - gt)^2 / (gt^2 + 1e-2) per frame; time to a target error is proportional to "
+                "ms/frame x mean relMSE; 95% CI by bootstrap over frames within each view")
+    studies = []
+    for r, rl in zip(runs, rls):
+        rl = np.stack(rl)
+        ml = rl.mean()
+        speedup, lo, hi, med_ratio = speedup_stats(rp, rl, plain_ms, r["ms"])
+        log(f"M={r['M']}, mix {r['mix']}: relMSE plain {mp:.3f} (median {np.median(rp):.3f}), learned {ml:.3f} "
+            f"(median {np.median(rl):.3f}); equal-error speedup learned vs plain {speedup:.3f} "
+            f"(95% CI {lo:.3f}-{hi:.3f}); of medians {med_ratio:.3f}")
+        studies.append({
             "relmse_learned": round(float(ml), 4),
             "relmse_learned_median": round(float(np.median(rl)), 4),
-            "relmse_plain_by_view": [round(float(v), 4) for v in rp.mean(axis=1)],
             "relmse_learned_by_view": [round(float(v), 4) for v in rl.mean(axis=1)],
-            "relmse_plain_p90_max": [round(float(np.quantile(rp, 0.9)), 3), round(float(rp.max()), 3)],
             "relmse_learned_p90_max": [round(float(np.quantile(rl, 0.9)), 3), round(float(rl.max()), 3)],
-            "ms_plain": round(plain_ms, 4), "ms_learned": round(learned_ms, 4),
+            "ms_learned": round(r["ms"], 4),
             "speedup_learned_vs_plain": round(float(speedup), 4),
             "speedup_ci95": [round(float(lo), 4), round(float(hi), 4)],
             "speedup_of_medians": round(float(med_ratio), 4),
             "target": ">= 2x (not met)" if speedup < 2 else ">= 2x (met)",
+            "value": learned_ms_value(r["ms"], ml, mp),
+        })
+    main, first = runs[0], studies[0]
+    out = {
+        "value": first.pop("value"),
+        "ms_per_step": round(main["ms"], 4),
+        "learned": {"ms_per_frame": round(main["ms"], 4), "frame_timing": main["timing"],
+                    "pass_a_schedule": main["sched"], "kernels": main["kernels"],
+                    "train_loss": round(main["lp"].loss.item(), 4),
+                    "pretrain_frames": args.pretrain, "mix": args.mix, "train_every": args.train_every,
+                    "train_on": args.train_on},
+        "equal_rmse": {
+            "protocol": protocol,
+            "views": list(views), "frames_per_view": per_view, "error_seed": args.error_seed,
+            "relmse_plain": round(float(mp), 4), "relmse_plain_median": round(float(np.median(rp)), 4),
+            "relmse_plain_by_view": [round(float(v), 4) for v in rp.mean(axis=1)],
+            "relmse_plain_p90_max": [round(float(np.quantile(rp, 0.9)), 3), round(float(rp.max()), 3)],
+            "ms_plain": round(plain_ms, 4), **first,
         },
     }
+    if len(runs) > 1:
+        tuned, st = runs[1], studies[1]
+        out["equal_rmse_tuned"] = {
+            "setting": f"learned M={tuned['M']}, mix {tuned['mix']} (not the workload's M={args.learned_M}, "
+                       f"mix {args.mix}, which `value` keeps); same training, ground truth, plain frames "
+                       "and protocol as equal_rmse",
+            "M": tuned["M"], "mix": tuned["mix"], "frame_timing": tuned["timing"],
+            "learned_ms_at_plain_error": st.pop("value"), **st,
+        }
+        main["lp"].bind()
+    return out
+
+
+def learned_ms_value(ms, ml, mp):
+    """The learned pipeline's ms/frame at the plain pipeline's per-frame error."""
+    return round(ms * ml / mp, 4)
 
 
 def scene_builds(torch, scene):

@@ -17,6 +17,10 @@ independent frames) gives the cost to equal error: frames of an unbiased estimat
 target error in proportion to their per-frame relative MSE, so the time to reach any error is
 proportional to ms/frame x relMSE.
 
+A second learned setting (M = 12, half the candidates from the alias table) is trained, timed and
+measured against the same ground truth and plain frames and reported beside the first
+(equal_rmse_tuned); value keeps the workload's setting.
+
 Prints one JSON line: value = the learned pipeline's ms/frame at the plain pipeline's per-frame
 error (ms_learned x relMSE_learned / relMSE_plain), the measured ms/frame of both, per-kernel
 times, traversal Mrays/s and its roofline, the roofline of the dominant kernel, and the CPU-oracle

@@ -617,8 +617,13 @@ __global__ __launch_bounds__(2 * kProposeTile) void k_propose(const ProposeArgs 
         int c = 32 * blk + 31;
         float run = base;
         bool hit = false;
-        for (int q = 0; q < 8 && !hit; ++q) {
-            const float4 f = unpack_h4(row[8 * blk + q]);
+        // the block's 32 numerators in one batch of LDS reads, then the same sequential sum
+        uint2 raw[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) raw[q] = row[8 * blk + q];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float4 f = unpack_h4(raw[q]);
             const float e4[4] = {f.x, f.y, f.z, f.w};
 #pragma unroll
             for (int k = 0; k < 4; ++k) {

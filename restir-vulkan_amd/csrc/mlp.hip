@@ -581,10 +581,14 @@ __global__ __launch_bounds__(2 * kProposeTile) void k_propose(const ProposeArgs 
     // compares and scans at most 32 numerators of that block
     const uint2 *row = reinterpret_cast<const uint2 *>(reinterpret_cast<const __half *>(lds) + (tid >> 1) * (KC + 4));
     constexpr int kBlocks = KC / 32;
+    // the two lanes of a quad read the same row: each sums every other block (block 2j + lane
+    // parity), and the partner's sums come by a lane swap; se then adds the blocks in order
+    static_assert(kBlocks % 2 == 0, "block pairs");
+    const bool odd = (tid & 1) != 0;
     float bs[kBlocks];
-    float se = 0.0f;
 #pragma unroll
-    for (int b = 0; b < kBlocks; ++b) {
+    for (int j = 0; j < kBlocks / 2; ++j) {
+        const int b = 2 * j + (odd ? 1 : 0);
         float sum = 0.0f;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
@@ -594,9 +598,13 @@ __global__ __launch_bounds__(2 * kProposeTile) void k_propose(const ProposeArgs 
             sum += f.z;
             sum += f.w;
         }
-        bs[b] = sum;
-        se += sum;
+        const float other = __shfl_xor(sum, 1);
+        bs[2 * j] = odd ? other : sum;
+        bs[2 * j + 1] = odd ? sum : other;
     }
+    float se = 0.0f;
+#pragma unroll
+    for (int b = 0; b < kBlocks; ++b) se += bs[b];
     const float rse = 1.0f / se;
     // cluster of the draw t in [0, se): the first c whose running sum exceeds t (numerators of
     // zero are never chosen); rounding overflow falls to the last cluster of the block
